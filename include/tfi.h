@@ -1,0 +1,195 @@
+/*
+ * tfi.h -- C-ABI of libtfi_hip.so, the MI355X-native body-fitted IGA
+ * Navier-Stokes hot path.
+ *
+ * The reference has no library or FFI for this path: its GPU backend is the
+ * process `iga_cuda navier-stokes` (solvers/cuda/src/iga_cuda.cu:734-1153)
+ * built from header-only CUDA/C++ classes.  Each entry point below names the
+ * reference routine it replaces.  Conventions:
+ *   - every function returns 0 on success, non-zero on failure; the message is
+ *     available from tfi_last_error(ctx) (no exceptions cross the ABI);
+ *   - host arrays are borrowed for the duration of the call only;
+ *   - device memory is owned by the library objects; one HIP stream per context;
+ *   - a context is not thread-safe (one host thread drives it, like the
+ *     reference's single-process driver, CudaExecution.hpp:13-32);
+ *   - vectors are node-major [u_x, u_y, u_z, p] per node (iga_cuda.cu:360-363);
+ *   - block values are 4x4 row-major (row field * 4 + column field)
+ *     (IgaCudaKernels.cuh:389-395).
+ */
+#ifndef TFI_H
+#define TFI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tfi_ctx tfi_ctx;
+typedef struct tfi_mesh tfi_mesh;
+typedef struct tfi_flow tfi_flow;
+
+enum { TFI_OK = 0, TFI_ERROR = 1 };
+enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
+enum { TFI_FP64 = 0, TFI_MIXED = 1 };
+enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2 };
+
+/* ------------------------------------------------------------------ context */
+/* Replaces the CUDA runtime setup + PrintDevice (iga_cuda.cu:278-303). */
+int tfi_ctx_create(int device, tfi_ctx** out);
+void tfi_ctx_destroy(tfi_ctx* ctx);
+const char* tfi_last_error(const tfi_ctx* ctx);
+/* One-line device description ("name=... cus=... memory_total_gib=..."). */
+int tfi_device_info(tfi_ctx* ctx, char* buffer, int length);
+/* Project device allocations: DeviceAllocationCounter (DeviceAllocation.hpp:12-28). */
+int tfi_allocation_stats(tfi_ctx* ctx, long long* peak_bytes, long long* live_bytes);
+
+/* ------------------------------------------------------------------ mesh */
+typedef struct {
+	int nodes, elements, maximum_basis;
+	long long basis_entries, extraction_nonzeros, block_nonzeros, element_pairs;
+	int colors;               /* element colours of the deterministic scatter */
+	double minimum_determinant; /* min over (element, q) of 0.125 det J */
+	long long bad_samples;    /* quadrature points with det <= 0 or non-finite */
+} tfi_mesh_stats;
+
+/* Uploads a FlatMesh (DeviceMesh.hpp:16-68, 70-118), builds the node-block
+ * pattern on the GPU (BlockPattern, BlockCsr.hpp:27-55), the element colouring
+ * and pair->block map of the deterministic scatter, and the per-quadrature-point
+ * geometry (BuildGeometryKernel, IgaCudaKernels.cuh:61-111). */
+int tfi_mesh_create(tfi_ctx* ctx, int nodes, int elements, const int32_t* element_offsets,
+	const int32_t* connectivity, const int32_t* extraction_offsets, const uint8_t* extraction_columns,
+	const double* extraction_values, const double* bezier_points, tfi_mesh** out);
+void tfi_mesh_destroy(tfi_mesh* mesh);
+int tfi_mesh_stats_get(tfi_mesh* mesh, tfi_mesh_stats* out);
+/* Downloads the block pattern: row_offsets[N+1], columns[nnzb], diagonal[N]. */
+int tfi_mesh_pattern(tfi_mesh* mesh, int32_t* row_offsets, int32_t* columns, int32_t* diagonal);
+/* Element colouring: colour of each element (int32[E]). */
+int tfi_mesh_colors(tfi_mesh* mesh, int32_t* element_color);
+
+/* ------------------------------------------------------------------ kernels (host in/out) */
+/* Negated residual and legacy tangent (AssembleNavierStokesKernel,
+ * IgaCudaKernels.cuh:265-409) with a deterministic colour-ordered scatter.
+ * jacobian may be NULL (residual only).  Arrays: state/previous [4N],
+ * jacobian [nnzb*16], negative_residual [4N]. */
+int tfi_ns_assemble(tfi_mesh* mesh, const double* state, const double* previous, double density,
+	double viscosity, double dt, double* jacobian, double* negative_residual);
+/* y = A x on the mesh pattern (BlockSpmvKernel<4>, SparseKernels.cuh:25-40);
+ * precision TFI_FP64 or TFI_MIXED (values rounded to fp32). */
+int tfi_bsr_spmv(tfi_mesh* mesh, const double* values, const double* x, double* y, int precision);
+/* 4x4 diagonal-block inverses (BuildBlockInverseKernel<4>, SparseKernels.cuh:42-90). */
+int tfi_block_inverse(tfi_mesh* mesh, const double* values, double* inverse, int* singular);
+/* Dirichlet rows and rhs (ApplyNavierStokesBoundaryKernel /
+ * SetNavierStokesBoundaryRhsKernel, IgaCudaKernels.cuh:436-463), in place. */
+int tfi_apply_dirichlet(tfi_mesh* mesh, const int32_t* velocity_mask, const int32_t* pressure_mask,
+	const double* velocity_target, const double* pressure_target, const double* state, double* jacobian,
+	double* rhs);
+/* Solve A x = b (x holds the initial guess) with FGMRES; options as in
+ * tfi_flow_options.  Replaces SolveGmres<4> (Gmres.hpp:69-213). */
+int tfi_linear_solve(tfi_mesh* mesh, const double* values, const double* b, double* x, int preconditioner,
+	int precision, int restart, int maximum_iterations, double relative_tolerance, int* iterations,
+	double* residual);
+/* Device-resident SpMV timing on the context stream with HIP events:
+ * average milliseconds per launch over `repetitions` launches on a synthetic
+ * matrix of the mesh pattern. */
+int tfi_bench_spmv(tfi_mesh* mesh, int precision, int repetitions, double* milliseconds);
+
+/* ------------------------------------------------------------------ flow solver */
+typedef struct {
+	double density, viscosity, dt;    /* dt == 0: steady */
+	int maximum_newton;               /* 12 (iga_cuda.cu:402) */
+	double nonlinear_rtol;            /* 1e-5 (iga_cuda.cu:403) */
+	double nonlinear_atol;            /* 1e-10 (iga_cuda.cu:404) */
+	double mass_rtol;                 /* 1e-3 (iga_cuda.cu:405) */
+	int linear_maximum_iterations;    /* 5000 (iga_cuda.cu:1015) */
+	double linear_rtol;               /* 1e-8 (iga_cuda.cu:1015) */
+	int restart;                      /* Krylov restart length */
+	int preconditioner;               /* TFI_PC_BJACOBI | TFI_PC_AMG */
+	int precision;                    /* TFI_FP64 | TFI_MIXED */
+	int warm_start;                   /* TFI_WARM_* */
+	int verbose;
+} tfi_flow_options;
+
+void tfi_flow_default_options(tfi_flow_options* options);
+
+#define TFI_MAX_NEWTON_RECORDS 64
+typedef struct {
+	double residual_l2, continuity_l2, continuity_sum, net_boundary_flow, relative_mass_imbalance;
+	double update_l2, linear_residual, assembly_s, linear_s;
+	int linear_iterations, singular_diagonal_blocks;
+} tfi_newton_record;
+
+typedef struct {
+	int converged, newton_iterations, linear_iterations, records;
+	double initial_residual, final_residual, tolerance, assembly_s, linear_s, total_s;
+	double warm_start_initial_ratio;  /* ||r(x0)|| / ||b|| of the first linear solve */
+	int warm_start_used;              /* 1 if the learned guess was accepted */
+	tfi_newton_record record[TFI_MAX_NEWTON_RECORDS];
+} tfi_newton_report;
+
+int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** out);
+void tfi_flow_destroy(tfi_flow* flow);
+/* state/previous [4N] (previous may be NULL: copied from state). */
+int tfi_flow_set_state(tfi_flow* flow, const double* state, const double* previous);
+int tfi_flow_get_state(tfi_flow* flow, double* state);
+/* Resolved Dirichlet data (ResolvedBoundaryConditions, CaseInput.hpp:35-47)
+ * and the assembled traction load [4N] (IntegratePressureTractionForces,
+ * PressureTractionFlow.hpp:41-64), may be NULL. */
+int tfi_flow_set_boundary(tfi_flow* flow, const int32_t* velocity_mask, const int32_t* pressure_mask,
+	const double* velocity_target, const double* pressure_target, const double* traction);
+/* Labelled boundary faces for the mass-balance test (MeasureFlowConvergence,
+ * iga_cuda.cu:520-557): count faces given as (element, local face, label). */
+int tfi_flow_set_boundary_faces(tfi_flow* flow, int count, const int32_t* element, const int32_t* face,
+	const int32_t* label);
+/* Start of a physical step: previous <- state (iga_cuda.cu:918-920), then the
+ * warm-start prediction of the step increment (learned mode). phase = inflow
+ * phase features for the learned predictor (may be NULL). */
+int tfi_flow_begin_step(tfi_flow* flow, const double* phase, int phase_count);
+/* Newton loop at the current boundary data (iga_cuda.cu:956-1046). */
+int tfi_flow_newton(tfi_flow* flow, tfi_newton_report* report);
+/* End of a physical step: history for the learned warm start. */
+int tfi_flow_commit_step(tfi_flow* flow);
+/* Per-face volumetric flows of the current state for labels (outlets):
+ * flows[i] = sum over faces with label labels[i] (IntegrateBoundaryFlow). */
+int tfi_flow_boundary_flows(tfi_flow* flow, int count, const int32_t* labels, double* flows);
+
+/* Learned warm start (MLP on 27-control-point stencils). */
+typedef struct {
+	int hidden;            /* width (128) */
+	int stencil;           /* control points per stencil (27) */
+	int train_steps;       /* optimisation steps per snapshot */
+	double learning_rate;
+	int min_snapshots;     /* snapshots before the predictor is used */
+} tfi_warm_options;
+int tfi_flow_warm_configure(tfi_flow* flow, const tfi_warm_options* options);
+/* Training statistics: snapshots seen, last training loss, accepted guesses. */
+int tfi_flow_warm_stats(tfi_flow* flow, int* snapshots, double* last_loss, int* accepted, int* rejected);
+
+/* Kernel timers accumulated on the context stream (HIP events). */
+typedef struct {
+	double spmv_ms, spmv_calls, spmv_bytes;           /* fine-level fp64 SpMV */
+	double spmv32_ms, spmv32_calls, spmv32_bytes;     /* fine-level fp32 SpMV */
+	double assembly_ms, assembly_calls, assembly_flops;
+} tfi_kernel_timers;
+int tfi_timers_get(tfi_ctx* ctx, tfi_kernel_timers* out);
+int tfi_timers_enable(tfi_ctx* ctx, int enabled);
+int tfi_timers_reset(tfi_ctx* ctx);
+
+/* ------------------------------------------------------------------ test hooks
+ * No reference counterpart: kernel-level entry points of the learned warm
+ * start, used by the parity tests against the CPU oracle.
+ * tfi_mlp_reference: FP32 forward + backward (MSE loss) of the stencil MLP on a
+ * host batch x [batch][320], target [batch][4]; weights w1 [320][H], b1 [H],
+ * w2 [H][H], b2 [H], w3 [H][4], b3 [4] (row-major, x W + b, tanh hidden).
+ * tfi_mlp_infer_bf16: the BF16-MFMA inference kernel on the same layout. */
+int tfi_mlp_reference(tfi_ctx* ctx, int hidden, int batch, const float* x, const float* target, const float* w1,
+	const float* b1, const float* w2, const float* b2, const float* w3, const float* b3, float* out, float* loss,
+	float* g_w1, float* g_b1, float* g_w2, float* g_b2, float* g_w3, float* g_b3);
+int tfi_mlp_infer_bf16(tfi_ctx* ctx, int hidden, int rows, const float* x, const float* w1, const float* b1,
+	const float* w2, const float* b2, const float* w3, const float* b3, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TFI_H */

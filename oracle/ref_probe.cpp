@@ -1,0 +1,138 @@
+// TEST INFRASTRUCTURE ONLY.  Our own driver around the reference's PETSc-free
+// headers (solvers/cpu/include/{IgaDatabase,ElementGeometry,Quadrature,
+// BoundaryFlow,BoundarySupport}.hpp), built by `make -C oracle ref` into
+// oracle/_ref/ref_probe.  It lets tests/golden/make_ref_vectors.py write
+// reference-computed vectors that pin the CPU oracle.  Nothing here ships.
+//
+// Commands (all numbers printed with 17 significant digits):
+//   ref_probe basis   ELEMENT u v w      -> det, inverse(9), per basis: value grad(3)
+//   ref_probe flow    ELEMENT face       -> IntegrateBoundaryFlow (state from file)
+//   ref_probe traction ELEMENT face p    -> IntegrateBoundaryPressureTraction (4*nen)
+//   ref_probe surface ELEMENT face       -> SurfaceJacobianMeasure at the 16 face points
+//   ref_probe ntiga   DATABASE           -> header + every element record as text
+//   ref_probe walltrace DATABASE VTK     -> WallTraceBasis node ids
+// ELEMENT file: nen, then nen connectivity ints, nen x 64 extraction values,
+// 64 x 3 Bezier points, nen x 4 nodal state values (whitespace separated).
+#include "BoundaryFlow.hpp"
+#include "BoundarySupport.hpp"
+#include "IgaDatabase.hpp"
+#include "Quadrature.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Input {
+	iga::Element element;
+	std::vector<std::array<double, 4>> state;
+};
+
+Input ReadElement(const char* path)
+{
+	std::ifstream in(path);
+	if (!in) throw std::runtime_error("cannot open element file");
+	Input r;
+	std::size_t nen = 0;
+	in >> nen;
+	r.element.connectivity.resize(nen);
+	r.element.extraction.resize(nen);
+	for (auto& c : r.element.connectivity) in >> c;
+	for (auto& row : r.element.extraction)
+		for (double& v : row) in >> v;
+	for (auto& p : r.element.bezier_points)
+		for (double& v : p) in >> v;
+	r.state.resize(nen);
+	for (auto& s : r.state)
+		for (double& v : s) in >> v;
+	if (!in) throw std::runtime_error("element file is truncated");
+	return r;
+}
+
+void Print(double v) { std::printf("%.17g\n", v); }
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+	try {
+		if (argc < 3) throw std::runtime_error("usage: ref_probe COMMAND ...");
+		const std::string cmd = argv[1];
+		if (cmd == "basis" && argc == 6) {
+			const auto in = ReadElement(argv[2]);
+			const auto b = iga::EvaluateBoundaryBasis(in.element, std::stod(argv[3]), std::stod(argv[4]),
+				std::stod(argv[5]));
+			Print(b.raw_determinant);
+			for (int i = 0; i < 3; ++i)
+				for (int j = 0; j < 3; ++j) Print(b.inverse_jacobian[i][j]);
+			for (std::size_t a = 0; a < in.element.connectivity.size(); ++a) {
+				Print(b.value[a]);
+				for (int d = 0; d < 3; ++d) Print(b.gradient[a][d]);
+			}
+		} else if (cmd == "flow" && argc == 4) {
+			const auto in = ReadElement(argv[2]);
+			Print(iga::IntegrateBoundaryFlow(in.element, std::stoul(argv[3]), in.state));
+		} else if (cmd == "traction" && argc == 5) {
+			const auto in = ReadElement(argv[2]);
+			for (double v : iga::IntegrateBoundaryPressureTraction(in.element, std::stoul(argv[3]), std::stod(argv[4])))
+				Print(v);
+		} else if (cmd == "surface" && argc == 4) {
+			const auto in = ReadElement(argv[2]);
+			const int face = std::stoi(argv[3]);
+			constexpr int fixed_axis[6] = {2, 1, 0, 1, 0, 2};
+			constexpr int varying[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 2}, {1, 2}, {0, 1}};
+			constexpr double fixed_value[6] = {0.0, 0.0, 1.0, 1.0, 0.0, 1.0};
+			for (int qi = 0; qi < 4; ++qi)
+				for (int qj = 0; qj < 4; ++qj) {
+					std::array<double, 3> c{};
+					c[fixed_axis[face]] = fixed_value[face];
+					c[varying[face][0]] = iga::kGaussFourPoints[qi];
+					c[varying[face][1]] = iga::kGaussFourPoints[qj];
+					const auto g = iga::EvaluateElementGeometry(in.element, c);
+					Print(iga::SurfaceJacobianMeasure(g, face));
+				}
+		} else if (cmd == "ntiga" && argc == 3) {
+			iga::Database db(argv[2]);
+			const auto& h = db.header();
+			std::printf("version %u ranks %u elements %llu nodes %llu\n", h.version, h.ranks,
+				static_cast<unsigned long long>(h.elements), static_cast<unsigned long long>(h.nodes));
+			std::printf("transform %.17g %.17g %.17g %.17g %.17g\n", h.geometry_transform.source_origin[0],
+				h.geometry_transform.source_origin[1], h.geometry_transform.source_origin[2],
+				h.geometry_transform.source_units_per_normalized_unit, h.geometry_transform.source_length_scale_to_m);
+			for (std::uint64_t e = 0; e < h.elements; ++e) {
+				const auto el = db.Load(e);
+				std::printf("element %llu type %d owner %d nen %zu labels", static_cast<unsigned long long>(el.id),
+					el.type, el.owner, el.connectivity.size());
+				for (int l : el.boundary_labels) std::printf(" %d", l);
+				std::printf("\nconn");
+				for (int c : el.connectivity) std::printf(" %d", c);
+				std::printf("\nextraction");
+				for (const auto& row : el.extraction)
+					for (std::size_t p = 0; p < 64; ++p)
+						if (row[p] != 0.0) std::printf(" %zu:%.17g", p, row[p]);
+				std::printf("\nbezier");
+				for (const auto& p : el.bezier_points) std::printf(" %.17g %.17g %.17g", p[0], p[1], p[2]);
+				std::printf("\n");
+			}
+			for (std::uint32_t r = 0; r < h.ranks; ++r) {
+				std::printf("rank %u required", r);
+				for (auto i : db.RequiredElementIndices(static_cast<std::int32_t>(r)))
+					std::printf(" %llu", static_cast<unsigned long long>(i));
+				std::printf("\n");
+			}
+		} else if (cmd == "walltrace" && argc == 4) {
+			iga::Database db(argv[2]);
+			const auto mesh = iga::ReadLabeledHexMesh(argv[3], db.header().nodes, db.header().elements);
+			for (auto n : iga::WallTraceBasis(db, mesh)) std::printf("%d\n", n);
+		} else {
+			throw std::runtime_error("unknown command or wrong argument count");
+		}
+		return 0;
+	} catch (const std::exception& e) {
+		std::fprintf(stderr, "ref_probe: %s\n", e.what());
+		return 1;
+	}
+}

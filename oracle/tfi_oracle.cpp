@@ -1,0 +1,912 @@
+// TEST INFRASTRUCTURE ONLY — the CPU oracle.
+//
+// A plain C++ (FP64) restatement of the reference's body-fitted IGA
+// Navier-Stokes arithmetic.  It is the checker for the HIP path: only the
+// tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it,
+// and never as the thing measured or shipped.  Every function cites the
+// reference file:line it follows (paths relative to the reference root).
+//
+// Pinning (which independent check fixes each function) is listed in
+// tests/test_oracle.py; functions without such a check are marked
+// "parity unpinned" below.
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_error;
+
+// Gauss-Legendre 4-point rule mapped to [0,1]; weights kept on [-1,1]
+// (sum 2), exactly as solvers/cpu/include/Quadrature.hpp:63-66 and
+// solvers/cuda/src/iga_cuda.cu:80-83 state them.
+constexpr double kGaussPoint[4] = {0.06943184420297371, 0.33000947820757187,
+	0.6699905217924281, 0.9305681557970262};
+constexpr double kGaussWeight[4] = {0.3478548451374539, 0.6521451548625461,
+	0.6521451548625461, 0.3478548451374539};
+
+// Cubic Bernstein polynomials and their first two derivatives
+// (solvers/cpu/include/TransportElement.hpp:120-134).
+void Bernstein1D(double x, double b[4], double db[4], double d2b[4])
+{
+	const double y = 1.0 - x;
+	b[0] = y*y*y; b[1] = 3.0*y*y*x; b[2] = 3.0*y*x*x; b[3] = x*x*x;
+	db[0] = -3.0*y*y; db[1] = 3.0 - 12.0*x + 9.0*x*x; db[2] = 3.0*(2.0 - 3.0*x)*x; db[3] = 3.0*x*x;
+	d2b[0] = 6.0*y; d2b[1] = -12.0 + 18.0*x; d2b[2] = 6.0 - 18.0*x; d2b[3] = 6.0*x;
+}
+
+struct BernsteinPoint {
+	double value[64];
+	double grad[64][3];
+	double hess[64][3][3];
+};
+
+// Tensor Bernstein basis at (u,v,w); Bezier index p = i + 4 j + 16 k with i
+// running along u (TransportElement.hpp:135-145, iga_cuda.cu:105-123).
+void TensorBernstein(double u, double v, double w, BernsteinPoint& out)
+{
+	double b[3][4], db[3][4], d2b[3][4];
+	Bernstein1D(u, b[0], db[0], d2b[0]);
+	Bernstein1D(v, b[1], db[1], d2b[1]);
+	Bernstein1D(w, b[2], db[2], d2b[2]);
+	for (int k = 0; k < 4; ++k)
+		for (int j = 0; j < 4; ++j)
+			for (int i = 0; i < 4; ++i) {
+				const int p = i + 4*j + 16*k;
+				out.value[p] = b[0][i]*b[1][j]*b[2][k];
+				out.grad[p][0] = db[0][i]*b[1][j]*b[2][k];
+				out.grad[p][1] = b[0][i]*db[1][j]*b[2][k];
+				out.grad[p][2] = b[0][i]*b[1][j]*db[2][k];
+				out.hess[p][0][0] = d2b[0][i]*b[1][j]*b[2][k];
+				out.hess[p][0][1] = out.hess[p][1][0] = db[0][i]*db[1][j]*b[2][k];
+				out.hess[p][0][2] = out.hess[p][2][0] = db[0][i]*b[1][j]*db[2][k];
+				out.hess[p][1][1] = b[0][i]*d2b[1][j]*b[2][k];
+				out.hess[p][1][2] = out.hess[p][2][1] = b[0][i]*db[1][j]*db[2][k];
+				out.hess[p][2][2] = b[0][i]*b[1][j]*d2b[2][k];
+			}
+}
+
+struct ElementData {
+	int nen = 0;
+	const double* bezier = nullptr;          // [64][3]
+	std::vector<double> extraction;          // dense [nen][64]
+};
+
+struct Basis {
+	std::vector<double> value;               // [nen]
+	std::vector<double> grad;                // [nen][3]
+	std::vector<double> hess;                // [nen][3][3]
+	double inv[3][3]{};                      // inv[param][phys] = d xi_param / d x_phys
+	double raw_det = 0.0;
+};
+
+double Det3(const double a[3][3])
+{
+	return a[0][0]*(a[1][1]*a[2][2] - a[1][2]*a[2][1])
+		- a[0][1]*(a[1][0]*a[2][2] - a[1][2]*a[2][0])
+		+ a[0][2]*(a[1][0]*a[2][1] - a[1][1]*a[2][0]);
+}
+
+// Extraction -> physical value / gradient / Hessian at one parametric point
+// (solvers/cpu/include/TransportElement.hpp:118-205).  Throws on
+// |det| < 1e-14 like the CPU reference (TransportElement.hpp:42).
+void EvaluateBasis(const ElementData& e, double u, double v, double w, bool with_hessian, Basis& out)
+{
+	BernsteinPoint bp;
+	TensorBernstein(u, v, w, bp);
+	double jac[3][3]{};
+	for (int p = 0; p < 64; ++p)
+		for (int x = 0; x < 3; ++x)
+			for (int d = 0; d < 3; ++d) jac[x][d] += e.bezier[3*p + x]*bp.grad[p][d];
+	const double det = Det3(jac);
+	if (!std::isfinite(det) || std::abs(det) < 1e-14) throw std::runtime_error("singular element Jacobian");
+	if (!(det > 0.0)) throw std::runtime_error("non-positive element Jacobian");
+	// cofactor inverse: inv = adj(J)/det
+	double inv[3][3];
+	inv[0][0] = (jac[1][1]*jac[2][2] - jac[1][2]*jac[2][1])/det;
+	inv[0][1] = (jac[0][2]*jac[2][1] - jac[0][1]*jac[2][2])/det;
+	inv[0][2] = (jac[0][1]*jac[1][2] - jac[0][2]*jac[1][1])/det;
+	inv[1][0] = (jac[1][2]*jac[2][0] - jac[1][0]*jac[2][2])/det;
+	inv[1][1] = (jac[0][0]*jac[2][2] - jac[0][2]*jac[2][0])/det;
+	inv[1][2] = (jac[0][2]*jac[1][0] - jac[0][0]*jac[1][2])/det;
+	inv[2][0] = (jac[1][0]*jac[2][1] - jac[1][1]*jac[2][0])/det;
+	inv[2][1] = (jac[0][1]*jac[2][0] - jac[0][0]*jac[2][1])/det;
+	inv[2][2] = (jac[0][0]*jac[1][1] - jac[0][1]*jac[1][0])/det;
+	std::memcpy(out.inv, inv, sizeof(inv));
+	out.raw_det = det;
+	const int nen = e.nen;
+	out.value.assign(nen, 0.0);
+	out.grad.assign(3*nen, 0.0);
+	// parametric gradients of each basis, then push forward
+	for (int a = 0; a < nen; ++a) {
+		const double* c = &e.extraction[64*a];
+		double val = 0.0, g[3] = {0, 0, 0};
+		for (int p = 0; p < 64; ++p) {
+			if (c[p] == 0.0) continue;
+			val += c[p]*bp.value[p];
+			for (int d = 0; d < 3; ++d) g[d] += c[p]*bp.grad[p][d];
+		}
+		out.value[a] = val;
+		for (int x = 0; x < 3; ++x) out.grad[3*a + x] = g[0]*inv[0][x] + g[1]*inv[1][x] + g[2]*inv[2][x];
+	}
+	if (!with_hessian) return;
+	out.hess.assign(9*nen, 0.0);
+	// second derivatives of the geometry map, then d^2 xi_c / dx_i dx_j
+	double x2[3][3][3]{};  // [phys][a][b]
+	for (int p = 0; p < 64; ++p)
+		for (int x = 0; x < 3; ++x)
+			for (int a = 0; a < 3; ++a)
+				for (int b = 0; b < 3; ++b) x2[x][a][b] += e.bezier[3*p + x]*bp.hess[p][a][b];
+	double xi2[3][3][3]{};  // [c][i][j]
+	for (int c = 0; c < 3; ++c)
+		for (int i = 0; i < 3; ++i)
+			for (int j = 0; j < 3; ++j) {
+				double s = 0.0;
+				for (int x = 0; x < 3; ++x)
+					for (int a = 0; a < 3; ++a)
+						for (int b = 0; b < 3; ++b) s += x2[x][a][b]*inv[a][i]*inv[b][j]*inv[c][x];
+				xi2[c][i][j] = -s;
+			}
+	for (int a = 0; a < nen; ++a) {
+		const double* c = &e.extraction[64*a];
+		double g[3] = {0, 0, 0}, h[3][3]{};
+		for (int p = 0; p < 64; ++p) {
+			if (c[p] == 0.0) continue;
+			for (int d = 0; d < 3; ++d) g[d] += c[p]*bp.grad[p][d];
+			for (int d = 0; d < 3; ++d)
+				for (int f = 0; f < 3; ++f) h[d][f] += c[p]*bp.hess[p][d][f];
+		}
+		for (int i = 0; i < 3; ++i)
+			for (int j = 0; j < 3; ++j) {
+				double s = 0.0;
+				for (int d = 0; d < 3; ++d) {
+					for (int f = 0; f < 3; ++f) s += h[d][f]*inv[d][i]*inv[f][j];
+					s += g[d]*xi2[d][i][j];
+				}
+				out.hess[9*a + 3*i + j] = s;
+			}
+	}
+}
+
+// VMS stabilisation parameters (solvers/cpu/include/NavierStokesElement.hpp:121-144,
+// solvers/cuda/include/IgaCudaKernels.cuh:239-263).
+void Stabilization(const double inv[3][3], const double u[3], double nu, double dt, double& tau_m, double& tau_c)
+{
+	double G[3][3]{}, g[3]{};
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j) {
+			for (int k = 0; k < 3; ++k) G[i][j] += inv[k][i]*inv[k][j];
+			g[i] += inv[j][i];
+		}
+	double GG = 0.0, gg = 0.0, uGu = 0.0;
+	for (int i = 0; i < 3; ++i) {
+		gg += g[i]*g[i];
+		for (int j = 0; j < 3; ++j) {
+			GG += G[i][j]*G[i][j];
+			uGu += u[i]*G[i][j]*u[j];
+		}
+	}
+	const double temporal = dt > 0.0 ? 4.0/(dt*dt) : 0.0;
+	tau_m = 1.0/std::sqrt(temporal + uGu + (1.0/12.0)*nu*nu*GG);
+	tau_c = 1.0/(tau_m*gg);
+}
+
+enum TangentKind { kLegacyTangent = 0, kExactTangent = 1 };
+
+// Element residual (negated) and tangent.  Residual: NavierStokesElement.hpp:173-241
+// (LegacyBodyFitted form, zero body force) == IgaCudaKernels.cuh:311-379.
+// Legacy (tau-frozen) tangent: IgaCudaKernels.cuh:380-396 and
+// solvers/cpu/tests/test_navier_stokes.cpp:145-168.  Exact tangent:
+// NavierStokesElement.hpp:244-313.
+void NavierStokesElement(const ElementData& e, const double* state, const double* previous,
+	double rho, double mu, double dt, int tangent, int nq, const double* qpts, const double* qw,
+	double* jac, double* negres)
+{
+	const int nen = e.nen;
+	const int ndof = 4*nen;
+	const double nu = mu/rho;
+	std::fill(negres, negres + ndof, 0.0);
+	if (jac) std::fill(jac, jac + static_cast<size_t>(ndof)*ndof, 0.0);
+	Basis basis;
+	for (int q = 0; q < nq; ++q) {
+		EvaluateBasis(e, qpts[3*q], qpts[3*q + 1], qpts[3*q + 2], true, basis);
+		const double measure = qw[q]*basis.raw_det;
+		double s[4] = {0, 0, 0, 0}, grad[4][3]{}, hess[4][3][3]{}, prev[3] = {0, 0, 0};
+		for (int a = 0; a < nen; ++a)
+			for (int f = 0; f < 4; ++f) {
+				const double c = state[4*a + f];
+				s[f] += c*basis.value[a];
+				for (int i = 0; i < 3; ++i) {
+					grad[f][i] += c*basis.grad[3*a + i];
+					for (int j = 0; j < 3; ++j) hess[f][i][j] += c*basis.hess[9*a + 3*i + j];
+				}
+			}
+		if (dt > 0.0)
+			for (int a = 0; a < nen; ++a)
+				for (int c = 0; c < 3; ++c) prev[c] += previous[4*a + c]*basis.value[a];
+		double dudt[3] = {0, 0, 0};
+		if (dt > 0.0)
+			for (int c = 0; c < 3; ++c) dudt[c] = (s[c] - prev[c])/dt;
+		double tau_m, tau_c;
+		Stabilization(basis.inv, s, nu, dt, tau_m, tau_c);
+		double strong[3], fine_u[3];
+		for (int c = 0; c < 3; ++c) {
+			const double conv = s[0]*grad[c][0] + s[1]*grad[c][1] + s[2]*grad[c][2];
+			const double lap = hess[c][0][0] + hess[c][1][1] + hess[c][2][2];
+			strong[c] = dudt[c] + conv + grad[3][c]/rho - nu*lap;
+			fine_u[c] = -tau_m*strong[c];
+		}
+		const double div = grad[0][0] + grad[1][1] + grad[2][2];
+		const double fine_p = -rho*tau_c*div;
+		double G[3][3]{};
+		for (int i = 0; i < 3; ++i)
+			for (int j = 0; j < 3; ++j)
+				for (int k = 0; k < 3; ++k) G[i][j] += basis.inv[k][i]*basis.inv[k][j];
+		for (int a = 0; a < nen; ++a) {
+			const double na = basis.value[a];
+			const double* ga = &basis.grad[3*a];
+			double r[4];
+			for (int c = 0; c < 3; ++c) {
+				r[c] = rho*na*dudt[c] - ga[c]*s[3] - ga[c]*fine_p;
+				for (int d = 0; d < 3; ++d) {
+					r[c] += mu*ga[d]*(grad[c][d] + grad[d][c]);
+					r[c] += rho*na*(s[d] + fine_u[d])*grad[c][d];
+					r[c] -= rho*ga[d]*fine_u[c]*(s[d] + fine_u[d]);
+				}
+			}
+			r[3] = na*div - ga[0]*fine_u[0] - ga[1]*fine_u[1] - ga[2]*fine_u[2];
+			for (int f = 0; f < 4; ++f) negres[4*a + f] -= r[f]*measure;
+			if (!jac) continue;
+			for (int b = 0; b < nen; ++b) {
+				const double nb = basis.value[b];
+				const double* gb = &basis.grad[3*b];
+				double* row[4];
+				for (int i = 0; i < 4; ++i) row[i] = jac + static_cast<size_t>(4*a + i)*ndof + 4*b;
+				if (tangent == kLegacyTangent) {
+					const double conv_b = s[0]*gb[0] + s[1]*gb[1] + s[2]*gb[2];
+					const double stream_a = s[0]*ga[0] + s[1]*ga[1] + s[2]*ga[2];
+					const double gdot = ga[0]*gb[0] + ga[1]*gb[1] + ga[2]*gb[2];
+					const double mass_b = dt > 0.0 ? nb/dt : 0.0;
+					const double diag = rho*na*(mass_b + conv_b) + mu*gdot + rho*tau_m*stream_a*(mass_b + conv_b);
+					double t[4][4];
+					for (int i = 0; i < 3; ++i)
+						for (int j = 0; j < 3; ++j) t[i][j] = mu*ga[j]*gb[i] + rho*tau_c*ga[i]*gb[j];
+					for (int i = 0; i < 3; ++i) t[i][i] += diag;
+					for (int i = 0; i < 3; ++i) {
+						t[i][3] = -ga[i]*nb + tau_m*stream_a*gb[i];
+						t[3][i] = na*gb[i] + tau_m*ga[i]*(mass_b + conv_b);
+					}
+					t[3][3] = (tau_m/rho)*gdot;
+					for (int i = 0; i < 4; ++i)
+						for (int j = 0; j < 4; ++j) row[i][j] += t[i][j]*measure;
+					continue;
+				}
+				// exact consistent tangent
+				const double lap_b = basis.hess[9*b] + basis.hess[9*b + 4] + basis.hess[9*b + 8];
+				for (int f = 0; f < 4; ++f) {
+					double du[3] = {0, 0, 0}, dlap[3] = {0, 0, 0}, dgp[3] = {0, 0, 0};
+					const double dp = f == 3 ? nb : 0.0;
+					if (f < 3) { du[f] = nb; dlap[f] = lap_b; }
+					else { dgp[0] = gb[0]; dgp[1] = gb[1]; dgp[2] = gb[2]; }
+					double uG = 0.0;
+					for (int i = 0; i < 3; ++i)
+						for (int j = 0; j < 3; ++j) uG += du[i]*G[i][j]*s[j];
+					const double dtau_m = -tau_m*tau_m*tau_m*uG;
+					const double dtau_c = tau_c*tau_m*tau_m*uG;
+					const double ddiv = f < 3 ? gb[f] : 0.0;
+					double dfine[3];
+					for (int c = 0; c < 3; ++c) {
+						double ds = 0.0;
+						for (int d = 0; d < 3; ++d) ds += du[d]*grad[c][d] + s[d]*(f == c ? gb[d] : 0.0);
+						if (dt > 0.0 && f == c) ds += nb/dt;
+						ds += dgp[c]/rho - nu*dlap[c];
+						dfine[c] = -dtau_m*strong[c] - tau_m*ds;
+					}
+					const double dfine_p = -rho*(dtau_c*div + tau_c*ddiv);
+					for (int i = 0; i < 3; ++i) {
+						double t = rho*na*(dt > 0.0 && f == i ? nb/dt : 0.0) - ga[i]*dfine_p - ga[i]*dp;
+						for (int d = 0; d < 3; ++d) {
+							t += mu*ga[d]*((f == i ? gb[d] : 0.0) + (f == d ? gb[i] : 0.0));
+							t += rho*na*((du[d] + dfine[d])*grad[i][d] + (s[d] + fine_u[d])*(f == i ? gb[d] : 0.0));
+							t -= rho*ga[d]*(dfine[i]*(s[d] + fine_u[d]) + fine_u[i]*(du[d] + dfine[d]));
+						}
+						row[i][f] += t*measure;
+					}
+					double tc = na*ddiv;
+					for (int d = 0; d < 3; ++d) tc -= ga[d]*dfine[d];
+					row[3][f] += tc*measure;
+				}
+			}
+		}
+	}
+}
+
+void FullRule(std::vector<double>& pts, std::vector<double>& w)
+{
+	pts.resize(192);
+	w.resize(64);
+	for (int qz = 0; qz < 4; ++qz)
+		for (int qy = 0; qy < 4; ++qy)
+			for (int qx = 0; qx < 4; ++qx) {
+				const int q = qx + 4*qy + 16*qz;
+				pts[3*q] = kGaussPoint[qx];
+				pts[3*q + 1] = kGaussPoint[qy];
+				pts[3*q + 2] = kGaussPoint[qz];
+				// FullCell4x4x4VolumeQuadratureProvider: weights divided by 8 (Quadrature.hpp:193-202)
+				w[q] = kGaussWeight[qx]*kGaussWeight[qy]*kGaussWeight[qz]/8.0;
+			}
+}
+
+struct FlatMeshView {
+	int nodes, elements;
+	const int* eoff;
+	const int* conn;
+	const int* xoff;
+	const uint8_t* xcol;
+	const double* xval;
+	const double* bezier;
+};
+
+ElementData LoadElement(const FlatMeshView& m, int e)
+{
+	ElementData d;
+	const int b = m.eoff[e];
+	d.nen = m.eoff[e + 1] - b;
+	d.bezier = m.bezier + static_cast<size_t>(e)*192;
+	d.extraction.assign(static_cast<size_t>(d.nen)*64, 0.0);
+	for (int a = 0; a < d.nen; ++a)
+		for (int k = m.xoff[b + a]; k < m.xoff[b + a + 1]; ++k) d.extraction[64*a + m.xcol[k]] = m.xval[k];
+	return d;
+}
+
+int FindBlock(const int* ro, const int* cols, int row, int col)
+{
+	const int* first = cols + ro[row];
+	const int* last = cols + ro[row + 1];
+	const int* it = std::lower_bound(first, last, col);
+	if (it == last || *it != col) throw std::runtime_error("block not in pattern");
+	return static_cast<int>(it - cols);
+}
+
+template <class F>
+int Guard(F&& f)
+{
+	try {
+		f();
+		return 0;
+	} catch (const std::exception& e) {
+		g_error = e.what();
+		return 1;
+	}
+}
+
+// --- Krylov helpers ------------------------------------------------------
+double Dot(const std::vector<double>& a, const std::vector<double>& b)
+{
+	double s = 0.0;
+	for (size_t i = 0; i < a.size(); ++i) s += a[i]*b[i];
+	return s;
+}
+
+void BsrSpmv(int n, const int* ro, const int* cols, const double* vals, const double* x, double* y)
+{
+#pragma omp parallel for schedule(static)
+	for (int r = 0; r < n; ++r) {
+		double acc[4] = {0, 0, 0, 0};
+		for (int k = ro[r]; k < ro[r + 1]; ++k) {
+			const double* A = vals + static_cast<size_t>(k)*16;
+			const double* xv = x + 4*static_cast<size_t>(cols[k]);
+			for (int i = 0; i < 4; ++i)
+				acc[i] += A[4*i]*xv[0] + A[4*i + 1]*xv[1] + A[4*i + 2]*xv[2] + A[4*i + 3]*xv[3];
+		}
+		for (int i = 0; i < 4; ++i) y[4*static_cast<size_t>(r) + i] = acc[i];
+	}
+}
+
+// 4x4 Gauss-Jordan with partial pivoting, threshold 1e-14, diagonal fallback
+// (solvers/cuda/include/SparseKernels.cuh:42-90).
+int BlockInverse(int n, const int* diag, const double* vals, double* inv)
+{
+	int singular = 0;
+	for (int r = 0; r < n; ++r) {
+		const double* s = vals + static_cast<size_t>(diag[r])*16;
+		double m[4][8];
+		for (int i = 0; i < 4; ++i)
+			for (int j = 0; j < 8; ++j) m[i][j] = j < 4 ? s[4*i + j] : (j - 4 == i ? 1.0 : 0.0);
+		bool failed = false;
+		for (int p = 0; p < 4; ++p) {
+			int best = p;
+			for (int i = p + 1; i < 4; ++i)
+				if (std::abs(m[i][p]) > std::abs(m[best][p])) best = i;
+			if (!std::isfinite(m[best][p]) || std::abs(m[best][p]) < 1e-14) { failed = true; break; }
+			if (best != p)
+				for (int j = 0; j < 8; ++j) std::swap(m[p][j], m[best][j]);
+			const double sc = 1.0/m[p][p];
+			for (int j = 0; j < 8; ++j) m[p][j] *= sc;
+			for (int i = 0; i < 4; ++i) {
+				if (i == p) continue;
+				const double f = m[i][p];
+				for (int j = 0; j < 8; ++j) m[i][j] -= f*m[p][j];
+			}
+		}
+		double* d = inv + static_cast<size_t>(r)*16;
+		if (!failed) {
+			for (int i = 0; i < 4; ++i)
+				for (int j = 0; j < 4; ++j) d[4*i + j] = m[i][4 + j];
+		} else {
+			++singular;
+			for (int i = 0; i < 4; ++i)
+				for (int j = 0; j < 4; ++j)
+					d[4*i + j] = i == j && std::abs(s[5*i]) > 1e-14 ? 1.0/s[5*i] : (i == j ? 1.0 : 0.0);
+		}
+	}
+	return singular;
+}
+
+void BlockApply(int n, const double* inv, const double* x, double* y)
+{
+	for (int r = 0; r < n; ++r)
+		for (int i = 0; i < 4; ++i) {
+			const double* d = inv + static_cast<size_t>(r)*16 + 4*i;
+			y[4*r + i] = d[0]*x[4*r] + d[1]*x[4*r + 1] + d[2]*x[4*r + 2] + d[3]*x[4*r + 3];
+		}
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_error.c_str(); }
+
+// Bernstein tables on the 4x4x4 Gauss points, layouts as in
+// solvers/cuda/src/iga_cuda.cu:84-129: basis[q*64+p], grad[(q*64+p)*3+d],
+// hess[(q*64+p)*9+a*3+b], weight[q] = w_x w_y w_z (NOT divided by 8).
+int oracle_reference_tables(double* basis, double* grad, double* hess, double* weight)
+{
+	return Guard([&] {
+		for (int qz = 0; qz < 4; ++qz)
+			for (int qy = 0; qy < 4; ++qy)
+				for (int qx = 0; qx < 4; ++qx) {
+					const int q = qx + 4*qy + 16*qz;
+					BernsteinPoint bp;
+					TensorBernstein(kGaussPoint[qx], kGaussPoint[qy], kGaussPoint[qz], bp);
+					for (int p = 0; p < 64; ++p) {
+						basis[q*64 + p] = bp.value[p];
+						for (int d = 0; d < 3; ++d) grad[(q*64 + p)*3 + d] = bp.grad[p][d];
+						for (int a = 0; a < 3; ++a)
+							for (int b = 0; b < 3; ++b) hess[(q*64 + p)*9 + 3*a + b] = bp.hess[p][a][b];
+					}
+					weight[q] = kGaussWeight[qx]*kGaussWeight[qy]*kGaussWeight[qz];
+				}
+	});
+}
+
+// EvaluateBasis for one element given dense extraction [nen][64].
+int oracle_evaluate_basis(int nen, const double* bezier, const double* extraction, double u, double v,
+	double w, double* value, double* grad, double* hess, double* inv, double* raw_det)
+{
+	return Guard([&] {
+		ElementData e;
+		e.nen = nen;
+		e.bezier = bezier;
+		e.extraction.assign(extraction, extraction + static_cast<size_t>(nen)*64);
+		Basis b;
+		EvaluateBasis(e, u, v, w, hess != nullptr, b);
+		std::copy(b.value.begin(), b.value.end(), value);
+		std::copy(b.grad.begin(), b.grad.end(), grad);
+		if (hess) std::copy(b.hess.begin(), b.hess.end(), hess);
+		if (inv) std::memcpy(inv, b.inv, sizeof(b.inv));
+		*raw_det = b.raw_det;
+	});
+}
+
+// One element's negated residual and tangent (dense (4nen)^2, row (a,i),
+// column (b,j)).  nq<=0 selects the full 4x4x4 rule (weights/8).
+int oracle_ns_element(int nen, const double* bezier, const double* extraction, const double* state,
+	const double* previous, double rho, double mu, double dt, int tangent, int nq, const double* qpts,
+	const double* qw, double* jac, double* negres)
+{
+	return Guard([&] {
+		if (!(rho > 0.0) || !(mu > 0.0) || !(dt >= 0.0)) throw std::runtime_error("invalid NS parameters");
+		ElementData e;
+		e.nen = nen;
+		e.bezier = bezier;
+		e.extraction.assign(extraction, extraction + static_cast<size_t>(nen)*64);
+		std::vector<double> pts, w;
+		if (nq <= 0) {
+			FullRule(pts, w);
+			nq = 64;
+			qpts = pts.data();
+			qw = w.data();
+		}
+		NavierStokesElement(e, state, previous, rho, mu, dt, tangent, nq, qpts, qw, jac, negres);
+	});
+}
+
+// BlockPattern: per node, the sorted unique union of the connectivities of the
+// elements containing it (solvers/cuda/include/BlockCsr.hpp:27-55).
+// Call with columns == nullptr to obtain the block count in *nnzb.
+int oracle_block_pattern(int nodes, int elements, const int* eoff, const int* conn, int* row_offsets,
+	int* columns, int* diagonal, long long* nnzb)
+{
+	return Guard([&] {
+		std::vector<std::vector<int>> adj(nodes);
+		for (int e = 0; e < elements; ++e)
+			for (int a = eoff[e]; a < eoff[e + 1]; ++a) {
+				auto& row = adj.at(conn[a]);
+				row.insert(row.end(), conn + eoff[e], conn + eoff[e + 1]);
+			}
+		long long total = 0;
+		for (int n = 0; n < nodes; ++n) {
+			auto& row = adj[n];
+			std::sort(row.begin(), row.end());
+			row.erase(std::unique(row.begin(), row.end()), row.end());
+			if (!std::binary_search(row.begin(), row.end(), n))
+				throw std::runtime_error("block pattern is missing a diagonal entry");
+			total += static_cast<long long>(row.size());
+		}
+		*nnzb = total;
+		if (!columns) return;
+		long long k = 0;
+		row_offsets[0] = 0;
+		for (int n = 0; n < nodes; ++n) {
+			const auto& row = adj[n];
+			diagonal[n] = static_cast<int>(k + (std::lower_bound(row.begin(), row.end(), n) - row.begin()));
+			std::copy(row.begin(), row.end(), columns + k);
+			k += static_cast<long long>(row.size());
+			row_offsets[n + 1] = static_cast<int>(k);
+		}
+	});
+}
+
+// Global assembly of the negated residual and (optionally) the block Jacobian
+// with the full 4x4x4 rule; elements are evaluated in parallel and scattered
+// serially in element order (the reference CPU adds element matrices with
+// MatSetValues ADD_VALUES, TransientFlowRuntime.hpp:1396-1436).
+int oracle_ns_assemble(int nodes, int elements, const int* eoff, const int* conn, const int* xoff,
+	const uint8_t* xcol, const double* xval, const double* bezier, const int* row_offsets, const int* columns,
+	const double* state, const double* previous, double rho, double mu, double dt, int tangent,
+	double* jac_values, double* negres)
+{
+	return Guard([&] {
+		const FlatMeshView m{nodes, elements, eoff, conn, xoff, xcol, xval, bezier};
+		std::vector<double> pts, w;
+		FullRule(pts, w);
+		std::fill(negres, negres + 4*static_cast<size_t>(nodes), 0.0);
+		if (jac_values) std::fill(jac_values, jac_values + 16*static_cast<size_t>(row_offsets[nodes]), 0.0);
+		const int chunk = 256;
+		std::string failure;
+		for (int e0 = 0; e0 < elements; e0 += chunk) {
+			const int e1 = std::min(elements, e0 + chunk);
+			std::vector<std::vector<double>> mats(e1 - e0), vecs(e1 - e0);
+#pragma omp parallel for schedule(dynamic, 4)
+			for (int e = e0; e < e1; ++e) {
+				try {
+					ElementData d = LoadElement(m, e);
+					std::vector<double> ls(4*d.nen), lp(4*d.nen);
+					for (int a = 0; a < d.nen; ++a)
+						for (int f = 0; f < 4; ++f) {
+							ls[4*a + f] = state[4*static_cast<size_t>(conn[eoff[e] + a]) + f];
+							lp[4*a + f] = previous[4*static_cast<size_t>(conn[eoff[e] + a]) + f];
+						}
+					const int ndof = 4*d.nen;
+					vecs[e - e0].assign(ndof, 0.0);
+					if (jac_values) mats[e - e0].assign(static_cast<size_t>(ndof)*ndof, 0.0);
+					NavierStokesElement(d, ls.data(), lp.data(), rho, mu, dt, tangent, 64, pts.data(), w.data(),
+						jac_values ? mats[e - e0].data() : nullptr, vecs[e - e0].data());
+				} catch (const std::exception& ex) {
+#pragma omp critical
+					failure = ex.what();
+				}
+			}
+			if (!failure.empty()) throw std::runtime_error(failure);
+			for (int e = e0; e < e1; ++e) {
+				const int b = eoff[e];
+				const int nen = eoff[e + 1] - b;
+				const int ndof = 4*nen;
+				for (int a = 0; a < nen; ++a) {
+					const int row = conn[b + a];
+					for (int f = 0; f < 4; ++f) negres[4*static_cast<size_t>(row) + f] += vecs[e - e0][4*a + f];
+					if (!jac_values) continue;
+					for (int bb = 0; bb < nen; ++bb) {
+						const int blk = FindBlock(row_offsets, columns, row, conn[b + bb]);
+						double* dst = jac_values + static_cast<size_t>(blk)*16;
+						for (int i = 0; i < 4; ++i)
+							for (int j = 0; j < 4; ++j)
+								dst[4*i + j] += mats[e - e0][static_cast<size_t>(4*a + i)*ndof + 4*bb + j];
+					}
+				}
+			}
+		}
+	});
+}
+
+// Dirichlet rows and rhs (solvers/cuda/include/IgaCudaKernels.cuh:436-463).
+int oracle_apply_dirichlet(int nodes, const int* row_offsets, const int* diagonal, const int* vmask,
+	const int* pmask, const double* vtarget, const double* ptarget, const double* state, double* jac_values,
+	double* rhs)
+{
+	return Guard([&] {
+		for (int n = 0; n < nodes; ++n)
+			for (int f = 0; f < 4; ++f) {
+				const bool c = f < 3 ? vmask[n] != 0 : pmask[n] != 0;
+				if (!c) continue;
+				if (jac_values) {
+					for (int k = row_offsets[n]; k < row_offsets[n + 1]; ++k)
+						for (int j = 0; j < 4; ++j) jac_values[static_cast<size_t>(k)*16 + 4*f + j] = 0.0;
+					jac_values[static_cast<size_t>(diagonal[n])*16 + 5*f] = 1.0;
+				}
+				if (rhs) {
+					const double target = f < 3 ? vtarget[3*n + f] : ptarget[n];
+					rhs[4*n + f] = target - state[4*n + f];
+				}
+			}
+	});
+}
+
+int oracle_bsr_spmv(int nodes, const int* row_offsets, const int* columns, const double* values,
+	const double* x, double* y)
+{
+	return Guard([&] { BsrSpmv(nodes, row_offsets, columns, values, x, y); });
+}
+
+int oracle_block_inverse(int nodes, const int* diagonal, const double* values, double* inverse, int* singular)
+{
+	return Guard([&] { *singular = BlockInverse(nodes, diagonal, values, inverse); });
+}
+
+// Restarted left-preconditioned GMRES with node-block Jacobi, modified
+// Gram-Schmidt, Givens on the host, tolerance rtol*||D^-1 r0||
+// (solvers/cuda/include/Gmres.hpp:69-213).  x holds the initial guess.
+int oracle_gmres_bjacobi(int nodes, const int* row_offsets, const int* columns, const int* diagonal,
+	const double* values, const double* b, double* x, int restart, int maxit, double rtol, int* iterations,
+	double* residual)
+{
+	return Guard([&] {
+		const size_t n = 4*static_cast<size_t>(nodes);
+		std::vector<double> inv(16*static_cast<size_t>(nodes));
+		BlockInverse(nodes, diagonal, values, inv.data());
+		std::vector<double> r(n), w(n), t(n);
+		std::vector<double> V(n*(restart + 1));
+		std::vector<double> H(static_cast<size_t>(restart + 1)*restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+		auto precond_residual = [&](std::vector<double>& out) {
+			BsrSpmv(nodes, row_offsets, columns, values, x, t.data());
+			for (size_t i = 0; i < n; ++i) r[i] = b[i] - t[i];
+			BlockApply(nodes, inv.data(), r.data(), out.data());
+		};
+		precond_residual(w);
+		double beta = std::sqrt(Dot(w, w));
+		int its = 0;
+		*iterations = 0;
+		*residual = beta;
+		if (beta == 0.0) return;
+		const double tol = std::max(1e-30, rtol*beta);
+		while (its < maxit) {
+			for (size_t i = 0; i < n; ++i) V[i] = w[i]/beta;
+			std::fill(H.begin(), H.end(), 0.0);
+			std::fill(g.begin(), g.end(), 0.0);
+			g[0] = beta;
+			int used = 0;
+			for (int j = 0; j < restart && its < maxit; ++j) {
+				BsrSpmv(nodes, row_offsets, columns, values, &V[n*j], t.data());
+				BlockApply(nodes, inv.data(), t.data(), w.data());
+				for (int i = 0; i <= j; ++i) {
+					double h = 0.0;
+					const double* vi = &V[n*i];
+					for (size_t k = 0; k < n; ++k) h += vi[k]*w[k];
+					H[static_cast<size_t>(i)*restart + j] = h;
+					for (size_t k = 0; k < n; ++k) w[k] -= h*vi[k];
+				}
+				const double hn = std::sqrt(Dot(w, w));
+				H[static_cast<size_t>(j + 1)*restart + j] = hn;
+				if (hn > 1e-30)
+					for (size_t k = 0; k < n; ++k) V[n*(j + 1) + k] = w[k]/hn;
+				for (int i = 0; i < j; ++i) {
+					const double h0 = H[static_cast<size_t>(i)*restart + j], h1 = H[static_cast<size_t>(i + 1)*restart + j];
+					H[static_cast<size_t>(i)*restart + j] = cs[i]*h0 + sn[i]*h1;
+					H[static_cast<size_t>(i + 1)*restart + j] = -sn[i]*h0 + cs[i]*h1;
+				}
+				const double h0 = H[static_cast<size_t>(j)*restart + j], h1 = H[static_cast<size_t>(j + 1)*restart + j];
+				const double mag = std::hypot(h0, h1);
+				cs[j] = mag > 0.0 ? h0/mag : 1.0;
+				sn[j] = mag > 0.0 ? h1/mag : 0.0;
+				H[static_cast<size_t>(j)*restart + j] = mag;
+				H[static_cast<size_t>(j + 1)*restart + j] = 0.0;
+				const double gj = g[j];
+				g[j] = cs[j]*gj;
+				g[j + 1] = -sn[j]*gj;
+				++its;
+				used = j + 1;
+				*residual = std::abs(g[j + 1]);
+				if (*residual <= tol || hn <= 1e-30) break;
+			}
+			for (int row = used - 1; row >= 0; --row) {
+				double v = g[row];
+				for (int c = row + 1; c < used; ++c) v -= H[static_cast<size_t>(row)*restart + c]*y[c];
+				const double d = H[static_cast<size_t>(row)*restart + row];
+				if (std::abs(d) < 1e-30) throw std::runtime_error("GMRES Hessenberg matrix is singular");
+				y[row] = v/d;
+			}
+			for (int i = 0; i < used; ++i)
+				for (size_t k = 0; k < n; ++k) x[k] += y[i]*V[n*i + k];
+			precond_residual(w);
+			beta = std::sqrt(Dot(w, w));
+			*residual = beta;
+			if (beta <= tol) break;
+		}
+		*iterations = its;
+	});
+}
+
+// Right-preconditioned flexible GMRES with node-block Jacobi and a true
+// (unpreconditioned) relative residual test ||b - A x|| <= rtol ||b||, the
+// criterion PETSc's KSPFGMRES applies with a zero initial guess
+// (solvers/cpu/include/TransientFlowRuntime.hpp:176-190).  The reference's
+// PETSc preconditioner is not restated (parity unpinned at that boundary; the
+// converged solution is what is compared).
+int oracle_fgmres_bjacobi(int nodes, const int* row_offsets, const int* columns, const int* diagonal,
+	const double* values, const double* b, double* x, int restart, int maxit, double rtol, int* iterations,
+	double* residual)
+{
+	return Guard([&] {
+		const size_t n = 4*static_cast<size_t>(nodes);
+		std::vector<double> inv(16*static_cast<size_t>(nodes));
+		BlockInverse(nodes, diagonal, values, inv.data());
+		std::vector<double> r(n), w(n), t(n);
+		std::vector<double> V(n*(restart + 1)), Z(n*restart);
+		std::vector<double> H(static_cast<size_t>(restart + 1)*restart), cs(restart), sn(restart), g(restart + 1), y(restart);
+		std::vector<double> bv(b, b + n);
+		const double bnorm = std::sqrt(Dot(bv, bv));
+		auto true_residual = [&]() {
+			BsrSpmv(nodes, row_offsets, columns, values, x, t.data());
+			for (size_t i = 0; i < n; ++i) r[i] = b[i] - t[i];
+			return std::sqrt(Dot(r, r));
+		};
+		double beta = true_residual();
+		int its = 0;
+		*iterations = 0;
+		*residual = beta;
+		if (bnorm == 0.0 || beta == 0.0) return;
+		const double tol = rtol*bnorm;
+		while (its < maxit && beta > tol) {
+			for (size_t i = 0; i < n; ++i) V[i] = r[i]/beta;
+			std::fill(H.begin(), H.end(), 0.0);
+			std::fill(g.begin(), g.end(), 0.0);
+			g[0] = beta;
+			int used = 0;
+			for (int j = 0; j < restart && its < maxit; ++j) {
+				BlockApply(nodes, inv.data(), &V[n*j], &Z[n*j]);
+				BsrSpmv(nodes, row_offsets, columns, values, &Z[n*j], w.data());
+				for (int pass = 0; pass < 2; ++pass)  // classical Gram-Schmidt, twice
+					for (int i = 0; i <= j; ++i) {
+						double h = 0.0;
+						const double* vi = &V[n*i];
+						for (size_t k = 0; k < n; ++k) h += vi[k]*w[k];
+						H[static_cast<size_t>(i)*restart + j] += h;
+						for (size_t k = 0; k < n; ++k) w[k] -= h*vi[k];
+					}
+				const double hn = std::sqrt(Dot(w, w));
+				H[static_cast<size_t>(j + 1)*restart + j] = hn;
+				if (hn > 1e-300)
+					for (size_t k = 0; k < n; ++k) V[n*(j + 1) + k] = w[k]/hn;
+				for (int i = 0; i < j; ++i) {
+					const double h0 = H[static_cast<size_t>(i)*restart + j], h1 = H[static_cast<size_t>(i + 1)*restart + j];
+					H[static_cast<size_t>(i)*restart + j] = cs[i]*h0 + sn[i]*h1;
+					H[static_cast<size_t>(i + 1)*restart + j] = -sn[i]*h0 + cs[i]*h1;
+				}
+				const double h0 = H[static_cast<size_t>(j)*restart + j], h1 = H[static_cast<size_t>(j + 1)*restart + j];
+				const double mag = std::hypot(h0, h1);
+				cs[j] = mag > 0.0 ? h0/mag : 1.0;
+				sn[j] = mag > 0.0 ? h1/mag : 0.0;
+				H[static_cast<size_t>(j)*restart + j] = mag;
+				H[static_cast<size_t>(j + 1)*restart + j] = 0.0;
+				const double gj = g[j];
+				g[j] = cs[j]*gj;
+				g[j + 1] = -sn[j]*gj;
+				++its;
+				used = j + 1;
+				if (std::abs(g[j + 1]) <= tol || hn <= 1e-300) break;
+			}
+			for (int row = used - 1; row >= 0; --row) {
+				double v = g[row];
+				for (int c = row + 1; c < used; ++c) v -= H[static_cast<size_t>(row)*restart + c]*y[c];
+				const double d = H[static_cast<size_t>(row)*restart + row];
+				if (std::abs(d) < 1e-300) throw std::runtime_error("FGMRES Hessenberg matrix is singular");
+				y[row] = v/d;
+			}
+			for (int i = 0; i < used; ++i)
+				for (size_t k = 0; k < n; ++k) x[k] += y[i]*Z[n*i + k];
+			beta = true_residual();
+		}
+		*iterations = its;
+		*residual = beta;
+	});
+}
+
+// Volumetric flow through one element face (solvers/cpu/include/BoundaryFlow.hpp:164-200).
+int oracle_boundary_flow(int nen, const double* bezier, const double* extraction, const double* state, int face,
+	double* flow)
+{
+	return Guard([&] {
+		if (face < 0 || face >= 6) throw std::runtime_error("boundary face index must be below six");
+		static const int fixed_axis[6] = {2, 1, 0, 1, 0, 2};
+		static const int varying[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 2}, {1, 2}, {0, 1}};
+		static const double fixed_value[6] = {0.0, 0.0, 1.0, 1.0, 0.0, 1.0};
+		static const double sign[6] = {-1.0, -1.0, 1.0, 1.0, -1.0, 1.0};
+		ElementData e;
+		e.nen = nen;
+		e.bezier = bezier;
+		e.extraction.assign(extraction, extraction + static_cast<size_t>(nen)*64);
+		double total = 0.0;
+		Basis b;
+		for (int qi = 0; qi < 4; ++qi)
+			for (int qj = 0; qj < 4; ++qj) {
+				double c[3];
+				c[fixed_axis[face]] = fixed_value[face];
+				c[varying[face][0]] = kGaussPoint[qi];
+				c[varying[face][1]] = kGaussPoint[qj];
+				EvaluateBasis(e, c[0], c[1], c[2], false, b);
+				double vel[3] = {0, 0, 0};
+				for (int a = 0; a < nen; ++a)
+					for (int k = 0; k < 3; ++k) vel[k] += b.value[a]*state[4*a + k];
+				double un = 0.0;
+				for (int k = 0; k < 3; ++k) un += vel[k]*b.inv[fixed_axis[face]][k];
+				total += sign[face]*kGaussWeight[qi]*kGaussWeight[qj]*2.0*(0.125*b.raw_det)*un;
+			}
+		*flow = total;
+	});
+}
+
+// -p n traction load of one face (solvers/cpu/include/BoundaryFlow.hpp:445-477).
+int oracle_pressure_traction(int nen, const double* bezier, const double* extraction, int face, double pressure,
+	double* local)
+{
+	return Guard([&] {
+		if (face < 0 || face >= 6) throw std::runtime_error("pressure-traction face index must be below six");
+		static const int fixed_axis[6] = {2, 1, 0, 1, 0, 2};
+		static const int varying[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 2}, {1, 2}, {0, 1}};
+		static const double fixed_value[6] = {0.0, 0.0, 1.0, 1.0, 0.0, 1.0};
+		static const double sign[6] = {-1.0, -1.0, 1.0, 1.0, -1.0, 1.0};
+		ElementData e;
+		e.nen = nen;
+		e.bezier = bezier;
+		e.extraction.assign(extraction, extraction + static_cast<size_t>(nen)*64);
+		std::fill(local, local + 4*nen, 0.0);
+		Basis b;
+		for (int qi = 0; qi < 4; ++qi)
+			for (int qj = 0; qj < 4; ++qj) {
+				double c[3];
+				c[fixed_axis[face]] = fixed_value[face];
+				c[varying[face][0]] = kGaussPoint[qi];
+				c[varying[face][1]] = kGaussPoint[qj];
+				EvaluateBasis(e, c[0], c[1], c[2], false, b);
+				for (int a = 0; a < nen; ++a)
+					for (int k = 0; k < 3; ++k)
+						local[4*a + k] -= pressure*sign[face]*kGaussWeight[qi]*kGaussWeight[qj]*2.0*(0.125*b.raw_det)
+							*b.inv[fixed_axis[face]][k]*b.value[a];
+			}
+	});
+}
+
+int oracle_num_threads()
+{
+#ifdef _OPENMP
+	return omp_get_max_threads();
+#else
+	return 1;
+#endif
+}
+
+}  // extern "C"

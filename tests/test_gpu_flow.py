@@ -1,0 +1,97 @@
+"""End-to-end Newton / backward-Euler steps on the GPU against the CPU oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tubularflowiga_amd import boundary, casegen
+
+pytestmark = pytest.mark.gpu
+
+RHO, MU = 1060.0 * (2e-3) ** 2, 3.5e-3
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tubularflowiga_amd import device
+    c = device.Context(0)
+    print(c.device_info(), flush=True)
+    return c
+
+
+@pytest.fixture(scope="module")
+def tube():
+    return casegen.workload("tube_c0")
+
+
+def _rel(a, b):
+    a = a.reshape(-1, 4)
+    b = b.reshape(-1, 4)
+    v = np.linalg.norm(a[:, :3] - b[:, :3]) / max(np.linalg.norm(b[:, :3]), 1e-300)
+    p = np.linalg.norm(a[:, 3] - b[:, 3]) / max(np.linalg.norm(b[:, 3]), 1e-300)
+    return v, p
+
+
+def _oracle_steps(case, steps, dt):
+    pattern = O.block_pattern(case.mesh)
+    vmask, pmask, vt, pt = boundary.resolve_flow_boundaries(case)
+    state = boundary.initial_state(vmask, pmask, vt, pt)
+    for _ in range(steps):
+        prev = state.copy()
+        state, its, lin, hist = O.ns_newton_step(case.mesh, pattern, state, prev, RHO, MU, dt, vmask, pmask, vt, pt,
+                                                 max_newton=40, rtol=1e-13, atol=1e-16, linear_rtol=1e-12)
+    return state
+
+
+@pytest.mark.parametrize("pc,precision", [(0, 0), (1, 0), (1, 1)])
+def test_backward_euler_matches_oracle(ctx, tube, pc, precision):
+    from tubularflowiga_amd import device
+    dm = device.DeviceMesh(ctx, tube.mesh)
+    vmask, pmask, vt, pt = boundary.resolve_flow_boundaries(tube)
+    s0 = boundary.initial_state(vmask, pmask, vt, pt)
+    flow = device.FlowSolver(dm, density=RHO, viscosity=MU, dt=1e-3, maximum_newton=40, nonlinear_rtol=1e-13,
+                             nonlinear_atol=1e-16, mass_rtol=1e30, linear_rtol=1e-12, preconditioner=pc,
+                             precision=precision)
+    flow.set_state(s0)
+    flow.set_boundary(vmask, pmask, vt, pt)
+    flow.set_boundary_faces(*boundary.boundary_faces(tube.mesh))
+    reports = []
+    for _ in range(2):
+        flow.begin_step()
+        reports.append(flow.newton())
+        flow.commit_step()
+    gpu = flow.state()
+    ref = _oracle_steps(tube, 2, 1e-3)
+    v, p = _rel(gpu, ref)
+    print(f"pc={pc} precision={precision} newton={[r['newton_iterations'] for r in reports]} "
+          f"linear={[r['linear_iterations'] for r in reports]} rel_l2 v={v:.3e} p={p:.3e}", flush=True)
+    assert v <= 1e-10 and p <= 1e-10
+    for r in reports:
+        assert r["converged"] == 1
+
+
+def test_mass_balance_reported(ctx, tube):
+    from tubularflowiga_amd import device
+    dm = device.DeviceMesh(ctx, tube.mesh)
+    vmask, pmask, vt, pt = boundary.resolve_flow_boundaries(tube)
+    flow = device.FlowSolver(dm, density=RHO, viscosity=MU, dt=1e-3, nonlinear_rtol=1e-8, nonlinear_atol=1e-14,
+                             mass_rtol=1e-3)
+    flow.set_state(boundary.initial_state(vmask, pmask, vt, pt))
+    flow.set_boundary(vmask, pmask, vt, pt)
+    flow.set_boundary_faces(*boundary.boundary_faces(tube.mesh))
+    flow.begin_step()
+    rep = flow.newton()
+    last = rep["records"][-1]
+    assert last["relative_mass_imbalance"] <= 1e-3
+    q_in, q_out = flow.boundary_flows([1, 2])
+    assert q_in < 0 < q_out
+    assert abs(q_in + q_out) <= 1e-3 * abs(q_in)
+    # the oracle's face flow of every labelled face, summed per label, equals the GPU sum
+    state = flow.state().reshape(-1, 4)
+    m = tube.mesh
+    oflows = {1: 0.0, 2: 0.0}
+    for e, f, lab in zip(*boundary.boundary_faces(m)):
+        if lab in oflows:
+            conn = m.element_connectivity(e)
+            oflows[lab] += O.boundary_flow(m.element_bezier(e), m.dense_extraction(e), state[conn], int(f))
+    assert abs(oflows[1] - q_in) <= 1e-12 * abs(q_in)
+    assert abs(oflows[2] - q_out) <= 1e-12 * abs(q_out)

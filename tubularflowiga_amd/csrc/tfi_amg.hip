@@ -1,0 +1,677 @@
+// Node-block (4x4) aggregation AMG preconditioner.
+//
+// The reference GPU path has no AMG (it preconditions GMRES with node-block
+// Jacobi, solvers/cuda/include/Gmres.hpp:47-67); its CPU path uses PETSc
+// FIELDSPLIT + GAMG (solvers/cpu/include/TransientFlowRuntime.hpp:176-215) and
+// the GPU tet solver a scalar plain-aggregation AMG
+// (solvers/cuda/include/AggregationAmg.hpp).  This is a coupled (all four
+// fields per node) plain-aggregation hierarchy:
+//   * symbolic phase (host, once per mesh, deterministic): strength graph from
+//     the first matrix (block Frobenius norms), greedy aggregation in node
+//     order, coarse block pattern, and for every coarse block the ascending
+//     list of fine blocks that sum into it;
+//   * numeric phase (GPU, every Newton step): Galerkin coarse blocks as
+//     fixed-order segmented sums (no atomics), 4x4 diagonal inverses, and a
+//     dense Gauss-Jordan inverse of the coarsest operator;
+//   * V-cycle: damped block-Jacobi pre/post smoothing (fused residual+smooth
+//     kernel, one wave per block row), restriction = aggregate sums,
+//     prolongation = injection; optionally entirely in FP32.
+// The prolongator is the 0/1 aggregate map, stored as integers, so it is exact
+// in any precision.
+#include "tfi_solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace tfi {
+
+namespace {
+
+constexpr double kStrength = 0.25;     // row-relative strength threshold
+constexpr int kCoarseMax = 192;        // nodes on the coarsest (direct) level
+constexpr int kMaxLevels = 12;
+constexpr double kOmega = 0.65;        // block-Jacobi damping
+constexpr int kPre = 2, kPost = 2;
+
+template <class T>
+__device__ inline T WaveSumT(T v)
+{
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;
+}
+
+// x_out = x_in + omega D^-1 (b - A x_in); x_in == nullptr means x_in = 0.
+template <class T>
+__global__ void __launch_bounds__(256) SmoothKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
+	const T* __restrict__ vals, const T* __restrict__ inv, const T* __restrict__ b, const T* __restrict__ xin,
+	T* __restrict__ xout, T omega)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 2, r = lane & 3;
+	T acc = 0;
+	if (xin) {
+		for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
+			const T* a = vals + (size_t)k*16 + 4*r;
+			const T* xv = xin + 4*(size_t)cols[k];
+			acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
+		}
+		acc += __shfl_xor(acc, 4, 64);
+		acc += __shfl_xor(acc, 8, 64);
+		acc += __shfl_xor(acc, 16, 64);
+		acc += __shfl_xor(acc, 32, 64);
+	}
+	// lanes 0..3 hold row residual components after the butterfly
+	const T res = b[4*(size_t)row + r] - acc;
+	const T r0 = __shfl(res, 0, 64), r1 = __shfl(res, 1, 64), r2 = __shfl(res, 2, 64), r3 = __shfl(res, 3, 64);
+	if (sub == 0) {
+		const T* d = inv + (size_t)row*16 + 4*r;
+		const T corr = d[0]*r0 + d[1]*r1 + d[2]*r2 + d[3]*r3;
+		xout[4*(size_t)row + r] = (xin ? xin[4*(size_t)row + r] : T(0)) + omega*corr;
+	}
+}
+
+// r = b - A x
+template <class T>
+__global__ void __launch_bounds__(256) ResidualKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
+	const T* __restrict__ vals, const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 2, rr = lane & 3;
+	T acc = 0;
+	for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
+		const T* a = vals + (size_t)k*16 + 4*rr;
+		const T* xv = x + 4*(size_t)cols[k];
+		acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
+	}
+	acc += __shfl_xor(acc, 4, 64);
+	acc += __shfl_xor(acc, 8, 64);
+	acc += __shfl_xor(acc, 16, 64);
+	acc += __shfl_xor(acc, 32, 64);
+	if (sub == 0) r[4*(size_t)row + rr] = b[4*(size_t)row + rr] - acc;
+}
+
+// bc[I] = sum_{i in aggregate I} r[i] (ascending fine order)
+template <class T>
+__global__ void RestrictKernel(int coarse, const int* __restrict__ aoff, const int* __restrict__ amem,
+	const T* __restrict__ r, T* __restrict__ bc)
+{
+	const int t = blockIdx.x*blockDim.x + threadIdx.x;
+	const int I = t >> 2, c = t & 3;
+	if (I >= coarse) return;
+	T s = 0;
+	for (int k = aoff[I]; k < aoff[I + 1]; ++k) s += r[4*(size_t)amem[k] + c];
+	bc[4*(size_t)I + c] = s;
+}
+
+// x[i] += xc[agg[i]]
+template <class T>
+__global__ void ProlongKernel(int fine, const int* __restrict__ agg, const T* __restrict__ xc, T* __restrict__ x)
+{
+	const int t = blockIdx.x*blockDim.x + threadIdx.x;
+	const int i = t >> 2, c = t & 3;
+	if (i >= fine) return;
+	x[4*(size_t)i + c] += xc[4*(size_t)agg[i] + c];
+}
+
+// coarse[K] = sum of fine blocks list[off[K]..off[K+1]) (16 lanes per block)
+__global__ void GalerkinKernel(long long cblocks, const long long* __restrict__ off, const int* __restrict__ list,
+	const double* __restrict__ fine, double* __restrict__ coarse)
+{
+	const long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x;
+	const long long K = t >> 4;
+	const int e = t & 15;
+	if (K >= cblocks) return;
+	double s = 0.0;
+	for (long long k = off[K]; k < off[K + 1]; ++k) s += fine[(size_t)list[k]*16 + e];
+	coarse[(size_t)K*16 + e] = s;
+}
+
+// Dense matrix of the coarsest level: D[(4i+a)*n + 4j+b] = A_ij[a][b]
+__global__ void DenseKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
+	const double* __restrict__ vals, double* __restrict__ dense)
+{
+	const int i = blockIdx.x;
+	const int n = 4*rows;
+	for (int k = ro[i]; k < ro[i + 1]; ++k)
+		for (int e = threadIdx.x; e < 16; e += blockDim.x) {
+			const int a = e >> 2, b = e & 3;
+			dense[(size_t)(4*i + a)*n + 4*cols[k] + b] = vals[(size_t)k*16 + e];
+		}
+}
+
+// In-place Gauss-Jordan inverse with partial pivoting of an n x n matrix
+// (single workgroup; n <= 1024).  Augmented [A | I] stored as a (n x 2n).
+__global__ void __launch_bounds__(1024) GaussJordanKernel(int n, double* __restrict__ aug, int* __restrict__ status)
+{
+	__shared__ double sval[1024];
+	__shared__ int sidx[1024];
+	__shared__ int piv;
+	const int w = 2*n;
+	for (int p = 0; p < n; ++p) {
+		double best = -1.0;
+		int bi = p;
+		for (int i = p + threadIdx.x; i < n; i += blockDim.x) {
+			const double v = fabs(aug[(size_t)i*w + p]);
+			if (v > best) { best = v; bi = i; }
+		}
+		sval[threadIdx.x] = best;
+		sidx[threadIdx.x] = bi;
+		__syncthreads();
+		for (int s = blockDim.x/2; s > 0; s >>= 1) {
+			if (threadIdx.x < s) {
+				const double o = sval[threadIdx.x + s];
+				const int oi = sidx[threadIdx.x + s];
+				if (o > sval[threadIdx.x] || (o == sval[threadIdx.x] && oi < sidx[threadIdx.x])) {
+					sval[threadIdx.x] = o;
+					sidx[threadIdx.x] = oi;
+				}
+			}
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) {
+			piv = sidx[0];
+			if (!(sval[0] > 1e-300) || !isfinite(sval[0])) *status = 1;
+		}
+		__syncthreads();
+		const int q = piv;
+		if (q != p)
+			for (int j = threadIdx.x; j < w; j += blockDim.x) {
+				const double t = aug[(size_t)p*w + j];
+				aug[(size_t)p*w + j] = aug[(size_t)q*w + j];
+				aug[(size_t)q*w + j] = t;
+			}
+		__syncthreads();
+		const double inv = 1.0/aug[(size_t)p*w + p];
+		__syncthreads();
+		for (int j = threadIdx.x; j < w; j += blockDim.x) aug[(size_t)p*w + j] *= inv;
+		__syncthreads();
+		for (int idx = threadIdx.x; idx < n*w; idx += blockDim.x) {
+			const int i = idx/w, j = idx - i*w;
+			if (i == p) continue;
+			const double f = aug[(size_t)i*w + p];
+			if (j == p) continue;
+			aug[(size_t)i*w + j] -= f*aug[(size_t)p*w + j];
+		}
+		__syncthreads();
+		for (int i = threadIdx.x; i < n; i += blockDim.x)
+			if (i != p) aug[(size_t)i*w + p] = 0.0;
+		__syncthreads();
+	}
+}
+
+// inverse (n x n) from the right half of the augmented matrix, in T
+template <class T>
+__global__ void ExtractInverseKernel(int n, const double* __restrict__ aug, T* __restrict__ inv)
+{
+	const long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x;
+	if (t >= (long long)n*n) return;
+	const int i = static_cast<int>(t/n), j = static_cast<int>(t - (long long)i*n);
+	inv[t] = static_cast<T>(aug[(size_t)i*2*n + n + j]);
+}
+
+// y = Dinv b (dense), one wave per row
+template <class T>
+__global__ void DenseApplyKernel(int n, const T* __restrict__ inv, const T* __restrict__ b, T* __restrict__ y)
+{
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (row >= n) return;
+	T acc = 0;
+	for (int j = lane; j < n; j += 64) acc += inv[(size_t)row*n + j]*b[j];
+	acc = WaveSumT(acc);
+	if (lane == 0) y[row] = acc;
+}
+
+__global__ void ToFloatKernel2(long long n, const double* __restrict__ x, float* __restrict__ y)
+{
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
+		y[i] = static_cast<float>(x[i]);
+}
+
+__global__ void ToDoubleKernel(long long n, const float* __restrict__ x, double* __restrict__ y)
+{
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
+		y[i] = static_cast<double>(x[i]);
+}
+
+__global__ void BlockInverseT32Kernel(long long n, const double* __restrict__ x, float* __restrict__ y)
+{
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
+		y[i] = static_cast<float>(x[i]);
+}
+
+int Grid(long long threads, int block = 256)
+{
+	long long b = (threads + block - 1)/block;
+	return static_cast<int>(std::max<long long>(1, std::min<long long>(b, 1LL << 30)));
+}
+
+}  // namespace
+
+struct Level {
+	int rows = 0;
+	long long nnzb = 0;
+	// pattern (level 0 borrows the mesh pattern)
+	DevBuf<int> ro_own, cols_own, diag_own;
+	const int* ro = nullptr;
+	const int* cols = nullptr;
+	const int* diag = nullptr;
+	std::vector<int> h_ro, h_cols, h_diag;
+	DevBuf<double> vals;          // own values (levels >= 1)
+	const double* vals64 = nullptr;
+	DevBuf<double> inv64;
+	DevBuf<float> vals32_own, inv32;
+	const float* vals32 = nullptr;
+	// transfer to the next level
+	DevBuf<int> agg, aoff, amem;
+	DevBuf<long long> goff;       // Galerkin lists for the NEXT level's blocks
+	DevBuf<int> glist;
+	// vectors
+	DevBuf<double> x64, y64, b64, r64;
+	DevBuf<float> x32, y32, b32, r32;
+	BsrView view() const { return {rows, nnzb, ro, cols, diag}; }
+};
+
+struct Amg::Impl {
+	tfi_mesh* mesh;
+	tfi_ctx* ctx;
+	int precision;
+	bool built = false;
+	std::vector<std::unique_ptr<Level>> L;
+	int nc = 0;  // coarsest dense size (4 * rows)
+	DevBuf<double> aug, dinv64, v64tmp;
+	DevBuf<float> dinv32;
+	DevBuf<int> status;
+
+	void Symbolic(const double* fine_values);
+	void Numeric(const double* fine_values64, const float* fine_values32);
+	template <class T>
+	void Cycle(int l);
+};
+
+namespace {
+
+// Strength of connection between block rows i and j: the largest coupling of
+// one field with itself, scaled by the two field diagonals,
+//   s_ij = max_f |A_ij[f][f]| / sqrt(|A_ii[f][f]| |A_jj[f][f]|),
+// symmetrised (max of s_ij, s_ji) so that Dirichlet rows (identity velocity
+// rows) stay connected through their pressure couplings.  j is strong for i
+// when s_ij >= theta * max(rowmax_i, rowmax_j) -- a row-relative threshold.
+// Greedy aggregation in node order (deterministic): roots whose strong
+// neighbours are all free, then attach, then leftovers.
+std::vector<int> Aggregate(int n, const std::vector<int>& ro, const std::vector<int>& cols,
+	const std::vector<double>& vals, double theta, int& count)
+{
+	std::vector<double> dg(4LL*n, 0.0);
+	for (int i = 0; i < n; ++i)
+		for (int k = ro[i]; k < ro[i + 1]; ++k)
+			if (cols[k] == i)
+				for (int f = 0; f < 4; ++f) dg[4LL*i + f] = std::abs(vals[(size_t)k*16 + 5*f]);
+	const long long nnz = ro[n];
+	std::vector<double> s(nnz, 0.0);
+	for (int i = 0; i < n; ++i)
+		for (int k = ro[i]; k < ro[i + 1]; ++k) {
+			const int j = cols[k];
+			if (j == i) continue;
+			double best = 0.0;
+			for (int f = 0; f < 4; ++f) {
+				const double d = dg[4LL*i + f]*dg[4LL*j + f];
+				if (d > 0.0) best = std::max(best, std::abs(vals[(size_t)k*16 + 5*f])/std::sqrt(d));
+			}
+			s[k] = best;
+		}
+	// symmetrise through the (structurally symmetric) pattern
+	std::vector<double> sym(s);
+	for (int i = 0; i < n; ++i)
+		for (int k = ro[i]; k < ro[i + 1]; ++k) {
+			const int j = cols[k];
+			if (j == i) continue;
+			const int* b = cols.data() + ro[j];
+			const int* e = cols.data() + ro[j + 1];
+			const int* it = std::lower_bound(b, e, i);
+			if (it != e && *it == i) sym[k] = std::max(s[k], s[it - cols.data()]);
+		}
+	std::vector<double> rowmax(n, 0.0);
+	for (int i = 0; i < n; ++i)
+		for (int k = ro[i]; k < ro[i + 1]; ++k) rowmax[i] = std::max(rowmax[i], sym[k]);
+	std::vector<int> sro(n + 1, 0), scols;
+	std::vector<double> sval;
+	for (int i = 0; i < n; ++i) {
+		for (int k = ro[i]; k < ro[i + 1]; ++k) {
+			const int j = cols[k];
+			if (j == i || sym[k] <= 0.0) continue;
+			if (sym[k] >= theta*std::max(rowmax[i], rowmax[j])) {
+				scols.push_back(j);
+				sval.push_back(sym[k]);
+			}
+		}
+		sro[i + 1] = static_cast<int>(scols.size());
+	}
+	std::vector<int> agg(n, -1);
+	count = 0;
+	for (int i = 0; i < n; ++i) {
+		if (agg[i] >= 0 || sro[i + 1] == sro[i]) continue;
+		bool free = true;
+		for (int k = sro[i]; k < sro[i + 1] && free; ++k) free = agg[scols[k]] < 0;
+		if (!free) continue;
+		agg[i] = count;
+		for (int k = sro[i]; k < sro[i + 1]; ++k) agg[scols[k]] = count;
+		++count;
+	}
+	std::vector<int> pass2(n, -1);
+	for (int i = 0; i < n; ++i) {
+		if (agg[i] >= 0) continue;
+		double best = -1.0;
+		for (int k = sro[i]; k < sro[i + 1]; ++k)
+			if (agg[scols[k]] >= 0 && sval[k] > best) {
+				best = sval[k];
+				pass2[i] = agg[scols[k]];
+			}
+	}
+	for (int i = 0; i < n; ++i)
+		if (agg[i] < 0 && pass2[i] >= 0) agg[i] = pass2[i];
+	for (int i = 0; i < n; ++i) {
+		if (agg[i] >= 0) continue;
+		agg[i] = count;
+		for (int k = sro[i]; k < sro[i + 1]; ++k)
+			if (agg[scols[k]] < 0) agg[scols[k]] = count;
+		++count;
+	}
+	return agg;
+}
+
+}  // namespace
+
+void Amg::Impl::Symbolic(const double* fine_values)
+{
+	L.clear();
+	auto l0 = std::make_unique<Level>();
+	l0->rows = mesh->nodes;
+	l0->nnzb = mesh->nnzb;
+	l0->ro = mesh->row_offsets.data();
+	l0->cols = mesh->columns.data();
+	l0->diag = mesh->diagonal.data();
+	l0->h_ro.resize(mesh->nodes + 1);
+	l0->h_cols.resize(mesh->nnzb);
+	l0->h_diag.resize(mesh->nodes);
+	mesh->row_offsets.Download(l0->h_ro.data(), mesh->nodes + 1);
+	mesh->columns.Download(l0->h_cols.data(), mesh->nnzb);
+	mesh->diagonal.Download(l0->h_diag.data(), mesh->nodes);
+	std::vector<double> hv(mesh->nnzb*16);
+	Check(hipMemcpyAsync(hv.data(), fine_values, hv.size()*sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+		"amg download");
+	Sync(ctx);
+	L.push_back(std::move(l0));
+	while ((int)L.size() < kMaxLevels && L.back()->rows > kCoarseMax) {
+		Level& f = *L.back();
+		int count = 0;
+		auto agg = Aggregate(f.rows, f.h_ro, f.h_cols, hv, kStrength, count);
+		if (count >= f.rows*0.9 || count < 1) break;  // coarsening stalled
+		// aggregate membership lists
+		std::vector<int> aoff(count + 1, 0), amem(f.rows);
+		for (int i = 0; i < f.rows; ++i) ++aoff[agg[i] + 1];
+		for (int c = 0; c < count; ++c) aoff[c + 1] += aoff[c];
+		{
+			std::vector<int> fill(aoff.begin(), aoff.end() - 1);
+			for (int i = 0; i < f.rows; ++i) amem[fill[agg[i]]++] = i;
+		}
+		// coarse pattern
+		auto c = std::make_unique<Level>();
+		c->rows = count;
+		std::vector<std::vector<int>> rows(count);
+		for (int i = 0; i < f.rows; ++i) {
+			auto& rw = rows[agg[i]];
+			for (int k = f.h_ro[i]; k < f.h_ro[i + 1]; ++k) rw.push_back(agg[f.h_cols[k]]);
+		}
+		c->h_ro.assign(count + 1, 0);
+		c->h_diag.assign(count, -1);
+		for (int I = 0; I < count; ++I) {
+			auto& rw = rows[I];
+			std::sort(rw.begin(), rw.end());
+			rw.erase(std::unique(rw.begin(), rw.end()), rw.end());
+			c->h_ro[I + 1] = c->h_ro[I] + static_cast<int>(rw.size());
+		}
+		c->nnzb = c->h_ro[count];
+		c->h_cols.resize(c->nnzb);
+		for (int I = 0; I < count; ++I) {
+			std::copy(rows[I].begin(), rows[I].end(), c->h_cols.begin() + c->h_ro[I]);
+			const auto it = std::lower_bound(rows[I].begin(), rows[I].end(), I);
+			c->h_diag[I] = c->h_ro[I] + static_cast<int>(it - rows[I].begin());
+			std::vector<int>().swap(rows[I]);
+		}
+		// fine block -> coarse block lists (ascending fine block index)
+		std::vector<long long> cnt(c->nnzb + 1, 0);
+		std::vector<int> target(f.nnzb);
+		for (int i = 0; i < f.rows; ++i) {
+			const int I = agg[i];
+			const int* b = c->h_cols.data() + c->h_ro[I];
+			const int* e = c->h_cols.data() + c->h_ro[I + 1];
+			for (int k = f.h_ro[i]; k < f.h_ro[i + 1]; ++k) {
+				const int J = agg[f.h_cols[k]];
+				const int pos = static_cast<int>(std::lower_bound(b, e, J) - c->h_cols.data());
+				target[k] = pos;
+				++cnt[pos + 1];
+			}
+		}
+		for (long long K = 0; K < c->nnzb; ++K) cnt[K + 1] += cnt[K];
+		std::vector<int> list(f.nnzb);
+		{
+			std::vector<long long> fill(cnt.begin(), cnt.end() - 1);
+			for (long long k = 0; k < f.nnzb; ++k) list[fill[target[k]]++] = static_cast<int>(k);
+		}
+		f.agg.Allocate(ctx, f.rows);
+		f.agg.Upload(agg.data(), f.rows);
+		f.aoff.Allocate(ctx, count + 1);
+		f.aoff.Upload(aoff.data(), count + 1);
+		f.amem.Allocate(ctx, f.rows);
+		f.amem.Upload(amem.data(), f.rows);
+		f.goff.Allocate(ctx, c->nnzb + 1);
+		f.goff.Upload(cnt.data(), c->nnzb + 1);
+		f.glist.Allocate(ctx, f.nnzb);
+		f.glist.Upload(list.data(), f.nnzb);
+		c->ro_own.Allocate(ctx, count + 1);
+		c->ro_own.Upload(c->h_ro.data(), count + 1);
+		c->cols_own.Allocate(ctx, c->nnzb);
+		c->cols_own.Upload(c->h_cols.data(), c->nnzb);
+		c->diag_own.Allocate(ctx, count);
+		c->diag_own.Upload(c->h_diag.data(), count);
+		c->ro = c->ro_own.data();
+		c->cols = c->cols_own.data();
+		c->diag = c->diag_own.data();
+		c->vals.Allocate(ctx, c->nnzb*16);
+		// coarse values now (the next aggregation needs them)
+		hipLaunchKernelGGL(GalerkinKernel, dim3(Grid(c->nnzb*16)), dim3(256), 0, ctx->stream, c->nnzb, f.goff.data(),
+			f.glist.data(), L.size() == 1 ? fine_values : f.vals.data(), c->vals.data());
+		TFI_LAUNCH_CHECK("GalerkinKernel");
+		hv.assign(c->nnzb*16, 0.0);
+		c->vals.Download(hv.data(), hv.size());
+		L.push_back(std::move(c));
+	}
+	// per-level buffers
+	for (size_t l = 0; l < L.size(); ++l) {
+		Level& v = *L[l];
+		const long long n = 4LL*v.rows;
+		v.inv64.Allocate(ctx, 16LL*v.rows);
+		if (precision == TFI_MIXED) {
+			v.inv32.Allocate(ctx, 16LL*v.rows);
+			v.x32.Allocate(ctx, n);
+			v.y32.Allocate(ctx, n);
+			v.b32.Allocate(ctx, n);
+			v.r32.Allocate(ctx, n);
+			if (l > 0) v.vals32_own.Allocate(ctx, v.nnzb*16);
+		} else {
+			v.x64.Allocate(ctx, n);
+			v.y64.Allocate(ctx, n);
+			v.b64.Allocate(ctx, n);
+			v.r64.Allocate(ctx, n);
+		}
+	}
+	nc = 4*L.back()->rows;
+	if (nc > 1024) throw Error("AMG coarsest level too large for the dense solve");
+	aug.Allocate(ctx, (size_t)nc*2*nc);
+	if (precision == TFI_MIXED) dinv32.Allocate(ctx, (size_t)nc*nc);
+	else dinv64.Allocate(ctx, (size_t)nc*nc);
+	status.Allocate(ctx, 1);
+	built = true;
+}
+
+void Amg::Impl::Numeric(const double* fine64, const float* fine32)
+{
+	L[0]->vals64 = fine64;
+	L[0]->vals32 = fine32;
+	for (size_t l = 0; l + 1 < L.size(); ++l) {
+		Level& f = *L[l];
+		Level& c = *L[l + 1];
+		hipLaunchKernelGGL(GalerkinKernel, dim3(Grid(c.nnzb*16)), dim3(256), 0, ctx->stream, c.nnzb, f.goff.data(),
+			f.glist.data(), l == 0 ? fine64 : f.vals.data(), c.vals.data());
+		TFI_LAUNCH_CHECK("GalerkinKernel");
+		c.vals64 = c.vals.data();
+	}
+	for (size_t l = 0; l < L.size(); ++l) {
+		Level& v = *L[l];
+		BlockInverse64(ctx, v.view(), v.vals64, v.inv64.data());
+		if (precision == TFI_MIXED) {
+			hipLaunchKernelGGL(BlockInverseT32Kernel, dim3(256), dim3(256), 0, ctx->stream, 16LL*v.rows,
+				v.inv64.data(), v.inv32.data());
+			TFI_LAUNCH_CHECK("BlockInverseT32Kernel");
+			if (l > 0) {
+				hipLaunchKernelGGL(ToFloatKernel2, dim3(256), dim3(256), 0, ctx->stream, v.nnzb*16, v.vals64,
+					v.vals32_own.data());
+				TFI_LAUNCH_CHECK("ToFloatKernel2");
+				v.vals32 = v.vals32_own.data();
+			}
+		}
+	}
+	// coarsest dense inverse
+	Level& c = *L.back();
+	Fill(ctx, aug.data(), 0.0, (long long)nc*2*nc);
+	// dense A into the left half with leading dimension 2n: build n x n then spread
+	DevBuf<double> dense(ctx, (size_t)nc*nc);
+	Fill(ctx, dense.data(), 0.0, (long long)nc*nc);
+	hipLaunchKernelGGL(DenseKernel, dim3(c.rows), dim3(64), 0, ctx->stream, c.rows, c.ro, c.cols, c.vals64,
+		dense.data());
+	TFI_LAUNCH_CHECK("DenseKernel");
+	std::vector<double> h((size_t)nc*nc), a2((size_t)nc*2*nc, 0.0);
+	dense.Download(h.data(), h.size());
+	for (int i = 0; i < nc; ++i) {
+		std::copy(h.begin() + (size_t)i*nc, h.begin() + (size_t)(i + 1)*nc, a2.begin() + (size_t)i*2*nc);
+		a2[(size_t)i*2*nc + nc + i] = 1.0;
+	}
+	aug.Upload(a2.data(), a2.size());
+	FillInt(ctx, status.data(), 0, 1);
+	hipLaunchKernelGGL(GaussJordanKernel, dim3(1), dim3(1024), 0, ctx->stream, nc, aug.data(), status.data());
+	TFI_LAUNCH_CHECK("GaussJordanKernel");
+	int st = 0;
+	status.Download(&st, 1);
+	if (st) throw Error("AMG coarsest operator is singular");
+	if (precision == TFI_MIXED) {
+		hipLaunchKernelGGL(ExtractInverseKernel<float>, dim3(Grid((long long)nc*nc)), dim3(256), 0, ctx->stream, nc,
+			aug.data(), dinv32.data());
+	} else {
+		hipLaunchKernelGGL(ExtractInverseKernel<double>, dim3(Grid((long long)nc*nc)), dim3(256), 0, ctx->stream, nc,
+			aug.data(), dinv64.data());
+	}
+	TFI_LAUNCH_CHECK("ExtractInverseKernel");
+}
+
+template <class T>
+void Amg::Impl::Cycle(int l)
+{
+	Level& v = *L[l];
+	const bool f32 = std::is_same<T, float>::value;
+	T* x = f32 ? reinterpret_cast<T*>(v.x32.data()) : reinterpret_cast<T*>(v.x64.data());
+	T* y = f32 ? reinterpret_cast<T*>(v.y32.data()) : reinterpret_cast<T*>(v.y64.data());
+	T* b = f32 ? reinterpret_cast<T*>(v.b32.data()) : reinterpret_cast<T*>(v.b64.data());
+	T* r = f32 ? reinterpret_cast<T*>(v.r32.data()) : reinterpret_cast<T*>(v.r64.data());
+	const T* vals = f32 ? reinterpret_cast<const T*>(v.vals32) : reinterpret_cast<const T*>(v.vals64);
+	const T* inv = f32 ? reinterpret_cast<const T*>(v.inv32.data()) : reinterpret_cast<const T*>(v.inv64.data());
+	const int grid = (v.rows + 3)/4;
+	if (l + 1 == (int)L.size()) {
+		const T* dinv = f32 ? reinterpret_cast<const T*>(dinv32.data()) : reinterpret_cast<const T*>(dinv64.data());
+		hipLaunchKernelGGL(DenseApplyKernel<T>, dim3((nc + 3)/4), dim3(256), 0, ctx->stream, nc, dinv, b, x);
+		TFI_LAUNCH_CHECK("DenseApplyKernel");
+		return;
+	}
+	const T omega = static_cast<T>(kOmega);
+	// pre-smoothing from x = 0
+	hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
+		static_cast<const T*>(nullptr), x, omega);
+	for (int s = 1; s < kPre; ++s) {
+		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
+			x, y, omega);
+		std::swap(x, y);
+	}
+	hipLaunchKernelGGL(ResidualKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, b, x, r);
+	TFI_LAUNCH_CHECK("ResidualKernel");
+	Level& c = *L[l + 1];
+	T* bc = f32 ? reinterpret_cast<T*>(c.b32.data()) : reinterpret_cast<T*>(c.b64.data());
+	T* xc = f32 ? reinterpret_cast<T*>(c.x32.data()) : reinterpret_cast<T*>(c.x64.data());
+	hipLaunchKernelGGL(RestrictKernel<T>, dim3(Grid(4LL*c.rows)), dim3(256), 0, ctx->stream, c.rows, v.aoff.data(),
+		v.amem.data(), r, bc);
+	TFI_LAUNCH_CHECK("RestrictKernel");
+	Cycle<T>(l + 1);
+	// the coarse solution lives in c.x (cycle output); when the coarse cycle
+	// ended with an odd number of swaps it is in c.y -- Cycle always leaves it in x
+	hipLaunchKernelGGL(ProlongKernel<T>, dim3(Grid(4LL*v.rows)), dim3(256), 0, ctx->stream, v.rows, v.agg.data(), xc,
+		x);
+	TFI_LAUNCH_CHECK("ProlongKernel");
+	for (int s = 0; s < kPost; ++s) {
+		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
+			x, y, omega);
+		std::swap(x, y);
+	}
+	TFI_LAUNCH_CHECK("SmoothKernel");
+	// leave the result in the level's x buffer
+	T* xbuf = f32 ? reinterpret_cast<T*>(v.x32.data()) : reinterpret_cast<T*>(v.x64.data());
+	if (x != xbuf) {
+		const long long n = 4LL*v.rows;
+		if (f32) {
+			Check(hipMemcpyAsync(xbuf, x, n*sizeof(T), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+		} else {
+			Copy(ctx, reinterpret_cast<const double*>(x), reinterpret_cast<double*>(xbuf), n);
+		}
+	}
+}
+
+Amg::Amg(tfi_mesh* mesh, int precision) : impl_(std::make_unique<Impl>())
+{
+	impl_->mesh = mesh;
+	impl_->ctx = mesh->ctx;
+	impl_->precision = precision;
+}
+
+Amg::~Amg() = default;
+
+int Amg::levels() const { return static_cast<int>(impl_->L.size()); }
+
+void Amg::Setup(const double* values64, const float* values32)
+{
+	if (!impl_->built) impl_->Symbolic(values64);
+	impl_->Numeric(values64, values32);
+	singular_ = 0;
+}
+
+void Amg::Apply(const double* v, double* z)
+{
+	Impl& I = *impl_;
+	Level& l0 = *I.L[0];
+	const long long n = 4LL*l0.rows;
+	if (I.precision == TFI_MIXED) {
+		hipLaunchKernelGGL(ToFloatKernel2, dim3(Grid(n)), dim3(256), 0, I.ctx->stream, n, v, l0.b32.data());
+		TFI_LAUNCH_CHECK("ToFloatKernel2");
+		I.Cycle<float>(0);
+		hipLaunchKernelGGL(ToDoubleKernel, dim3(Grid(n)), dim3(256), 0, I.ctx->stream, n, l0.x32.data(), z);
+		TFI_LAUNCH_CHECK("ToDoubleKernel");
+	} else {
+		Copy(I.ctx, v, l0.b64.data(), n);
+		I.Cycle<double>(0);
+		Copy(I.ctx, l0.x64.data(), z, n);
+	}
+}
+
+}  // namespace tfi

@@ -1,0 +1,579 @@
+// Flow solver (Newton loop over device-resident state) and the host-in/out
+// kernel entry points of the C-ABI.
+//
+// tfi_flow_newton restates the nonlinear loop of the reference driver
+// (solvers/cuda/src/iga_cuda.cu:956-1046): assemble residual + tangent,
+// add the traction load, measure continuity and boundary mass imbalance,
+// apply Dirichlet rows, test ||r|| <= max(atol sqrt(4N), rtol ||r0||) together
+// with the imbalance tolerance, solve, update.  Unlike the reference it copies
+// no vector to the host per Newton iteration: the boundary flows are
+// integrated on the GPU and only per-face scalars come back.
+#include "tfi_internal.hpp"
+#include "tfi_solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+
+namespace tfi {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+__global__ void ContinityPartialKernel(const double* __restrict__ rhs, int nodes, double* __restrict__ partial)
+{
+	__shared__ double l2[4], sm[4];
+	double a = 0.0, s = 0.0;
+	for (int n = blockIdx.x*blockDim.x + threadIdx.x; n < nodes; n += gridDim.x*blockDim.x) {
+		const double c = rhs[4*(size_t)n + 3];
+		a += c*c;
+		s += c;
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		a += __shfl_xor(a, o, 64);
+		s += __shfl_xor(s, o, 64);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		l2[threadIdx.x >> 6] = a;
+		sm[threadIdx.x >> 6] = s;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		partial[2*blockIdx.x] = l2[0] + l2[1] + l2[2] + l2[3];
+		partial[2*blockIdx.x + 1] = sm[0] + sm[1] + sm[2] + sm[3];
+	}
+}
+
+}  // namespace
+
+LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart)
+	: mesh_(mesh), ctx_(mesh->ctx), pc_(preconditioner), precision_(precision),
+	  krylov_(mesh->ctx, 4LL*mesh->nodes, restart)
+{
+	const long long n = 4LL*mesh->nodes;
+	inverse_.Allocate(ctx_, 16LL*mesh->nodes);
+	work_.Allocate(ctx_, n);
+	if (precision_ == TFI_MIXED) {
+		values32_.Allocate(ctx_, mesh->nnzb*16);
+		correction_.Allocate(ctx_, n);
+		residual_.Allocate(ctx_, n);
+	}
+	if (pc_ == TFI_PC_AMG) amg_ = std::make_unique<Amg>(mesh, precision_);
+}
+
+LinearSolver::~LinearSolver() = default;
+
+void LinearSolver::Setup(const double* values)
+{
+	values_ = values;
+	const BsrView a = mesh_->view();
+	if (precision_ == TFI_MIXED) ToFloat(ctx_, values, values32_.data(), mesh_->nnzb*16);
+	if (pc_ == TFI_PC_BJACOBI) {
+		singular_ = BlockInverse64(ctx_, a, values, inverse_.data());
+	} else {
+		amg_->Setup(values, precision_ == TFI_MIXED ? values32_.data() : nullptr);
+		singular_ = amg_->singular_blocks();
+	}
+}
+
+KrylovResult LinearSolver::Solve(const double* b, double* x, int maxit, double rtol)
+{
+	const BsrView a = mesh_->view();
+	const long long n = 4LL*mesh_->nodes;
+	tfi_ctx* ctx = ctx_;
+	Operator A64 = [&](const double* in, double* out) {
+		StreamTimer t(ctx, &ctx->timers.spmv_ms);
+		Spmv64(ctx, a, values_, in, out);
+		ctx->timers.spmv_calls += 1;
+	};
+	Precond M = [&](const double* v, double* z) {
+		if (pc_ == TFI_PC_BJACOBI) BlockApply64(ctx, mesh_->nodes, inverse_.data(), v, z);
+		else amg_->Apply(v, z);
+	};
+	if (precision_ == TFI_FP64) return krylov_.Solve(A64, M, b, x, maxit, rtol);
+	// Mixed precision: iterative refinement.  Outer residual, Gram-Schmidt and
+	// updates in FP64; the inner operator reads FP32 matrix values and the
+	// preconditioner runs in FP32.
+	Operator A32 = [&](const double* in, double* out) {
+		StreamTimer t(ctx, &ctx->timers.spmv32_ms);
+		Spmv32(ctx, a, values32_.data(), in, out);
+		ctx->timers.spmv32_calls += 1;
+	};
+	KrylovResult total;
+	total.rhs_norm = Norm2(ctx, b, n);
+	const double target = rtol*total.rhs_norm;
+	for (int refinement = 0; refinement < 30; ++refinement) {
+		Residual64(ctx, a, values_, x, b, residual_.data());
+		const double rn = Norm2(ctx, residual_.data(), n);
+		total.residual = rn;
+		if (rn <= target || total.rhs_norm == 0.0) {
+			total.converged = true;
+			break;
+		}
+		if (total.iterations >= maxit) break;
+		Fill(ctx, correction_.data(), 0.0, n);
+		// ask the inner solve for what FP32 storage can deliver, never more
+		// than the remaining reduction
+		const double inner = std::max(target/rn, 1e-5);
+		const auto r = krylov_.Solve(A32, M, residual_.data(), correction_.data(), maxit - total.iterations, inner);
+		total.iterations += r.iterations;
+		Axpy(ctx, 1.0, correction_.data(), x, n);
+		++refinements_;
+	}
+	return total;
+}
+
+}  // namespace tfi
+
+using namespace tfi;
+
+struct tfi_flow {
+	tfi_mesh* mesh = nullptr;
+	tfi_ctx* ctx = nullptr;
+	tfi_flow_options opt{};
+	DevBuf<double> state, previous, update, rhs, traction, jac, vtarget, ptarget, face_flow, cont_partial;
+	DevBuf<int> vmask, pmask, face_elem, face_id;
+	std::vector<int> face_label;
+	bool has_traction = false, has_boundary = false;
+	std::unique_ptr<LinearSolver> solver;
+	std::unique_ptr<WarmStart> warm;
+	double phase[8] = {};
+	int phase_count = 0;
+};
+
+namespace {
+
+struct Metrics {
+	double continuity_l2 = 0, continuity_sum = 0, net = 0, absolute = 0, imbalance = 0;
+};
+
+Metrics Measure(tfi_flow* f)
+{
+	tfi_ctx* ctx = f->ctx;
+	Metrics m;
+	const int blocks = 256;
+	hipLaunchKernelGGL(ContinityPartialKernel, dim3(blocks), dim3(256), 0, ctx->stream, f->rhs.data(), f->mesh->nodes,
+		f->cont_partial.data());
+	TFI_LAUNCH_CHECK("ContinityPartialKernel");
+	std::vector<double> part(2*blocks);
+	f->cont_partial.Download(part.data(), part.size());
+	for (int b = 0; b < blocks; ++b) {
+		m.continuity_l2 += part[2*b];
+		m.continuity_sum += part[2*b + 1];
+	}
+	m.continuity_l2 = std::sqrt(m.continuity_l2);
+	if (!f->face_label.empty()) {
+		const int count = static_cast<int>(f->face_label.size());
+		BoundaryFaceFlows(f->mesh, count, f->face_elem.data(), f->face_id.data(), f->state.data(), f->face_flow.data());
+		std::vector<double> flows(count);
+		f->face_flow.Download(flows.data(), count);
+		std::map<int, double> per_label;
+		for (int i = 0; i < count; ++i) per_label[f->face_label[i]] += flows[i];
+		for (const auto& kv : per_label) {
+			m.net += kv.second;
+			m.absolute += std::abs(kv.second);
+		}
+		if (m.absolute > 0.0) m.imbalance = 2.0*std::abs(m.net)/m.absolute;
+	}
+	return m;
+}
+
+template <class F>
+int Guard(tfi_ctx* ctx, F&& fn)
+{
+	try {
+		fn();
+		return TFI_OK;
+	} catch (const std::exception& e) {
+		if (ctx) ctx->error = e.what();
+		return TFI_ERROR;
+	}
+}
+
+}  // namespace
+
+extern "C" {
+
+void tfi_flow_default_options(tfi_flow_options* o)
+{
+	if (!o) return;
+	*o = tfi_flow_options{};
+	o->density = 1.0;
+	o->viscosity = 0.1;
+	o->dt = 0.0;
+	o->maximum_newton = 12;
+	o->nonlinear_rtol = 1e-5;
+	o->nonlinear_atol = 1e-10;
+	o->mass_rtol = 1e-3;
+	o->linear_maximum_iterations = 5000;
+	o->linear_rtol = 1e-8;
+	o->restart = 200;
+	o->preconditioner = TFI_PC_AMG;
+	o->precision = TFI_FP64;
+	o->warm_start = TFI_WARM_PREVIOUS;
+	o->verbose = 0;
+}
+
+int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** out)
+{
+	if (!mesh || !options || !out) return TFI_ERROR;
+	*out = nullptr;
+	return Guard(mesh->ctx, [&] {
+		if (!(options->density > 0.0) || !(options->viscosity > 0.0) || !(options->dt >= 0.0))
+			throw Error("invalid Navier-Stokes density, dynamic viscosity, or time step");
+		auto f = std::make_unique<tfi_flow>();
+		f->mesh = mesh;
+		f->ctx = mesh->ctx;
+		f->opt = *options;
+		tfi_ctx* ctx = mesh->ctx;
+		const long long n = 4LL*mesh->nodes;
+		f->state.Allocate(ctx, n);
+		f->previous.Allocate(ctx, n);
+		f->update.Allocate(ctx, n);
+		f->rhs.Allocate(ctx, n);
+		f->traction.Allocate(ctx, n);
+		f->jac.Allocate(ctx, mesh->nnzb*16);
+		f->vtarget.Allocate(ctx, 3LL*mesh->nodes);
+		f->ptarget.Allocate(ctx, mesh->nodes);
+		f->vmask.Allocate(ctx, mesh->nodes);
+		f->pmask.Allocate(ctx, mesh->nodes);
+		f->cont_partial.Allocate(ctx, 512);
+		for (auto* b : {&f->state, &f->previous, &f->update, &f->rhs, &f->traction, &f->vtarget, &f->ptarget})
+			Fill(ctx, b->data(), 0.0, b->size());
+		FillInt(ctx, f->vmask.data(), 0, mesh->nodes);
+		FillInt(ctx, f->pmask.data(), 0, mesh->nodes);
+		f->solver = std::make_unique<LinearSolver>(mesh, options->preconditioner, options->precision,
+			options->restart);
+		f->warm = std::make_unique<WarmStart>(mesh);
+		Sync(ctx);
+		*out = f.release();
+	});
+}
+
+void tfi_flow_destroy(tfi_flow* f)
+{
+	if (f && f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
+	delete f;
+}
+
+int tfi_flow_set_state(tfi_flow* f, const double* state, const double* previous)
+{
+	return Guard(f->ctx, [&] {
+		const size_t n = 4ULL*f->mesh->nodes;
+		f->state.Upload(state, n);
+		f->previous.Upload(previous ? previous : state, n);
+		f->warm->Reset();
+	});
+}
+
+int tfi_flow_get_state(tfi_flow* f, double* state)
+{
+	return Guard(f->ctx, [&] { f->state.Download(state, 4ULL*f->mesh->nodes); });
+}
+
+int tfi_flow_set_boundary(tfi_flow* f, const int32_t* vmask, const int32_t* pmask, const double* vtarget,
+	const double* ptarget, const double* traction)
+{
+	return Guard(f->ctx, [&] {
+		const int N = f->mesh->nodes;
+		f->vmask.Upload(vmask, N);
+		f->pmask.Upload(pmask, N);
+		f->vtarget.Upload(vtarget, 3ULL*N);
+		f->ptarget.Upload(ptarget, N);
+		f->has_traction = traction != nullptr;
+		if (traction) f->traction.Upload(traction, 4ULL*N);
+		f->has_boundary = true;
+	});
+}
+
+int tfi_flow_set_boundary_faces(tfi_flow* f, int count, const int32_t* element, const int32_t* face,
+	const int32_t* label)
+{
+	return Guard(f->ctx, [&] {
+		if (count < 0) throw Error("negative face count");
+		for (int i = 0; i < count; ++i)
+			if (element[i] < 0 || element[i] >= f->mesh->elements || face[i] < 0 || face[i] > 5)
+				throw Error("invalid boundary face");
+		f->face_label.assign(label, label + count);
+		f->face_elem.Allocate(f->ctx, count);
+		f->face_id.Allocate(f->ctx, count);
+		f->face_flow.Allocate(f->ctx, count);
+		if (count) {
+			f->face_elem.Upload(element, count);
+			f->face_id.Upload(face, count);
+		}
+	});
+}
+
+int tfi_flow_begin_step(tfi_flow* f, const double* phase, int phase_count)
+{
+	return Guard(f->ctx, [&] {
+		Copy(f->ctx, f->state.data(), f->previous.data(), 4LL*f->mesh->nodes);
+		f->phase_count = std::min(phase_count, 8);
+		for (int i = 0; i < f->phase_count; ++i) f->phase[i] = phase[i];
+	});
+}
+
+int tfi_flow_commit_step(tfi_flow* f)
+{
+	return Guard(f->ctx, [&] { f->warm->Commit(f->state.data(), f->phase, f->phase_count); });
+}
+
+int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
+{
+	return Guard(f->ctx, [&] {
+		if (!f->has_boundary) throw Error("tfi_flow_set_boundary must be called before tfi_flow_newton");
+		tfi_ctx* ctx = f->ctx;
+		tfi_mesh* m = f->mesh;
+		const long long n = 4LL*m->nodes;
+		const NsParams params{f->opt.density, f->opt.viscosity, f->opt.dt};
+		tfi_newton_report rep{};
+		const auto t_start = Clock::now();
+		const double scale = std::sqrt(static_cast<double>(n));
+		double initial = -1.0;
+		// learned / extrapolated initial state (accepted only if it lowers the residual)
+		if (f->opt.warm_start != TFI_WARM_PREVIOUS && f->warm->Ready(f->opt.warm_start)) {
+			AssembleNs(m, f->state.data(), f->previous.data(), params, nullptr, f->rhs.data());
+			if (f->has_traction) Axpy(ctx, 1.0, f->traction.data(), f->rhs.data(), n);
+			ApplyDirichlet(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(), f->state.data(),
+				nullptr, f->rhs.data());
+			const double base = Norm2(ctx, f->rhs.data(), n);
+			Copy(ctx, f->state.data(), f->update.data(), n);  // keep the plain guess
+			f->warm->Predict(f->opt.warm_start, f->state.data(), f->phase, f->phase_count);
+			SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
+				f->state.data());
+			AssembleNs(m, f->state.data(), f->previous.data(), params, nullptr, f->rhs.data());
+			if (f->has_traction) Axpy(ctx, 1.0, f->traction.data(), f->rhs.data(), n);
+			ApplyDirichlet(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(), f->state.data(),
+				nullptr, f->rhs.data());
+			const double guessed = Norm2(ctx, f->rhs.data(), n);
+			rep.warm_start_initial_ratio = base > 0.0 ? guessed/base : 0.0;
+			if (guessed < base) {
+				rep.warm_start_used = 1;
+				f->warm->Accepted(true);
+				initial = base;  // relative tolerance measured against the plain start
+			} else {
+				Copy(ctx, f->update.data(), f->state.data(), n);
+				f->warm->Accepted(false);
+			}
+		}
+		bool converged = false;
+		for (int it = 0; it < f->opt.maximum_newton; ++it) {
+			tfi_newton_record rec{};
+			const auto a0 = Clock::now();
+			AssembleNs(m, f->state.data(), f->previous.data(), params, f->jac.data(), f->rhs.data());
+			if (f->has_traction) Axpy(ctx, 1.0, f->traction.data(), f->rhs.data(), n);
+			const Metrics mt = Measure(f);
+			ApplyDirichlet(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(), f->state.data(),
+				f->jac.data(), f->rhs.data());
+			Sync(ctx);
+			const auto a1 = Clock::now();
+			rec.assembly_s = std::chrono::duration<double>(a1 - a0).count();
+			rep.assembly_s += rec.assembly_s;
+			const double norm = Norm2(ctx, f->rhs.data(), n);
+			rec.residual_l2 = norm;
+			rec.continuity_l2 = mt.continuity_l2;
+			rec.continuity_sum = mt.continuity_sum;
+			rec.net_boundary_flow = mt.net;
+			rec.relative_mass_imbalance = mt.imbalance;
+			if (initial < 0.0) initial = norm;
+			if (it == 0) rep.initial_residual = norm;
+			const double tol = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*initial);
+			rep.tolerance = tol;
+			rep.final_residual = norm;
+			if (norm <= tol && mt.imbalance <= f->opt.mass_rtol) {
+				converged = true;
+				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				rep.newton_iterations = it;
+				break;
+			}
+			const auto l0 = Clock::now();
+			Fill(ctx, f->update.data(), 0.0, n);
+			f->solver->Setup(f->jac.data());
+			const auto r = f->solver->Solve(f->rhs.data(), f->update.data(), f->opt.linear_maximum_iterations,
+				f->opt.linear_rtol);
+			Sync(ctx);
+			const auto l1 = Clock::now();
+			rec.linear_s = std::chrono::duration<double>(l1 - l0).count();
+			rep.linear_s += rec.linear_s;
+			rec.linear_iterations = r.iterations;
+			rec.linear_residual = r.residual;
+			rec.singular_diagonal_blocks = f->solver->singular_blocks();
+			rep.linear_iterations += r.iterations;
+			if (!r.converged)
+				throw Error("Navier-Stokes linear solve failed at nonlinear iteration " + std::to_string(it)
+					+ " (residual=" + std::to_string(r.residual) + ", rhs=" + std::to_string(r.rhs_norm) + ")");
+			rec.update_l2 = Norm2(ctx, f->update.data(), n);
+			Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
+			if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+			rep.newton_iterations = it + 1;
+		}
+		Sync(ctx);
+		rep.converged = converged ? 1 : 0;
+		rep.total_s = std::chrono::duration<double>(Clock::now() - t_start).count();
+		if (report) *report = rep;
+		if (!converged) throw Error("Navier-Stokes nonlinear solve reached MAX_NEWTON");
+	});
+}
+
+int tfi_flow_boundary_flows(tfi_flow* f, int count, const int32_t* labels, double* flows)
+{
+	return Guard(f->ctx, [&] {
+		const int faces = static_cast<int>(f->face_label.size());
+		std::vector<double> per(faces, 0.0);
+		if (faces) {
+			BoundaryFaceFlows(f->mesh, faces, f->face_elem.data(), f->face_id.data(), f->state.data(),
+				f->face_flow.data());
+			f->face_flow.Download(per.data(), faces);
+		}
+		for (int i = 0; i < count; ++i) {
+			double s = 0.0;
+			for (int k = 0; k < faces; ++k)
+				if (f->face_label[k] == labels[i]) s += per[k];
+			flows[i] = s;
+		}
+	});
+}
+
+int tfi_flow_warm_configure(tfi_flow* f, const tfi_warm_options* o)
+{
+	return Guard(f->ctx, [&] { f->warm->Configure(*o); });
+}
+
+int tfi_flow_warm_stats(tfi_flow* f, int* snapshots, double* last_loss, int* accepted, int* rejected)
+{
+	return Guard(f->ctx, [&] { f->warm->Stats(snapshots, last_loss, accepted, rejected); });
+}
+
+// ---------------------------------------------------------------- host in/out kernels
+
+int tfi_ns_assemble(tfi_mesh* m, const double* state, const double* previous, double density, double viscosity,
+	double dt, double* jacobian, double* negative_residual)
+{
+	return Guard(m->ctx, [&] {
+		if (!(density > 0.0) || !(viscosity > 0.0) || !(dt >= 0.0))
+			throw Error("invalid Navier-Stokes density, dynamic viscosity, or time step");
+		tfi_ctx* ctx = m->ctx;
+		const size_t n = 4ULL*m->nodes;
+		DevBuf<double> s(ctx, n), p(ctx, n), r(ctx, n), j;
+		s.Upload(state, n);
+		p.Upload(previous ? previous : state, n);
+		if (jacobian) j.Allocate(ctx, m->nnzb*16);
+		AssembleNs(m, s.data(), p.data(), NsParams{density, viscosity, dt}, jacobian ? j.data() : nullptr, r.data());
+		r.Download(negative_residual, n);
+		if (jacobian) j.Download(jacobian, m->nnzb*16);
+	});
+}
+
+int tfi_bsr_spmv(tfi_mesh* m, const double* values, const double* x, double* y, int precision)
+{
+	return Guard(m->ctx, [&] {
+		tfi_ctx* ctx = m->ctx;
+		const size_t n = 4ULL*m->nodes;
+		DevBuf<double> v(ctx, m->nnzb*16), xd(ctx, n), yd(ctx, n);
+		v.Upload(values, m->nnzb*16);
+		xd.Upload(x, n);
+		if (precision == TFI_FP64) {
+			Spmv64(ctx, m->view(), v.data(), xd.data(), yd.data());
+		} else {
+			DevBuf<float> v32(ctx, m->nnzb*16);
+			ToFloat(ctx, v.data(), v32.data(), m->nnzb*16);
+			Spmv32(ctx, m->view(), v32.data(), xd.data(), yd.data());
+			Sync(ctx);
+		}
+		yd.Download(y, n);
+	});
+}
+
+int tfi_block_inverse(tfi_mesh* m, const double* values, double* inverse, int* singular)
+{
+	return Guard(m->ctx, [&] {
+		tfi_ctx* ctx = m->ctx;
+		DevBuf<double> v(ctx, m->nnzb*16), inv(ctx, 16ULL*m->nodes);
+		v.Upload(values, m->nnzb*16);
+		const int s = BlockInverse64(ctx, m->view(), v.data(), inv.data());
+		inv.Download(inverse, 16ULL*m->nodes);
+		if (singular) *singular = s;
+	});
+}
+
+int tfi_apply_dirichlet(tfi_mesh* m, const int32_t* vmask, const int32_t* pmask, const double* vtarget,
+	const double* ptarget, const double* state, double* jacobian, double* rhs)
+{
+	return Guard(m->ctx, [&] {
+		tfi_ctx* ctx = m->ctx;
+		const int N = m->nodes;
+		DevBuf<int> vm(ctx, N), pm(ctx, N);
+		DevBuf<double> vt(ctx, 3ULL*N), pt(ctx, N), s(ctx, 4ULL*N), r, j;
+		vm.Upload(vmask, N);
+		pm.Upload(pmask, N);
+		vt.Upload(vtarget, 3ULL*N);
+		pt.Upload(ptarget, N);
+		s.Upload(state, 4ULL*N);
+		if (jacobian) {
+			j.Allocate(ctx, m->nnzb*16);
+			j.Upload(jacobian, m->nnzb*16);
+		}
+		if (rhs) {
+			r.Allocate(ctx, 4ULL*N);
+			r.Upload(rhs, 4ULL*N);
+		}
+		ApplyDirichlet(m, vm.data(), pm.data(), vt.data(), pt.data(), s.data(), jacobian ? j.data() : nullptr,
+			rhs ? r.data() : nullptr);
+		if (jacobian) j.Download(jacobian, m->nnzb*16);
+		if (rhs) r.Download(rhs, 4ULL*N);
+	});
+}
+
+int tfi_linear_solve(tfi_mesh* m, const double* values, const double* b, double* x, int preconditioner,
+	int precision, int restart, int maxit, double rtol, int* iterations, double* residual)
+{
+	return Guard(m->ctx, [&] {
+		tfi_ctx* ctx = m->ctx;
+		const size_t n = 4ULL*m->nodes;
+		DevBuf<double> v(ctx, m->nnzb*16), bd(ctx, n), xd(ctx, n);
+		v.Upload(values, m->nnzb*16);
+		bd.Upload(b, n);
+		xd.Upload(x, n);
+		LinearSolver solver(m, preconditioner, precision, restart);
+		solver.Setup(v.data());
+		const auto r = solver.Solve(bd.data(), xd.data(), maxit, rtol);
+		xd.Download(x, n);
+		if (iterations) *iterations = r.iterations;
+		if (residual) *residual = r.residual;
+		if (!r.converged) throw Error("linear solve did not converge");
+	});
+}
+
+int tfi_bench_spmv(tfi_mesh* m, int precision, int repetitions, double* milliseconds)
+{
+	return Guard(m->ctx, [&] {
+		tfi_ctx* ctx = m->ctx;
+		const size_t n = 4ULL*m->nodes;
+		DevBuf<double> v(ctx, m->nnzb*16), xd(ctx, n), yd(ctx, n);
+		Fill(ctx, v.data(), 1e-3, m->nnzb*16);
+		Fill(ctx, xd.data(), 1.0, n);
+		DevBuf<float> v32;
+		if (precision != TFI_FP64) {
+			v32.Allocate(ctx, m->nnzb*16);
+			ToFloat(ctx, v.data(), v32.data(), m->nnzb*16);
+		}
+		auto run = [&] {
+			if (precision == TFI_FP64) Spmv64(ctx, m->view(), v.data(), xd.data(), yd.data());
+			else Spmv32(ctx, m->view(), v32.data(), xd.data(), yd.data());
+		};
+		for (int i = 0; i < 3; ++i) run();
+		Sync(ctx);
+		Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event");
+		for (int i = 0; i < repetitions; ++i) run();
+		Check(hipEventRecord(ctx->ev_end, ctx->stream), "event");
+		Check(hipEventSynchronize(ctx->ev_end), "event sync");
+		float ms = 0.0f;
+		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
+		*milliseconds = ms/std::max(1, repetitions);
+	});
+}
+
+}  // extern "C"

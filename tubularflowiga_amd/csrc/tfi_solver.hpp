@@ -1,0 +1,68 @@
+// Linear solver, algebraic multigrid and warm-start declarations.
+#pragma once
+
+#include "tfi_internal.hpp"
+
+namespace tfi {
+
+// Node-block smoothed-free aggregation AMG (tfi_amg.hip).
+class Amg {
+public:
+	Amg(tfi_mesh* mesh, int precision);
+	~Amg();
+	// Numeric setup for new fine values (symbolic hierarchy built on first call).
+	void Setup(const double* values64, const float* values32);
+	// z = M^-1 v (one V-cycle), FP64 vectors at the interface.
+	void Apply(const double* v, double* z);
+	int singular_blocks() const { return singular_; }
+	int levels() const;
+	struct Impl;
+
+private:
+	std::unique_ptr<Impl> impl_;
+	int singular_ = 0;
+};
+
+class LinearSolver {
+public:
+	LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart);
+	~LinearSolver();
+	void Setup(const double* values);
+	KrylovResult Solve(const double* b, double* x, int maxit, double rtol);
+	int singular_blocks() const { return singular_; }
+	int refinements() const { return refinements_; }
+
+private:
+	tfi_mesh* mesh_;
+	tfi_ctx* ctx_;
+	int pc_, precision_;
+	Fgmres krylov_;
+	const double* values_ = nullptr;
+	DevBuf<double> inverse_, work_, correction_, residual_;
+	DevBuf<float> values32_;
+	std::unique_ptr<Amg> amg_;
+	int singular_ = 0;
+	int refinements_ = 0;
+};
+
+// Initial-state predictor for a time step (tfi_warmstart.hip).
+class WarmStart {
+public:
+	explicit WarmStart(tfi_mesh* mesh);
+	~WarmStart();
+	void Configure(const tfi_warm_options& options);
+	void Reset();
+	// Record the converged state of a finished step (and train when learned).
+	void Commit(const double* state, const double* phase, int phase_count);
+	bool Ready(int mode) const;
+	// state <- predicted state of the next step (state holds u_n on entry).
+	void Predict(int mode, double* state, const double* phase, int phase_count);
+	void Accepted(bool used);
+	void Stats(int* snapshots, double* last_loss, int* accepted, int* rejected) const;
+	struct Impl;
+
+private:
+	std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace tfi

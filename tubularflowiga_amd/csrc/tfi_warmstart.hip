@@ -1,0 +1,110 @@
+// Initial-state prediction for a time step.  Modes:
+//   TFI_WARM_PREVIOUS    the step starts from u_n (the reference behaviour,
+//                        iga_cuda.cu:917-920);
+//   TFI_WARM_EXTRAPOLATE u_n + (u_n - u_{n-1});
+//   TFI_WARM_LEARNED     u_n + MLP(stencil features), see tfi_mlp.hip.
+// A prediction is only a guess: the Newton loop accepts it when it lowers the
+// nonlinear residual and always converges to the same tolerance.
+#include "tfi_solver.hpp"
+#include "tfi_mlp.hpp"
+
+namespace tfi {
+
+namespace {
+
+__global__ void ExtrapolateKernel(long long n, const double* __restrict__ un, const double* __restrict__ um,
+	double* __restrict__ out)
+{
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
+		out[i] = 2.0*un[i] - um[i];
+}
+
+}  // namespace
+
+struct WarmStart::Impl {
+	tfi_mesh* mesh;
+	tfi_ctx* ctx;
+	DevBuf<double> hist[3];   // u_{n}, u_{n-1}, u_{n-2} (ring)
+	int count = 0;
+	int head = 0;             // index of u_n
+	double phase_hist[3][8] = {};
+	int accepted = 0, rejected = 0;
+	std::unique_ptr<Mlp> mlp;
+	tfi_warm_options opt{128, 27, 40, 1e-3, 3};
+	const double* at(int back) const { return hist[(head - back + 3)%3].data(); }
+};
+
+WarmStart::WarmStart(tfi_mesh* mesh) : impl_(std::make_unique<Impl>())
+{
+	impl_->mesh = mesh;
+	impl_->ctx = mesh->ctx;
+	for (auto& h : impl_->hist) h.Allocate(mesh->ctx, 4LL*mesh->nodes);
+}
+
+WarmStart::~WarmStart() = default;
+
+void WarmStart::Configure(const tfi_warm_options& o)
+{
+	impl_->opt = o;
+	impl_->mlp.reset();
+}
+
+void WarmStart::Reset()
+{
+	impl_->count = 0;
+	impl_->head = 0;
+}
+
+void WarmStart::Commit(const double* state, const double* phase, int phase_count)
+{
+	Impl& I = *impl_;
+	const long long n = 4LL*I.mesh->nodes;
+	I.head = (I.head + 1)%3;
+	Copy(I.ctx, state, I.hist[I.head].data(), n);
+	for (int i = 0; i < 8; ++i) I.phase_hist[I.head][i] = i < phase_count ? phase[i] : 0.0;
+	++I.count;
+	// learned mode: a new (u_{n-2}, u_{n-1}) -> u_n snapshot trains the network
+	if (I.count >= 3) {
+		if (!I.mlp) I.mlp = std::make_unique<Mlp>(I.mesh, I.opt);
+		I.mlp->Train(I.at(0), I.at(1), I.at(2), I.phase_hist[I.head]);
+	}
+}
+
+bool WarmStart::Ready(int mode) const
+{
+	const Impl& I = *impl_;
+	if (mode == TFI_WARM_EXTRAPOLATE) return I.count >= 2;
+	if (mode == TFI_WARM_LEARNED) return I.count >= 2 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
+	return false;
+}
+
+void WarmStart::Predict(int mode, double* state, const double* phase, int phase_count)
+{
+	Impl& I = *impl_;
+	const long long n = 4LL*I.mesh->nodes;
+	if (mode == TFI_WARM_EXTRAPOLATE) {
+		hipLaunchKernelGGL(ExtrapolateKernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), state);
+		TFI_LAUNCH_CHECK("ExtrapolateKernel");
+	} else if (mode == TFI_WARM_LEARNED) {
+		double ph[8] = {};
+		for (int i = 0; i < 8 && i < phase_count; ++i) ph[i] = phase[i];
+		I.mlp->Predict(I.at(0), I.at(1), ph, state);
+	}
+}
+
+void WarmStart::Accepted(bool used)
+{
+	if (used) ++impl_->accepted;
+	else ++impl_->rejected;
+}
+
+void WarmStart::Stats(int* snapshots, double* last_loss, int* accepted, int* rejected) const
+{
+	const Impl& I = *impl_;
+	if (snapshots) *snapshots = I.mlp ? I.mlp->snapshots() : 0;
+	if (last_loss) *last_loss = I.mlp ? I.mlp->last_loss() : 0.0;
+	if (accepted) *accepted = I.accepted;
+	if (rejected) *rejected = I.rejected;
+}
+
+}  // namespace tfi

@@ -1,0 +1,258 @@
+"""Python front-end of the HIP hot path (thin wrappers over the C-ABI).
+
+Names follow the reference's types: ``DeviceMesh`` uploads a FlatMesh and owns
+the node-block pattern (DeviceMesh.hpp / BlockCsr.hpp); ``FlowSolver`` owns the
+device-resident Navier-Stokes state and runs the Newton loop of
+``iga_cuda navier-stokes`` (iga_cuda.cu:956-1046).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, WarmOptions, TFI_FP64, TFI_MIXED,
+                   TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_LEARNED, TFI_WARM_PREVIOUS)
+
+__all__ = ["Context", "DeviceMesh", "FlowSolver", "TFI_FP64", "TFI_MIXED", "TFI_PC_AMG", "TFI_PC_BJACOBI",
+           "TFI_WARM_PREVIOUS", "TFI_WARM_LEARNED", "TFI_WARM_EXTRAPOLATE"]
+
+
+class TfiError(RuntimeError):
+    pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load()
+        h = ctypes.c_void_p()
+        if self.lib.tfi_ctx_create(device, ctypes.byref(h)) != 0:
+            raise TfiError("tfi_ctx_create failed (no HIP device?)")
+        self.handle = h
+
+    def check(self, status: int) -> None:
+        if status != 0:
+            raise TfiError(self.lib.tfi_last_error(self.handle).decode())
+
+    def device_info(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        self.check(self.lib.tfi_device_info(self.handle, buf, 512))
+        return buf.value.decode()
+
+    def allocations(self):
+        peak, live = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tfi_allocation_stats(self.handle, ctypes.byref(peak), ctypes.byref(live)))
+        return peak.value, live.value
+
+    def timers(self, enable: bool | None = None, reset: bool = False) -> dict:
+        if enable is not None:
+            self.check(self.lib.tfi_timers_enable(self.handle, int(enable)))
+        if reset:
+            self.check(self.lib.tfi_timers_reset(self.handle))
+        t = KernelTimers()
+        self.check(self.lib.tfi_timers_get(self.handle, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in KernelTimers._fields_}
+
+    def close(self):
+        if self.handle:
+            self.lib.tfi_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- test hooks (learned warm start kernels)
+    def mlp_reference(self, x, target, weights, hidden):
+        x = np.ascontiguousarray(x, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        w = [np.ascontiguousarray(v, np.float32).ravel() for v in weights]
+        out = np.zeros((x.shape[0], 4), np.float32)
+        loss = ctypes.c_float()
+        grads = [np.zeros_like(v) for v in w]
+        self.check(self.lib.tfi_mlp_reference(self.handle, hidden, x.shape[0], x.ravel(), target.ravel(), *w,
+                                              out.ravel(), ctypes.byref(loss), *grads))
+        return out, loss.value, grads
+
+    def mlp_infer_bf16(self, x, weights, hidden):
+        x = np.ascontiguousarray(x, np.float32)
+        w = [np.ascontiguousarray(v, np.float32).ravel() for v in weights]
+        out = np.zeros((x.shape[0], 4), np.float32)
+        self.check(self.lib.tfi_mlp_infer_bf16(self.handle, hidden, x.shape[0], x.ravel(), *w, out.ravel()))
+        return out
+
+
+class DeviceMesh:
+    """Uploads a FlatMesh and builds pattern, colouring and geometry on the GPU."""
+
+    def __init__(self, ctx: Context, mesh):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.host = mesh
+        h = ctypes.c_void_p()
+        ctx.check(self.lib.tfi_mesh_create(
+            ctx.handle, int(mesh.nodes), int(mesh.elements),
+            np.ascontiguousarray(mesh.element_offsets, np.int32), np.ascontiguousarray(mesh.connectivity, np.int32),
+            np.ascontiguousarray(mesh.extraction_offsets, np.int32),
+            np.ascontiguousarray(mesh.extraction_columns, np.uint8),
+            np.ascontiguousarray(mesh.extraction_values, np.float64),
+            np.ascontiguousarray(mesh.bezier_points, np.float64), ctypes.byref(h)))
+        self.handle = h
+        s = MeshStats()
+        ctx.check(self.lib.tfi_mesh_stats_get(h, ctypes.byref(s)))
+        self.stats = {k: getattr(s, k) for k, _ in MeshStats._fields_}
+        self.nodes = s.nodes
+        self.nnzb = s.block_nonzeros
+
+    def pattern(self):
+        ro = np.zeros(self.nodes + 1, np.int32)
+        cols = np.zeros(self.nnzb, np.int32)
+        diag = np.zeros(self.nodes, np.int32)
+        self.ctx.check(self.lib.tfi_mesh_pattern(self.handle, ro, cols, diag))
+        return ro, cols, diag
+
+    def colors(self):
+        c = np.zeros(self.host.elements, np.int32)
+        self.ctx.check(self.lib.tfi_mesh_colors(self.handle, c))
+        return c
+
+    def assemble(self, state, previous, density, viscosity, dt, jacobian=True):
+        n4 = 4 * self.nodes
+        negres = np.zeros(n4)
+        jac = np.zeros(16 * self.nnzb) if jacobian else None
+        self.ctx.check(self.lib.tfi_ns_assemble(
+            self.handle, np.ascontiguousarray(state, np.float64), np.ascontiguousarray(previous, np.float64),
+            density, viscosity, dt, None if jac is None else jac.ctypes.data_as(ctypes.c_void_p), negres))
+        return (None if jac is None else jac.reshape(-1, 4, 4)), negres
+
+    def spmv(self, values, x, precision=TFI_FP64):
+        y = np.zeros(4 * self.nodes)
+        self.ctx.check(self.lib.tfi_bsr_spmv(self.handle, np.ascontiguousarray(values, np.float64).ravel(),
+                                             np.ascontiguousarray(x, np.float64), y, precision))
+        return y
+
+    def block_inverse(self, values):
+        inv = np.zeros(16 * self.nodes)
+        s = ctypes.c_int()
+        self.ctx.check(self.lib.tfi_block_inverse(self.handle, np.ascontiguousarray(values, np.float64).ravel(),
+                                                  inv, ctypes.byref(s)))
+        return inv.reshape(-1, 4, 4), s.value
+
+    def apply_dirichlet(self, vmask, pmask, vtarget, ptarget, state, jac=None, rhs=None):
+        jac = None if jac is None else np.ascontiguousarray(jac, np.float64)
+        rhs = None if rhs is None else np.ascontiguousarray(rhs, np.float64)
+        self.ctx.check(self.lib.tfi_apply_dirichlet(
+            self.handle, np.ascontiguousarray(vmask, np.int32), np.ascontiguousarray(pmask, np.int32),
+            np.ascontiguousarray(vtarget, np.float64).ravel(), np.ascontiguousarray(ptarget, np.float64),
+            np.ascontiguousarray(state, np.float64),
+            None if jac is None else jac.ctypes.data_as(ctypes.c_void_p),
+            None if rhs is None else rhs.ctypes.data_as(ctypes.c_void_p)))
+        return jac, rhs
+
+    def solve(self, values, b, x0=None, preconditioner=TFI_PC_AMG, precision=TFI_FP64, restart=200,
+              maxit=5000, rtol=1e-8):
+        x = np.zeros(4 * self.nodes) if x0 is None else np.array(x0, np.float64)
+        its, res = ctypes.c_int(), ctypes.c_double()
+        self.ctx.check(self.lib.tfi_linear_solve(
+            self.handle, np.ascontiguousarray(values, np.float64).ravel(), np.ascontiguousarray(b, np.float64), x,
+            preconditioner, precision, restart, maxit, rtol, ctypes.byref(its), ctypes.byref(res)))
+        return x, its.value, res.value
+
+    def bench_spmv(self, precision=TFI_FP64, repetitions=50):
+        ms = ctypes.c_double()
+        self.ctx.check(self.lib.tfi_bench_spmv(self.handle, precision, repetitions, ctypes.byref(ms)))
+        return ms.value
+
+    def spmv_bytes(self, value_bytes=8):
+        """Algorithmic bytes of one SpMV: nnzb (16 s_v + 4) + 4 (N+1) + 2 * 4N * 8 (x read once, y written)."""
+        return self.nnzb * (16 * value_bytes + 4) + 4 * (self.nodes + 1) + 2 * 4 * self.nodes * 8
+
+    def close(self):
+        if self.handle:
+            self.lib.tfi_mesh_destroy(self.handle)
+            self.handle = None
+
+
+class FlowSolver:
+    """Device-resident backward-Euler / steady Navier-Stokes solver."""
+
+    def __init__(self, dmesh: DeviceMesh, **options):
+        self.dmesh = dmesh
+        self.ctx = dmesh.ctx
+        self.lib = dmesh.lib
+        o = FlowOptions()
+        self.lib.tfi_flow_default_options(ctypes.byref(o))
+        for k, v in options.items():
+            if not hasattr(o, k):
+                raise KeyError(k)
+            setattr(o, k, v)
+        self.options = o
+        h = ctypes.c_void_p()
+        self.ctx.check(self.lib.tfi_flow_create(dmesh.handle, ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+        self.nodes = dmesh.nodes
+
+    def set_state(self, state, previous=None):
+        prev = None if previous is None else np.ascontiguousarray(previous, np.float64)
+        self.ctx.check(self.lib.tfi_flow_set_state(self.handle, np.ascontiguousarray(state, np.float64),
+                                                   None if prev is None else prev.ctypes.data_as(ctypes.c_void_p)))
+
+    def state(self):
+        out = np.zeros(4 * self.nodes)
+        self.ctx.check(self.lib.tfi_flow_get_state(self.handle, out))
+        return out
+
+    def set_boundary(self, vmask, pmask, vtarget, ptarget, traction=None):
+        tr = None if traction is None else np.ascontiguousarray(traction, np.float64)
+        self.ctx.check(self.lib.tfi_flow_set_boundary(
+            self.handle, np.ascontiguousarray(vmask, np.int32), np.ascontiguousarray(pmask, np.int32),
+            np.ascontiguousarray(vtarget, np.float64).ravel(), np.ascontiguousarray(ptarget, np.float64),
+            None if tr is None else tr.ctypes.data_as(ctypes.c_void_p)))
+
+    def set_boundary_faces(self, element, face, label):
+        element = np.ascontiguousarray(element, np.int32)
+        self.ctx.check(self.lib.tfi_flow_set_boundary_faces(self.handle, len(element), element,
+                                                            np.ascontiguousarray(face, np.int32),
+                                                            np.ascontiguousarray(label, np.int32)))
+
+    def begin_step(self, phase=None):
+        ph = None if phase is None else np.ascontiguousarray(phase, np.float64)
+        self.ctx.check(self.lib.tfi_flow_begin_step(self.handle,
+                                                    None if ph is None else ph.ctypes.data_as(ctypes.c_void_p),
+                                                    0 if ph is None else len(ph)))
+
+    def newton(self) -> dict:
+        r = NewtonReport()
+        status = self.lib.tfi_flow_newton(self.handle, ctypes.byref(r))
+        rep = {k: getattr(r, k) for k, _ in NewtonReport._fields_ if k != "record"}
+        rep["records"] = [{k: getattr(r.record[i], k) for k, _ in r.record[i]._fields_} for i in range(r.records)]
+        if status != 0:
+            raise TfiError(self.lib.tfi_last_error(self.ctx.handle).decode())
+        return rep
+
+    def commit_step(self):
+        self.ctx.check(self.lib.tfi_flow_commit_step(self.handle))
+
+    def boundary_flows(self, labels):
+        labels = np.ascontiguousarray(labels, np.int32)
+        out = np.zeros(len(labels))
+        self.ctx.check(self.lib.tfi_flow_boundary_flows(self.handle, len(labels), labels, out))
+        return out
+
+    def configure_warm_start(self, hidden=128, train_steps=40, learning_rate=1e-3, min_snapshots=3):
+        o = WarmOptions(hidden, 27, train_steps, learning_rate, min_snapshots)
+        self.ctx.check(self.lib.tfi_flow_warm_configure(self.handle, ctypes.byref(o)))
+
+    def warm_stats(self):
+        s, l, a, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.lib.tfi_flow_warm_stats(self.handle, ctypes.byref(s), ctypes.byref(l), ctypes.byref(a),
+                                                    ctypes.byref(r)))
+        return {"snapshots": s.value, "last_loss": l.value, "accepted": a.value, "rejected": r.value}
+
+    def close(self):
+        if self.handle:
+            self.lib.tfi_flow_destroy(self.handle)
+            self.handle = None
