@@ -1,0 +1,210 @@
+"""Benchmark of the body-fitted IGA Navier-Stokes time step on one MI355X.
+
+Workload: the symmetric Y-bifurcation (parent R = 2 mm, daughters of 1.6 mm
+equivalent radius at +-35 deg, 20 mm branches, quadratic C1 splines refined to
+~0.6M velocity+pressure DOFs), steady inflow Q = 5 ml/s, backward Euler with
+dt = 1e-3 s, rho = 1060 kg/m^3, mu = 3.5e-3 Pa s, every linear solve to a true
+relative residual of 1e-8.  Two methods run the same physical steps from the
+same initial state:
+  * plain: FP64 FGMRES + FP64 block AMG, Newton from the previous step;
+  * mixed + learned: FP32 SpMV/AMG inside FP64 FGMRES with iterative
+    refinement, Newton from the online-trained MLP prediction (residual-checked).
+`value` is the mixed+learned wall time per time step (lower is better); the
+plain FP64 time, the speedup and the SpMV roofline sit beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD public spec
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector / matrix, AMD public spec
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol):
+    from tubularflowiga_amd import boundary, device
+    vmask, pmask, vt, pt = bc
+    flow = device.FlowSolver(dm, density=rho, viscosity=mu, dt=dt, nonlinear_rtol=tol["rtol"],
+                             nonlinear_atol=tol["atol"], mass_rtol=tol["mass"], linear_rtol=1e-8,
+                             preconditioner=device.TFI_PC_AMG, precision=precision, warm_start=warm,
+                             restart=100, linear_maximum_iterations=4000)
+    if warm == device.TFI_WARM_LEARNED:
+        flow.configure_warm_start(hidden=128, train_steps=40, learning_rate=1e-3, min_snapshots=2)
+    flow.set_state(boundary.initial_state(vmask, pmask, vt, pt))
+    flow.set_boundary(vmask, pmask, vt, pt)
+    flow.set_boundary_faces(*faces)
+    ctx = dm.ctx
+    records = []
+    timed = []
+    for step in range(warmup + steps):
+        is_timed = step >= warmup
+        if is_timed and step == warmup:
+            ctx.timers(enable=True, reset=True)
+        t0 = time.perf_counter()
+        flow.begin_step()
+        rep = flow.newton()
+        flow.commit_step()
+        t1 = time.perf_counter()
+        rec = {"step": step + 1, "wall_s": t1 - t0, "newton": rep["newton_iterations"],
+               "linear": rep["linear_iterations"], "assembly_s": rep["assembly_s"], "linear_s": rep["linear_s"],
+               "warm_used": rep["warm_start_used"], "warm_ratio": rep["warm_start_initial_ratio"]}
+        records.append(rec)
+        log(f"  step {step + 1} {'timed' if is_timed else 'warmup'} wall={rec['wall_s']*1e3:.1f} ms "
+            f"newton={rec['newton']} linear={rec['linear']} asm={rec['assembly_s']*1e3:.1f} ms "
+            f"lin={rec['linear_s']*1e3:.1f} ms warm={rec['warm_used']} ratio={rec['warm_ratio']:.3g}")
+        if is_timed:
+            timed.append(rec)
+    timers = ctx.timers(enable=False)
+    state = flow.state()
+    warm_stats = flow.warm_stats()
+    flow.close()
+    return timed, records, timers, state, warm_stats
+
+
+def cpu_baseline(case, dm, rho, mu, dt, plain_timed, budget_s=20.0):
+    """CPU oracle (FP64 C++ restatement, OpenMP) on a bounded sample: element
+    assembly of a slice of the mesh and FP64 SpMVs on the full pattern, scaled
+    to one time step with the plain GPU run's Newton / Krylov counts and the
+    AMG V-cycle's SpMV-equivalent cost."""
+    from oracle import oracle as O
+    mesh = case.mesh
+    state = np.zeros(4 * mesh.nodes)
+    n_el = min(mesh.elements, 2000)
+    t_el = O.time_elements(mesh, state, rho, mu, dt, 0, n_el)
+    per_element = t_el / n_el
+    pattern = dm.pattern()
+    vals = np.full((len(pattern[1]), 4, 4), 1e-3)
+    x = np.ones(4 * mesh.nodes)
+    O.bsr_spmv(pattern, vals, x)
+    reps = 0
+    t0 = time.perf_counter()
+    while reps < 20 and time.perf_counter() - t0 < budget_s * 0.5:
+        O.bsr_spmv(pattern, vals, x)
+        reps += 1
+    t_spmv = (time.perf_counter() - t0) / max(reps, 1)
+    newton = np.mean([r["newton"] for r in plain_timed])
+    linear = np.mean([r["linear"] for r in plain_timed])
+    assemblies = newton + 1
+    spmv_equivalents_per_iteration = 1.0 + 5.5   # outer SpMV + V-cycle (GS pre/post, residual, coarse)
+    step_s = assemblies * per_element * mesh.elements + linear * spmv_equivalents_per_iteration * t_spmv
+    return {"value": step_s * 1e3, "unit": "ms/step", "cores": O.num_threads(), "kind": "port",
+            "sample": (f"FP64 C++ oracle: {n_el} element systems ({t_el:.2f} s) and {reps} full-pattern BSR SpMVs "
+                       f"({t_spmv*1e3:.1f} ms each); one step = {assemblies:.1f} assemblies x {mesh.elements} "
+                       f"elements + {linear:.0f} Krylov iterations x {spmv_equivalents_per_iteration} "
+                       f"SpMV-equivalents (counts from the plain GPU run)")}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--workload", default="ybif_c1")
+    ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
+              "the multi-GPU decomposition is design-only", file=sys.stderr)
+        return 2
+    from tubularflowiga_amd import boundary, casegen, device
+
+    t0 = time.perf_counter()
+    case = casegen.workload(a.workload)
+    rho, mu = casegen.flow_parameters(case)
+    log(f"case {a.workload}: elements={case.mesh.elements} nodes={case.mesh.nodes} dofs={4*case.mesh.nodes} "
+        f"generated in {time.perf_counter()-t0:.1f} s")
+    ctx = device.Context(0)
+    log(ctx.device_info())
+    t0 = time.perf_counter()
+    dm = device.DeviceMesh(ctx, case.mesh)
+    log(f"mesh upload + pattern + colouring + geometry: {time.perf_counter()-t0:.2f} s; {dm.stats}")
+    bc = boundary.resolve_flow_boundaries(case)
+    faces = boundary.boundary_faces(case.mesh)
+    tol = {"rtol": 1e-8, "atol": 1e-12, "mass": 1e-6}   # the reference's HPC reference run settings
+
+    log("plain FP64 (AMG, previous-step start)")
+    plain, plain_all, plain_timers, plain_state, _ = run_method(
+        dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol)
+    log("mixed precision + learned warm start")
+    fast, fast_all, fast_timers, fast_state, warm_stats = run_method(
+        dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_MIXED, device.TFI_WARM_LEARNED, tol)
+
+    ms_plain = 1e3 * float(np.mean([r["wall_s"] for r in plain]))
+    ms_fast = 1e3 * float(np.mean([r["wall_s"] for r in fast]))
+    ps, fs = plain_state.reshape(-1, 4), fast_state.reshape(-1, 4)
+    rel_v = float(np.linalg.norm(fs[:, :3] - ps[:, :3]) / np.linalg.norm(ps[:, :3]))
+    rel_p = float(np.linalg.norm(fs[:, 3] - ps[:, 3]) / max(np.linalg.norm(ps[:, 3]), 1e-300))
+
+    # SpMV roofline: FP64 SpMVs of the plain run's timed region (HIP events on the library stream)
+    spmv_ms = plain_timers["spmv_ms"] / max(plain_timers["spmv_calls"], 1)
+    spmv_bytes = plain_timers["spmv_bytes"] / max(plain_timers["spmv_calls"], 1)
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    asm_ms = plain_timers["assembly_ms"] / max(plain_timers["assembly_calls"], 1)
+    asm_flops = plain_timers["assembly_flops"] / max(plain_timers["assembly_calls"], 1)
+    asm_tflops = asm_flops / (asm_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": "NS time-step wall time @ rel. residual 1e-8: FP64 vs mixed/learned; SpMV GB/s",
+        "value": round(ms_fast, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_fast, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (plain) / f32+f64 refinement (mixed); mlp bf16 inference",
+        "data": "synthetic (generated Y-bifurcation case; no external data)",
+        "config": {"workload": f"{a.workload}: symmetric Y-bifurcation, quadratic IGA, {4*case.mesh.nodes} DOFs, "
+                               f"{case.mesh.elements} elements, steady Q=5 ml/s, backward Euler dt={a.dt}",
+                   "nodes": case.mesh.nodes, "elements": case.mesh.elements, "dofs": 4 * case.mesh.nodes,
+                   "block_nonzeros": int(dm.nnzb), "newton_rtol": tol["rtol"], "linear_rtol": 1e-8},
+        "fp64_ms_per_step": round(ms_plain, 3),
+        "speedup_vs_fp64": round(ms_plain / ms_fast, 3),
+        "plain_fp64": {"ms_per_step": round(ms_plain, 3),
+                       "assembly_ms_per_step": round(1e3 * float(np.mean([r["assembly_s"] for r in plain])), 3),
+                       "linear_ms_per_step": round(1e3 * float(np.mean([r["linear_s"] for r in plain])), 3),
+                       "newton_per_step": float(np.mean([r["newton"] for r in plain])),
+                       "krylov_per_step": float(np.mean([r["linear"] for r in plain]))},
+        "mixed_learned": {"ms_per_step": round(ms_fast, 3),
+                          "assembly_ms_per_step": round(1e3 * float(np.mean([r["assembly_s"] for r in fast])), 3),
+                          "linear_ms_per_step": round(1e3 * float(np.mean([r["linear_s"] for r in fast])), 3),
+                          "newton_per_step": float(np.mean([r["newton"] for r in fast])),
+                          "krylov_per_step": float(np.mean([r["linear"] for r in fast])),
+                          "learned_guess_accepted": int(sum(r["warm_used"] for r in fast)),
+                          "warm_start": warm_stats},
+        "rel_l2_mixed_vs_fp64": {"velocity": rel_v, "pressure": rel_p},
+        "spmv_gbs": round(spmv_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "BsrSpmvKernel<double> (FGMRES operator, FP64 values)",
+                     "avg_ms": round(spmv_ms, 4), "algorithmic_bytes": int(spmv_bytes)},
+        "assembly_roofline": {"bound": "fp64", "achieved": round(asm_tflops, 2), "peak": FP64_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": round(asm_tflops / FP64_PEAK_TFLOPS, 4),
+                              "avg_ms": round(asm_ms, 3), "algorithmic_flops": asm_flops},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, dm, rho, mu, a.dt, plain)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -33,6 +33,8 @@ enum { TFI_OK = 0, TFI_ERROR = 1 };
 enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
 enum { TFI_FP64 = 0, TFI_MIXED = 1 };
 enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2 };
+enum { TFI_SMOOTH_JACOBI = 0, TFI_SMOOTH_GAUSS_SEIDEL = 1 };
+enum { TFI_TANGENT_LEGACY = 0, TFI_TANGENT_EXACT = 1 };
 
 /* ------------------------------------------------------------------ context */
 /* Replaces the CUDA runtime setup + PrintDevice (iga_cuda.cu:278-303). */
@@ -68,12 +70,14 @@ int tfi_mesh_pattern(tfi_mesh* mesh, int32_t* row_offsets, int32_t* columns, int
 int tfi_mesh_colors(tfi_mesh* mesh, int32_t* element_color);
 
 /* ------------------------------------------------------------------ kernels (host in/out) */
-/* Negated residual and legacy tangent (AssembleNavierStokesKernel,
- * IgaCudaKernels.cuh:265-409) with a deterministic colour-ordered scatter.
+/* Negated residual and tangent with a deterministic colour-ordered scatter:
+ * tangent TFI_TANGENT_LEGACY = the GPU reference's tau-frozen tangent
+ * (AssembleNavierStokesKernel, IgaCudaKernels.cuh:265-409), TFI_TANGENT_EXACT =
+ * the CPU reference's consistent tangent (NavierStokesElement.hpp:244-313).
  * jacobian may be NULL (residual only).  Arrays: state/previous [4N],
  * jacobian [nnzb*16], negative_residual [4N]. */
 int tfi_ns_assemble(tfi_mesh* mesh, const double* state, const double* previous, double density,
-	double viscosity, double dt, double* jacobian, double* negative_residual);
+	double viscosity, double dt, double* jacobian, double* negative_residual, int tangent);
 /* y = A x on the mesh pattern (BlockSpmvKernel<4>, SparseKernels.cuh:25-40);
  * precision TFI_FP64 or TFI_MIXED (values rounded to fp32). */
 int tfi_bsr_spmv(tfi_mesh* mesh, const double* values, const double* x, double* y, int precision);
@@ -108,6 +112,8 @@ typedef struct {
 	int precision;                    /* TFI_FP64 | TFI_MIXED */
 	int warm_start;                   /* TFI_WARM_* */
 	int verbose;
+	int amg_smoother;                 /* TFI_SMOOTH_* */
+	int tangent;                      /* TFI_TANGENT_* */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);

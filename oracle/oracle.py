@@ -58,6 +58,8 @@ def lib():
         L.oracle_boundary_flow.argtypes = [c_int, _dp, _dp, _dp, c_int, ctypes.POINTER(c_dbl)]
         L.oracle_pressure_traction.argtypes = [c_int, _dp, _dp, c_int, c_dbl, _dp]
         L.oracle_num_threads.restype = c_int
+        L.oracle_time_elements.argtypes = [c_int, c_int, _ip, _ip, _ip, _u8p, _dp, _dp, _dp, c_dbl, c_dbl, c_dbl,
+                                           c_int, c_int, ctypes.POINTER(c_dbl)]
         _LIB = L
     return _LIB
 
@@ -267,3 +269,13 @@ def ns_newton_step(mesh, pattern, state, previous, rho, mu, dt, vmask, pmask, vt
         if log:
             log(f"oracle newton={it} residual={norm:.6e} linear_iterations={its}")
     raise RuntimeError("oracle Newton reached max_newton")
+
+
+def time_elements(mesh, state, rho, mu, dt, first, count):
+    """oracle_time_elements: wall seconds to build `count` element systems (CPU baseline sample)."""
+    sec = ctypes.c_double()
+    _check(lib().oracle_time_elements(mesh.nodes, mesh.elements, mesh.element_offsets, mesh.connectivity,
+                                      mesh.extraction_offsets, mesh.extraction_columns, mesh.extraction_values,
+                                      mesh.bezier_points, np.ascontiguousarray(state, dtype=np.float64),
+                                      rho, mu, dt, first, count, ctypes.byref(sec)))
+    return sec.value

@@ -633,6 +633,40 @@ int oracle_ns_assemble(int nodes, int elements, const int* eoff, const int* conn
 	});
 }
 
+// Element-level assembly timing sample: builds the element residual and
+// tangent of elements [first, first+count) (OpenMP over elements) without the
+// scatter, returning the wall time in seconds.  Used only as the CPU baseline.
+int oracle_time_elements(int nodes, int elements, const int* eoff, const int* conn, const int* xoff,
+	const uint8_t* xcol, const double* xval, const double* bezier, const double* state, double rho, double mu,
+	double dt, int first, int count, double* seconds)
+{
+	return Guard([&] {
+		const FlatMeshView m{nodes, elements, eoff, conn, xoff, xcol, xval, bezier};
+		std::vector<double> pts, w;
+		FullRule(pts, w);
+		const int last = std::min(elements, first + count);
+		std::string failure;
+		const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 4)
+		for (int e = first; e < last; ++e) {
+			try {
+				ElementData d = LoadElement(m, e);
+				std::vector<double> ls(4*d.nen);
+				for (int a = 0; a < d.nen; ++a)
+					for (int f = 0; f < 4; ++f) ls[4*a + f] = state[4*static_cast<size_t>(conn[eoff[e] + a]) + f];
+				std::vector<double> mat(static_cast<size_t>(16)*d.nen*d.nen), vec(4*d.nen);
+				NavierStokesElement(d, ls.data(), ls.data(), rho, mu, dt, kLegacyTangent, 64, pts.data(), w.data(),
+					mat.data(), vec.data());
+			} catch (const std::exception& ex) {
+#pragma omp critical
+				failure = ex.what();
+			}
+		}
+		*seconds = omp_get_wtime() - t0;
+		if (!failure.empty()) throw std::runtime_error(failure);
+	});
+}
+
 // Dirichlet rows and rhs (solvers/cuda/include/IgaCudaKernels.cuh:436-463).
 int oracle_apply_dirichlet(int nodes, const int* row_offsets, const int* diagonal, const int* vmask,
 	const int* pmask, const double* vtarget, const double* ptarget, const double* state, double* jac_values,

@@ -1,0 +1,114 @@
+"""Write tests/golden/ref_vectors.json from the reference's own PETSc-free headers.
+
+Runs oracle/_ref/ref_probe (our driver around solvers/cpu/include/{IgaDatabase,
+ElementGeometry,Quadrature,BoundaryFlow,BoundarySupport}.hpp, built by
+`make -C oracle ref`) on a few elements and on a generated case, and stores the
+outputs as fixtures:
+  * basis values and physical gradients (EvaluateBoundaryBasis) at points of a
+    unit, a curved, an affine-skewed and a degree-elevated quadratic element;
+  * face flows (IntegrateBoundaryFlow) and pressure-traction loads
+    (IntegrateBoundaryPressureTraction) with a seeded nodal state;
+  * surface Jacobian measures (SurfaceJacobianMeasure);
+  * the SHA-256 of the reference reader's text dump of a generated `.ntiga`
+    database (IgaDatabase.hpp) and its wall-trace basis (WallTraceBasis).
+
+Run in the build container only (the reference tree is not shipped):
+    python tests/golden/make_ref_vectors.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+PROBE = os.path.join(ROOT, "oracle", "_ref", "ref_probe")
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.json")
+
+
+def element_file(path, conn, extraction, bezier, state):
+    with open(path, "w") as f:
+        f.write(f"{len(conn)}\n")
+        f.write(" ".join(str(int(c)) for c in conn) + "\n")
+        for row in extraction:
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")
+        for p in bezier:
+            f.write(" ".join(repr(float(v)) for v in p) + "\n")
+        for s in state:
+            f.write(" ".join(repr(float(v)) for v in s) + "\n")
+
+
+def probe(*args):
+    out = subprocess.run([PROBE, *map(str, args)], check=True, capture_output=True, text=True).stdout
+    return [float(v) for v in out.split()]
+
+
+def elements():
+    from tubularflowiga_amd import casegen
+    grid = np.array([[i / 3.0, j / 3.0, k / 3.0] for k in range(4) for j in range(4) for i in range(4)])
+    eye = np.eye(64)
+    out = {"unit": (np.arange(64), eye, grid)}
+    curved = grid.copy()
+    curved[:, 0] += 0.08 * grid[:, 0] * grid[:, 1]
+    curved[:, 1] += 0.05 * grid[:, 1] * grid[:, 2]
+    curved[:, 2] += 0.04 * grid[:, 0] * grid[:, 2]
+    out["curved"] = (np.arange(64), eye, curved)
+    A = np.array([[2.0, 0.3, 0.2], [0.1, 3.0, 0.4], [0.2, 0.3, 4.0]])
+    out["skewed"] = (np.arange(64), eye, grid @ A.T)
+    case = casegen.workload("tube_c0")
+    m = case.mesh
+    for e in (0, 37):
+        out[f"tube_c0_e{e}"] = (m.element_connectivity(e), m.dense_extraction(e), m.element_bezier(e))
+    return out
+
+
+def main():
+    if not os.path.exists(PROBE):
+        raise SystemExit("build the probe first: make -C oracle ref")
+    from tubularflowiga_amd import boundary, casegen
+    from tubularflowiga_amd.mesh import write_ntiga
+    rng = np.random.default_rng(20261202)
+    points = [(0.2, 0.3, 0.4), (0.21, 0.37, 0.64), (0.9, 0.05, 0.5)]
+    result = {"points": points, "elements": {}}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, (conn, ext, bez) in elements().items():
+            state = rng.standard_normal((len(conn), 4))
+            path = os.path.join(tmp, name + ".txt")
+            element_file(path, conn, ext, bez, state)
+            rec = {"conn": [int(c) for c in conn], "bezier": np.asarray(bez).tolist(), "state": state.tolist(),
+                   "extraction_nonzeros": [[int(a), int(p), float(ext[a, p])] for a, p in zip(*np.nonzero(ext))],
+                   "basis": [], "flow": [], "traction": [], "surface": []}
+            for (u, v, w) in points:
+                vals = probe("basis", path, u, v, w)
+                rec["basis"].append({"raw_determinant": vals[0], "inverse": vals[1:10],
+                                     "value_gradient": vals[10:]})
+            for face in range(6):
+                rec["flow"].append(probe("flow", path, face)[0])
+                rec["traction"].append(probe("traction", path, face, 1.7))
+                rec["surface"].append(probe("surface", path, face))
+            result["elements"][name] = rec
+        # database round trip through the reference reader
+        case = casegen.workload("tube_c0")
+        db = os.path.join(tmp, "case.ntiga")
+        write_ntiga(db, case.mesh, version=5)
+        dump = subprocess.run([PROBE, "ntiga", db], check=True, capture_output=True, text=True).stdout
+        result["tube_c0_ntiga_dump_sha256"] = hashlib.sha256(dump.encode()).hexdigest()
+        result["tube_c0_ntiga_dump_head"] = dump.splitlines()[:2]
+        casegen.write_case(case, tmp)
+        trace = subprocess.run([PROBE, "walltrace", os.path.join(tmp, "case.ntiga"),
+                                os.path.join(tmp, "controlmesh.vtk")], check=True, capture_output=True,
+                               text=True).stdout
+        result["tube_c0_wall_trace"] = [int(v) for v in trace.split()]
+    with open(OUT, "w") as f:
+        json.dump(result, f)
+    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+
+
+if __name__ == "__main__":
+    main()

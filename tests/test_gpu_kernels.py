@@ -62,13 +62,14 @@ def test_geometry_valid(dmesh):
     assert dmesh.stats["minimum_determinant"] > 0.0
 
 
+@pytest.mark.parametrize("tangent", ["legacy", "exact"])
 @pytest.mark.parametrize("dt", [1e-3, 0.0])
-def test_assembly_matches_oracle(dmesh, case, dt):
+def test_assembly_matches_oracle(dmesh, case, dt, tangent):
     state = _state(case, 1)
     prev = _state(case, 2)
     pattern = O.block_pattern(case.mesh)
-    jac, negres = dmesh.assemble(state, prev, RHO, MU, dt)
-    ojac, onegres = O.ns_assemble(case.mesh, pattern, state, prev, RHO, MU, dt, "legacy")
+    jac, negres = dmesh.assemble(state, prev, RHO, MU, dt, tangent=tangent)
+    ojac, onegres = O.ns_assemble(case.mesh, pattern, state, prev, RHO, MU, dt, tangent)
     # tolerance: FP64 summation-order differences only
     assert np.max(np.abs(negres - onegres)) <= 1e-11 * np.max(np.abs(onegres))
     assert np.max(np.abs(jac - ojac)) <= 1e-11 * np.max(np.abs(ojac))
@@ -76,8 +77,8 @@ def test_assembly_matches_oracle(dmesh, case, dt):
 
 def test_assembly_deterministic(dmesh, case):
     state = _state(case, 3)
-    j1, r1 = dmesh.assemble(state, state, RHO, MU, 1e-3)
-    j2, r2 = dmesh.assemble(state, state, RHO, MU, 1e-3)
+    j1, r1 = dmesh.assemble(state, state, RHO, MU, 1e-3, tangent="exact")
+    j2, r2 = dmesh.assemble(state, state, RHO, MU, 1e-3, tangent="exact")
     assert np.array_equal(j1, j2) and np.array_equal(r1, r2)
 
 

@@ -17,6 +17,8 @@ LIB_PATH = PKG / "libtfi_hip.so"
 TFI_PC_BJACOBI, TFI_PC_AMG = 0, 1
 TFI_FP64, TFI_MIXED = 0, 1
 TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE = 0, 1, 2
+TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL = 0, 1
+TFI_TANGENT_LEGACY, TFI_TANGENT_EXACT = 0, 1
 MAX_NEWTON_RECORDS = 64
 
 c_int, c_double, c_ll, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_void_p
@@ -34,7 +36,7 @@ class FlowOptions(ctypes.Structure):
                 ("maximum_newton", c_int), ("nonlinear_rtol", c_double), ("nonlinear_atol", c_double),
                 ("mass_rtol", c_double), ("linear_maximum_iterations", c_int), ("linear_rtol", c_double),
                 ("restart", c_int), ("preconditioner", c_int), ("precision", c_int), ("warm_start", c_int),
-                ("verbose", c_int)]
+                ("verbose", c_int), ("amg_smoother", c_int), ("tangent", c_int)]
 
 
 class NewtonRecord(ctypes.Structure):
@@ -82,7 +84,7 @@ SIGNATURES = {
     "tfi_mesh_stats_get": (c_int, [c_void_p, P(MeshStats)]),
     "tfi_mesh_pattern": (c_int, [c_void_p, _i32, _i32, _i32]),
     "tfi_mesh_colors": (c_int, [c_void_p, _i32]),
-    "tfi_ns_assemble": (c_int, [c_void_p, _d, _d, c_double, c_double, c_double, c_void_p, _d]),
+    "tfi_ns_assemble": (c_int, [c_void_p, _d, _d, c_double, c_double, c_double, c_void_p, _d, c_int]),
     "tfi_bsr_spmv": (c_int, [c_void_p, _d, _d, _d, c_int]),
     "tfi_block_inverse": (c_int, [c_void_p, _d, _d, P(c_int)]),
     "tfi_apply_dirichlet": (c_int, [c_void_p, _i32, _i32, _d, _d, _d, c_void_p, c_void_p]),

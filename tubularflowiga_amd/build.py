@@ -47,7 +47,7 @@ def _run(cmd):
     return r.stdout
 
 
-def build_library(jobs: int | None = None, verbose: bool = False) -> Path:
+def build_library(jobs: int | None = None, verbose: bool = False, force: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     cc = hipcc()
     jobs = jobs or min(8, os.cpu_count() or 4)
@@ -57,7 +57,7 @@ def build_library(jobs: int | None = None, verbose: bool = False) -> Path:
     def compile_one(src: str) -> Path:
         s = CSRC / src
         o = BUILD / (s.stem + ".o")
-        if o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, newest_header):
+        if not force and o.exists() and o.stat().st_mtime > max(s.stat().st_mtime, newest_header):
             return o
         out = _run([cc, *HIPCC_FLAGS, "-c", str(s), "-o", str(o)])
         if verbose and out.strip():
@@ -89,8 +89,9 @@ def main(argv=None):
     ap.add_argument("--lib-only", action="store_true")
     ap.add_argument("-v", "--verbose", action="store_true")
     ap.add_argument("-j", "--jobs", type=int, default=None)
+    ap.add_argument("--force", action="store_true", help="recompile every translation unit")
     a = ap.parse_args(argv)
-    print("built", build_library(a.jobs, a.verbose))
+    print("built", build_library(a.jobs, a.verbose, a.force))
     if not a.lib_only:
         cli = build_cli()
         if cli:

@@ -173,47 +173,52 @@ def build_patches(patches, degree: int = 2, scale_mm: float = 1.0, merge_tol: fl
         ra, rb = find(a), find(b)
         if ra != rb:
             parent[max(ra, rb)] = min(ra, rb)
-    root = np.array([find(i) for i in range(len(pts))])
+    root = np.arange(len(pts))
+    for i in np.unique(pairs):
+        root[i] = find(i)
 
-    # element order: (branch, zeta layer, patch, eta, xi)
-    elems = []
-    for pi, P in enumerate(patches):
-        n1, n2, n3 = P.n
-        for k in range(n3):
-            for j in range(n2):
-                for i in range(n1):
-                    elems.append((P.branch, k, pi, j, i))
-    elems.sort()
-    ext = [tuple(extraction_1d(k, p) for k in P.n) for P in patches]
+    # per-patch element arrays, vectorised; element order (branch, zeta layer, patch, eta, xi)
     nen = (p + 1) ** 3
-    E = len(elems)
-    conn_raw = np.zeros((E, nen), np.int64)
-    rows = np.zeros((E, nen, 64))
-    labels = np.full((E, 6), -1, np.int32)
-    cells_raw = np.zeros((E, 8), np.int64)
+    lr = np.arange(p + 1)
+    keys, conns, rowss, labs, cells_l = [], [], [], [], []
 
     def corner_index(c, n):
-        return c if c < n else n + p - 1
-    local = [(r, s, t) for t in range(p + 1) for s in range(p + 1) for r in range(p + 1)]
-    for e, (_b, k, pi, j, i) in enumerate(elems):
-        P = patches[pi]
+        return np.where(c < n, c, n + p - 1)
+    corner_off = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+    for pi, P in enumerate(patches):
         n1, n2, n3 = P.n
         s1, s2, s3 = layouts[pi]
         base = patch_base[pi]
-        ex, ey, ez = ext[pi]
-        for a, (r, s, t) in enumerate(local):
-            conn_raw[e, a] = base + ((i + r) * s2 + (j + s)) * s3 + (k + t)
-            rows[e, a] = np.einsum("i,j,k->kji", ex[i, r], ey[j, s], ez[k, t]).reshape(64)
+        ex, ey, ez = (extraction_1d(k, p) for k in P.n)
+        I, J, K = np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij")
+        I, J, K = I.ravel(), J.ravel(), K.ravel()
+        # local basis a = r + (p+1) s + (p+1)^2 t
+        R = (I[:, None, None, None] + lr[None, None, None, :])
+        S = (J[:, None, None, None] + lr[None, None, :, None])
+        T = (K[:, None, None, None] + lr[None, :, None, None])
+        conns.append((base + (R*s2 + S)*s3 + T).reshape(-1, nen))
+        # extraction rows: ex[i,r,a] ey[j,s,b] ez[k,t,c] -> [(t,s,r), (c,b,a)]
+        rows = np.einsum("ira,jsb,ktc->ijktsrcba", ex, ey, ez, optimize=True).reshape(n1*n2*n3, nen, 64)
+        rowss.append(rows)
+        lab = np.full((len(I), 6), -1, np.int32)
         for f, (axis, side) in _FACE_AXIS.items():
             if f in P.face_labels:
-                idx = (i, j, k)[axis]
-                nn = P.n[axis]
-                if (side == 0 and idx == 0) or (side == 1 and idx == nn - 1):
-                    labels[e, f] = P.face_labels[f]
-        for c, (a_, b_, c_) in enumerate([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
-                                          (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]):
-            ii, jj, kk = corner_index(i + a_, n1), corner_index(j + b_, n2), corner_index(k + c_, n3)
-            cells_raw[e, c] = base + (ii * s2 + jj) * s3 + kk
+                idx = (I, J, K)[axis]
+                hit = (idx == 0) if side == 0 else (idx == P.n[axis] - 1)
+                lab[hit, f] = P.face_labels[f]
+        labs.append(lab)
+        ci = corner_index(I[:, None] + corner_off[None, :, 0], n1)
+        cj = corner_index(J[:, None] + corner_off[None, :, 1], n2)
+        ck = corner_index(K[:, None] + corner_off[None, :, 2], n3)
+        cells_l.append(base + (ci*s2 + cj)*s3 + ck)
+        keys.append(np.stack([np.full(len(I), P.branch), K, np.full(len(I), pi), J, I], axis=1))
+    keys = np.concatenate(keys)
+    order_e = np.lexsort(keys.T[::-1])
+    conn_raw = np.concatenate(conns)[order_e]
+    rows = np.concatenate(rowss)[order_e]
+    labels = np.concatenate(labs)[order_e]
+    cells_raw = np.concatenate(cells_l)[order_e]
+    E = len(order_e)
     # node numbering by first appearance in element order (banded matrices)
     roots_in_order = root[conn_raw.ravel()]
     _, first = np.unique(roots_in_order, return_index=True)
@@ -227,7 +232,8 @@ def build_patches(patches, degree: int = 2, scale_mm: float = 1.0, merge_tol: fl
     cps = np.zeros((N, 3))
     cps[node_of] = pts
     conn = node_of[conn_raw].astype(np.int32)
-    if any(len(set(c)) != nen for c in conn):
+    sc = np.sort(conn, axis=1)
+    if np.any(sc[:, 1:] == sc[:, :-1]):
         raise RuntimeError("element connectivity repeats a node (degenerate patch)")
     bez = np.einsum("eap,eax->epx", rows, cps[conn])
     # sparse extraction
@@ -533,7 +539,7 @@ WORKLOADS = {
     "tube_c0": dict(kind="tube", radius_mm=1.0, length_mm=10.0, n_core=1, m_ring=1, n_axial=16, u_max=0.1),
     # symmetric Y, parent 2 mm, daughters 1.6 mm at +-35 deg, 20 mm branches, Q = 5 ml/s, ~0.6M DOFs
     "ybif_c1": dict(kind="ybif", r_parent_mm=2.0, r_daughter_mm=1.6, angle_deg=35.0, l_parent_mm=20.0,
-                    l_daughter_mm=20.0, n_core=12, m_ring=8, n_parent=130, n_daughter=130, flow_ml_s=5.0),
+                    l_daughter_mm=20.0, n_core=12, m_ring=8, n_parent=104, n_daughter=104, flow_ml_s=5.0),
     # small Y for CPU tests
     "ybif_small": dict(kind="ybif", r_parent_mm=2.0, r_daughter_mm=1.6, angle_deg=35.0, l_parent_mm=20.0,
                        l_daughter_mm=20.0, n_core=2, m_ring=1, n_parent=8, n_daughter=8, flow_ml_s=5.0),

@@ -7,15 +7,17 @@
 // (solvers/cuda/include/AggregationAmg.hpp).  This is a coupled (all four
 // fields per node) plain-aggregation hierarchy:
 //   * symbolic phase (host, once per mesh, deterministic): strength graph from
-//     the first matrix (block Frobenius norms), greedy aggregation in node
+//     the first matrix (field-diagonal couplings), greedy aggregation in node
 //     order, coarse block pattern, and for every coarse block the ascending
 //     list of fine blocks that sum into it;
 //   * numeric phase (GPU, every Newton step): Galerkin coarse blocks as
 //     fixed-order segmented sums (no atomics), 4x4 diagonal inverses, and a
 //     dense Gauss-Jordan inverse of the coarsest operator;
-//   * V-cycle: damped block-Jacobi pre/post smoothing (fused residual+smooth
-//     kernel, one wave per block row), restriction = aggregate sums,
-//     prolongation = injection; optionally entirely in FP32.
+//   * V-cycle: symmetric multicolour block Gauss-Seidel (greedy node colouring
+//     per level, one launch per colour, in place -- rows of one colour are
+//     uncoupled) or damped block Jacobi with omega = 1.33 / lambda_max(D^-1 A)
+//     (power-iteration estimate); restriction = aggregate sums, prolongation =
+//     injection; optionally entirely in FP32.
 // The prolongator is the 0/1 aggregate map, stored as integers, so it is exact
 // in any precision.
 #include "tfi_solver.hpp"
@@ -31,8 +33,9 @@ namespace {
 constexpr double kStrength = 0.25;     // row-relative strength threshold
 constexpr int kCoarseMax = 192;        // nodes on the coarsest (direct) level
 constexpr int kMaxLevels = 12;
-constexpr double kOmega = 0.65;        // block-Jacobi damping
-constexpr int kPre = 2, kPost = 2;
+constexpr double kOmegaScale = 1.33;   // block-Jacobi damping = kOmegaScale / lambda_max(D^-1 A)
+constexpr int kPre = 2, kPost = 2;      // Jacobi sweeps
+constexpr int kPowerIterations = 12;
 
 template <class T>
 __device__ inline T WaveSumT(T v)
@@ -93,6 +96,74 @@ __global__ void __launch_bounds__(256) ResidualKernel(int rows, const int* __res
 	acc += __shfl_xor(acc, 16, 64);
 	acc += __shfl_xor(acc, 32, 64);
 	if (sub == 0) r[4*(size_t)row + rr] = b[4*(size_t)row + rr] - acc;
+}
+
+// y = D^-1 (A x)  (power iteration for lambda_max of the Jacobi-preconditioned operator)
+__global__ void __launch_bounds__(256) DinvAKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
+	const double* __restrict__ vals, const double* __restrict__ inv, const double* __restrict__ x,
+	double* __restrict__ y)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 2, r = lane & 3;
+	double acc = 0.0;
+	for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
+		const double* a = vals + (size_t)k*16 + 4*r;
+		const double* xv = x + 4*(size_t)cols[k];
+		acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
+	}
+	acc += __shfl_xor(acc, 4, 64);
+	acc += __shfl_xor(acc, 8, 64);
+	acc += __shfl_xor(acc, 16, 64);
+	acc += __shfl_xor(acc, 32, 64);
+	const double a0 = __shfl(acc, 0, 64), a1 = __shfl(acc, 1, 64), a2 = __shfl(acc, 2, 64), a3 = __shfl(acc, 3, 64);
+	if (sub == 0) {
+		const double* d = inv + (size_t)row*16 + 4*r;
+		y[4*(size_t)row + r] = d[0]*a0 + d[1]*a1 + d[2]*a2 + d[3]*a3;
+	}
+}
+
+// Deterministic pseudo-random start vector in [-1, 1).
+__global__ void HashVectorKernel(long long n, double* __restrict__ v)
+{
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x) {
+		unsigned long long z = (unsigned long long)i*0x9e3779b97f4a7c15ULL + 0x632be59bd9b4e019ULL;
+		z = (z ^ (z >> 30))*0xbf58476d1ce4e5b9ULL;
+		z = (z ^ (z >> 27))*0x94d049bb133111ebULL;
+		z ^= z >> 31;
+		v[i] = (double)(z >> 11)*(2.0/9007199254740992.0) - 1.0;
+	}
+}
+
+// One colour of a block Gauss-Seidel sweep, in place: for the rows of this
+// colour (no two of them are coupled) x_i += D_i^-1 (b_i - sum_j A_ij x_j).
+template <class T>
+__global__ void __launch_bounds__(256) GsColorKernel(int count, const int* __restrict__ list, const int* __restrict__ ro,
+	const int* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ inv, const T* __restrict__ b,
+	T* __restrict__ x)
+{
+	const int lane = threadIdx.x & 63;
+	const int item = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (item >= count) return;
+	const int row = list[item];
+	const int sub = lane >> 2, r = lane & 3;
+	T acc = 0;
+	for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
+		const T* a = vals + (size_t)k*16 + 4*r;
+		const T* xv = x + 4*(size_t)cols[k];
+		acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
+	}
+	acc += __shfl_xor(acc, 4, 64);
+	acc += __shfl_xor(acc, 8, 64);
+	acc += __shfl_xor(acc, 16, 64);
+	acc += __shfl_xor(acc, 32, 64);
+	const T res = b[4*(size_t)row + r] - acc;
+	const T r0 = __shfl(res, 0, 64), r1 = __shfl(res, 1, 64), r2 = __shfl(res, 2, 64), r3 = __shfl(res, 3, 64);
+	if (sub == 0) {
+		const T* d = inv + (size_t)row*16 + 4*r;
+		x[4*(size_t)row + r] += d[0]*r0 + d[1]*r1 + d[2]*r2 + d[3]*r3;
+	}
 }
 
 // bc[I] = sum_{i in aggregate I} r[i] (ascending fine order)
@@ -271,6 +342,10 @@ struct Level {
 	DevBuf<int> agg, aoff, amem;
 	DevBuf<long long> goff;       // Galerkin lists for the NEXT level's blocks
 	DevBuf<int> glist;
+	// multicolour Gauss-Seidel: rows grouped by colour
+	std::vector<int> color_offsets;
+	DevBuf<int> color_rows;
+	double lambda_max = 1.0;
 	// vectors
 	DevBuf<double> x64, y64, b64, r64;
 	DevBuf<float> x32, y32, b32, r32;
@@ -281,6 +356,7 @@ struct Amg::Impl {
 	tfi_mesh* mesh;
 	tfi_ctx* ctx;
 	int precision;
+	int smoother = 1;  // 0 damped block Jacobi, 1 multicolour symmetric block Gauss-Seidel
 	bool built = false;
 	std::vector<std::unique_ptr<Level>> L;
 	int nc = 0;  // coarsest dense size (4 * rows)
@@ -492,9 +568,32 @@ void Amg::Impl::Symbolic(const double* fine_values)
 		c->vals.Download(hv.data(), hv.size());
 		L.push_back(std::move(c));
 	}
-	// per-level buffers
+	// per-level buffers; greedy node colouring of every smoothed level
 	for (size_t l = 0; l < L.size(); ++l) {
 		Level& v = *L[l];
+		if (l + 1 < L.size()) {
+			std::vector<int> col(v.rows, -1), mark;
+			int ncol = 0;
+			for (int i = 0; i < v.rows; ++i) {
+				if ((int)mark.size() < ncol + 1) mark.resize(ncol + 1, -1);
+				for (int k = v.h_ro[i]; k < v.h_ro[i + 1]; ++k) {
+					const int c = col[v.h_cols[k]];
+					if (c >= 0) mark[c] = i;
+				}
+				int c = 0;
+				while (c < ncol && mark[c] == i) ++c;
+				if (c == ncol) ++ncol;
+				col[i] = c;
+			}
+			v.color_offsets.assign(ncol + 1, 0);
+			for (int i = 0; i < v.rows; ++i) ++v.color_offsets[col[i] + 1];
+			for (int c = 0; c < ncol; ++c) v.color_offsets[c + 1] += v.color_offsets[c];
+			std::vector<int> rows(v.rows);
+			std::vector<int> fill(v.color_offsets.begin(), v.color_offsets.end() - 1);
+			for (int i = 0; i < v.rows; ++i) rows[fill[col[i]]++] = i;
+			v.color_rows.Allocate(ctx, v.rows);
+			v.color_rows.Upload(rows.data(), v.rows);
+		}
 		const long long n = 4LL*v.rows;
 		v.inv64.Allocate(ctx, 16LL*v.rows);
 		if (precision == TFI_MIXED) {
@@ -547,6 +646,25 @@ void Amg::Impl::Numeric(const double* fine64, const float* fine32)
 			}
 		}
 	}
+	// lambda_max(D^-1 A) per smoothed level by power iteration (FP64)
+	if (smoother == 0)
+		for (size_t l = 0; l + 1 < L.size(); ++l) {
+			Level& v = *L[l];
+			const long long n = 4LL*v.rows;
+			DevBuf<double> a(ctx, n), b(ctx, n);
+			hipLaunchKernelGGL(HashVectorKernel, dim3(Grid(n)), dim3(256), 0, ctx->stream, n, a.data());
+			double lam = 1.0;
+			for (int it = 0; it < kPowerIterations; ++it) {
+				const double nrm = Norm2(ctx, a.data(), n);
+				Scale(ctx, 1.0/nrm, a.data(), n);
+				hipLaunchKernelGGL(DinvAKernel, dim3((v.rows + 3)/4), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols,
+					v.vals64, v.inv64.data(), a.data(), b.data());
+				TFI_LAUNCH_CHECK("DinvAKernel");
+				lam = Norm2(ctx, b.data(), n);
+				std::swap(a, b);
+			}
+			v.lambda_max = lam > 0.0 ? lam : 1.0;
+		}
 	// coarsest dense inverse
 	Level& c = *L.back();
 	Fill(ctx, aug.data(), 0.0, (long long)nc*2*nc);
@@ -597,14 +715,34 @@ void Amg::Impl::Cycle(int l)
 		TFI_LAUNCH_CHECK("DenseApplyKernel");
 		return;
 	}
-	const T omega = static_cast<T>(kOmega);
-	// pre-smoothing from x = 0
-	hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
-		static_cast<const T*>(nullptr), x, omega);
-	for (int s = 1; s < kPre; ++s) {
+	const T omega = static_cast<T>(kOmegaScale/v.lambda_max);
+	const int ncol = static_cast<int>(v.color_offsets.size()) - 1;
+	auto gs_sweep = [&](bool forward) {
+		for (int cc = 0; cc < ncol; ++cc) {
+			const int c = forward ? cc : ncol - 1 - cc;
+			const int count = v.color_offsets[c + 1] - v.color_offsets[c];
+			if (count == 0) continue;
+			hipLaunchKernelGGL(GsColorKernel<T>, dim3((count + 3)/4), dim3(256), 0, ctx->stream, count,
+				v.color_rows.data() + v.color_offsets[c], v.ro, v.cols, vals, inv, b, x);
+		}
+		TFI_LAUNCH_CHECK("GsColorKernel");
+	};
+	if (smoother == 1) {
+		// symmetric multicolour Gauss-Seidel from x = 0
+		const long long n = 4LL*v.rows;
+		if (f32) Check(hipMemsetAsync(x, 0, n*sizeof(T), ctx->stream), "memset");
+		else Fill(ctx, reinterpret_cast<double*>(x), 0.0, n);
+		gs_sweep(true);
+		gs_sweep(false);
+	} else {
+		// damped block Jacobi from x = 0
 		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
-			x, y, omega);
-		std::swap(x, y);
+			static_cast<const T*>(nullptr), x, omega);
+		for (int s = 1; s < kPre; ++s) {
+			hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv,
+				b, x, y, omega);
+			std::swap(x, y);
+		}
 	}
 	hipLaunchKernelGGL(ResidualKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, b, x, r);
 	TFI_LAUNCH_CHECK("ResidualKernel");
@@ -620,10 +758,15 @@ void Amg::Impl::Cycle(int l)
 	hipLaunchKernelGGL(ProlongKernel<T>, dim3(Grid(4LL*v.rows)), dim3(256), 0, ctx->stream, v.rows, v.agg.data(), xc,
 		x);
 	TFI_LAUNCH_CHECK("ProlongKernel");
-	for (int s = 0; s < kPost; ++s) {
-		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
-			x, y, omega);
-		std::swap(x, y);
+	if (smoother == 1) {
+		gs_sweep(true);
+		gs_sweep(false);
+	} else {
+		for (int s = 0; s < kPost; ++s) {
+			hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv,
+				b, x, y, omega);
+			std::swap(x, y);
+		}
 	}
 	TFI_LAUNCH_CHECK("SmoothKernel");
 	// leave the result in the level's x buffer
@@ -638,11 +781,12 @@ void Amg::Impl::Cycle(int l)
 	}
 }
 
-Amg::Amg(tfi_mesh* mesh, int precision) : impl_(std::make_unique<Impl>())
+Amg::Amg(tfi_mesh* mesh, int precision, int smoother) : impl_(std::make_unique<Impl>())
 {
 	impl_->mesh = mesh;
 	impl_->ctx = mesh->ctx;
 	impl_->precision = precision;
+	impl_->smoother = smoother;
 }
 
 Amg::~Amg() = default;

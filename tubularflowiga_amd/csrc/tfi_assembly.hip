@@ -1,9 +1,10 @@
-// Element assembly of the stabilised (VMS) Navier-Stokes residual and the
-// legacy (tau-frozen) tangent, with a deterministic scatter.
+// Element assembly of the stabilised (VMS) Navier-Stokes residual and tangent
+// (legacy tau-frozen or exact consistent), with a deterministic scatter.
 //
 // Arithmetic restated from solvers/cuda/include/IgaCudaKernels.cuh:239-409
-// (StabilizationDevice + AssembleNavierStokesKernel) and the CPU element
-// solvers/cpu/include/NavierStokesElement.hpp:121-241.  The work is organised
+// (StabilizationDevice + AssembleNavierStokesKernel, legacy tangent) and the
+// CPU element solvers/cpu/include/NavierStokesElement.hpp:121-313 (residual and
+// exact tangent, which differentiates tau_M, tau_C and the strong viscous term).  The work is organised
 // for CDNA4 instead of the reference's thread-per-pair tiles with a serial
 // thread-0 section:
 //   * one workgroup (4 waves) per (element, tile of test rows); the element's
@@ -30,8 +31,8 @@ namespace {
 
 constexpr int kSlab = 32;
 constexpr int kThreads = 256;
-constexpr int kQd = 24;       // per-point record in LDS
-constexpr int kBasisRec = 5;  // N, dN/dx, dN/dy, dN/dz, (lap N | u.grad N)
+constexpr int kQd = 56;       // per-point record in LDS
+constexpr int kBasisRec = 7;  // N, dN/dx, dN/dy, dN/dz, lap N, u.grad N, w.grad N (w = u + u')
 
 __constant__ double kGp[4] = {0.06943184420297371, 0.33000947820757187, 0.6699905217924281, 0.9305681557970262};
 
@@ -64,6 +65,7 @@ struct AsmArgs {
 	int with_jacobian;
 };
 
+template <bool kExact>
 __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 {
 	extern __shared__ double lds[];
@@ -225,6 +227,28 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				}
 				r[18] = div;
 				for (int c = 0; c < 3; ++c) r[19 + c] = fu[c];
+				for (int c = 0; c < 3; ++c) r[22 + c] = w[c];
+				for (int c = 0; c < 3; ++c)
+					for (int d = 0; d < 3; ++d) r[25 + 3*c + d] = grad[c][d];
+				if (kExact) {
+					// d(u')/d(u_f) = N_b K[.][f] - delta tau_M s_b with
+					// K_cf = tau_M^3 strong_c (G u)_f - tau_M du_c/dx_f, and
+					// R = grad u + grad u K, c1_f = rho tau_C tau_M^2 div (G u)_f.
+					double Gu[3];
+					for (int i = 0; i < 3; ++i) Gu[i] = G[i][0]*u[0] + G[i][1]*u[1] + G[i][2]*u[2];
+					const double tm3 = tau_m*tau_m*tau_m;
+					double K[3][3];
+					for (int c = 0; c < 3; ++c) {
+						const double strong = -fu[c]/tau_m;
+						for (int f = 0; f < 3; ++f) K[c][f] = tm3*strong*Gu[f] - tau_m*grad[c][f];
+					}
+					for (int c = 0; c < 3; ++c)
+						for (int f = 0; f < 3; ++f) {
+							r[34 + 3*c + f] = K[c][f];
+							r[43 + 3*c + f] = grad[c][f] + grad[c][0]*K[0][f] + grad[c][1]*K[1][f] + grad[c][2]*K[2][f];
+						}
+					for (int f = 0; f < 3; ++f) r[52 + f] = rho*tau_c*tau_m*tau_m*div*Gu[f];
+				}
 			}
 		}
 		__syncthreads();
@@ -237,7 +261,8 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 			if (valid) {
 				double* bb = basis + ((size_t)a*kSlab + ql)*kBasisRec;
 				const double N = bb[0], gx = bb[1], gy = bb[2], gz = bb[3];
-				bb[4] = r[3]*gx + r[4]*gy + r[5]*gz;
+				bb[5] = r[3]*gx + r[4]*gy + r[5]*gz;
+				bb[6] = r[22]*gx + r[23]*gy + r[24]*gz;
 				if (a >= a0 && a < a1) {
 					const double m = r[0];
 					for (int c = 0; c < 3; ++c)
@@ -254,7 +279,64 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 		}
 		__syncthreads();
 		// ---- P4: tangent on 2x2 (test, trial) tiles
-		if (pair_active) {
+		if (kExact && pair_active) {
+			// consistent tangent, NavierStokesElement.hpp:244-313, in outer-product form
+			const double nu_ = nu;
+			for (int ql = 0; ql < kSlab; ++ql) {
+				const double* r = qd + ql*kQd;
+				const double m = r[0], tm = r[1], tc = r[2];
+				const double* up = r + 19;
+				const double* gu = r + 25;
+				const double* K = r + 34;
+				const double* R = r + 43;
+				const double* c1 = r + 52;
+				double mN[2], mG[2][3], mA[2], mP[2][3], Z[2][9];
+#pragma unroll
+				for (int i = 0; i < 2; ++i) {
+					const double* bb = basis + ((size_t)ra[i]*kSlab + ql)*kBasisRec;
+					mN[i] = m*bb[0];
+					for (int d = 0; d < 3; ++d) mG[i][d] = m*bb[1 + d];
+					mA[i] = m*bb[6];
+					for (int f = 0; f < 3; ++f) mP[i][f] = K[f]*mG[i][0] + K[3 + f]*mG[i][1] + K[6 + f]*mG[i][2];
+					for (int x = 0; x < 3; ++x)
+						for (int f = 0; f < 3; ++f)
+							Z[i][3*x + f] = rho*(mN[i]*R[3*x + f] - mA[i]*K[3*x + f] - up[x]*(mG[i][f] + mP[i][f]));
+				}
+				double Nb[2], B[2][3], sb[2], wb[2], Qb[2][3], Xb[2][3], Yb[2][3];
+#pragma unroll
+				for (int j = 0; j < 2; ++j) {
+					const double* bb = basis + ((size_t)rb[j]*kSlab + ql)*kBasisRec;
+					Nb[j] = bb[0];
+					B[j][0] = bb[1]; B[j][1] = bb[2]; B[j][2] = bb[3];
+					sb[j] = Nb[j]*idt + bb[5] - nu_*bb[4];
+					wb[j] = bb[6];
+					for (int x = 0; x < 3; ++x) {
+						Qb[j][x] = gu[3*x]*B[j][0] + gu[3*x + 1]*B[j][1] + gu[3*x + 2]*B[j][2];
+						Xb[j][x] = mu*B[j][x] + rho*tm*up[x]*sb[j];
+						Yb[j][x] = rho*tc*B[j][x] + c1[x]*Nb[j];
+					}
+				}
+#pragma unroll
+				for (int i = 0; i < 2; ++i)
+#pragma unroll
+					for (int j = 0; j < 2; ++j) {
+						const int pr = 2*i + j;
+						const double gd = mG[i][0]*B[j][0] + mG[i][1]*B[j][1] + mG[i][2]*B[j][2];
+						dg[pr] += rho*mN[i]*(Nb[j]*idt + wb[j]) + mu*gd + rho*tm*mA[i]*sb[j];
+						k33[pr] += (tm/rho)*gd;
+						const double c4 = -rho*tm*mN[i]*sb[j];
+#pragma unroll
+						for (int x = 0; x < 3; ++x) {
+#pragma unroll
+							for (int y = 0; y < 3; ++y)
+								uu[pr][3*x + y] += mG[i][y]*Xb[j][x] + mG[i][x]*Yb[j][y] + Nb[j]*Z[i][3*x + y] + c4*gu[3*x + y];
+							i3[pr][x] += -mG[i][x]*Nb[j] - tm*mN[i]*Qb[j][x] + tm*mA[i]*B[j][x] + tm*up[x]*gd;
+							t3[pr][x] += mN[i]*B[j][x] - Nb[j]*mP[i][x] + tm*sb[j]*mG[i][x];
+						}
+					}
+			}
+		}
+		if (!kExact && pair_active) {
 			for (int ql = 0; ql < kSlab; ++ql) {
 				const double* r = qd + ql*kQd;
 				const double m = r[0], tm = r[1], tc = r[2];
@@ -263,7 +345,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 #pragma unroll
 				for (int i = 0; i < 2; ++i) {
 					const double* bb = basis + ((size_t)ra[i]*kSlab + ql)*kBasisRec;
-					const double N = bb[0], gu = bb[4];
+					const double N = bb[0], gu = bb[5];
 					for (int d = 0; d < 3; ++d) {
 						mA[i][d] = m*bb[1 + d];
 						P[i][d] = mu*mA[i][d];
@@ -280,7 +362,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 					const double* bb = basis + ((size_t)rb[j]*kSlab + ql)*kBasisRec;
 					Nb[j] = bb[0];
 					B[j][0] = bb[1]; B[j][1] = bb[2]; B[j][2] = bb[3];
-					sb[j] = Nb[j]*idt + bb[4];
+					sb[j] = Nb[j]*idt + bb[5];
 				}
 #pragma unroll
 				for (int i = 0; i < 2; ++i)
@@ -454,6 +536,9 @@ __global__ void FaceFlowKernel(int count, const int* __restrict__ felem, const i
 void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const NsParams& p, double* jacobian,
 	double* negative_residual)
 {
+	const bool exact = p.tangent == TFI_TANGENT_EXACT;
+	const void* kernel = exact ? reinterpret_cast<const void*>(AssembleNsKernel<true>)
+		: reinterpret_cast<const void*>(AssembleNsKernel<false>);
 	tfi_ctx* ctx = m->ctx;
 	StreamTimer timer(ctx, &ctx->timers.assembly_ms);
 	if (jacobian) Fill(ctx, jacobian, 0.0, m->nnzb*16);
@@ -485,21 +570,40 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	a.negres = negative_residual;
 	a.with_jacobian = jacobian != nullptr;
 	if (lds > 65536) {
-		static size_t raised = 0;
-		if (lds > raised) {
-			Check(hipFuncSetAttribute(reinterpret_cast<const void*>(AssembleNsKernel),
-				hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), "hipFuncSetAttribute");
-			raised = lds;
+		static size_t raised[2] = {0, 0};
+		if (lds > raised[exact]) {
+			Check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+				"hipFuncSetAttribute");
+			raised[exact] = lds;
 		}
 	}
 	for (int c = 0; c < m->colors; ++c) {
 		a.elements = m->color_elements.data() + m->color_offsets[c];
 		a.count = m->color_offsets[c + 1] - m->color_offsets[c];
 		if (a.count == 0) continue;
-		hipLaunchKernelGGL(AssembleNsKernel, dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a);
+		if (exact) hipLaunchKernelGGL(AssembleNsKernel<true>, dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a);
+		else hipLaunchKernelGGL(AssembleNsKernel<false>, dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a);
 		TFI_LAUNCH_CHECK("AssembleNsKernel");
 	}
 	ctx->timers.assembly_calls += 1;
+	ctx->timers.assembly_flops += AssemblyFlops(m, jacobian != nullptr, exact);
+}
+
+// Counted arithmetic of AssembleNsKernel per element with nen basis functions
+// and Q = 64 points: sum-factorised basis (656 flops per (a, q)), push-forward
+// (42), field interpolation (44), u.grad N and residual row (38), per-point
+// scalars (~300 per q) and the 2x2-tiled tangent (72 flops per (a, b, q) for
+// the legacy tangent; 132 per (a, b, q) plus ~100 per (a, q) for the exact one).
+double AssemblyFlops(const tfi_mesh* m, bool with_jacobian, bool exact)
+{
+	double total = 0.0;
+	for (int e = 0; e < m->elements; ++e) {
+		const double nen = m->h_eoff[e + 1] - m->h_eoff[e];
+		double pair = 0.0;
+		if (with_jacobian) pair = exact ? 132.0*nen*nen + 100.0*nen : 72.0*nen*nen;
+		total += kQ*(nen*(656.0 + 42.0 + 44.0 + 38.0) + 300.0 + pair);
+	}
+	return total;
 }
 
 void ApplyDirichlet(tfi_mesh* m, const int* vmask, const int* pmask, const double* vtarget, const double* ptarget,

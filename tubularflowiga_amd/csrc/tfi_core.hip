@@ -181,6 +181,31 @@ void Sync(tfi_ctx* ctx)
 	Check(hipStreamSynchronize(ctx->stream), "stream synchronize");
 }
 
+hipEvent_t TakeEvent(tfi_ctx* ctx)
+{
+	if (!ctx->event_pool.empty()) {
+		hipEvent_t e = ctx->event_pool.back();
+		ctx->event_pool.pop_back();
+		return e;
+	}
+	hipEvent_t e = nullptr;
+	Check(hipEventCreate(&e), "hipEventCreate");
+	return e;
+}
+
+void ResolveTimers(tfi_ctx* ctx)
+{
+	for (auto& p : ctx->pending) {
+		float ms = 0.0f;
+		Check(hipEventSynchronize(p.end), "timer sync");
+		Check(hipEventElapsedTime(&ms, p.begin, p.end), "timer elapsed");
+		*p.accumulate += ms;
+		ctx->event_pool.push_back(p.begin);
+		ctx->event_pool.push_back(p.end);
+	}
+	ctx->pending.clear();
+}
+
 }  // namespace tfi
 
 using namespace tfi;
@@ -221,6 +246,11 @@ void tfi_ctx_destroy(tfi_ctx* ctx)
 	if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 	if (ctx->scratch) (void)hipFree(ctx->scratch);
 	if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+	for (auto& p : ctx->pending) {
+		(void)hipEventDestroy(p.begin);
+		(void)hipEventDestroy(p.end);
+	}
+	for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
 	if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
 	if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -265,6 +295,12 @@ int tfi_allocation_stats(tfi_ctx* ctx, long long* peak_bytes, long long* live_by
 int tfi_timers_get(tfi_ctx* ctx, tfi_kernel_timers* out)
 {
 	if (!ctx || !out) return TFI_ERROR;
+	try {
+		ResolveTimers(ctx);
+	} catch (const std::exception& e) {
+		ctx->error = e.what();
+		return TFI_ERROR;
+	}
 	*out = ctx->timers;
 	return TFI_OK;
 }
@@ -279,6 +315,12 @@ int tfi_timers_enable(tfi_ctx* ctx, int enabled)
 int tfi_timers_reset(tfi_ctx* ctx)
 {
 	if (!ctx) return TFI_ERROR;
+	try {
+		ResolveTimers(ctx);
+	} catch (const std::exception& e) {
+		ctx->error = e.what();
+		return TFI_ERROR;
+	}
 	ctx->timers = tfi_kernel_timers{};
 	return TFI_OK;
 }

@@ -48,7 +48,7 @@ __global__ void ContinityPartialKernel(const double* __restrict__ rhs, int nodes
 
 }  // namespace
 
-LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart)
+LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart, int smoother)
 	: mesh_(mesh), ctx_(mesh->ctx), pc_(preconditioner), precision_(precision),
 	  krylov_(mesh->ctx, 4LL*mesh->nodes, restart)
 {
@@ -60,7 +60,7 @@ LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, in
 		correction_.Allocate(ctx_, n);
 		residual_.Allocate(ctx_, n);
 	}
-	if (pc_ == TFI_PC_AMG) amg_ = std::make_unique<Amg>(mesh, precision_);
+	if (pc_ == TFI_PC_AMG) amg_ = std::make_unique<Amg>(mesh, precision_, smoother);
 }
 
 LinearSolver::~LinearSolver() = default;
@@ -87,6 +87,7 @@ KrylovResult LinearSolver::Solve(const double* b, double* x, int maxit, double r
 		StreamTimer t(ctx, &ctx->timers.spmv_ms);
 		Spmv64(ctx, a, values_, in, out);
 		ctx->timers.spmv_calls += 1;
+		ctx->timers.spmv_bytes += SpmvBytes(a, 8);
 	};
 	Precond M = [&](const double* v, double* z) {
 		if (pc_ == TFI_PC_BJACOBI) BlockApply64(ctx, mesh_->nodes, inverse_.data(), v, z);
@@ -100,6 +101,7 @@ KrylovResult LinearSolver::Solve(const double* b, double* x, int maxit, double r
 		StreamTimer t(ctx, &ctx->timers.spmv32_ms);
 		Spmv32(ctx, a, values32_.data(), in, out);
 		ctx->timers.spmv32_calls += 1;
+		ctx->timers.spmv32_bytes += SpmvBytes(a, 4);
 	};
 	KrylovResult total;
 	total.rhs_norm = Norm2(ctx, b, n);
@@ -214,6 +216,8 @@ void tfi_flow_default_options(tfi_flow_options* o)
 	o->precision = TFI_FP64;
 	o->warm_start = TFI_WARM_PREVIOUS;
 	o->verbose = 0;
+	o->amg_smoother = TFI_SMOOTH_GAUSS_SEIDEL;
+	o->tangent = TFI_TANGENT_EXACT;
 }
 
 int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** out)
@@ -245,7 +249,7 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 		FillInt(ctx, f->vmask.data(), 0, mesh->nodes);
 		FillInt(ctx, f->pmask.data(), 0, mesh->nodes);
 		f->solver = std::make_unique<LinearSolver>(mesh, options->preconditioner, options->precision,
-			options->restart);
+			options->restart, options->amg_smoother);
 		f->warm = std::make_unique<WarmStart>(mesh);
 		Sync(ctx);
 		*out = f.release();
@@ -328,7 +332,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		tfi_ctx* ctx = f->ctx;
 		tfi_mesh* m = f->mesh;
 		const long long n = 4LL*m->nodes;
-		const NsParams params{f->opt.density, f->opt.viscosity, f->opt.dt};
+		const NsParams params{f->opt.density, f->opt.viscosity, f->opt.dt, f->opt.tangent};
 		tfi_newton_report rep{};
 		const auto t_start = Clock::now();
 		const double scale = std::sqrt(static_cast<double>(n));
@@ -450,7 +454,7 @@ int tfi_flow_warm_stats(tfi_flow* f, int* snapshots, double* last_loss, int* acc
 // ---------------------------------------------------------------- host in/out kernels
 
 int tfi_ns_assemble(tfi_mesh* m, const double* state, const double* previous, double density, double viscosity,
-	double dt, double* jacobian, double* negative_residual)
+	double dt, double* jacobian, double* negative_residual, int tangent)
 {
 	return Guard(m->ctx, [&] {
 		if (!(density > 0.0) || !(viscosity > 0.0) || !(dt >= 0.0))
@@ -461,7 +465,8 @@ int tfi_ns_assemble(tfi_mesh* m, const double* state, const double* previous, do
 		s.Upload(state, n);
 		p.Upload(previous ? previous : state, n);
 		if (jacobian) j.Allocate(ctx, m->nnzb*16);
-		AssembleNs(m, s.data(), p.data(), NsParams{density, viscosity, dt}, jacobian ? j.data() : nullptr, r.data());
+		AssembleNs(m, s.data(), p.data(), NsParams{density, viscosity, dt, tangent}, jacobian ? j.data() : nullptr,
+			r.data());
 		r.Download(negative_residual, n);
 		if (jacobian) j.Download(jacobian, m->nnzb*16);
 	});

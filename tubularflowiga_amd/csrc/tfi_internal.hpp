@@ -41,6 +41,13 @@ struct tfi_ctx {
 	double* scratch = nullptr;       // reduction partials
 	size_t scratch_count = 0;
 	double* host_pinned = nullptr;   // small pinned host buffer for reductions
+	// kernel timers: event pairs recorded on the stream, resolved lazily
+	struct PendingTimer {
+		hipEvent_t begin, end;
+		double* accumulate;
+	};
+	std::vector<PendingTimer> pending;
+	std::vector<hipEvent_t> event_pool;
 };
 
 namespace tfi {
@@ -168,23 +175,28 @@ void FillInt(tfi_ctx* ctx, int* x, int value, long long n);
 void ToFloat(tfi_ctx* ctx, const double* x, float* y, long long n);
 void Sync(tfi_ctx* ctx);
 
-// Event timer on the context stream (no-op unless timers are enabled).
+// Event timer on the context stream (no-op unless timers are enabled).  The
+// begin/end events are recorded around the timed launches without any host
+// synchronisation; ResolveTimers adds the elapsed times later.
+hipEvent_t TakeEvent(tfi_ctx* ctx);
+void ResolveTimers(tfi_ctx* ctx);
 struct StreamTimer {
 	tfi_ctx* ctx;
 	double* accumulate;
 	bool on;
+	hipEvent_t begin = nullptr;
 	StreamTimer(tfi_ctx* c, double* acc) : ctx(c), accumulate(acc), on(c->timers_enabled)
 	{
-		if (on) Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event record");
+		if (!on) return;
+		begin = TakeEvent(ctx);
+		Check(hipEventRecord(begin, ctx->stream), "event record");
 	}
 	~StreamTimer()
 	{
 		if (!on) return;
-		float ms = 0.0f;
-		if (hipEventRecord(ctx->ev_end, ctx->stream) == hipSuccess
-			&& hipEventSynchronize(ctx->ev_end) == hipSuccess
-			&& hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end) == hipSuccess)
-			*accumulate += ms;
+		hipEvent_t end = TakeEvent(ctx);
+		if (hipEventRecord(end, ctx->stream) == hipSuccess) ctx->pending.push_back({begin, end, accumulate});
+		if (ctx->pending.size() > 8192) ResolveTimers(ctx);
 	}
 };
 
@@ -194,9 +206,11 @@ void BuildMesh(tfi_mesh* mesh);
 // ---- assembly (tfi_assembly.hip)
 struct NsParams {
 	double density, viscosity, dt;
+	int tangent = TFI_TANGENT_LEGACY;
 };
 void AssembleNs(tfi_mesh* mesh, const double* state, const double* previous, const NsParams& p, double* jacobian,
 	double* negative_residual);
+double AssemblyFlops(const tfi_mesh* mesh, bool with_jacobian, bool exact);
 void ApplyDirichlet(tfi_mesh* mesh, const int* vmask, const int* pmask, const double* vtarget, const double* ptarget,
 	const double* state, double* jacobian, double* rhs);
 // state[constrained] = target
@@ -207,6 +221,12 @@ void BoundaryFaceFlows(tfi_mesh* mesh, int count, const int* element, const int*
 	double* flows_out);
 
 // ---- sparse (tfi_sparse.hip)
+// Algorithmic bytes of one SpMV: values + column indices once, row offsets,
+// x (FP64) read once and y (FP64) written once.
+inline double SpmvBytes(const BsrView& a, int value_bytes)
+{
+	return (double)a.nnzb*(16.0*value_bytes + 4.0) + 4.0*(a.rows + 1) + 2.0*4.0*a.rows*8.0;
+}
 void Spmv64(tfi_ctx* ctx, const BsrView& a, const double* values, const double* x, double* y);
 void Spmv32(tfi_ctx* ctx, const BsrView& a, const float* values, const double* x, double* y);
 void Spmv32f(tfi_ctx* ctx, const BsrView& a, const float* values, const float* x, float* y);

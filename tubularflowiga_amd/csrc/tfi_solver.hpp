@@ -5,10 +5,10 @@
 
 namespace tfi {
 
-// Node-block smoothed-free aggregation AMG (tfi_amg.hip).
+// Node-block plain-aggregation AMG (tfi_amg.hip).
 class Amg {
 public:
-	Amg(tfi_mesh* mesh, int precision);
+	Amg(tfi_mesh* mesh, int precision, int smoother);
 	~Amg();
 	// Numeric setup for new fine values (symbolic hierarchy built on first call).
 	void Setup(const double* values64, const float* values32);
@@ -25,7 +25,7 @@ private:
 
 class LinearSolver {
 public:
-	LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart);
+	LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart, int smoother = 1);
 	~LinearSolver();
 	void Setup(const double* values);
 	KrylovResult Solve(const double* b, double* x, int maxit, double rtol);

@@ -119,13 +119,14 @@ class DeviceMesh:
         self.ctx.check(self.lib.tfi_mesh_colors(self.handle, c))
         return c
 
-    def assemble(self, state, previous, density, viscosity, dt, jacobian=True):
+    def assemble(self, state, previous, density, viscosity, dt, jacobian=True, tangent="exact"):
         n4 = 4 * self.nodes
         negres = np.zeros(n4)
         jac = np.zeros(16 * self.nnzb) if jacobian else None
         self.ctx.check(self.lib.tfi_ns_assemble(
             self.handle, np.ascontiguousarray(state, np.float64), np.ascontiguousarray(previous, np.float64),
-            density, viscosity, dt, None if jac is None else jac.ctypes.data_as(ctypes.c_void_p), negres))
+            density, viscosity, dt, None if jac is None else jac.ctypes.data_as(ctypes.c_void_p), negres,
+            {"legacy": 0, "exact": 1}[tangent]))
         return (None if jac is None else jac.reshape(-1, 4, 4)), negres
 
     def spmv(self, values, x, precision=TFI_FP64):
