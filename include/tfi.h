@@ -69,6 +69,15 @@ int tfi_mesh_pattern(tfi_mesh* mesh, int32_t* row_offsets, int32_t* columns, int
 /* Element colouring: colour of each element (int32[E]). */
 int tfi_mesh_colors(tfi_mesh* mesh, int32_t* element_color);
 
+/* Pressure-traction load of labelled faces (IntegratePressureTractionForces,
+ * solvers/cpu/include/PressureTractionFlow.hpp:41-64, with the face integral of
+ * BoundaryFlow.hpp:445-477): traction [4N] is overwritten with
+ * sum_k -pressure[k] int_{face k} N_a n dA in the velocity rows, summed in the
+ * order the faces are given (pass them element-major, faces 0..5, to match the
+ * reference's order).  Computed on the GPU, gathered on the host. */
+int tfi_pressure_traction(tfi_mesh* mesh, int count, const int32_t* element, const int32_t* face,
+	const double* pressure, double* traction);
+
 /* ------------------------------------------------------------------ kernels (host in/out) */
 /* Negated residual and tangent with a deterministic colour-ordered scatter:
  * tangent TFI_TANGENT_LEGACY = the GPU reference's tau-frozen tangent

@@ -169,3 +169,20 @@ def test_mlp_bf16_inference_vs_oracle(ctx):
         assert np.max(np.abs(y - oy)) <= 2e-2 * (1 + np.max(np.abs(oy)))
         fy, _ = MO.forward(x, *[v.astype(np.float64) for v in w])
         assert np.max(np.abs(y - fy)) <= 5e-2 * (1 + np.max(np.abs(fy)))
+
+
+def test_pressure_traction_matches_oracle(dmesh, case):
+    """Traction load of every labelled face with a label-dependent pressure
+    (IntegratePressureTractionForces order: elements ascending, faces 0..5)."""
+    m = case.mesh
+    e, f, lab = boundary.boundary_faces(m)
+    pressure = 1.5 + 0.25 * lab.astype(np.float64)
+    gpu = dmesh.pressure_traction(e, f, pressure)
+    ref = np.zeros(4 * m.nodes)
+    for ei, fi, pi in zip(e, f, pressure):
+        conn = m.element_connectivity(ei)
+        loc = O.pressure_traction(m.element_bezier(ei), m.dense_extraction(ei), int(fi), float(pi)).reshape(-1, 4)
+        for a, node in enumerate(conn):
+            ref[4 * node:4 * node + 3] += loc[a, :3]
+    assert np.all(gpu.reshape(-1, 4)[:, 3] == 0.0)
+    assert np.max(np.abs(gpu - ref)) <= 1e-13 * np.max(np.abs(ref))

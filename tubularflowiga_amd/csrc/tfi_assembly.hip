@@ -531,6 +531,79 @@ __global__ void FaceFlowKernel(int count, const int* __restrict__ felem, const i
 	}
 }
 
+// Pressure-traction load of one labelled face (IntegrateBoundaryPressureTraction,
+// BoundaryFlow.hpp:445-477): local[a][c] = -p sum_pt s w_i w_j / 4 * cof_c(pt) N_a(pt),
+// with cof = det J * (row `fixed axis` of J^-1) the face's area vector.  One
+// 64-lane block per face; the per-face vectors are summed on the host in face
+// order, so the load is deterministic and ordered like the reference's.
+__global__ void FaceTractionKernel(int count, const int* __restrict__ felem, const int* __restrict__ fface,
+	const double* __restrict__ pressure, const int* __restrict__ eoff, const int* __restrict__ xoff,
+	const uint8_t* __restrict__ xcol, const double* __restrict__ xval, const double* __restrict__ bezier,
+	double* __restrict__ local)
+{
+	const int f = blockIdx.x;
+	if (f >= count) return;
+	__shared__ double bern[16][64];
+	__shared__ double wcof[16][3];
+	const int e = felem[f], face = fface[f];
+	const int fixed_axis[6] = {2, 1, 0, 1, 0, 2};
+	const int var0[6] = {0, 0, 1, 0, 1, 0}, var1[6] = {1, 2, 2, 2, 2, 1};
+	const double fixed_value[6] = {0.0, 0.0, 1.0, 1.0, 0.0, 1.0};
+	const double sign[6] = {-1.0, -1.0, 1.0, 1.0, -1.0, 1.0};
+	const double gw[4] = {0.3478548451374539, 0.6521451548625461, 0.6521451548625461, 0.3478548451374539};
+	const int lane = threadIdx.x;  // 64 threads
+	for (int idx = lane; idx < 16*64; idx += 64) {
+		const int pt = idx >> 6, p = idx & 63;
+		const int qi = pt >> 2, qj = pt & 3;
+		double c[3];
+		c[fixed_axis[face]] = fixed_value[face];
+		c[var0[face]] = kGp[qi];
+		c[var1[face]] = kGp[qj];
+		double b[3][4], db[3][4], d2b[3][4];
+		for (int d = 0; d < 3; ++d) Bern3(c[d], b[d], db[d], d2b[d]);
+		bern[pt][p] = b[0][p & 3]*b[1][(p >> 2) & 3]*b[2][p >> 4];
+	}
+	if (lane < 16) {
+		const int pt = lane, qi = pt >> 2, qj = pt & 3;
+		double c[3];
+		c[fixed_axis[face]] = fixed_value[face];
+		c[var0[face]] = kGp[qi];
+		c[var1[face]] = kGp[qj];
+		double b[3][4], db[3][4], d2b[3][4];
+		for (int d = 0; d < 3; ++d) Bern3(c[d], b[d], db[d], d2b[d]);
+		const double* X = bezier + (size_t)e*192;
+		double J[3][3] = {};
+		for (int k = 0; k < 4; ++k)
+			for (int j = 0; j < 4; ++j)
+				for (int i = 0; i < 4; ++i) {
+					const int p = i + 4*j + 16*k;
+					const double g[3] = {db[0][i]*b[1][j]*b[2][k], b[0][i]*db[1][j]*b[2][k], b[0][i]*b[1][j]*db[2][k]};
+					for (int x = 0; x < 3; ++x)
+						for (int d = 0; d < 3; ++d) J[x][d] += X[3*p + x]*g[d];
+				}
+		const int a = fixed_axis[face];
+		const int a1 = (a + 1)%3, a2 = (a + 2)%3;
+		const double w = sign[face]*gw[qi]*gw[qj]*0.25;
+		for (int x = 0; x < 3; ++x) {
+			const int x1 = (x + 1)%3, x2 = (x + 2)%3;
+			wcof[pt][x] = w*(J[x1][a1]*J[x2][a2] - J[x1][a2]*J[x2][a1]);
+		}
+	}
+	__syncthreads();
+	const int b0 = eoff[e], nen = eoff[e + 1] - b0;
+	const double pf = pressure[f];
+	for (int aa = lane; aa < nen; aa += 64) {
+		double acc[3] = {0.0, 0.0, 0.0};
+		for (int pt = 0; pt < 16; ++pt) {
+			double N = 0.0;
+			for (int k = xoff[b0 + aa]; k < xoff[b0 + aa + 1]; ++k) N += xval[k]*bern[pt][xcol[k]];
+			for (int c = 0; c < 3; ++c) acc[c] += N*wcof[pt][c];
+		}
+		double* out = local + ((size_t)f*kMaxBasis + aa)*3;
+		for (int c = 0; c < 3; ++c) out[c] = -pf*acc[c];
+	}
+}
+
 }  // namespace
 
 void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const NsParams& p, double* jacobian,
@@ -639,6 +712,15 @@ void BoundaryFaceFlows(tfi_mesh* m, int count, const int* element, const int* fa
 	hipLaunchKernelGGL(FaceFlowKernel, dim3(count), dim3(64), 0, ctx->stream, count, element, face, m->eoff.data(),
 		m->conn.data(), m->xoff.data(), m->xcol.data(), m->xval.data(), m->bezier.data(), state, flows_out);
 	TFI_LAUNCH_CHECK("FaceFlowKernel");
+}
+
+void FaceTraction(tfi_mesh* m, int count, const int* element, const int* face, const double* pressure,
+	double* local)
+{
+	if (count <= 0) return;
+	hipLaunchKernelGGL(FaceTractionKernel, dim3(count), dim3(64), 0, m->ctx->stream, count, element, face, pressure,
+		m->eoff.data(), m->xoff.data(), m->xcol.data(), m->xval.data(), m->bezier.data(), local);
+	TFI_LAUNCH_CHECK("FaceTractionKernel");
 }
 
 }  // namespace tfi

@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <string>
 #include <map>
 
 namespace tfi {
@@ -194,6 +196,16 @@ int Guard(tfi_ctx* ctx, F&& fn)
 	}
 }
 
+}  // namespace
+
+namespace {
+// %.16e, the precision of the reference driver's failure messages
+std::string Precise(double v)
+{
+	char buf[64];
+	std::snprintf(buf, sizeof buf, "%.16e", v);
+	return buf;
+}
 }  // namespace
 
 extern "C" {
@@ -406,9 +418,15 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			rec.linear_residual = r.residual;
 			rec.singular_diagonal_blocks = f->solver->singular_blocks();
 			rep.linear_iterations += r.iterations;
-			if (!r.converged)
+			if (!r.converged) {
+				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				rep.total_s = std::chrono::duration<double>(Clock::now() - t_start).count();
+				if (report) *report = rep;
 				throw Error("Navier-Stokes linear solve failed at nonlinear iteration " + std::to_string(it)
-					+ " (residual=" + std::to_string(r.residual) + ", rhs=" + std::to_string(r.rhs_norm) + ")");
+					+ " (residual=" + Precise(r.residual) + ", nonlinear_residual=" + Precise(norm)
+					+ ", continuity_l2=" + Precise(mt.continuity_l2) + ", net_boundary_flow=" + Precise(mt.net)
+					+ ", relative_mass_imbalance=" + Precise(mt.imbalance) + ")");
+			}
 			rec.update_l2 = Norm2(ctx, f->update.data(), n);
 			Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
 			if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
@@ -418,7 +436,13 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		rep.converged = converged ? 1 : 0;
 		rep.total_s = std::chrono::duration<double>(Clock::now() - t_start).count();
 		if (report) *report = rep;
-		if (!converged) throw Error("Navier-Stokes nonlinear solve reached MAX_NEWTON");
+		if (!converged) {
+			const tfi_newton_record& last = rep.record[rep.records > 0 ? rep.records - 1 : 0];
+			throw Error("Navier-Stokes nonlinear solve reached MAX_NEWTON (nonlinear_residual="
+				+ Precise(rep.final_residual) + ", continuity_l2=" + Precise(last.continuity_l2)
+				+ ", net_boundary_flow=" + Precise(last.net_boundary_flow) + ", relative_mass_imbalance="
+				+ Precise(last.relative_mass_imbalance) + ")");
+		}
 	});
 }
 
@@ -529,6 +553,38 @@ int tfi_apply_dirichlet(tfi_mesh* m, const int32_t* vmask, const int32_t* pmask,
 			rhs ? r.data() : nullptr);
 		if (jacobian) j.Download(jacobian, m->nnzb*16);
 		if (rhs) r.Download(rhs, 4ULL*N);
+	});
+}
+
+int tfi_pressure_traction(tfi_mesh* m, int count, const int32_t* element, const int32_t* face,
+	const double* pressure, double* traction)
+{
+	return Guard(m->ctx, [&] {
+		tfi_ctx* ctx = m->ctx;
+		const size_t n = 4ULL*m->nodes;
+		std::fill(traction, traction + n, 0.0);
+		if (count <= 0) return;
+		for (int k = 0; k < count; ++k)
+			if (element[k] < 0 || element[k] >= m->elements || face[k] < 0 || face[k] >= 6)
+				throw Error("pressure-traction face " + std::to_string(k) + " has an invalid element or face index");
+		DevBuf<int> el(ctx, count), fa(ctx, count);
+		DevBuf<double> pr(ctx, count), local(ctx, static_cast<size_t>(count)*kMaxBasis*3);
+		el.Upload(element, count);
+		fa.Upload(face, count);
+		pr.Upload(pressure, count);
+		FaceTraction(m, count, el.data(), fa.data(), pr.data(), local.data());
+		std::vector<double> host(static_cast<size_t>(count)*kMaxBasis*3);
+		local.Download(host.data(), host.size());
+		// sum in face order (elements ascending, faces 0..5 within an element), as
+		// IntegratePressureTractionForces does
+		for (int k = 0; k < count; ++k) {
+			const int e = element[k];
+			const int b0 = m->h_eoff[e], nen = m->h_eoff[e + 1] - b0;
+			for (int a = 0; a < nen; ++a) {
+				const size_t node = static_cast<size_t>(m->h_conn[b0 + a]);
+				for (int c = 0; c < 3; ++c) traction[4*node + c] += host[(static_cast<size_t>(k)*kMaxBasis + a)*3 + c];
+			}
+		}
 	});
 }
 

@@ -219,6 +219,9 @@ void SetDirichletState(tfi_mesh* mesh, const int* vmask, const int* pmask, const
 // Boundary faces flows: per face flow into flows_out[face] (deterministic).
 void BoundaryFaceFlows(tfi_mesh* mesh, int count, const int* element, const int* face, const double* state,
 	double* flows_out);
+// Per-face pressure-traction vectors local[count][kMaxBasis][3] (device).
+void FaceTraction(tfi_mesh* mesh, int count, const int* element, const int* face, const double* pressure,
+	double* local);
 
 // ---- sparse (tfi_sparse.hip)
 // Algorithmic bytes of one SpMV: values + column indices once, row offsets,

@@ -153,6 +153,16 @@ class DeviceMesh:
             None if rhs is None else rhs.ctypes.data_as(ctypes.c_void_p)))
         return jac, rhs
 
+    def pressure_traction(self, element, face, pressure):
+        """Traction load [4N] of faces (element, local face) at pressures (GPU face
+        integrals, summed in the given face order)."""
+        element = np.ascontiguousarray(element, np.int32)
+        out = np.zeros(4 * self.nodes)
+        self.ctx.check(self.lib.tfi_pressure_traction(
+            self.handle, len(element), element, np.ascontiguousarray(face, np.int32),
+            np.ascontiguousarray(pressure, np.float64), out))
+        return out
+
     def solve(self, values, b, x0=None, preconditioner=TFI_PC_AMG, precision=TFI_FP64, restart=200,
               maxit=5000, rtol=1e-8):
         x = np.zeros(4 * self.nodes) if x0 is None else np.array(x0, np.float64)

@@ -1,0 +1,510 @@
+// iga_hip: the MI355X drop-in for `iga_cuda navier-stokes | mesh-check | device-info`
+// (solvers/cuda/src/iga_cuda.cu:447-498 options, :734-1151 driver, :1157-1194 main).
+// Same command line, input files, output files and stdout key=value records;
+// the solve runs in libtfi_hip through the C-ABI of include/tfi.h.  Options the
+// reference does not have select the MI355X solver path (preconditioner,
+// precision, warm start, tangent) and default to the reproducing settings.
+#include "case_input.hpp"
+#include "tfi.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace {
+
+using namespace tfi_host;
+using Clock = std::chrono::steady_clock;
+
+double Seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+struct Ctx {
+	tfi_ctx* ctx = nullptr;
+	explicit Ctx(int device)
+	{
+		if (tfi_ctx_create(device, &ctx) != TFI_OK) {
+			const std::string msg = tfi_last_error(nullptr);
+			throw std::runtime_error("cannot create the HIP context: " + msg);
+		}
+	}
+	~Ctx() { tfi_ctx_destroy(ctx); }
+	void Check(int status, const char* what) const
+	{
+		if (status != TFI_OK) throw std::runtime_error(std::string(what) + ": " + tfi_last_error(ctx));
+	}
+};
+
+struct Mesh {
+	tfi_mesh* mesh = nullptr;
+	~Mesh() { tfi_mesh_destroy(mesh); }
+};
+
+struct Flow {
+	tfi_flow* flow = nullptr;
+	~Flow() { tfi_flow_destroy(flow); }
+};
+
+double Gib(double bytes) { return bytes/(1024.0*1024.0*1024.0); }
+
+void PrintDevice(Ctx& c)
+{
+	char buf[512];
+	c.Check(tfi_device_info(c.ctx, buf, sizeof buf), "device info");
+	std::cout << buf << '\n';
+}
+
+void PrintMesh(const FlatDatabase& db, const tfi_mesh_stats* st)
+{
+	std::cout << "mesh nodes=" << db.nodes << " elements=" << db.elements() << " basis_entries=" << db.connectivity.size()
+		<< " extraction_nonzeros=" << db.extraction_values.size() << " max_basis=" << db.maximum_basis;
+	if (st)
+		std::cout << " block_nonzeros=" << st->block_nonzeros
+			<< " mean_block_row=" << static_cast<double>(st->block_nonzeros)/static_cast<double>(db.nodes);
+	std::cout << '\n';
+}
+
+void CreateMesh(Ctx& c, const FlatDatabase& db, Mesh& m)
+{
+	c.Check(tfi_mesh_create(c.ctx, static_cast<int>(db.nodes), db.elements(), db.element_offsets.data(),
+		db.connectivity.data(), db.extraction_offsets.data(), db.extraction_columns.data(), db.extraction_values.data(),
+		db.bezier.data(), &m.mesh), "mesh upload");
+}
+
+int MeshCheck(Ctx& c, const std::string& path)
+{
+	const FlatDatabase db = ReadDatabase(path);
+	Mesh m;
+	CreateMesh(c, db, m);
+	tfi_mesh_stats st{};
+	c.Check(tfi_mesh_stats_get(m.mesh, &st), "mesh statistics");
+	PrintMesh(db, &st);
+	std::cout << std::setprecision(17) << "geometry min_detJ=" << st.minimum_determinant
+		<< " bad_samples=" << st.bad_samples << " colors=" << st.colors << '\n';
+	return st.bad_samples ? 1 : 0;
+}
+
+struct Options {
+	fs::path database, case_dir, output, checkpoint, restart;
+	std::string system;
+	int maximum_newton = 12, output_every = 0, checkpoint_every = 0, stop_after_step = 0;
+	double nonlinear_rtol = 1e-5, nonlinear_atol = 1e-10, mass_rtol = 1e-3;
+	std::string visualization = "auto";
+	// MI355X solver path
+	int preconditioner = TFI_PC_AMG, precision = TFI_FP64, warm_start = TFI_WARM_PREVIOUS;
+	int tangent = TFI_TANGENT_EXACT, smoother = TFI_SMOOTH_GAUSS_SEIDEL, restart_length = 200;
+	int linear_maximum_iterations = 5000;
+	double linear_rtol = 1e-8;
+};
+
+int PositiveInt(const std::string& t, const std::string& opt)
+{
+	std::size_t used = 0;
+	int v = 0;
+	try {
+		v = std::stoi(t, &used);
+	} catch (const std::exception&) {
+		throw std::runtime_error(opt + " requires a positive integer");
+	}
+	if (used != t.size() || v <= 0) throw std::runtime_error(opt + " requires a positive integer");
+	return v;
+}
+
+double PositiveDouble(const std::string& t, const std::string& opt)
+{
+	std::size_t used = 0;
+	double v = 0.0;
+	try {
+		v = std::stod(t, &used);
+	} catch (const std::exception&) {
+		throw std::runtime_error(opt + " requires a finite positive value");
+	}
+	if (used != t.size() || !std::isfinite(v) || v <= 0.0) throw std::runtime_error(opt + " requires a finite positive value");
+	return v;
+}
+
+int Choice(const std::string& v, const std::string& opt, std::initializer_list<std::pair<const char*, int>> table)
+{
+	std::string names;
+	for (const auto& t : table) {
+		if (v == t.first) return t.second;
+		names += std::string(names.empty() ? "" : "|") + t.first;
+	}
+	throw std::runtime_error(opt + " must be " + names);
+}
+
+const char* kUsage =
+	"usage: iga_hip navier-stokes DATABASE.ntiga CASE_DIR [MAX_NEWTON] [OUTPUT] "
+	"[--system NAME] [--max-newton N] [--output PATH] [--output-every N] "
+	"[--checkpoint PREFIX] [--checkpoint-every N] [--restart PREFIX] "
+	"[--stop-after-step N] [--nonlinear-rtol R] [--nonlinear-atol A] "
+	"[--mass-rtol R] [--visualization-format auto|vtu] "
+	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|learned] "
+	"[--tangent exact|legacy] [--amg-smoother gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
+	"[--linear-max-iterations N]";
+
+// ParseCudaFlowOptions (iga_cuda.cu:447-498) plus the solver-path options
+Options ParseOptions(int argc, char** argv)
+{
+	if (argc < 4) throw std::runtime_error(kUsage);
+	Options o;
+	o.database = argv[2];
+	o.case_dir = argv[3];
+	int positional = 0;
+	for (int i = 4; i < argc; ++i) {
+		const std::string arg(argv[i]);
+		if (arg.rfind("--", 0) != 0) {
+			if (positional == 0) o.maximum_newton = PositiveInt(arg, "MAX_NEWTON");
+			else if (positional == 1) o.output = arg;
+			else throw std::runtime_error("too many positional arguments");
+			++positional;
+			continue;
+		}
+		if (i + 1 >= argc) throw std::runtime_error(arg + " requires a value");
+		const std::string v(argv[++i]);
+		if (arg == "--system") o.system = v;
+		else if (arg == "--max-newton") o.maximum_newton = PositiveInt(v, arg);
+		else if (arg == "--output") o.output = v;
+		else if (arg == "--output-every") o.output_every = PositiveInt(v, arg);
+		else if (arg == "--checkpoint") o.checkpoint = v;
+		else if (arg == "--checkpoint-every") o.checkpoint_every = PositiveInt(v, arg);
+		else if (arg == "--restart") o.restart = v;
+		else if (arg == "--stop-after-step") o.stop_after_step = PositiveInt(v, arg);
+		else if (arg == "--nonlinear-rtol") o.nonlinear_rtol = PositiveDouble(v, arg);
+		else if (arg == "--nonlinear-atol") o.nonlinear_atol = PositiveDouble(v, arg);
+		else if (arg == "--mass-rtol") o.mass_rtol = PositiveDouble(v, arg);
+		else if (arg == "--visualization-format") {
+			if (v == "vtkhdf" || v == "bezier-vtkhdf")
+				throw std::runtime_error("--visualization-format vtkhdf is not available in iga_hip; use vtu");
+			if (v != "auto" && v != "vtu") throw std::runtime_error("--visualization-format must be auto, vtu, or vtkhdf");
+			o.visualization = v;
+		} else if (arg == "--preconditioner") o.preconditioner = Choice(v, arg, {{"amg", TFI_PC_AMG}, {"bjacobi", TFI_PC_BJACOBI}});
+		else if (arg == "--precision") o.precision = Choice(v, arg, {{"fp64", TFI_FP64}, {"mixed", TFI_MIXED}});
+		else if (arg == "--warm-start")
+			o.warm_start = Choice(v, arg, {{"previous", TFI_WARM_PREVIOUS}, {"extrapolate", TFI_WARM_EXTRAPOLATE},
+				{"learned", TFI_WARM_LEARNED}});
+		else if (arg == "--tangent") o.tangent = Choice(v, arg, {{"exact", TFI_TANGENT_EXACT}, {"legacy", TFI_TANGENT_LEGACY}});
+		else if (arg == "--amg-smoother")
+			o.smoother = Choice(v, arg, {{"gauss-seidel", TFI_SMOOTH_GAUSS_SEIDEL}, {"jacobi", TFI_SMOOTH_JACOBI}});
+		else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
+		else if (arg == "--restart-length") o.restart_length = PositiveInt(v, arg);
+		else if (arg == "--linear-max-iterations") o.linear_maximum_iterations = PositiveInt(v, arg);
+		else throw std::runtime_error("unknown option: " + arg);
+	}
+	if (o.output_every > 0 && o.output.empty()) throw std::runtime_error("--output-every requires --output or legacy OUTPUT");
+	if (o.checkpoint_every > 0 && o.checkpoint.empty()) throw std::runtime_error("--checkpoint-every requires --checkpoint");
+	return o;
+}
+
+void ApplyWallTrace(ResolvedFlowBoundaries& b, const std::vector<std::int32_t>& trace)
+{
+	for (const auto n : trace) {
+		if (!b.velocity_mask.at(n)) ++b.velocity_nodes;
+		b.velocity_mask[n] = 1;
+		for (int k = 0; k < 3; ++k) b.velocity[3*n + k] = 0.0;
+	}
+}
+
+double Norm(const std::vector<double>& v, int first, int stride)
+{
+	double s = 0.0;
+	for (std::size_t i = first; i < v.size(); i += stride) s += v[i]*v[i];
+	return std::sqrt(s);
+}
+
+int NavierStokes(Ctx& c, int argc, char** argv)
+{
+	const Options o = ParseOptions(argc, argv);
+	const auto total_start = Clock::now();
+	const FlatDatabase db = ReadDatabase(o.database.string());
+	const auto labeled = ReadLabeledHexMesh((o.case_dir/"controlmesh.vtk").string(), db.nodes,
+		static_cast<std::uint64_t>(db.elements()));
+	const auto wall_trace = WallTraceBasis(db, labeled);
+	const auto profile = ReadVelocity((o.case_dir/"initial_velocityfield.txt").string(), db.nodes);
+	if (!fs::exists(o.case_dir/"simulation_config.json"))
+		throw std::runtime_error("legacy simulation_parameter.txt/case_config.json input is not supported by iga_hip; "
+			"write simulation_config.json");
+	const SimulationConfig config = ReadSimulationConfig((o.case_dir/"simulation_config.json").string());
+	const FlowSystem& sys = FindFlowSystem(config, o.system);
+	const bool transient = sys.time_integration == "backward_euler";
+	const double dt = transient ? config.dt : 0.0;
+	const int physical_steps = transient ? config.steps : 1;
+	auto step_config = [&](double time) { return transient ? MaterializeWaveforms(config, o.case_dir, time) : config; };
+	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(step_config(0.0), sys, labeled.labels, profile);
+	ApplyWallTrace(bc, wall_trace);
+	if (o.stop_after_step > physical_steps) throw std::runtime_error("--stop-after-step exceeds configured physical steps");
+	const int run_end = o.stop_after_step > 0 ? o.stop_after_step : physical_steps;
+	std::cout << "boundary_config=simulation_config.json viscosity=" << sys.viscosity << " density=" << sys.density
+		<< " time_integration=" << (transient ? "backward_euler" : "steady") << " dt=" << dt
+		<< " steps=" << physical_steps << " run_end_step=" << run_end << " velocity_nodes=" << bc.velocity_nodes
+		<< " pressure_nodes=" << bc.pressure_nodes << '\n';
+	if (!transient && (!o.restart.empty() || !o.checkpoint.empty() || o.output_every > 0 || o.checkpoint_every > 0))
+		throw std::runtime_error("restart, checkpoint, and time-indexed output require transient flow");
+
+	Mesh m;
+	CreateMesh(c, db, m);
+	tfi_mesh_stats st{};
+	c.Check(tfi_mesh_stats_get(m.mesh, &st), "mesh statistics");
+	PrintMesh(db, &st);
+	if (st.bad_samples)
+		throw std::runtime_error("mesh has " + std::to_string(st.bad_samples)
+			+ " non-positive quadrature Jacobians; minimum detJ " + std::to_string(st.minimum_determinant));
+	const auto preprocess_end = Clock::now();
+
+	// labelled faces (mass balance) and pressure-traction faces, element-major
+	std::vector<std::int32_t> face_el, face_id, face_lab;
+	for (int e = 0; e < db.elements(); ++e)
+		for (int f = 0; f < 6; ++f)
+			if (db.labels[6*e + f] >= 0) {
+				face_el.push_back(e);
+				face_id.push_back(f);
+				face_lab.push_back(db.labels[6*e + f]);
+			}
+	for (const auto& t : bc.tractions)
+		if (std::find(face_lab.begin(), face_lab.end(), t.first) == face_lab.end())
+			throw std::runtime_error("pressure traction label " + std::to_string(t.first)
+				+ " has no boundary faces in the .ntiga database; repack with iga_pack");
+
+	tfi_flow_options opt;
+	tfi_flow_default_options(&opt);
+	opt.density = sys.density;
+	opt.viscosity = sys.viscosity;
+	opt.dt = dt;
+	opt.maximum_newton = o.maximum_newton;
+	opt.nonlinear_rtol = o.nonlinear_rtol;
+	opt.nonlinear_atol = o.nonlinear_atol;
+	opt.mass_rtol = o.mass_rtol;
+	opt.linear_rtol = o.linear_rtol;
+	opt.linear_maximum_iterations = o.linear_maximum_iterations;
+	opt.restart = o.restart_length;
+	opt.preconditioner = o.preconditioner;
+	opt.precision = o.precision;
+	opt.warm_start = transient ? o.warm_start : TFI_WARM_PREVIOUS;
+	opt.amg_smoother = o.smoother;
+	opt.tangent = o.tangent;
+	Flow flow;
+	c.Check(tfi_flow_create(m.mesh, &opt, &flow.flow), "flow solver");
+	c.Check(tfi_flow_set_boundary_faces(flow.flow, static_cast<int>(face_el.size()), face_el.data(), face_id.data(),
+		face_lab.data()), "boundary faces");
+
+	const std::size_t N = db.nodes;
+	std::vector<double> state(4*N, 0.0);
+	int start_step = 0;
+	if (transient && !o.restart.empty()) {
+		const auto meta = ReadCheckpointMeta(o.restart);
+		ValidateCheckpoint(meta, db.nodes, physical_steps, dt, sys.density, sys.viscosity);
+		state = ReadCheckpointState(o.restart, meta);
+		start_step = meta.completed_step;
+		std::cout << "restart=" << o.restart.string() << " completed_step=" << start_step
+			<< " physical_time=" << meta.physical_time << '\n';
+	} else if (transient) {
+		for (std::size_t n = 0; n < N; ++n) {
+			if (bc.velocity_mask[n])
+				for (int k = 0; k < 3; ++k) state[4*n + k] = bc.velocity[3*n + k];
+			if (bc.pressure_mask[n]) state[4*n + 3] = bc.pressure[n];
+		}
+	}
+	c.Check(tfi_flow_set_state(flow.flow, state.data(), nullptr), "initial state");
+
+	const fs::path mesh_vtk = o.case_dir/"controlmesh.vtk";
+	std::vector<std::pair<double, fs::path>> vtu_snaps, text_snaps;
+	if (transient && o.output_every > 0) {
+		const auto text = TimeIndexedPath(o.output, start_step);
+		const auto vtu = VtuStepPath(o.output, start_step);
+		WriteStateText(text, state);
+		WriteStateVtu(mesh_vtk, vtu, state, start_step*dt);
+		vtu_snaps.emplace_back(start_step*dt, vtu);
+		text_snaps.emplace_back(start_step*dt, text);
+	}
+	long long total_linear_iterations = 0;
+	double total_assembly = 0.0, total_linear = 0.0;
+	std::vector<double> traction(4*N, 0.0);
+	for (int step = start_step; step < run_end; ++step) {
+		const double time = transient ? (step + 1)*dt : 0.0;
+		bc = ResolveFlowBoundaries(step_config(time), sys, labeled.labels, profile);
+		ApplyWallTrace(bc, wall_trace);
+		std::vector<std::int32_t> te, tf;
+		std::vector<double> tp;
+		for (std::size_t k = 0; k < face_el.size(); ++k) {
+			const auto it = bc.tractions.find(face_lab[k]);
+			if (it == bc.tractions.end()) continue;
+			te.push_back(face_el[k]);
+			tf.push_back(face_id[k]);
+			tp.push_back(it->second);
+		}
+		c.Check(tfi_pressure_traction(m.mesh, static_cast<int>(te.size()), te.data(), tf.data(), tp.data(),
+			traction.data()), "pressure traction");
+		const bool any_traction = std::any_of(traction.begin(), traction.end(), [](double v) { return v != 0.0; });
+		c.Check(tfi_flow_begin_step(flow.flow, nullptr, 0), "begin step");
+		c.Check(tfi_flow_set_boundary(flow.flow, bc.velocity_mask.data(), bc.pressure_mask.data(), bc.velocity.data(),
+			bc.pressure.data(), any_traction ? traction.data() : nullptr), "boundary data");
+		tfi_newton_report rep{};
+		const int status = tfi_flow_newton(flow.flow, &rep);
+		const double scale = std::sqrt(static_cast<double>(4*N));
+		for (int r = 0; r < rep.records; ++r) {
+			const auto& rec = rep.record[r];
+			const bool last_converged = status == TFI_OK && r == rep.records - 1;
+			if (last_converged) {
+				std::cout << "step=" << step + 1 << " time=" << time << " converged newton=" << r
+					<< " residual_l2=" << rec.residual_l2 << " residual_rms=" << rec.residual_l2/scale
+					<< " tolerance=" << rep.tolerance << " absolute_rms_tolerance=" << o.nonlinear_atol
+					<< " continuity_l2=" << rec.continuity_l2 << " continuity_sum=" << rec.continuity_sum
+					<< " net_boundary_flow=" << rec.net_boundary_flow
+					<< " relative_mass_imbalance=" << rec.relative_mass_imbalance
+					<< " mass_tolerance=" << o.mass_rtol << '\n';
+			} else {
+				std::cout << "step=" << step + 1 << " time=" << time << " newton=" << r
+					<< " residual_l2=" << rec.residual_l2 << " residual_rms=" << rec.residual_l2/scale
+					<< " continuity_l2=" << rec.continuity_l2 << " continuity_sum=" << rec.continuity_sum
+					<< " net_boundary_flow=" << rec.net_boundary_flow
+					<< " relative_mass_imbalance=" << rec.relative_mass_imbalance << " update_l2=" << rec.update_l2
+					<< " linear_iterations=" << rec.linear_iterations << " linear_residual=" << rec.linear_residual
+					<< " singular_diagonal_blocks=" << rec.singular_diagonal_blocks << " assembly_s=" << rec.assembly_s
+					<< " linear_s=" << rec.linear_s << '\n';
+			}
+		}
+		if (status != TFI_OK)
+			throw std::runtime_error(std::string(tfi_last_error(c.ctx)) + " at physical step " + std::to_string(step + 1));
+		if (opt.warm_start != TFI_WARM_PREVIOUS)
+			std::cout << "warm_start step=" << step + 1 << " used=" << rep.warm_start_used
+				<< " initial_ratio=" << rep.warm_start_initial_ratio << '\n';
+		c.Check(tfi_flow_commit_step(flow.flow), "commit step");
+		total_assembly += rep.assembly_s;
+		total_linear += rep.linear_s;
+		total_linear_iterations += rep.linear_iterations;
+		const int completed = step + 1;
+		if (o.output_every > 0 && completed % o.output_every == 0) {
+			c.Check(tfi_flow_get_state(flow.flow, state.data()), "state download");
+			const auto text = TimeIndexedPath(o.output, completed);
+			const auto vtu = VtuStepPath(o.output, completed);
+			WriteStateText(text, state);
+			WriteStateVtu(mesh_vtk, vtu, state, completed*dt);
+			vtu_snaps.emplace_back(completed*dt, vtu);
+			text_snaps.emplace_back(completed*dt, text);
+		}
+		if (!o.checkpoint.empty() && (completed == run_end || (o.checkpoint_every > 0 && completed % o.checkpoint_every == 0))) {
+			c.Check(tfi_flow_get_state(flow.flow, state.data()), "state download");
+			CheckpointMeta meta;
+			meta.nodes = db.nodes;
+			meta.completed_step = completed;
+			meta.physical_time = completed*dt;
+			meta.dt = dt;
+			meta.density = sys.density;
+			meta.viscosity = sys.viscosity;
+			WriteCheckpoint(o.checkpoint, meta, state);
+			std::cout << "checkpoint=" << o.checkpoint.string() << " completed_step=" << completed << '\n';
+		}
+	}
+	c.Check(tfi_flow_get_state(flow.flow, state.data()), "state download");
+	if (!o.output.empty()) {
+		WriteStateText(o.output, state);
+		WriteStateVtk(mesh_vtk, o.output.string() + ".vtk", state);
+		const double final_time = transient ? run_end*dt : 0.0;
+		WriteStateVtu(mesh_vtk, VtuFinalPath(o.output), state, final_time);
+		if (vtu_snaps.empty()) vtu_snaps.emplace_back(final_time, VtuFinalPath(o.output));
+		WritePvd(PvdPath(o.output), vtu_snaps);
+		if (!text_snaps.empty()) WriteSeries(SeriesPath(o.output), text_snaps);
+	}
+	long long peak = 0, live = 0;
+	tfi_allocation_stats(c.ctx, &peak, &live);
+	// record name and keys of the reference driver (parsed by its scripts)
+	std::cout << "navier_stokes_cuda nodes=" << db.nodes << " elements=" << db.elements() << " steps=" << physical_steps
+		<< " run_end_step=" << run_end << " preprocess_s=" << Seconds(total_start, preprocess_end)
+		<< " assembly_s=" << total_assembly << " linear_s=" << total_linear
+		<< " total_linear_iterations=" << total_linear_iterations << " state_l2=" << Norm(state, 0, 1)
+		<< " velocity_l2=" << std::sqrt(std::pow(Norm(state, 0, 4), 2) + std::pow(Norm(state, 1, 4), 2)
+			+ std::pow(Norm(state, 2, 4), 2))
+		<< " pressure_l2=" << Norm(state, 3, 4) << " gpu_used_gib=" << Gib(static_cast<double>(peak))
+		<< " backend=hip" << '\n';
+	return 0;
+}
+
+void PrintAllocations(std::ostream& out, tfi_ctx* ctx)
+{
+	long long peak = 0, live = 0;
+	if (ctx) tfi_allocation_stats(ctx, &peak, &live);
+	out << "cuda_allocations scope=project_device_buffers requested_peak_bytes=" << peak
+		<< " requested_live_bytes=" << live << '\n';
+}
+
+}  // namespace
+
+// case-check: the host half of navier-stokes without a GPU -- reads the
+// database, control mesh, inflow profile and configuration, resolves the
+// boundary data at t = dt (or steady) and prints a summary.  No reference
+// counterpart; it lets the input path be tested on machines without a GPU.
+int CaseCheck(int argc, char** argv)
+{
+	if (argc < 4 || argc > 6) throw std::runtime_error("usage: iga_hip case-check DATABASE.ntiga CASE_DIR [--system NAME]");
+	std::string system;
+	if (argc == 6) {
+		if (std::string(argv[4]) != "--system") throw std::runtime_error("unknown option: " + std::string(argv[4]));
+		system = argv[5];
+	} else if (argc == 5) {
+		throw std::runtime_error("usage: iga_hip case-check DATABASE.ntiga CASE_DIR [--system NAME]");
+	}
+	const fs::path case_dir(argv[3]);
+	const FlatDatabase db = ReadDatabase(argv[2]);
+	const auto labeled = ReadLabeledHexMesh((case_dir/"controlmesh.vtk").string(), db.nodes,
+		static_cast<std::uint64_t>(db.elements()));
+	const auto trace = WallTraceBasis(db, labeled);
+	const auto profile = ReadVelocity((case_dir/"initial_velocityfield.txt").string(), db.nodes);
+	const SimulationConfig config = ReadSimulationConfig((case_dir/"simulation_config.json").string());
+	const FlowSystem& sys = FindFlowSystem(config, system);
+	const bool transient = sys.time_integration == "backward_euler";
+	const double time = transient ? config.dt : 0.0;
+	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(transient ? MaterializeWaveforms(config, case_dir, time) : config,
+		sys, labeled.labels, profile);
+	ApplyWallTrace(bc, trace);
+	double vsum = 0.0, psum = 0.0;
+	long long vhash = 0, phash = 0;
+	for (std::size_t n = 0; n < db.nodes; ++n) {
+		if (bc.velocity_mask[n]) vhash += static_cast<long long>(n);
+		if (bc.pressure_mask[n]) phash += static_cast<long long>(n);
+		for (int k = 0; k < 3; ++k) vsum += bc.velocity[3*n + k];
+		psum += bc.pressure[n];
+	}
+	std::cout << std::setprecision(17) << "case_check nodes=" << db.nodes << " elements=" << db.elements()
+		<< " max_basis=" << db.maximum_basis << " version=" << db.version << " wall_trace=" << trace.size()
+		<< " velocity_nodes=" << bc.velocity_nodes << " pressure_nodes=" << bc.pressure_nodes
+		<< " velocity_mask_index_sum=" << vhash << " pressure_mask_index_sum=" << phash
+		<< " velocity_target_sum=" << vsum << " pressure_target_sum=" << psum << " time=" << time
+		<< " density=" << sys.density << " viscosity=" << sys.viscosity << " traction_labels=";
+	bool first = true;
+	for (const auto& t : bc.tractions) {
+		std::cout << (first ? "" : ",") << t.first << ':' << t.second;
+		first = false;
+	}
+	std::cout << '\n';
+	return 0;
+}
+
+int main(int argc, char** argv)
+{
+	std::unique_ptr<Ctx> ctx;
+	try {
+		if (argc < 2) throw std::runtime_error("usage: iga_hip device-info|mesh-check|navier-stokes|case-check ...");
+		const std::string command(argv[1]);
+		if (command == "case-check") return CaseCheck(argc, argv);
+		if (command == "navier-stokes") ParseOptions(argc, argv);  // usage errors before the device is touched
+		ctx = std::make_unique<Ctx>(0);
+		PrintDevice(*ctx);
+		if (command == "device-info") return 0;
+		if (command == "mesh-check") {
+			if (argc != 3) throw std::runtime_error("usage: iga_hip mesh-check DATABASE.ntiga");
+			return MeshCheck(*ctx, argv[2]);
+		}
+		int status = 0;
+		if (command == "navier-stokes") status = NavierStokes(*ctx, argc, argv);
+		else throw std::runtime_error("unknown command: " + command);
+		PrintAllocations(std::cout, ctx->ctx);
+		return status;
+	} catch (const std::exception& e) {
+		std::cerr << "iga_hip: " << e.what() << '\n';
+		PrintAllocations(std::cerr, ctx ? ctx->ctx : nullptr);
+		return 1;
+	}
+}
