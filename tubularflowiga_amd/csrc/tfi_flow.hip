@@ -334,7 +334,11 @@ int tfi_flow_begin_step(tfi_flow* f, const double* phase, int phase_count)
 
 int tfi_flow_commit_step(tfi_flow* f)
 {
-	return Guard(f->ctx, [&] { f->warm->Commit(f->state.data(), f->phase, f->phase_count); });
+	return Guard(f->ctx, [&] {
+		// history is kept for the predicting modes only; the network trains in learned mode
+		if (f->opt.warm_start != TFI_WARM_PREVIOUS)
+			f->warm->Commit(f->state.data(), f->phase, f->phase_count, f->opt.warm_start == TFI_WARM_LEARNED);
+	});
 }
 
 int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)

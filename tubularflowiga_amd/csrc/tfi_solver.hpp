@@ -53,7 +53,7 @@ public:
 	void Configure(const tfi_warm_options& options);
 	void Reset();
 	// Record the converged state of a finished step (and train when learned).
-	void Commit(const double* state, const double* phase, int phase_count);
+	void Commit(const double* state, const double* phase, int phase_count, bool train);
 	bool Ready(int mode) const;
 	// state <- predicted state of the next step (state holds u_n on entry).
 	void Predict(int mode, double* state, const double* phase, int phase_count);

@@ -55,7 +55,7 @@ void WarmStart::Reset()
 	impl_->head = 0;
 }
 
-void WarmStart::Commit(const double* state, const double* phase, int phase_count)
+void WarmStart::Commit(const double* state, const double* phase, int phase_count, bool train)
 {
 	Impl& I = *impl_;
 	const long long n = 4LL*I.mesh->nodes;
@@ -64,7 +64,7 @@ void WarmStart::Commit(const double* state, const double* phase, int phase_count
 	for (int i = 0; i < 8; ++i) I.phase_hist[I.head][i] = i < phase_count ? phase[i] : 0.0;
 	++I.count;
 	// learned mode: a new (u_{n-2}, u_{n-1}) -> u_n snapshot trains the network
-	if (I.count >= 3) {
+	if (train && I.count >= 3) {
 		if (!I.mlp) I.mlp = std::make_unique<Mlp>(I.mesh, I.opt);
 		I.mlp->Train(I.at(0), I.at(1), I.at(2), I.phase_hist[I.head]);
 	}
