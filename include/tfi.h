@@ -106,6 +106,10 @@ int tfi_linear_solve(tfi_mesh* mesh, const double* values, const double* b, doub
  * average milliseconds per launch over `repetitions` launches on a synthetic
  * matrix of the mesh pattern. */
 int tfi_bench_spmv(tfi_mesh* mesh, int precision, int repetitions, double* milliseconds);
+/* Calibration copy for HBM counters: y = x over `count` doubles (a plain
+ * grid-stride copy kernel), average milliseconds per launch.  No reference
+ * counterpart. */
+int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milliseconds);
 
 /* ------------------------------------------------------------------ flow solver */
 typedef struct {
@@ -121,8 +125,10 @@ typedef struct {
 	int precision;                    /* TFI_FP64 | TFI_MIXED */
 	int warm_start;                   /* TFI_WARM_* */
 	int verbose;
-	int amg_smoother;                 /* TFI_SMOOTH_* */
+	int amg_smoother;                 /* TFI_SMOOTH_* on the large levels */
 	int tangent;                      /* TFI_TANGENT_* */
+	int amg_coarse_nodes;             /* block rows of the coarsest dense level (0: 64) */
+	int amg_gs_min_rows;              /* levels below this many rows use block Jacobi (0: 8192) */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);
@@ -167,6 +173,13 @@ int tfi_flow_commit_step(tfi_flow* flow);
 /* Per-face volumetric flows of the current state for labels (outlets):
  * flows[i] = sum over faces with label labels[i] (IntegrateBoundaryFlow). */
 int tfi_flow_boundary_flows(tfi_flow* flow, int count, const int32_t* labels, double* flows);
+
+/* AMG hierarchy of the flow solver after its first setup: per level the block
+ * rows, block nonzeros and Gauss-Seidel colours (0 = Jacobi-smoothed or the
+ * dense coarsest level); *levels receives the level count.  No reference
+ * counterpart (the reference GPU solver has no multigrid). */
+int tfi_flow_amg_levels(tfi_flow* flow, int capacity, int* levels, int* rows, long long* block_nonzeros,
+	int* colors);
 
 /* Learned warm start (MLP on 27-control-point stencils). */
 typedef struct {

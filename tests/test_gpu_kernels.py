@@ -138,9 +138,10 @@ def test_linear_solve(dmesh, case, pc, precision):
     assert np.linalg.norm(x - ox) <= 1e-6 * np.linalg.norm(ox)
 
 
-def test_mlp_forward_backward_vs_oracle(ctx):
+@pytest.mark.parametrize("B", [200, 1500])   # 1500 rows: split-K weight gradients and column sums
+def test_mlp_forward_backward_vs_oracle(ctx, B):
     rng = np.random.default_rng(11)
-    H, B = 64, 200
+    H = 64
     x = rng.standard_normal((B, 320)).astype(np.float32)
     t = rng.standard_normal((B, 4)).astype(np.float32)
     w = [0.1 * rng.standard_normal((320, H)), 0.1 * rng.standard_normal(H), 0.1 * rng.standard_normal((H, H)),

@@ -36,7 +36,8 @@ class FlowOptions(ctypes.Structure):
                 ("maximum_newton", c_int), ("nonlinear_rtol", c_double), ("nonlinear_atol", c_double),
                 ("mass_rtol", c_double), ("linear_maximum_iterations", c_int), ("linear_rtol", c_double),
                 ("restart", c_int), ("preconditioner", c_int), ("precision", c_int), ("warm_start", c_int),
-                ("verbose", c_int), ("amg_smoother", c_int), ("tangent", c_int)]
+                ("verbose", c_int), ("amg_smoother", c_int), ("tangent", c_int), ("amg_coarse_nodes", c_int),
+                ("amg_gs_min_rows", c_int)]
 
 
 class NewtonRecord(ctypes.Structure):
@@ -92,6 +93,7 @@ SIGNATURES = {
     "tfi_linear_solve": (c_int, [c_void_p, _d, _d, _d, c_int, c_int, c_int, c_int, c_double, P(c_int),
                                  P(c_double)]),
     "tfi_bench_spmv": (c_int, [c_void_p, c_int, c_int, P(c_double)]),
+    "tfi_bench_copy": (c_int, [c_void_p, c_ll, c_int, P(c_double)]),
     "tfi_flow_default_options": (None, [P(FlowOptions)]),
     "tfi_flow_create": (c_int, [c_void_p, P(FlowOptions), P(c_void_p)]),
     "tfi_flow_destroy": (None, [c_void_p]),
@@ -104,6 +106,7 @@ SIGNATURES = {
     "tfi_flow_commit_step": (c_int, [c_void_p]),
     "tfi_flow_boundary_flows": (c_int, [c_void_p, c_int, _i32, _d]),
     "tfi_flow_warm_configure": (c_int, [c_void_p, P(WarmOptions)]),
+    "tfi_flow_amg_levels": (c_int, [c_void_p, c_int, P(c_int), P(c_int), P(c_ll), P(c_int)]),
     "tfi_flow_warm_stats": (c_int, [c_void_p, P(c_int), P(c_double), P(c_int), P(c_int)]),
     "tfi_timers_get": (c_int, [c_void_p, P(KernelTimers)]),
     "tfi_timers_enable": (c_int, [c_void_p, c_int]),

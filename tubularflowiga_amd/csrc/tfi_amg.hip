@@ -31,8 +31,7 @@ namespace tfi {
 namespace {
 
 constexpr double kStrength = 0.25;     // row-relative strength threshold
-constexpr int kCoarseMax = 192;        // nodes on the coarsest (direct) level
-constexpr int kMaxLevels = 12;
+constexpr int kMaxLevels = 16;
 constexpr double kOmegaScale = 1.33;   // block-Jacobi damping = kOmegaScale / lambda_max(D^-1 A)
 constexpr int kPre = 2, kPost = 2;      // Jacobi sweeps
 constexpr int kPowerIterations = 12;
@@ -202,17 +201,19 @@ __global__ void GalerkinKernel(long long cblocks, const long long* __restrict__ 
 	coarse[(size_t)K*16 + e] = s;
 }
 
-// Dense matrix of the coarsest level: D[(4i+a)*n + 4j+b] = A_ij[a][b]
-__global__ void DenseKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
-	const double* __restrict__ vals, double* __restrict__ dense)
+// Augmented [A | I] of the coarsest level (n = 4 rows, leading dimension 2n),
+// on a zero-filled buffer: aug[(4i+a)*2n + 4j+b] = A_ij[a][b], identity on the right.
+__global__ void DenseAugKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
+	const double* __restrict__ vals, double* __restrict__ aug)
 {
 	const int i = blockIdx.x;
-	const int n = 4*rows;
+	const int n = 4*rows, w = 2*n;
 	for (int k = ro[i]; k < ro[i + 1]; ++k)
 		for (int e = threadIdx.x; e < 16; e += blockDim.x) {
 			const int a = e >> 2, b = e & 3;
-			dense[(size_t)(4*i + a)*n + 4*cols[k] + b] = vals[(size_t)k*16 + e];
+			aug[(size_t)(4*i + a)*w + 4*cols[k] + b] = vals[(size_t)k*16 + e];
 		}
+	if (threadIdx.x < 4) aug[(size_t)(4*i + threadIdx.x)*w + n + 4*i + threadIdx.x] = 1.0;
 }
 
 // In-place Gauss-Jordan inverse with partial pivoting of an n x n matrix
@@ -261,12 +262,14 @@ __global__ void __launch_bounds__(1024) GaussJordanKernel(int n, double* __restr
 		__syncthreads();
 		for (int j = threadIdx.x; j < w; j += blockDim.x) aug[(size_t)p*w + j] *= inv;
 		__syncthreads();
-		for (int idx = threadIdx.x; idx < n*w; idx += blockDim.x) {
-			const int i = idx/w, j = idx - i*w;
+		// one wave per row, lanes along the row (row p is read-only here, and
+		// column p of row i is read before any lane of the wave writes row i)
+		const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+		for (int i = wave; i < n; i += waves) {
 			if (i == p) continue;
 			const double f = aug[(size_t)i*w + p];
-			if (j == p) continue;
-			aug[(size_t)i*w + j] -= f*aug[(size_t)p*w + j];
+			for (int j = lane; j < w; j += 64)
+				if (j != p) aug[(size_t)i*w + j] -= f*aug[(size_t)p*w + j];
 		}
 		__syncthreads();
 		for (int i = threadIdx.x; i < n; i += blockDim.x)
@@ -342,7 +345,8 @@ struct Level {
 	DevBuf<int> agg, aoff, amem;
 	DevBuf<long long> goff;       // Galerkin lists for the NEXT level's blocks
 	DevBuf<int> glist;
-	// multicolour Gauss-Seidel: rows grouped by colour
+	// multicolour Gauss-Seidel (gs) or damped block Jacobi; rows grouped by colour
+	bool gs = false;
 	std::vector<int> color_offsets;
 	DevBuf<int> color_rows;
 	double lambda_max = 1.0;
@@ -356,7 +360,7 @@ struct Amg::Impl {
 	tfi_mesh* mesh;
 	tfi_ctx* ctx;
 	int precision;
-	int smoother = 1;  // 0 damped block Jacobi, 1 multicolour symmetric block Gauss-Seidel
+	AmgOptions opt;
 	bool built = false;
 	std::vector<std::unique_ptr<Level>> L;
 	int nc = 0;  // coarsest dense size (4 * rows)
@@ -483,7 +487,7 @@ void Amg::Impl::Symbolic(const double* fine_values)
 		"amg download");
 	Sync(ctx);
 	L.push_back(std::move(l0));
-	while ((int)L.size() < kMaxLevels && L.back()->rows > kCoarseMax) {
+	while ((int)L.size() < kMaxLevels && L.back()->rows > opt.coarse_nodes) {
 		Level& f = *L.back();
 		int count = 0;
 		auto agg = Aggregate(f.rows, f.h_ro, f.h_cols, hv, kStrength, count);
@@ -568,10 +572,11 @@ void Amg::Impl::Symbolic(const double* fine_values)
 		c->vals.Download(hv.data(), hv.size());
 		L.push_back(std::move(c));
 	}
-	// per-level buffers; greedy node colouring of every smoothed level
+	// per-level buffers; greedy node colouring of every Gauss-Seidel level
 	for (size_t l = 0; l < L.size(); ++l) {
 		Level& v = *L[l];
-		if (l + 1 < L.size()) {
+		v.gs = l + 1 < L.size() && opt.smoother == TFI_SMOOTH_GAUSS_SEIDEL && v.rows >= opt.gs_min_rows;
+		if (v.gs) {
 			std::vector<int> col(v.rows, -1), mark;
 			int ncol = 0;
 			for (int i = 0; i < v.rows; ++i) {
@@ -646,10 +651,10 @@ void Amg::Impl::Numeric(const double* fine64, const float* fine32)
 			}
 		}
 	}
-	// lambda_max(D^-1 A) per smoothed level by power iteration (FP64)
-	if (smoother == 0)
-		for (size_t l = 0; l + 1 < L.size(); ++l) {
+	// lambda_max(D^-1 A) of every Jacobi-smoothed level by power iteration (FP64)
+	for (size_t l = 0; l + 1 < L.size(); ++l) {
 			Level& v = *L[l];
+			if (v.gs) continue;
 			const long long n = 4LL*v.rows;
 			DevBuf<double> a(ctx, n), b(ctx, n);
 			hipLaunchKernelGGL(HashVectorKernel, dim3(Grid(n)), dim3(256), 0, ctx->stream, n, a.data());
@@ -665,22 +670,12 @@ void Amg::Impl::Numeric(const double* fine64, const float* fine32)
 			}
 			v.lambda_max = lam > 0.0 ? lam : 1.0;
 		}
-	// coarsest dense inverse
+	// coarsest dense inverse, built and inverted on the device
 	Level& c = *L.back();
 	Fill(ctx, aug.data(), 0.0, (long long)nc*2*nc);
-	// dense A into the left half with leading dimension 2n: build n x n then spread
-	DevBuf<double> dense(ctx, (size_t)nc*nc);
-	Fill(ctx, dense.data(), 0.0, (long long)nc*nc);
-	hipLaunchKernelGGL(DenseKernel, dim3(c.rows), dim3(64), 0, ctx->stream, c.rows, c.ro, c.cols, c.vals64,
-		dense.data());
-	TFI_LAUNCH_CHECK("DenseKernel");
-	std::vector<double> h((size_t)nc*nc), a2((size_t)nc*2*nc, 0.0);
-	dense.Download(h.data(), h.size());
-	for (int i = 0; i < nc; ++i) {
-		std::copy(h.begin() + (size_t)i*nc, h.begin() + (size_t)(i + 1)*nc, a2.begin() + (size_t)i*2*nc);
-		a2[(size_t)i*2*nc + nc + i] = 1.0;
-	}
-	aug.Upload(a2.data(), a2.size());
+	hipLaunchKernelGGL(DenseAugKernel, dim3(c.rows), dim3(64), 0, ctx->stream, c.rows, c.ro, c.cols, c.vals64,
+		aug.data());
+	TFI_LAUNCH_CHECK("DenseAugKernel");
 	FillInt(ctx, status.data(), 0, 1);
 	hipLaunchKernelGGL(GaussJordanKernel, dim3(1), dim3(1024), 0, ctx->stream, nc, aug.data(), status.data());
 	TFI_LAUNCH_CHECK("GaussJordanKernel");
@@ -716,6 +711,7 @@ void Amg::Impl::Cycle(int l)
 		return;
 	}
 	const T omega = static_cast<T>(kOmegaScale/v.lambda_max);
+	const bool gs = v.gs;
 	const int ncol = static_cast<int>(v.color_offsets.size()) - 1;
 	auto gs_sweep = [&](bool forward) {
 		for (int cc = 0; cc < ncol; ++cc) {
@@ -727,7 +723,7 @@ void Amg::Impl::Cycle(int l)
 		}
 		TFI_LAUNCH_CHECK("GsColorKernel");
 	};
-	if (smoother == 1) {
+	if (gs) {
 		// symmetric multicolour Gauss-Seidel from x = 0
 		const long long n = 4LL*v.rows;
 		if (f32) Check(hipMemsetAsync(x, 0, n*sizeof(T), ctx->stream), "memset");
@@ -758,7 +754,7 @@ void Amg::Impl::Cycle(int l)
 	hipLaunchKernelGGL(ProlongKernel<T>, dim3(Grid(4LL*v.rows)), dim3(256), 0, ctx->stream, v.rows, v.agg.data(), xc,
 		x);
 	TFI_LAUNCH_CHECK("ProlongKernel");
-	if (smoother == 1) {
+	if (gs) {
 		gs_sweep(true);
 		gs_sweep(false);
 	} else {
@@ -781,12 +777,26 @@ void Amg::Impl::Cycle(int l)
 	}
 }
 
-Amg::Amg(tfi_mesh* mesh, int precision, int smoother) : impl_(std::make_unique<Impl>())
+Amg::Amg(tfi_mesh* mesh, int precision, const AmgOptions& options) : impl_(std::make_unique<Impl>())
 {
+	if (options.coarse_nodes < 1 || options.coarse_nodes > 256)
+		throw Error("AMG coarsest level must hold between 1 and 256 block rows");
 	impl_->mesh = mesh;
 	impl_->ctx = mesh->ctx;
 	impl_->precision = precision;
-	impl_->smoother = smoother;
+	impl_->opt = options;
+}
+
+void Amg::Describe(std::vector<int>& rows, std::vector<long long>& nnzb, std::vector<int>& colors) const
+{
+	rows.clear();
+	nnzb.clear();
+	colors.clear();
+	for (const auto& l : impl_->L) {
+		rows.push_back(l->rows);
+		nnzb.push_back(l->nnzb);
+		colors.push_back(l->gs ? static_cast<int>(l->color_offsets.size()) - 1 : 0);
+	}
 }
 
 Amg::~Amg() = default;

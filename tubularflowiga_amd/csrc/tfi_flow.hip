@@ -50,7 +50,7 @@ __global__ void ContinityPartialKernel(const double* __restrict__ rhs, int nodes
 
 }  // namespace
 
-LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart, int smoother)
+LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart, const AmgOptions& amg)
 	: mesh_(mesh), ctx_(mesh->ctx), pc_(preconditioner), precision_(precision),
 	  krylov_(mesh->ctx, 4LL*mesh->nodes, restart)
 {
@@ -62,7 +62,7 @@ LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, in
 		correction_.Allocate(ctx_, n);
 		residual_.Allocate(ctx_, n);
 	}
-	if (pc_ == TFI_PC_AMG) amg_ = std::make_unique<Amg>(mesh, precision_, smoother);
+	if (pc_ == TFI_PC_AMG) amg_ = std::make_unique<Amg>(mesh, precision_, amg);
 }
 
 LinearSolver::~LinearSolver() = default;
@@ -145,6 +145,7 @@ struct tfi_flow {
 	std::unique_ptr<WarmStart> warm;
 	double phase[8] = {};
 	int phase_count = 0;
+	double newton_contraction = 0.0;  // largest ||r_k|| / ||r_{k-1}||^2 of the last step
 };
 
 namespace {
@@ -230,6 +231,8 @@ void tfi_flow_default_options(tfi_flow_options* o)
 	o->verbose = 0;
 	o->amg_smoother = TFI_SMOOTH_GAUSS_SEIDEL;
 	o->tangent = TFI_TANGENT_EXACT;
+	o->amg_coarse_nodes = 0;
+	o->amg_gs_min_rows = 0;
 }
 
 int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** out)
@@ -260,8 +263,12 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 			Fill(ctx, b->data(), 0.0, b->size());
 		FillInt(ctx, f->vmask.data(), 0, mesh->nodes);
 		FillInt(ctx, f->pmask.data(), 0, mesh->nodes);
+		AmgOptions amg;
+		amg.smoother = options->amg_smoother;
+		if (options->amg_coarse_nodes > 0) amg.coarse_nodes = options->amg_coarse_nodes;
+		if (options->amg_gs_min_rows > 0) amg.gs_min_rows = options->amg_gs_min_rows;
 		f->solver = std::make_unique<LinearSolver>(mesh, options->preconditioner, options->precision,
-			options->restart, options->amg_smoother);
+			options->restart, amg);
 		f->warm = std::make_unique<WarmStart>(mesh);
 		Sync(ctx);
 		*out = f.release();
@@ -353,46 +360,74 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		const auto t_start = Clock::now();
 		const double scale = std::sqrt(static_cast<double>(n));
 		double initial = -1.0;
-		// learned / extrapolated initial state (accepted only if it lowers the residual)
-		if (f->opt.warm_start != TFI_WARM_PREVIOUS && f->warm->Ready(f->opt.warm_start)) {
-			AssembleNs(m, f->state.data(), f->previous.data(), params, nullptr, f->rhs.data());
+		// Residual (and tangent when jac is given) at the current state: traction
+		// load, continuity / boundary-flow measurement, Dirichlet rows.
+		auto assemble = [&](bool with_jacobian, Metrics* mt) {
+			AssembleNs(m, f->state.data(), f->previous.data(), params, with_jacobian ? f->jac.data() : nullptr,
+				f->rhs.data());
 			if (f->has_traction) Axpy(ctx, 1.0, f->traction.data(), f->rhs.data(), n);
+			if (mt) *mt = Measure(f);
 			ApplyDirichlet(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(), f->state.data(),
-				nullptr, f->rhs.data());
-			const double base = Norm2(ctx, f->rhs.data(), n);
+				with_jacobian ? f->jac.data() : nullptr, f->rhs.data());
+			return Norm2(ctx, f->rhs.data(), n);
+		};
+		// learned / extrapolated initial state, accepted only if it lowers the
+		// residual; the tangent assembled at an accepted guess is reused by the
+		// first Newton iteration
+		bool preassembled = false;
+		Metrics pre_mt;
+		double pre_norm = 0.0, pre_asm = 0.0;
+		if (f->opt.warm_start != TFI_WARM_PREVIOUS && f->warm->Ready(f->opt.warm_start)) {
+			const auto a0 = Clock::now();
+			const double base = assemble(false, nullptr);
 			Copy(ctx, f->state.data(), f->update.data(), n);  // keep the plain guess
 			f->warm->Predict(f->opt.warm_start, f->state.data(), f->phase, f->phase_count);
 			SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
 				f->state.data());
-			AssembleNs(m, f->state.data(), f->previous.data(), params, nullptr, f->rhs.data());
-			if (f->has_traction) Axpy(ctx, 1.0, f->traction.data(), f->rhs.data(), n);
-			ApplyDirichlet(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(), f->state.data(),
-				nullptr, f->rhs.data());
-			const double guessed = Norm2(ctx, f->rhs.data(), n);
+			const double guessed = assemble(true, &pre_mt);
+			pre_asm = std::chrono::duration<double>(Clock::now() - a0).count();
+			rep.assembly_s += pre_asm;
 			rep.warm_start_initial_ratio = base > 0.0 ? guessed/base : 0.0;
 			if (guessed < base) {
 				rep.warm_start_used = 1;
 				f->warm->Accepted(true);
 				initial = base;  // relative tolerance measured against the plain start
+				preassembled = true;
+				pre_norm = guessed;
 			} else {
 				Copy(ctx, f->update.data(), f->state.data(), n);
 				f->warm->Accepted(false);
 			}
 		}
 		bool converged = false;
+		double prev_norm = -1.0, contraction = 0.0;  // observed ||r_k|| / ||r_{k-1}||^2 this step
 		for (int it = 0; it < f->opt.maximum_newton; ++it) {
 			tfi_newton_record rec{};
 			const auto a0 = Clock::now();
-			AssembleNs(m, f->state.data(), f->previous.data(), params, f->jac.data(), f->rhs.data());
-			if (f->has_traction) Axpy(ctx, 1.0, f->traction.data(), f->rhs.data(), n);
-			const Metrics mt = Measure(f);
-			ApplyDirichlet(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(), f->state.data(),
-				f->jac.data(), f->rhs.data());
+			Metrics mt;
+			double norm = 0.0;
+			bool have_jacobian = true;
+			if (preassembled) {
+				mt = pre_mt;
+				norm = pre_norm;
+				preassembled = false;
+			} else {
+				// When quadratic convergence predicts this iterate converged, check
+				// it with a residual-only assembly first and build the tangent only
+				// if it did not.
+				const double c = std::max(contraction, f->newton_contraction);
+				const bool predict = prev_norm > 0.0 && initial > 0.0 && c > 0.0
+					&& c*prev_norm*prev_norm <= 0.25*std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*initial);
+				have_jacobian = !predict;
+				norm = assemble(have_jacobian, &mt);
+			}
+			const double tol_now = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*(initial < 0.0 ? norm : initial));
+			if (!have_jacobian && !(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol))
+				norm = assemble(true, &mt);  // prediction missed: full assembly at the same state
 			Sync(ctx);
 			const auto a1 = Clock::now();
-			rec.assembly_s = std::chrono::duration<double>(a1 - a0).count();
-			rep.assembly_s += rec.assembly_s;
-			const double norm = Norm2(ctx, f->rhs.data(), n);
+			rec.assembly_s = std::chrono::duration<double>(a1 - a0).count() + (it == 0 ? pre_asm : 0.0);
+			rep.assembly_s += std::chrono::duration<double>(a1 - a0).count();
 			rec.residual_l2 = norm;
 			rec.continuity_l2 = mt.continuity_l2;
 			rec.continuity_sum = mt.continuity_sum;
@@ -400,6 +435,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			rec.relative_mass_imbalance = mt.imbalance;
 			if (initial < 0.0) initial = norm;
 			if (it == 0) rep.initial_residual = norm;
+			if (prev_norm > 0.0) contraction = std::max(contraction, norm/(prev_norm*prev_norm));
 			const double tol = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*initial);
 			rep.tolerance = tol;
 			rep.final_residual = norm;
@@ -435,7 +471,9 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
 			if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
 			rep.newton_iterations = it + 1;
+			prev_norm = norm;
 		}
+		if (contraction > 0.0) f->newton_contraction = contraction;
 		Sync(ctx);
 		rep.converged = converged ? 1 : 0;
 		rep.total_s = std::chrono::duration<double>(Clock::now() - t_start).count();
@@ -465,6 +503,21 @@ int tfi_flow_boundary_flows(tfi_flow* f, int count, const int32_t* labels, doubl
 			for (int k = 0; k < faces; ++k)
 				if (f->face_label[k] == labels[i]) s += per[k];
 			flows[i] = s;
+		}
+	});
+}
+
+int tfi_flow_amg_levels(tfi_flow* f, int capacity, int* levels, int* rows, long long* block_nonzeros, int* colors)
+{
+	return Guard(f->ctx, [&] {
+		std::vector<int> r, c;
+		std::vector<long long> z;
+		if (f->solver->amg()) f->solver->amg()->Describe(r, z, c);
+		*levels = static_cast<int>(r.size());
+		for (int l = 0; l < static_cast<int>(r.size()) && l < capacity; ++l) {
+			rows[l] = r[l];
+			block_nonzeros[l] = z[l];
+			colors[l] = c[l];
 		}
 	});
 }
@@ -609,6 +662,24 @@ int tfi_linear_solve(tfi_mesh* m, const double* values, const double* b, double*
 		if (iterations) *iterations = r.iterations;
 		if (residual) *residual = r.residual;
 		if (!r.converged) throw Error("linear solve did not converge");
+	});
+}
+
+int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milliseconds)
+{
+	return Guard(ctx, [&] {
+		if (count <= 0 || repetitions <= 0) throw Error("copy benchmark needs a positive size and repetition count");
+		DevBuf<double> a(ctx, count), b(ctx, count);
+		Fill(ctx, a.data(), 1.0, count);
+		Copy(ctx, a.data(), b.data(), count);
+		Sync(ctx);
+		Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event");
+		for (int i = 0; i < repetitions; ++i) Copy(ctx, a.data(), b.data(), count);
+		Check(hipEventRecord(ctx->ev_end, ctx->stream), "event");
+		Check(hipEventSynchronize(ctx->ev_end), "event sync");
+		float ms = 0.0f;
+		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
+		*milliseconds = ms/repetitions;
 	});
 }
 

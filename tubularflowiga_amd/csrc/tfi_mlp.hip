@@ -32,6 +32,10 @@ struct GemmArgs {
 	const float* H;
 	long long ldh;
 	int mode;
+	// split-K (mode 0 only): blockIdx.z sums k in [z*kchunk, (z+1)*kchunk) into
+	// partial[z][M][N]; SplitReduceKernel then adds the slices in order.
+	int kchunk = 0;
+	float* partial = nullptr;
 };
 
 __global__ void __launch_bounds__(kT) GemmKernel(GemmArgs g)
@@ -40,14 +44,16 @@ __global__ void __launch_bounds__(kT) GemmKernel(GemmArgs g)
 	__shared__ float Bs[16][65];
 	const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
 	const int m0 = blockIdx.y*64, n0 = blockIdx.x*64;
+	const int kbeg = g.kchunk ? blockIdx.z*g.kchunk : 0;
+	const int kend = g.kchunk ? min(g.K, kbeg + g.kchunk) : g.K;
 	float acc[4][4] = {};
-	for (int k0 = 0; k0 < g.K; k0 += 16) {
+	for (int k0 = kbeg; k0 < kend; k0 += 16) {
 		for (int i = threadIdx.x; i < 16*64; i += kT) {
 			const int kk = i >> 6, mm = i & 63;
 			const int m = m0 + mm, k = k0 + kk;
-			As[kk][mm] = (m < g.M && k < g.K) ? g.A[m*g.sam + k*g.sak] : 0.0f;
+			As[kk][mm] = (m < g.M && k < kend) ? g.A[m*g.sam + k*g.sak] : 0.0f;
 			const int n = n0 + mm;
-			Bs[kk][mm] = (n < g.N && k < g.K) ? g.B[k*g.sbk + n*g.sbn] : 0.0f;
+			Bs[kk][mm] = (n < g.N && k < kend) ? g.B[k*g.sbk + n*g.sbn] : 0.0f;
 		}
 		__syncthreads();
 #pragma unroll
@@ -64,6 +70,15 @@ __global__ void __launch_bounds__(kT) GemmKernel(GemmArgs g)
 		}
 		__syncthreads();
 	}
+	if (g.kchunk) {
+		float* out = g.partial + (size_t)blockIdx.z*g.M*g.N;
+		for (int i = 0; i < 4; ++i)
+			for (int j = 0; j < 4; ++j) {
+				const int m = m0 + ty*4 + i, n = n0 + tx*4 + j;
+				if (m < g.M && n < g.N) out[(size_t)m*g.N + n] = acc[i][j];
+			}
+		return;
+	}
 	for (int i = 0; i < 4; ++i)
 		for (int j = 0; j < 4; ++j) {
 			const int m = m0 + ty*4 + i, n = n0 + tx*4 + j;
@@ -79,21 +94,81 @@ __global__ void __launch_bounds__(kT) GemmKernel(GemmArgs g)
 		}
 }
 
-void Gemm(tfi_ctx* ctx, const GemmArgs& g)
+// C[i] = sum_z partial[z][i] in z order (deterministic split-K reduction)
+__global__ void SplitReduceKernel(long long count, int splits, const float* __restrict__ partial,
+	float* __restrict__ C)
 {
+	const long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x;
+	if (i >= count) return;
+	float s = 0.0f;
+	for (int z = 0; z < splits; ++z) s += partial[(size_t)z*count + i];
+	C[i] = s;
+}
+
+constexpr int kSplitMax = 64;
+
+// Tall-K products (weight gradients, K = batch) split K over up to kSplitMax
+// workgroup slices so that the launch fills the chip; `scratch` holds
+// kSplitMax*M*N floats.  Other products run as one K pass.
+void Gemm(tfi_ctx* ctx, GemmArgs g, float* scratch = nullptr)
+{
+	const int tiles = ((g.N + 63)/64)*((g.M + 63)/64);
+	int splits = 1;
+	if (scratch && g.mode == 0 && tiles < 64 && g.K >= 512)
+		splits = std::min(kSplitMax, std::max(1, g.K/256));
+	if (splits > 1) {
+		g.kchunk = ((g.K + splits - 1)/splits + 15)/16*16;
+		splits = (g.K + g.kchunk - 1)/g.kchunk;
+		g.partial = scratch;
+		dim3 grid((g.N + 63)/64, (g.M + 63)/64, splits);
+		hipLaunchKernelGGL(GemmKernel, grid, dim3(kT), 0, ctx->stream, g);
+		TFI_LAUNCH_CHECK("GemmKernel");
+		const long long count = (long long)g.M*g.N;
+		hipLaunchKernelGGL(SplitReduceKernel, dim3((count + kT - 1)/kT), dim3(kT), 0, ctx->stream, count, splits,
+			scratch, g.C);
+		TFI_LAUNCH_CHECK("SplitReduceKernel");
+		return;
+	}
 	dim3 grid((g.N + 63)/64, (g.M + 63)/64);
 	hipLaunchKernelGGL(GemmKernel, grid, dim3(kT), 0, ctx->stream, g);
 	TFI_LAUNCH_CHECK("GemmKernel");
 }
 
-// db[n] = sum_m D[m][n]  (one thread per column, fixed order)
-__global__ void ColumnSumKernel(int M, int N, const float* __restrict__ D, float* __restrict__ out)
+// partial[z][n] = sum of rows [z*chunk, (z+1)*chunk) of D[:, n] (fixed order)
+__global__ void ColumnSumPartialKernel(int M, int N, int chunk, const float* __restrict__ D,
+	float* __restrict__ partial)
 {
 	const int n = blockIdx.x*blockDim.x + threadIdx.x;
 	if (n >= N) return;
+	const int m0 = blockIdx.y*chunk, m1 = min(M, m0 + chunk);
 	float s = 0.0f;
-	for (int m = 0; m < M; ++m) s += D[(long long)m*N + n];
-	out[n] = s;
+	for (int m = m0; m < m1; ++m) s += D[(long long)m*N + n];
+	partial[(size_t)blockIdx.y*N + n] = s;
+}
+
+// out[n] = sum_m D[m][n] in two fixed-order passes
+void ColumnSum(tfi_ctx* ctx, int M, int N, const float* D, float* out, float* scratch)
+{
+	const int splits = std::min(kSplitMax, std::max(1, M/64));
+	const int chunk = (M + splits - 1)/splits;
+	hipLaunchKernelGGL(ColumnSumPartialKernel, dim3((N + 63)/64, splits), dim3(64), 0, ctx->stream, M, N, chunk, D,
+		scratch);
+	TFI_LAUNCH_CHECK("ColumnSumPartialKernel");
+	hipLaunchKernelGGL(SplitReduceKernel, dim3((N + kT - 1)/kT), dim3(kT), 0, ctx->stream, (long long)N, splits,
+		scratch, out);
+	TFI_LAUNCH_CHECK("SplitReduceKernel");
+}
+
+// batch rows of one optimisation step: rows[b] = hash(seed, b) mod N
+__global__ void BatchRowsKernel(int batch, int nodes, unsigned long long seed, int* __restrict__ rows)
+{
+	const int b = blockIdx.x*blockDim.x + threadIdx.x;
+	if (b >= batch) return;
+	unsigned long long z = seed + (unsigned long long)b*0x9e3779b97f4a7c15ULL;
+	z = (z ^ (z >> 30))*0xbf58476d1ce4e5b9ULL;
+	z = (z ^ (z >> 27))*0x94d049bb133111ebULL;
+	z ^= z >> 31;
+	rows[b] = static_cast<int>(z%static_cast<unsigned long long>(nodes));
 }
 
 // delta3 = 2 (Y - T)/(B*4), partial loss per block
@@ -315,7 +390,7 @@ struct Mlp::Impl {
 	// parameters + Adam state: packed [w1 b1 w2 b2 w3 b3]
 	DevBuf<float> params, grads, adam_m, adam_v;
 	long long off[7] = {};
-	DevBuf<float> X, H1, H2, Y, T, D3, D2, D1, loss_partial;
+	DevBuf<float> X, H1, H2, Y, T, D3, D2, D1, loss_partial, split;
 	DevBuf<double> scales, scale_partial, phase;
 	DevBuf<short> Xb, W1t, W2t, W3t;
 	int adam_t = 0;
@@ -348,25 +423,27 @@ struct Mlp::Impl {
 		Gemm(ctx, {count, kOut, H, H2.data(), H, 1, P(4), kOut, 1, Y.data(), kOut, P(5), nullptr, 0, 2});
 	}
 
-	// gradients of the mean squared error for `count` rows (targets in T)
-	float Backward(int count, const float* x)
+	// gradients of the mean squared error for `count` rows (targets in T); the
+	// loss is read back only when asked for (it costs a stream synchronisation)
+	float Backward(int count, const float* x, bool want_loss = true)
 	{
 		const int lb = 64;
 		hipLaunchKernelGGL(LossKernel, dim3(lb), dim3(kT), 0, ctx->stream, count, Y.data(), T.data(), D3.data(),
 			loss_partial.data());
 		TFI_LAUNCH_CHECK("LossKernel");
+		float* sc = split.data();
 		// dW3 = H2^T D3 ; db3
-		Gemm(ctx, {H, kOut, count, H2.data(), 1, H, D3.data(), kOut, 1, G(4), kOut, nullptr, nullptr, 0, 0});
-		hipLaunchKernelGGL(ColumnSumKernel, dim3(1), dim3(64), 0, ctx->stream, count, kOut, D3.data(), G(5));
+		Gemm(ctx, {H, kOut, count, H2.data(), 1, H, D3.data(), kOut, 1, G(4), kOut, nullptr, nullptr, 0, 0}, sc);
+		ColumnSum(ctx, count, kOut, D3.data(), G(5), sc);
 		// D2 = (D3 W3^T) * (1 - H2^2)
 		Gemm(ctx, {count, H, kOut, D3.data(), kOut, 1, P(4), 1, kOut, D2.data(), H, nullptr, H2.data(), H, 3});
-		Gemm(ctx, {H, H, count, H1.data(), 1, H, D2.data(), H, 1, G(2), H, nullptr, nullptr, 0, 0});
-		hipLaunchKernelGGL(ColumnSumKernel, dim3((H + 63)/64), dim3(64), 0, ctx->stream, count, H, D2.data(), G(3));
+		Gemm(ctx, {H, H, count, H1.data(), 1, H, D2.data(), H, 1, G(2), H, nullptr, nullptr, 0, 0}, sc);
+		ColumnSum(ctx, count, H, D2.data(), G(3), sc);
 		// D1 = (D2 W2^T) * (1 - H1^2)
 		Gemm(ctx, {count, H, H, D2.data(), H, 1, P(2), 1, H, D1.data(), H, nullptr, H1.data(), H, 3});
-		Gemm(ctx, {kIn, H, count, x, 1, kIn, D1.data(), H, 1, G(0), H, nullptr, nullptr, 0, 0});
-		hipLaunchKernelGGL(ColumnSumKernel, dim3((H + 63)/64), dim3(64), 0, ctx->stream, count, H, D1.data(), G(1));
-		TFI_LAUNCH_CHECK("ColumnSumKernel");
+		Gemm(ctx, {kIn, H, count, x, 1, kIn, D1.data(), H, 1, G(0), H, nullptr, nullptr, 0, 0}, sc);
+		ColumnSum(ctx, count, H, D1.data(), G(1), sc);
+		if (!want_loss) return 0.0f;
 		std::vector<float> part(lb);
 		loss_partial.Download(part.data(), lb);
 		float l = 0.0f;
@@ -385,6 +462,7 @@ struct Mlp::Impl {
 		D2.Allocate(ctx, (size_t)rows_max*H);
 		D1.Allocate(ctx, (size_t)rows_max*H);
 		loss_partial.Allocate(ctx, 64);
+		split.Allocate(ctx, (size_t)kSplitMax*std::max<long long>((long long)kIn*H, (long long)H*H));
 	}
 
 	void Layout()
@@ -495,11 +573,11 @@ void Mlp::Train(const double* un1, const double* un, const double* um, const dou
 	I.Scales(un, um);
 	I.phase.Upload(phase, 8);
 	const int steps = std::max(1, I.opt.train_steps);
-	std::vector<int> idx(I.batch);
 	float loss = 0.0f;
 	for (int step = 0; step < steps; ++step) {
-		for (int b = 0; b < I.batch; ++b) idx[b] = static_cast<int>(SplitMix(I.rng)%static_cast<uint64_t>(N));
-		I.rows.Upload(idx.data(), I.batch);
+		hipLaunchKernelGGL(BatchRowsKernel, dim3((I.batch + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch, N,
+			static_cast<unsigned long long>(SplitMix(I.rng)), I.rows.data());
+		TFI_LAUNCH_CHECK("BatchRowsKernel");
 		const long long items = (long long)I.batch*kIn;
 		hipLaunchKernelGGL(FeatureKernel, dim3((items + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch, I.rows.data(),
 			I.stencil.data(), I.pos.data(), I.hscale.data(), un, um, I.scales.data(), I.phase.data(), I.X.data());
@@ -508,7 +586,7 @@ void Mlp::Train(const double* un1, const double* un, const double* um, const dou
 			I.rows.data(), un1, un, um, I.scales.data(), I.T.data());
 		TFI_LAUNCH_CHECK("TargetKernel");
 		I.Forward(I.batch, I.X.data());
-		loss = I.Backward(I.batch, I.X.data());
+		loss = I.Backward(I.batch, I.X.data(), step + 1 == steps);
 		++I.adam_t;
 		const float b1 = 0.9f, b2 = 0.999f;
 		const float c1 = 1.0f - std::pow(b1, static_cast<float>(I.adam_t));

@@ -6,9 +6,15 @@
 namespace tfi {
 
 // Node-block plain-aggregation AMG (tfi_amg.hip).
+struct AmgOptions {
+	int smoother = TFI_SMOOTH_GAUSS_SEIDEL;  // on levels with at least gs_min_rows rows
+	int coarse_nodes = 64;                   // block rows on the coarsest (dense direct) level
+	int gs_min_rows = 8192;                  // smaller levels use damped block Jacobi
+};
+
 class Amg {
 public:
-	Amg(tfi_mesh* mesh, int precision, int smoother);
+	Amg(tfi_mesh* mesh, int precision, const AmgOptions& options);
 	~Amg();
 	// Numeric setup for new fine values (symbolic hierarchy built on first call).
 	void Setup(const double* values64, const float* values32);
@@ -16,6 +22,8 @@ public:
 	void Apply(const double* v, double* z);
 	int singular_blocks() const { return singular_; }
 	int levels() const;
+	// rows, block nonzeros and Gauss-Seidel colours (0: Jacobi / direct) per level
+	void Describe(std::vector<int>& rows, std::vector<long long>& nnzb, std::vector<int>& colors) const;
 	struct Impl;
 
 private:
@@ -25,12 +33,14 @@ private:
 
 class LinearSolver {
 public:
-	LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart, int smoother = 1);
+	LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart,
+		const AmgOptions& amg = AmgOptions());
 	~LinearSolver();
 	void Setup(const double* values);
 	KrylovResult Solve(const double* b, double* x, int maxit, double rtol);
 	int singular_blocks() const { return singular_; }
 	int refinements() const { return refinements_; }
+	const Amg* amg() const { return amg_.get(); }
 
 private:
 	tfi_mesh* mesh_;

@@ -35,6 +35,12 @@ class Context:
         if status != 0:
             raise TfiError(self.lib.tfi_last_error(self.handle).decode())
 
+    def bench_copy(self, count, repetitions=5):
+        """Average ms of a device copy of `count` doubles (HBM counter calibration)."""
+        ms = ctypes.c_double()
+        self.check(self.lib.tfi_bench_copy(self.handle, int(count), repetitions, ctypes.byref(ms)))
+        return ms.value
+
     def device_info(self) -> str:
         buf = ctypes.create_string_buffer(512)
         self.check(self.lib.tfi_device_info(self.handle, buf, 512))
@@ -256,6 +262,16 @@ class FlowSolver:
     def configure_warm_start(self, hidden=128, train_steps=40, learning_rate=1e-3, min_snapshots=3):
         o = WarmOptions(hidden, 27, train_steps, learning_rate, min_snapshots)
         self.ctx.check(self.lib.tfi_flow_warm_configure(self.handle, ctypes.byref(o)))
+
+    def amg_levels(self):
+        """[(rows, block_nonzeros, gauss_seidel_colours)] of the AMG hierarchy (after the first solve)."""
+        cap = 16
+        n = ctypes.c_int()
+        rows = (ctypes.c_int * cap)()
+        nnzb = (ctypes.c_longlong * cap)()
+        cols = (ctypes.c_int * cap)()
+        self.ctx.check(self.lib.tfi_flow_amg_levels(self.handle, cap, ctypes.byref(n), rows, nnzb, cols))
+        return [(rows[i], nnzb[i], cols[i]) for i in range(min(n.value, cap))]
 
     def warm_stats(self):
         s, l, a, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
