@@ -125,6 +125,18 @@ def spmv_only(dm, workload, repetitions=20):
     return 0
 
 
+def assembly_only(dm, workload, repetitions=3):
+    """Device-timed assemblies of the workload (synthetic state): exact and
+    legacy tangent with residual, and the residual alone."""
+    out = {"workload": workload, "elements": int(dm.host.elements)}
+    for name, tangent, jac in (("exact", "exact", True), ("legacy", "legacy", True), ("residual", "exact", False)):
+        ms = dm.bench_assembly(tangent, jac, repetitions)
+        fl = dm.assembly_flops(tangent, jac)
+        out[name] = {"ms": ms, "algorithmic_flops": fl, "tflops": fl / (ms * 1e-3) / 1e12}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +147,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernel on the workload's pattern plus a 2 GiB copy (for counter passes)")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="only the element assembly (full legacy / exact, residual-only) on the workload")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
@@ -154,6 +168,8 @@ def main(argv=None):
     log(f"mesh upload + pattern + colouring + geometry: {time.perf_counter()-t0:.2f} s; {dm.stats}")
     if a.spmv_only:
         return spmv_only(dm, a.workload)
+    if a.assembly_only:
+        return assembly_only(dm, a.workload)
     bc = boundary.resolve_flow_boundaries(case)
     faces = boundary.boundary_faces(case.mesh)
     tol = {"rtol": 1e-8, "atol": 1e-12, "mass": 1e-6}   # the reference's HPC reference run settings

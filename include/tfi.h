@@ -32,7 +32,7 @@ typedef struct tfi_flow tfi_flow;
 enum { TFI_OK = 0, TFI_ERROR = 1 };
 enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
 enum { TFI_FP64 = 0, TFI_MIXED = 1 };
-enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2 };
+enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2, TFI_WARM_EXTRAPOLATE2 = 3 };
 enum { TFI_SMOOTH_JACOBI = 0, TFI_SMOOTH_GAUSS_SEIDEL = 1 };
 enum { TFI_TANGENT_LEGACY = 0, TFI_TANGENT_EXACT = 1 };
 
@@ -106,6 +106,10 @@ int tfi_linear_solve(tfi_mesh* mesh, const double* values, const double* b, doub
  * average milliseconds per launch over `repetitions` launches on a synthetic
  * matrix of the mesh pattern. */
 int tfi_bench_spmv(tfi_mesh* mesh, int precision, int repetitions, double* milliseconds);
+/* Device-resident assembly timing on a synthetic state (HIP events, average
+ * milliseconds per assembly, all colour launches included); with_jacobian = 0
+ * times the residual-only assembly.  No reference counterpart. */
+int tfi_bench_assembly(tfi_mesh* mesh, int tangent, int with_jacobian, int repetitions, double* milliseconds);
 /* Calibration copy for HBM counters: y = x over `count` doubles (a plain
  * grid-stride copy kernel), average milliseconds per launch.  No reference
  * counterpart. */

@@ -16,7 +16,7 @@ LIB_PATH = PKG / "libtfi_hip.so"
 
 TFI_PC_BJACOBI, TFI_PC_AMG = 0, 1
 TFI_FP64, TFI_MIXED = 0, 1
-TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE = 0, 1, 2
+TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2 = 0, 1, 2, 3
 TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL = 0, 1
 TFI_TANGENT_LEGACY, TFI_TANGENT_EXACT = 0, 1
 MAX_NEWTON_RECORDS = 64
@@ -94,6 +94,7 @@ SIGNATURES = {
                                  P(c_double)]),
     "tfi_bench_spmv": (c_int, [c_void_p, c_int, c_int, P(c_double)]),
     "tfi_bench_copy": (c_int, [c_void_p, c_ll, c_int, P(c_double)]),
+    "tfi_bench_assembly": (c_int, [c_void_p, c_int, c_int, c_int, P(c_double)]),
     "tfi_flow_default_options": (None, [P(FlowOptions)]),
     "tfi_flow_create": (c_int, [c_void_p, P(FlowOptions), P(c_void_p)]),
     "tfi_flow_destroy": (None, [c_void_p]),

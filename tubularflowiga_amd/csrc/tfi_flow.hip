@@ -665,6 +665,34 @@ int tfi_linear_solve(tfi_mesh* m, const double* values, const double* b, double*
 	});
 }
 
+int tfi_bench_assembly(tfi_mesh* m, int tangent, int with_jacobian, int repetitions, double* milliseconds)
+{
+	return Guard(m->ctx, [&] {
+		if (repetitions <= 0) throw Error("assembly benchmark needs a positive repetition count");
+		tfi_ctx* ctx = m->ctx;
+		const size_t n = 4ULL*m->nodes;
+		DevBuf<double> s(ctx, n), p(ctx, n), r(ctx, n), j;
+		std::vector<double> h(n);
+		for (size_t i = 0; i < n; ++i) h[i] = 0.01*std::sin(0.37*static_cast<double>(i)) + (i%4 == 2 ? 0.05 : 0.0);
+		s.Upload(h.data(), n);
+		p.Upload(h.data(), n);
+		if (with_jacobian) j.Allocate(ctx, m->nnzb*16);
+		const NsParams params{1060.0*4e-6, 3.5e-3, 1e-3, tangent};
+		auto run = [&] {
+			AssembleNs(m, s.data(), p.data(), params, with_jacobian ? j.data() : nullptr, r.data());
+		};
+		run();
+		Sync(ctx);
+		Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event");
+		for (int i = 0; i < repetitions; ++i) run();
+		Check(hipEventRecord(ctx->ev_end, ctx->stream), "event");
+		Check(hipEventSynchronize(ctx->ev_end), "event sync");
+		float ms = 0.0f;
+		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
+		*milliseconds = ms/repetitions;
+	});
+}
+
 int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milliseconds)
 {
 	return Guard(ctx, [&] {

@@ -2,6 +2,7 @@
 //   TFI_WARM_PREVIOUS    the step starts from u_n (the reference behaviour,
 //                        iga_cuda.cu:917-920);
 //   TFI_WARM_EXTRAPOLATE u_n + (u_n - u_{n-1});
+//   TFI_WARM_EXTRAPOLATE2 3 u_n - 3 u_{n-1} + u_{n-2} (quadratic in time);
 //   TFI_WARM_LEARNED     u_n + MLP(stencil features), see tfi_mlp.hip.
 // A prediction is only a guess: the Newton loop accepts it when it lowers the
 // nonlinear residual and always converges to the same tolerance.
@@ -17,6 +18,13 @@ __global__ void ExtrapolateKernel(long long n, const double* __restrict__ un, co
 {
 	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
 		out[i] = 2.0*un[i] - um[i];
+}
+
+__global__ void Extrapolate2Kernel(long long n, const double* __restrict__ un, const double* __restrict__ um,
+	const double* __restrict__ umm, double* __restrict__ out)
+{
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
+		out[i] = 3.0*(un[i] - um[i]) + umm[i];
 }
 
 }  // namespace
@@ -74,6 +82,7 @@ bool WarmStart::Ready(int mode) const
 {
 	const Impl& I = *impl_;
 	if (mode == TFI_WARM_EXTRAPOLATE) return I.count >= 2;
+	if (mode == TFI_WARM_EXTRAPOLATE2) return I.count >= 3;
 	if (mode == TFI_WARM_LEARNED) return I.count >= 2 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
 	return false;
 }
@@ -85,6 +94,10 @@ void WarmStart::Predict(int mode, double* state, const double* phase, int phase_
 	if (mode == TFI_WARM_EXTRAPOLATE) {
 		hipLaunchKernelGGL(ExtrapolateKernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), state);
 		TFI_LAUNCH_CHECK("ExtrapolateKernel");
+	} else if (mode == TFI_WARM_EXTRAPOLATE2) {
+		hipLaunchKernelGGL(Extrapolate2Kernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), I.at(2),
+			state);
+		TFI_LAUNCH_CHECK("Extrapolate2Kernel");
 	} else if (mode == TFI_WARM_LEARNED) {
 		double ph[8] = {};
 		for (int i = 0; i < 8 && i < phase_count; ++i) ph[i] = phase[i];

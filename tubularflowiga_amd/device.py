@@ -13,10 +13,12 @@ import numpy as np
 
 from . import _abi
 from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, WarmOptions, TFI_FP64, TFI_MIXED,
-                   TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_LEARNED, TFI_WARM_PREVIOUS)
+                   TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_LEARNED,
+                   TFI_WARM_PREVIOUS)
 
 __all__ = ["Context", "DeviceMesh", "FlowSolver", "TFI_FP64", "TFI_MIXED", "TFI_PC_AMG", "TFI_PC_BJACOBI",
-           "TFI_WARM_PREVIOUS", "TFI_WARM_LEARNED", "TFI_WARM_EXTRAPOLATE"]
+           "TFI_WARM_PREVIOUS", "TFI_WARM_LEARNED", "TFI_WARM_EXTRAPOLATE",
+           "TFI_WARM_EXTRAPOLATE2"]
 
 
 class TfiError(RuntimeError):
@@ -182,6 +184,18 @@ class DeviceMesh:
         ms = ctypes.c_double()
         self.ctx.check(self.lib.tfi_bench_spmv(self.handle, precision, repetitions, ctypes.byref(ms)))
         return ms.value
+
+    def bench_assembly(self, tangent="exact", with_jacobian=True, repetitions=3):
+        ms = ctypes.c_double()
+        self.ctx.check(self.lib.tfi_bench_assembly(self.handle, {"legacy": 0, "exact": 1}[tangent],
+                                                   1 if with_jacobian else 0, repetitions, ctypes.byref(ms)))
+        return ms.value
+
+    def assembly_flops(self, tangent="exact", with_jacobian=True):
+        """Algorithmic flops of one assembly (the library's per-element count)."""
+        nen = np.diff(self.host.element_offsets).astype(np.float64)
+        pair = (132.0 * nen * nen + 100.0 * nen if tangent == "exact" else 72.0 * nen * nen) if with_jacobian else 0.0
+        return float(np.sum(64.0 * (nen * (656.0 + 42.0 + 44.0 + 38.0) + 300.0 + pair)))
 
     def spmv_bytes(self, value_bytes=8):
         """Algorithmic bytes of one SpMV: nnzb (16 s_v + 4) + 4 (N+1) + 2 * 4N * 8 (x read once, y written)."""
