@@ -140,8 +140,8 @@ def assembly_only(dm, workload, repetitions=3):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=25)
     ap.add_argument("--workload", default="ybif_c1")
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")

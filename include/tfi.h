@@ -131,7 +131,7 @@ typedef struct {
 	int verbose;
 	int amg_smoother;                 /* TFI_SMOOTH_* on the large levels */
 	int tangent;                      /* TFI_TANGENT_* */
-	int amg_coarse_nodes;             /* block rows of the coarsest dense level (0: 64) */
+	int amg_coarse_nodes;             /* block rows of the coarsest dense level (0: 256, at most 256) */
 	int amg_gs_min_rows;              /* levels below this many rows use block Jacobi (0: 8192) */
 } tfi_flow_options;
 
@@ -211,7 +211,7 @@ int tfi_timers_reset(tfi_ctx* ctx);
  * No reference counterpart: kernel-level entry points of the learned warm
  * start, used by the parity tests against the CPU oracle.
  * tfi_mlp_reference: FP32 forward + backward (MSE loss) of the stencil MLP on a
- * host batch x [batch][320], target [batch][4]; weights w1 [320][H], b1 [H],
+ * host batch x [batch][416], target [batch][4]; weights w1 [416][H], b1 [H],
  * w2 [H][H], b2 [H], w3 [H][4], b3 [4] (row-major, x W + b, tanh hidden).
  * tfi_mlp_infer_bf16: the BF16-MFMA inference kernel on the same layout. */
 int tfi_mlp_reference(tfi_ctx* ctx, int hidden, int batch, const float* x, const float* target, const float* w1,

@@ -6,6 +6,8 @@ from oracle import mlp as MO
 from oracle import oracle as O
 from tubularflowiga_amd import boundary, casegen
 
+KIN = 416   # stencil-MLP input width (27 nodes x 15 features + 8 phase, padded)
+
 pytestmark = pytest.mark.gpu
 
 
@@ -142,9 +144,9 @@ def test_linear_solve(dmesh, case, pc, precision):
 def test_mlp_forward_backward_vs_oracle(ctx, B):
     rng = np.random.default_rng(11)
     H = 64
-    x = rng.standard_normal((B, 320)).astype(np.float32)
+    x = rng.standard_normal((B, KIN)).astype(np.float32)
     t = rng.standard_normal((B, 4)).astype(np.float32)
-    w = [0.1 * rng.standard_normal((320, H)), 0.1 * rng.standard_normal(H), 0.1 * rng.standard_normal((H, H)),
+    w = [0.1 * rng.standard_normal((KIN, H)), 0.1 * rng.standard_normal(H), 0.1 * rng.standard_normal((H, H)),
          0.1 * rng.standard_normal(H), 0.1 * rng.standard_normal((H, 4)), 0.1 * rng.standard_normal(4)]
     w = [v.astype(np.float32) for v in w]
     y, loss, grads = ctx.mlp_reference(x, t, w, H)
@@ -159,8 +161,8 @@ def test_mlp_bf16_inference_vs_oracle(ctx):
     rng = np.random.default_rng(12)
     for H in (64, 128):
         rows = 333
-        x = rng.standard_normal((rows, 320)).astype(np.float32)
-        w = [0.05 * rng.standard_normal((320, H)), 0.1 * rng.standard_normal(H),
+        x = rng.standard_normal((rows, KIN)).astype(np.float32)
+        w = [0.05 * rng.standard_normal((KIN, H)), 0.1 * rng.standard_normal(H),
              0.1 * rng.standard_normal((H, H)), 0.1 * rng.standard_normal(H),
              0.1 * rng.standard_normal((H, 4)), 0.1 * rng.standard_normal(4)]
         w = [v.astype(np.float32) for v in w]

@@ -23,6 +23,8 @@ from oracle import mlp as MO
 from oracle import oracle as O
 from tubularflowiga_amd import casegen
 
+KIN = 416   # stencil-MLP input width (27 nodes x 15 features + 8 phase, padded)
+
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.json")
 
 GAUSS4 = np.array([0.06943184420297371, 0.33000947820757187, 0.6699905217924281, 0.9305681557970262])
@@ -492,9 +494,9 @@ def test_krylov_solvers_match_direct_solve():
 def test_mlp_gradients_by_finite_difference():
     rng = np.random.default_rng(8)
     H, B = 8, 5
-    x = rng.standard_normal((B, 320))
+    x = rng.standard_normal((B, KIN))
     t = rng.standard_normal((B, 4))
-    w = [0.1 * rng.standard_normal((320, H)), 0.1 * rng.standard_normal(H), 0.3 * rng.standard_normal((H, H)),
+    w = [0.1 * rng.standard_normal((KIN, H)), 0.1 * rng.standard_normal(H), 0.3 * rng.standard_normal((H, H)),
          0.1 * rng.standard_normal(H), 0.3 * rng.standard_normal((H, 4)), 0.1 * rng.standard_normal(4)]
     _, loss, grads = MO.loss_and_grads(x, t, *w)
     h = 1e-6

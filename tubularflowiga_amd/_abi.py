@@ -136,12 +136,3 @@ def load(path: Path | str | None = None):
         fn.argtypes = args
     _LIB = lib
     return lib
-
-
-def gpu_available() -> bool:
-    """True when a HIP device is visible (does not initialise the runtime)."""
-    try:
-        import torch
-        return torch.cuda.device_count() > 0
-    except Exception:
-        return os.path.exists("/dev/kfd")

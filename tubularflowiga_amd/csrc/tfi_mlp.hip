@@ -208,67 +208,79 @@ __global__ void AdamKernel(int n, float* __restrict__ w, const float* __restrict
 // features of target nodes rows[r] (or r itself when rows == nullptr)
 __global__ void FeatureKernel(int count, const int* __restrict__ rows, const int* __restrict__ stencil,
 	const float* __restrict__ pos, const float* __restrict__ hscale, const double* __restrict__ un,
-	const double* __restrict__ um, const double* __restrict__ scales, const double* __restrict__ phase,
-	float* __restrict__ X)
+	const double* __restrict__ um, const double* __restrict__ umm, const double* __restrict__ scales,
+	const double* __restrict__ phase, float* __restrict__ X)
 {
 	const long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x;
 	if (t >= (long long)count*kIn) return;
 	const int r = static_cast<int>(t/kIn), c = static_cast<int>(t - (long long)r*kIn);
 	const int i = rows ? rows[r] : r;
 	float v = 0.0f;
-	if (c < kStencil*11) {
-		const int s = c/11, f = c - s*11;
+	if (c < kStencil*kPerNode) {
+		const int s = c/kPerNode, f = c - s*kPerNode;
 		const int j = stencil[(long long)i*kStencil + s];
-		if (f < 4) v = static_cast<float>(un[4LL*j + f]/scales[f]);
-		else if (f < 8) v = static_cast<float>((un[4LL*j + f - 4] - um[4LL*j + f - 4])/scales[f]);
-		else v = (pos[3LL*j + f - 8] - pos[3LL*i + f - 8])/hscale[i];
-	} else if (c < kStencil*11 + 8) {
-		v = static_cast<float>(phase[c - kStencil*11]);
+		if (f < 4) {
+			v = static_cast<float>(un[4LL*j + f]/scales[f]);
+		} else if (f < 8) {
+			const long long k = 4LL*j + f - 4;
+			v = static_cast<float>((un[k] - um[k])/scales[f]);
+		} else if (f < 12) {
+			const long long k = 4LL*j + f - 8;
+			v = static_cast<float>(((un[k] - um[k]) - (um[k] - umm[k]))/scales[f]);
+		} else {
+			v = (pos[3LL*j + f - 12] - pos[3LL*i + f - 12])/hscale[i];
+		}
+	} else if (c < kStencil*kPerNode + 8) {
+		v = static_cast<float>(phase[c - kStencil*kPerNode]);
 	}
 	X[t] = v;
 }
 
+// third difference in time, scaled: d3 = u_{n+1} - 3 u_n + 3 u_{n-1} - u_{n-2}
 __global__ void TargetKernel(int count, const int* __restrict__ rows, const double* __restrict__ un1,
-	const double* __restrict__ un, const double* __restrict__ um, const double* __restrict__ scales,
-	float* __restrict__ T)
+	const double* __restrict__ un, const double* __restrict__ um, const double* __restrict__ umm,
+	const double* __restrict__ scales, float* __restrict__ T)
 {
 	const int t = blockIdx.x*blockDim.x + threadIdx.x;
 	if (t >= count*kOut) return;
 	const int r = t >> 2, f = t & 3;
 	const long long k = 4LL*rows[r] + f;
-	T[t] = static_cast<float>(((un1[k] - un[k]) - (un[k] - um[k]))/scales[4 + f]);
+	T[t] = static_cast<float>((un1[k] - 3.0*(un[k] - um[k]) - umm[k])/scales[8 + f]);
 }
 
 __global__ void ApplyPredictionKernel(int nodes, const double* __restrict__ un, const double* __restrict__ um,
-	const float* __restrict__ Y, const double* __restrict__ scales, double* __restrict__ state)
+	const double* __restrict__ umm, const float* __restrict__ Y, const double* __restrict__ scales,
+	double* __restrict__ state)
 {
 	const int t = blockIdx.x*blockDim.x + threadIdx.x;
 	if (t >= nodes*4) return;
 	const int f = t & 3;
-	state[t] = un[t] + (un[t] - um[t]) + scales[4 + f]*static_cast<double>(Y[t]);
+	state[t] = 3.0*(un[t] - um[t]) + umm[t] + scales[8 + f]*static_cast<double>(Y[t]);
 }
 
-// per-field sums of squares of u_n and du_n (partials per block)
+// per-field sums of squares of u_n, d1_n and d2_n (partials per block)
 __global__ void ScalePartialKernel(int nodes, const double* __restrict__ un, const double* __restrict__ um,
-	double* __restrict__ partial)
+	const double* __restrict__ umm, double* __restrict__ partial)
 {
-	__shared__ double lds[8][kT];
-	double s[8] = {};
+	__shared__ double lds[12][kT];
+	double s[12] = {};
 	for (int n = blockIdx.x*blockDim.x + threadIdx.x; n < nodes; n += gridDim.x*blockDim.x)
 		for (int f = 0; f < 4; ++f) {
-			const double a = un[4LL*n + f], d = a - um[4LL*n + f];
+			const long long k = 4LL*n + f;
+			const double a = un[k], d1 = a - um[k], d2 = d1 - (um[k] - umm[k]);
 			s[f] += a*a;
-			s[4 + f] += d*d;
+			s[4 + f] += d1*d1;
+			s[8 + f] += d2*d2;
 		}
-	for (int f = 0; f < 8; ++f) lds[f][threadIdx.x] = s[f];
+	for (int f = 0; f < 12; ++f) lds[f][threadIdx.x] = s[f];
 	__syncthreads();
 	for (int o = kT/2; o > 0; o >>= 1) {
 		if (threadIdx.x < o)
-			for (int f = 0; f < 8; ++f) lds[f][threadIdx.x] += lds[f][threadIdx.x + o];
+			for (int f = 0; f < 12; ++f) lds[f][threadIdx.x] += lds[f][threadIdx.x + o];
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
-		for (int f = 0; f < 8; ++f) partial[8*blockIdx.x + f] = lds[f][0];
+		for (int f = 0; f < 12; ++f) partial[12*blockIdx.x + f] = lds[f][0];
 }
 
 // ------------------------------------------------------------------ BF16 MFMA inference
@@ -399,21 +411,21 @@ struct Mlp::Impl {
 	float* P(int i) { return params.data() + off[i]; }
 	float* G(int i) { return grads.data() + off[i]; }
 
-	void Scales(const double* un, const double* um)
+	void Scales(const double* un, const double* um, const double* umm)
 	{
 		const int blocks = 128;
-		hipLaunchKernelGGL(ScalePartialKernel, dim3(blocks), dim3(kT), 0, ctx->stream, mesh->nodes, un, um,
+		hipLaunchKernelGGL(ScalePartialKernel, dim3(blocks), dim3(kT), 0, ctx->stream, mesh->nodes, un, um, umm,
 			scale_partial.data());
 		TFI_LAUNCH_CHECK("ScalePartialKernel");
-		std::vector<double> part(8*blocks), s(8, 0.0);
+		std::vector<double> part(12*blocks), s(12, 0.0);
 		scale_partial.Download(part.data(), part.size());
 		for (int b = 0; b < blocks; ++b)
-			for (int f = 0; f < 8; ++f) s[f] += part[8*b + f];
-		for (int f = 0; f < 8; ++f) {
+			for (int f = 0; f < 12; ++f) s[f] += part[12*b + f];
+		for (int f = 0; f < 12; ++f) {
 			s[f] = std::sqrt(s[f]/std::max(1, mesh->nodes));
 			if (!(s[f] > 1e-300)) s[f] = 1.0;
 		}
-		scales.Upload(s.data(), 8);
+		scales.Upload(s.data(), 12);
 	}
 
 	void Forward(int count, const float* x)
@@ -558,19 +570,19 @@ Mlp::Mlp(tfi_mesh* mesh, const tfi_warm_options& options) : impl_(std::make_uniq
 	I.adam_m.Upload(z.data(), z.size());
 	I.adam_v.Upload(z.data(), z.size());
 	I.Allocate(I.batch);
-	I.scales.Allocate(I.ctx, 8);
-	I.scale_partial.Allocate(I.ctx, 8*128);
+	I.scales.Allocate(I.ctx, 12);
+	I.scale_partial.Allocate(I.ctx, 12*128);
 	I.phase.Allocate(I.ctx, 8);
 }
 
 Mlp::~Mlp() = default;
 
-void Mlp::Train(const double* un1, const double* un, const double* um, const double* phase)
+void Mlp::Train(const double* un1, const double* un, const double* um, const double* umm, const double* phase)
 {
 	Impl& I = *impl_;
 	tfi_ctx* ctx = I.ctx;
 	const int N = I.mesh->nodes;
-	I.Scales(un, um);
+	I.Scales(un, um, umm);
 	I.phase.Upload(phase, 8);
 	const int steps = std::max(1, I.opt.train_steps);
 	float loss = 0.0f;
@@ -580,10 +592,10 @@ void Mlp::Train(const double* un1, const double* un, const double* um, const dou
 		TFI_LAUNCH_CHECK("BatchRowsKernel");
 		const long long items = (long long)I.batch*kIn;
 		hipLaunchKernelGGL(FeatureKernel, dim3((items + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch, I.rows.data(),
-			I.stencil.data(), I.pos.data(), I.hscale.data(), un, um, I.scales.data(), I.phase.data(), I.X.data());
+			I.stencil.data(), I.pos.data(), I.hscale.data(), un, um, umm, I.scales.data(), I.phase.data(), I.X.data());
 		TFI_LAUNCH_CHECK("FeatureKernel");
 		hipLaunchKernelGGL(TargetKernel, dim3((I.batch*kOut + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch,
-			I.rows.data(), un1, un, um, I.scales.data(), I.T.data());
+			I.rows.data(), un1, un, um, umm, I.scales.data(), I.T.data());
 		TFI_LAUNCH_CHECK("TargetKernel");
 		I.Forward(I.batch, I.X.data());
 		loss = I.Backward(I.batch, I.X.data(), step + 1 == steps);
@@ -601,12 +613,12 @@ void Mlp::Train(const double* un1, const double* un, const double* um, const dou
 	++snapshots_;
 }
 
-void Mlp::Predict(const double* un, const double* um, const double* phase, double* state)
+void Mlp::Predict(const double* un, const double* um, const double* umm, const double* phase, double* state)
 {
 	Impl& I = *impl_;
 	tfi_ctx* ctx = I.ctx;
 	const int N = I.mesh->nodes;
-	I.Scales(un, um);
+	I.Scales(un, um, umm);
 	I.phase.Upload(phase, 8);
 	if (I.Xb.size() < (size_t)N*kIn) {
 		I.Xb.Allocate(ctx, (size_t)N*kIn);
@@ -617,7 +629,7 @@ void Mlp::Predict(const double* un, const double* um, const double* phase, doubl
 	DevBuf<float> X(ctx, (size_t)N*kIn), Y(ctx, (size_t)N*kOut);
 	const long long items = (long long)N*kIn;
 	hipLaunchKernelGGL(FeatureKernel, dim3((items + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, nullptr,
-		I.stencil.data(), I.pos.data(), I.hscale.data(), un, um, I.scales.data(), I.phase.data(), X.data());
+		I.stencil.data(), I.pos.data(), I.hscale.data(), un, um, umm, I.scales.data(), I.phase.data(), X.data());
 	TFI_LAUNCH_CHECK("FeatureKernel");
 	hipLaunchKernelGGL(ToBf16Kernel, dim3(4096), dim3(kT), 0, ctx->stream, items, X.data(), I.Xb.data());
 	hipLaunchKernelGGL(TransposeBf16Kernel, dim3((kIn*I.H + kT - 1)/kT), dim3(kT), 0, ctx->stream, kIn, I.H, I.P(0),
@@ -638,8 +650,8 @@ void Mlp::Predict(const double* un, const double* um, const double* phase, doubl
 		throw Error("BF16 inference kernels are built for widths 64 and 128");
 	}
 	TFI_LAUNCH_CHECK("MlpInferKernel");
-	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, un, um, Y.data(),
-		I.scales.data(), state);
+	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, un, um, umm,
+		Y.data(), I.scales.data(), state);
 	TFI_LAUNCH_CHECK("ApplyPredictionKernel");
 	Sync(ctx);
 }

@@ -5,25 +5,28 @@
 
 namespace tfi {
 
-// Feature layout per target node i (kIn = 320 columns, fp32):
+// Feature layout per target node i (kIn = 416 columns, fp32):
 //   for each of the 27 stencil nodes s (i itself, then its 26 nearest pattern
-//   neighbours by control-point distance): u_n(s)/su (4), du_n(s)/sd (4),
-//   (x_s - x_i)/h_i (3)  -> 297 columns;  then up to 8 phase features;  zeros.
-// Output (4): (du_{n+1}(i) - du_n(i))/sd, so the prediction is
-//   u_{n+1} ~ u_n + du_n + sd * MLP(x),  du_n = u_n - u_{n-1},
-// with su, sd the per-field RMS of u_n and du_n.
+//   neighbours by control-point distance): u_n(s)/su (4), d1_n(s)/s1 (4),
+//   d2_n(s)/s2 (4), (x_s - x_i)/h_i (3)  -> 405 columns;  then up to 8 phase
+//   features;  zeros.  d1_n = u_n - u_{n-1}, d2_n = d1_n - d1_{n-1}.
+// Output (4): d3_{n+1}(i)/s2 with d3_{n+1} = u_{n+1} - (3 u_n - 3 u_{n-1} + u_{n-2}),
+// so the prediction is the quadratic-in-time extrapolation plus a learned
+// correction, u_{n+1} ~ 3 u_n - 3 u_{n-1} + u_{n-2} + s2 * MLP(x),
+// with su, s1, s2 the per-field RMS of u_n, d1_n, d2_n.
 constexpr int kStencil = 27;
-constexpr int kIn = 320;
+constexpr int kPerNode = 15;
+constexpr int kIn = 416;
 constexpr int kOut = 4;
 
 class Mlp {
 public:
 	Mlp(tfi_mesh* mesh, const tfi_warm_options& options);
 	~Mlp();
-	// One snapshot: (u_{n-1}, u_n) -> u_{n+1} given as (un1 = u_{n+1}, un = u_n, um = u_{n-1}).
-	void Train(const double* un1, const double* un, const double* um, const double* phase);
-	// state <- u_n + du_n + sd * MLP(features(u_n, u_{n-1}, phase)).
-	void Predict(const double* un, const double* um, const double* phase, double* state);
+	// One snapshot: (u_{n-2}, u_{n-1}, u_n) -> u_{n+1}.
+	void Train(const double* un1, const double* un, const double* um, const double* umm, const double* phase);
+	// state <- 3 u_n - 3 u_{n-1} + u_{n-2} + s2 * MLP(features(u_n, u_{n-1}, u_{n-2}, phase)).
+	void Predict(const double* un, const double* um, const double* umm, const double* phase, double* state);
 	int snapshots() const { return snapshots_; }
 	double last_loss() const { return last_loss_; }
 

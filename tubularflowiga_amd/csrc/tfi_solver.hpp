@@ -8,7 +8,7 @@ namespace tfi {
 // Node-block plain-aggregation AMG (tfi_amg.hip).
 struct AmgOptions {
 	int smoother = TFI_SMOOTH_GAUSS_SEIDEL;  // on levels with at least gs_min_rows rows
-	int coarse_nodes = 64;                   // block rows on the coarsest (dense direct) level
+	int coarse_nodes = 256;                  // block rows on the coarsest (dense direct) level
 	int gs_min_rows = 8192;                  // smaller levels use damped block Jacobi
 };
 

@@ -3,7 +3,8 @@
 //                        iga_cuda.cu:917-920);
 //   TFI_WARM_EXTRAPOLATE u_n + (u_n - u_{n-1});
 //   TFI_WARM_EXTRAPOLATE2 3 u_n - 3 u_{n-1} + u_{n-2} (quadratic in time);
-//   TFI_WARM_LEARNED     u_n + MLP(stencil features), see tfi_mlp.hip.
+//   TFI_WARM_LEARNED     3 u_n - 3 u_{n-1} + u_{n-2} + MLP(stencil features),
+//                        see tfi_mlp.hip.
 // A prediction is only a guess: the Newton loop accepts it when it lowers the
 // nonlinear residual and always converges to the same tolerance.
 #include "tfi_solver.hpp"
@@ -32,14 +33,14 @@ __global__ void Extrapolate2Kernel(long long n, const double* __restrict__ un, c
 struct WarmStart::Impl {
 	tfi_mesh* mesh;
 	tfi_ctx* ctx;
-	DevBuf<double> hist[3];   // u_{n}, u_{n-1}, u_{n-2} (ring)
+	DevBuf<double> hist[4];   // u_{n}, u_{n-1}, u_{n-2}, u_{n-3} (ring)
 	int count = 0;
 	int head = 0;             // index of u_n
-	double phase_hist[3][8] = {};
+	double phase_hist[4][8] = {};
 	int accepted = 0, rejected = 0;
 	std::unique_ptr<Mlp> mlp;
 	tfi_warm_options opt{128, 27, 40, 1e-3, 3};
-	const double* at(int back) const { return hist[(head - back + 3)%3].data(); }
+	const double* at(int back) const { return hist[(head - back + 4)%4].data(); }
 };
 
 WarmStart::WarmStart(tfi_mesh* mesh) : impl_(std::make_unique<Impl>())
@@ -67,14 +68,14 @@ void WarmStart::Commit(const double* state, const double* phase, int phase_count
 {
 	Impl& I = *impl_;
 	const long long n = 4LL*I.mesh->nodes;
-	I.head = (I.head + 1)%3;
+	I.head = (I.head + 1)%4;
 	Copy(I.ctx, state, I.hist[I.head].data(), n);
 	for (int i = 0; i < 8; ++i) I.phase_hist[I.head][i] = i < phase_count ? phase[i] : 0.0;
 	++I.count;
-	// learned mode: a new (u_{n-2}, u_{n-1}) -> u_n snapshot trains the network
-	if (train && I.count >= 3) {
+	// learned mode: a new (u_{n-3}, u_{n-2}, u_{n-1}) -> u_n snapshot trains the network
+	if (train && I.count >= 4) {
 		if (!I.mlp) I.mlp = std::make_unique<Mlp>(I.mesh, I.opt);
-		I.mlp->Train(I.at(0), I.at(1), I.at(2), I.phase_hist[I.head]);
+		I.mlp->Train(I.at(0), I.at(1), I.at(2), I.at(3), I.phase_hist[I.head]);
 	}
 }
 
@@ -83,7 +84,7 @@ bool WarmStart::Ready(int mode) const
 	const Impl& I = *impl_;
 	if (mode == TFI_WARM_EXTRAPOLATE) return I.count >= 2;
 	if (mode == TFI_WARM_EXTRAPOLATE2) return I.count >= 3;
-	if (mode == TFI_WARM_LEARNED) return I.count >= 2 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
+	if (mode == TFI_WARM_LEARNED) return I.count >= 3 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
 	return false;
 }
 
@@ -101,7 +102,7 @@ void WarmStart::Predict(int mode, double* state, const double* phase, int phase_
 	} else if (mode == TFI_WARM_LEARNED) {
 		double ph[8] = {};
 		for (int i = 0; i < 8 && i < phase_count; ++i) ph[i] = phase[i];
-		I.mlp->Predict(I.at(0), I.at(1), ph, state);
+		I.mlp->Predict(I.at(0), I.at(1), I.at(2), ph, state);
 	}
 }
 
