@@ -6,7 +6,6 @@ fails loudly when the library has not been built.
 from __future__ import annotations
 
 import ctypes
-import os
 from pathlib import Path
 
 import numpy as np
