@@ -32,7 +32,8 @@ typedef struct tfi_flow tfi_flow;
 enum { TFI_OK = 0, TFI_ERROR = 1 };
 enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
 enum { TFI_FP64 = 0, TFI_MIXED = 1 };
-enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2, TFI_WARM_EXTRAPOLATE2 = 3 };
+enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2, TFI_WARM_EXTRAPOLATE2 = 3,
+	TFI_WARM_EXTRAPOLATE3 = 4 };
 enum { TFI_SMOOTH_JACOBI = 0, TFI_SMOOTH_GAUSS_SEIDEL = 1 };
 enum { TFI_TANGENT_LEGACY = 0, TFI_TANGENT_EXACT = 1 };
 
@@ -133,6 +134,7 @@ typedef struct {
 	int tangent;                      /* TFI_TANGENT_* */
 	int amg_coarse_nodes;             /* block rows of the coarsest dense level (0: 256, at most 256) */
 	int amg_gs_min_rows;              /* levels below this many rows use block Jacobi (0: 8192) */
+	int amg_gs_sweeps;                /* 2 (0): symmetric Gauss-Seidel pre and post; 1: forward pre, backward post */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);

@@ -15,7 +15,7 @@ LIB_PATH = PKG / "libtfi_hip.so"
 
 TFI_PC_BJACOBI, TFI_PC_AMG = 0, 1
 TFI_FP64, TFI_MIXED = 0, 1
-TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2 = 0, 1, 2, 3
+TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3 = 0, 1, 2, 3, 4
 TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL = 0, 1
 TFI_TANGENT_LEGACY, TFI_TANGENT_EXACT = 0, 1
 MAX_NEWTON_RECORDS = 64
@@ -36,7 +36,7 @@ class FlowOptions(ctypes.Structure):
                 ("mass_rtol", c_double), ("linear_maximum_iterations", c_int), ("linear_rtol", c_double),
                 ("restart", c_int), ("preconditioner", c_int), ("precision", c_int), ("warm_start", c_int),
                 ("verbose", c_int), ("amg_smoother", c_int), ("tangent", c_int), ("amg_coarse_nodes", c_int),
-                ("amg_gs_min_rows", c_int)]
+                ("amg_gs_min_rows", c_int), ("amg_gs_sweeps", c_int)]
 
 
 class NewtonRecord(ctypes.Structure):

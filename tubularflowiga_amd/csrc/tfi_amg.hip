@@ -216,66 +216,61 @@ __global__ void DenseAugKernel(int rows, const int* __restrict__ ro, const int* 
 	if (threadIdx.x < 4) aug[(size_t)(4*i + threadIdx.x)*w + n + 4*i + threadIdx.x] = 1.0;
 }
 
-// In-place Gauss-Jordan inverse with partial pivoting of an n x n matrix
-// (single workgroup; n <= 1024).  Augmented [A | I] stored as a (n x 2n).
-__global__ void __launch_bounds__(1024) GaussJordanKernel(int n, double* __restrict__ aug, int* __restrict__ status)
+// Gauss-Jordan step p, part 1 (one workgroup): partial pivot search in column
+// p over rows >= p (largest magnitude, lowest row on ties), row swap, scaling
+// of the pivot row, and the elimination multipliers m[i] = aug[i][p] (i != p).
+__global__ void __launch_bounds__(1024) PivotKernel(int n, int p, double* __restrict__ aug, double* __restrict__ mult,
+	int* __restrict__ status)
 {
 	__shared__ double sval[1024];
 	__shared__ int sidx[1024];
-	__shared__ int piv;
 	const int w = 2*n;
-	for (int p = 0; p < n; ++p) {
-		double best = -1.0;
-		int bi = p;
-		for (int i = p + threadIdx.x; i < n; i += blockDim.x) {
-			const double v = fabs(aug[(size_t)i*w + p]);
-			if (v > best) { best = v; bi = i; }
-		}
-		sval[threadIdx.x] = best;
-		sidx[threadIdx.x] = bi;
-		__syncthreads();
-		for (int s = blockDim.x/2; s > 0; s >>= 1) {
-			if (threadIdx.x < s) {
-				const double o = sval[threadIdx.x + s];
-				const int oi = sidx[threadIdx.x + s];
-				if (o > sval[threadIdx.x] || (o == sval[threadIdx.x] && oi < sidx[threadIdx.x])) {
-					sval[threadIdx.x] = o;
-					sidx[threadIdx.x] = oi;
-				}
+	double best = -1.0;
+	int bi = p;
+	for (int i = p + threadIdx.x; i < n; i += blockDim.x) {
+		const double v = fabs(aug[(size_t)i*w + p]);
+		if (v > best) { best = v; bi = i; }
+	}
+	sval[threadIdx.x] = best;
+	sidx[threadIdx.x] = bi;
+	__syncthreads();
+	for (int s = blockDim.x/2; s > 0; s >>= 1) {
+		if (threadIdx.x < s) {
+			const double o = sval[threadIdx.x + s];
+			const int oi = sidx[threadIdx.x + s];
+			if (o > sval[threadIdx.x] || (o == sval[threadIdx.x] && oi < sidx[threadIdx.x])) {
+				sval[threadIdx.x] = o;
+				sidx[threadIdx.x] = oi;
 			}
-			__syncthreads();
 		}
-		if (threadIdx.x == 0) {
-			piv = sidx[0];
-			if (!(sval[0] > 1e-300) || !isfinite(sval[0])) *status = 1;
-		}
-		__syncthreads();
-		const int q = piv;
-		if (q != p)
-			for (int j = threadIdx.x; j < w; j += blockDim.x) {
-				const double t = aug[(size_t)p*w + j];
-				aug[(size_t)p*w + j] = aug[(size_t)q*w + j];
-				aug[(size_t)q*w + j] = t;
-			}
-		__syncthreads();
-		const double inv = 1.0/aug[(size_t)p*w + p];
-		__syncthreads();
-		for (int j = threadIdx.x; j < w; j += blockDim.x) aug[(size_t)p*w + j] *= inv;
-		__syncthreads();
-		// one wave per row, lanes along the row (row p is read-only here, and
-		// column p of row i is read before any lane of the wave writes row i)
-		const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
-		for (int i = wave; i < n; i += waves) {
-			if (i == p) continue;
-			const double f = aug[(size_t)i*w + p];
-			for (int j = lane; j < w; j += 64)
-				if (j != p) aug[(size_t)i*w + j] -= f*aug[(size_t)p*w + j];
-		}
-		__syncthreads();
-		for (int i = threadIdx.x; i < n; i += blockDim.x)
-			if (i != p) aug[(size_t)i*w + p] = 0.0;
 		__syncthreads();
 	}
+	const int q = sidx[0];
+	if (threadIdx.x == 0 && (!(sval[0] > 1e-300) || !isfinite(sval[0]))) *status = 1;
+	if (q != p)
+		for (int j = threadIdx.x; j < w; j += blockDim.x) {
+			const double t = aug[(size_t)p*w + j];
+			aug[(size_t)p*w + j] = aug[(size_t)q*w + j];
+			aug[(size_t)q*w + j] = t;
+		}
+	__syncthreads();
+	const double piv = aug[(size_t)p*w + p];
+	const double inv = piv != 0.0 ? 1.0/piv : 0.0;
+	__syncthreads();
+	for (int j = threadIdx.x; j < w; j += blockDim.x) aug[(size_t)p*w + j] *= inv;
+	for (int i = threadIdx.x; i < n; i += blockDim.x) mult[i] = i == p ? 0.0 : aug[(size_t)i*w + p];
+}
+
+// Gauss-Jordan step p, part 2 (whole grid): aug[i][j] -= m[i] aug[p][j] for
+// i != p; column p of the other rows becomes exactly zero.
+__global__ void EliminateKernel(int n, int p, double* __restrict__ aug, const double* __restrict__ mult)
+{
+	const int w = 2*n;
+	const long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x;
+	if (t >= (long long)n*w) return;
+	const int i = static_cast<int>(t/w), j = static_cast<int>(t - (long long)i*w);
+	if (i == p) return;
+	aug[t] = j == p ? 0.0 : aug[t] - mult[i]*aug[(size_t)p*w + j];
 }
 
 // inverse (n x n) from the right half of the augmented matrix, in T
@@ -364,7 +359,7 @@ struct Amg::Impl {
 	bool built = false;
 	std::vector<std::unique_ptr<Level>> L;
 	int nc = 0;  // coarsest dense size (4 * rows)
-	DevBuf<double> aug, dinv64, v64tmp;
+	DevBuf<double> aug, mult, dinv64, v64tmp;
 	DevBuf<float> dinv32;
 	DevBuf<int> status;
 
@@ -618,6 +613,7 @@ void Amg::Impl::Symbolic(const double* fine_values)
 	nc = 4*L.back()->rows;
 	if (nc > 1024) throw Error("AMG coarsest level too large for the dense solve");
 	aug.Allocate(ctx, (size_t)nc*2*nc);
+	mult.Allocate(ctx, nc);
 	if (precision == TFI_MIXED) dinv32.Allocate(ctx, (size_t)nc*nc);
 	else dinv64.Allocate(ctx, (size_t)nc*nc);
 	status.Allocate(ctx, 1);
@@ -677,8 +673,16 @@ void Amg::Impl::Numeric(const double* fine64, const float* fine32)
 		aug.data());
 	TFI_LAUNCH_CHECK("DenseAugKernel");
 	FillInt(ctx, status.data(), 0, 1);
-	hipLaunchKernelGGL(GaussJordanKernel, dim3(1), dim3(1024), 0, ctx->stream, nc, aug.data(), status.data());
-	TFI_LAUNCH_CHECK("GaussJordanKernel");
+	// n pivot steps, each a pivot/scale launch and a grid-wide elimination
+	// launch (the single-workgroup form is bound by one CU's L2 bandwidth)
+	const long long cells = (long long)nc*2*nc;
+	for (int p = 0; p < nc; ++p) {
+		hipLaunchKernelGGL(PivotKernel, dim3(1), dim3(1024), 0, ctx->stream, nc, p, aug.data(), mult.data(),
+			status.data());
+		hipLaunchKernelGGL(EliminateKernel, dim3(Grid(cells)), dim3(256), 0, ctx->stream, nc, p, aug.data(),
+			mult.data());
+	}
+	TFI_LAUNCH_CHECK("EliminateKernel");
 	int st = 0;
 	status.Download(&st, 1);
 	if (st) throw Error("AMG coarsest operator is singular");
@@ -729,7 +733,7 @@ void Amg::Impl::Cycle(int l)
 		if (f32) Check(hipMemsetAsync(x, 0, n*sizeof(T), ctx->stream), "memset");
 		else Fill(ctx, reinterpret_cast<double*>(x), 0.0, n);
 		gs_sweep(true);
-		gs_sweep(false);
+		if (opt.gs_sweeps > 1) gs_sweep(false);
 	} else {
 		// damped block Jacobi from x = 0
 		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
@@ -755,7 +759,7 @@ void Amg::Impl::Cycle(int l)
 		x);
 	TFI_LAUNCH_CHECK("ProlongKernel");
 	if (gs) {
-		gs_sweep(true);
+		if (opt.gs_sweeps > 1) gs_sweep(true);
 		gs_sweep(false);
 	} else {
 		for (int s = 0; s < kPost; ++s) {

@@ -233,6 +233,7 @@ void tfi_flow_default_options(tfi_flow_options* o)
 	o->tangent = TFI_TANGENT_EXACT;
 	o->amg_coarse_nodes = 0;
 	o->amg_gs_min_rows = 0;
+	o->amg_gs_sweeps = 0;
 }
 
 int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** out)
@@ -267,6 +268,7 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 		amg.smoother = options->amg_smoother;
 		if (options->amg_coarse_nodes > 0) amg.coarse_nodes = options->amg_coarse_nodes;
 		if (options->amg_gs_min_rows > 0) amg.gs_min_rows = options->amg_gs_min_rows;
+		if (options->amg_gs_sweeps > 0) amg.gs_sweeps = options->amg_gs_sweeps;
 		f->solver = std::make_unique<LinearSolver>(mesh, options->preconditioner, options->precision,
 			options->restart, amg);
 		f->warm = std::make_unique<WarmStart>(mesh);

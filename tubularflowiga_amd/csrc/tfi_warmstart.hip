@@ -3,6 +3,7 @@
 //                        iga_cuda.cu:917-920);
 //   TFI_WARM_EXTRAPOLATE u_n + (u_n - u_{n-1});
 //   TFI_WARM_EXTRAPOLATE2 3 u_n - 3 u_{n-1} + u_{n-2} (quadratic in time);
+//   TFI_WARM_EXTRAPOLATE3 4 u_n - 6 u_{n-1} + 4 u_{n-2} - u_{n-3} (cubic in time);
 //   TFI_WARM_LEARNED     3 u_n - 3 u_{n-1} + u_{n-2} + MLP(stencil features),
 //                        see tfi_mlp.hip.
 // A prediction is only a guess: the Newton loop accepts it when it lowers the
@@ -26,6 +27,13 @@ __global__ void Extrapolate2Kernel(long long n, const double* __restrict__ un, c
 {
 	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
 		out[i] = 3.0*(un[i] - um[i]) + umm[i];
+}
+
+__global__ void Extrapolate3Kernel(long long n, const double* __restrict__ un, const double* __restrict__ um,
+	const double* __restrict__ umm, const double* __restrict__ um3, double* __restrict__ out)
+{
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
+		out[i] = 4.0*(un[i] + umm[i]) - 6.0*um[i] - um3[i];
 }
 
 }  // namespace
@@ -84,6 +92,7 @@ bool WarmStart::Ready(int mode) const
 	const Impl& I = *impl_;
 	if (mode == TFI_WARM_EXTRAPOLATE) return I.count >= 2;
 	if (mode == TFI_WARM_EXTRAPOLATE2) return I.count >= 3;
+	if (mode == TFI_WARM_EXTRAPOLATE3) return I.count >= 4;
 	if (mode == TFI_WARM_LEARNED) return I.count >= 3 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
 	return false;
 }
@@ -99,6 +108,10 @@ void WarmStart::Predict(int mode, double* state, const double* phase, int phase_
 		hipLaunchKernelGGL(Extrapolate2Kernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), I.at(2),
 			state);
 		TFI_LAUNCH_CHECK("Extrapolate2Kernel");
+	} else if (mode == TFI_WARM_EXTRAPOLATE3) {
+		hipLaunchKernelGGL(Extrapolate3Kernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), I.at(2),
+			I.at(3), state);
+		TFI_LAUNCH_CHECK("Extrapolate3Kernel");
 	} else if (mode == TFI_WARM_LEARNED) {
 		double ph[8] = {};
 		for (int i = 0; i < 8 && i < phase_count; ++i) ph[i] = phase[i];
