@@ -206,9 +206,35 @@ __global__ void AdamKernel(int n, float* __restrict__ w, const float* __restrict
 }
 
 // features of target nodes rows[r] (or r itself when rows == nullptr)
+// history u_{n-j}, j = 0..kOrder
+struct Hist {
+	const double* h[kOrder + 1];
+};
+
+// backward difference d_k at index t (d_0 = u_n)
+__device__ inline double Diff(const Hist& H, int k, long long t)
+{
+	double c = 1.0, s = 0.0;
+	for (int j = 0; j <= k; ++j) {
+		s += c*H.h[j][t];
+		c = -c*static_cast<double>(k - j)/static_cast<double>(j + 1);
+	}
+	return s;
+}
+
+// order-kOrder extrapolation in time: sum_{j=0..P} (-1)^j C(P+1, j+1) u_{n-j}
+__device__ inline double Extrapolate(const Hist& H, long long t)
+{
+	double c = static_cast<double>(kOrder + 1), s = 0.0;
+	for (int j = 0; j <= kOrder; ++j) {
+		s += c*H.h[j][t];
+		c = -c*static_cast<double>(kOrder - j)/static_cast<double>(j + 2);
+	}
+	return s;
+}
+
 __global__ void FeatureKernel(int count, const int* __restrict__ rows, const int* __restrict__ stencil,
-	const float* __restrict__ pos, const float* __restrict__ hscale, const double* __restrict__ un,
-	const double* __restrict__ um, const double* __restrict__ umm, const double* __restrict__ scales,
+	const float* __restrict__ pos, const float* __restrict__ hscale, Hist H, const double* __restrict__ scales,
 	const double* __restrict__ phase, float* __restrict__ X)
 {
 	const long long t = blockIdx.x*(long long)blockDim.x + threadIdx.x;
@@ -219,16 +245,12 @@ __global__ void FeatureKernel(int count, const int* __restrict__ rows, const int
 	if (c < kStencil*kPerNode) {
 		const int s = c/kPerNode, f = c - s*kPerNode;
 		const int j = stencil[(long long)i*kStencil + s];
-		if (f < 4) {
-			v = static_cast<float>(un[4LL*j + f]/scales[f]);
-		} else if (f < 8) {
-			const long long k = 4LL*j + f - 4;
-			v = static_cast<float>((un[k] - um[k])/scales[f]);
-		} else if (f < 12) {
-			const long long k = 4LL*j + f - 8;
-			v = static_cast<float>(((un[k] - um[k]) - (um[k] - umm[k]))/scales[f]);
+		if (f < 4*(kOrder + 1)) {
+			const int k = f >> 2, field = f & 3;
+			v = static_cast<float>(Diff(H, k, 4LL*j + field)/scales[f]);
 		} else {
-			v = (pos[3LL*j + f - 12] - pos[3LL*i + f - 12])/hscale[i];
+			const int d = f - 4*(kOrder + 1);
+			v = (pos[3LL*j + d] - pos[3LL*i + d])/hscale[i];
 		}
 	} else if (c < kStencil*kPerNode + 8) {
 		v = static_cast<float>(phase[c - kStencil*kPerNode]);
@@ -236,51 +258,48 @@ __global__ void FeatureKernel(int count, const int* __restrict__ rows, const int
 	X[t] = v;
 }
 
-// third difference in time, scaled: d3 = u_{n+1} - 3 u_n + 3 u_{n-1} - u_{n-2}
-__global__ void TargetKernel(int count, const int* __restrict__ rows, const double* __restrict__ un1,
-	const double* __restrict__ un, const double* __restrict__ um, const double* __restrict__ umm,
+// extrapolation error of the next state, scaled by s_P
+__global__ void TargetKernel(int count, const int* __restrict__ rows, const double* __restrict__ next, Hist H,
 	const double* __restrict__ scales, float* __restrict__ T)
 {
 	const int t = blockIdx.x*blockDim.x + threadIdx.x;
 	if (t >= count*kOut) return;
 	const int r = t >> 2, f = t & 3;
 	const long long k = 4LL*rows[r] + f;
-	T[t] = static_cast<float>((un1[k] - 3.0*(un[k] - um[k]) - umm[k])/scales[8 + f]);
+	T[t] = static_cast<float>((next[k] - Extrapolate(H, k))/scales[4*kOrder + f]);
 }
 
-__global__ void ApplyPredictionKernel(int nodes, const double* __restrict__ un, const double* __restrict__ um,
-	const double* __restrict__ umm, const float* __restrict__ Y, const double* __restrict__ scales,
+__global__ void ApplyPredictionKernel(int nodes, Hist H, const float* __restrict__ Y, const double* __restrict__ scales,
 	double* __restrict__ state)
 {
 	const int t = blockIdx.x*blockDim.x + threadIdx.x;
 	if (t >= nodes*4) return;
 	const int f = t & 3;
-	state[t] = 3.0*(un[t] - um[t]) + umm[t] + scales[8 + f]*static_cast<double>(Y[t]);
+	state[t] = Extrapolate(H, t) + scales[4*kOrder + f]*static_cast<double>(Y[t]);
 }
 
-// per-field sums of squares of u_n, d1_n and d2_n (partials per block)
-__global__ void ScalePartialKernel(int nodes, const double* __restrict__ un, const double* __restrict__ um,
-	const double* __restrict__ umm, double* __restrict__ partial)
+constexpr int kScales = 4*(kOrder + 1);
+
+// per-field sums of squares of u_n and d_1..d_P (partials per block)
+__global__ void ScalePartialKernel(int nodes, Hist H, double* __restrict__ partial)
 {
-	__shared__ double lds[12][kT];
-	double s[12] = {};
+	__shared__ double lds[kScales][kT];
+	double s[kScales] = {};
 	for (int n = blockIdx.x*blockDim.x + threadIdx.x; n < nodes; n += gridDim.x*blockDim.x)
-		for (int f = 0; f < 4; ++f) {
-			const long long k = 4LL*n + f;
-			const double a = un[k], d1 = a - um[k], d2 = d1 - (um[k] - umm[k]);
-			s[f] += a*a;
-			s[4 + f] += d1*d1;
-			s[8 + f] += d2*d2;
-		}
-	for (int f = 0; f < 12; ++f) lds[f][threadIdx.x] = s[f];
+		for (int f = 0; f < 4; ++f)
+			for (int k = 0; k <= kOrder; ++k) {
+				const double d = Diff(H, k, 4LL*n + f);
+				s[4*k + f] += d*d;
+			}
+	for (int f = 0; f < kScales; ++f) lds[f][threadIdx.x] = s[f];
 	__syncthreads();
 	for (int o = kT/2; o > 0; o >>= 1) {
 		if (threadIdx.x < o)
-			for (int f = 0; f < 12; ++f) lds[f][threadIdx.x] += lds[f][threadIdx.x + o];
+			for (int f = 0; f < kScales; ++f) lds[f][threadIdx.x] += lds[f][threadIdx.x + o];
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
-		for (int f = 0; f < 12; ++f) partial[12*blockIdx.x + f] = lds[f][0];
+		for (int f = 0; f < kScales; ++f) partial[kScales*blockIdx.x + f] = lds[f][0];
 }
 
 // ------------------------------------------------------------------ BF16 MFMA inference
@@ -411,21 +430,21 @@ struct Mlp::Impl {
 	float* P(int i) { return params.data() + off[i]; }
 	float* G(int i) { return grads.data() + off[i]; }
 
-	void Scales(const double* un, const double* um, const double* umm)
+	void Scales(const Hist& H)
 	{
 		const int blocks = 128;
-		hipLaunchKernelGGL(ScalePartialKernel, dim3(blocks), dim3(kT), 0, ctx->stream, mesh->nodes, un, um, umm,
+		hipLaunchKernelGGL(ScalePartialKernel, dim3(blocks), dim3(kT), 0, ctx->stream, mesh->nodes, H,
 			scale_partial.data());
 		TFI_LAUNCH_CHECK("ScalePartialKernel");
-		std::vector<double> part(12*blocks), s(12, 0.0);
+		std::vector<double> part(kScales*blocks), s(kScales, 0.0);
 		scale_partial.Download(part.data(), part.size());
 		for (int b = 0; b < blocks; ++b)
-			for (int f = 0; f < 12; ++f) s[f] += part[12*b + f];
-		for (int f = 0; f < 12; ++f) {
+			for (int f = 0; f < kScales; ++f) s[f] += part[kScales*b + f];
+		for (int f = 0; f < kScales; ++f) {
 			s[f] = std::sqrt(s[f]/std::max(1, mesh->nodes));
 			if (!(s[f] > 1e-300)) s[f] = 1.0;
 		}
-		scales.Upload(s.data(), 12);
+		scales.Upload(s.data(), kScales);
 	}
 
 	void Forward(int count, const float* x)
@@ -570,19 +589,21 @@ Mlp::Mlp(tfi_mesh* mesh, const tfi_warm_options& options) : impl_(std::make_uniq
 	I.adam_m.Upload(z.data(), z.size());
 	I.adam_v.Upload(z.data(), z.size());
 	I.Allocate(I.batch);
-	I.scales.Allocate(I.ctx, 12);
-	I.scale_partial.Allocate(I.ctx, 12*128);
+	I.scales.Allocate(I.ctx, kScales);
+	I.scale_partial.Allocate(I.ctx, kScales*128);
 	I.phase.Allocate(I.ctx, 8);
 }
 
 Mlp::~Mlp() = default;
 
-void Mlp::Train(const double* un1, const double* un, const double* um, const double* umm, const double* phase)
+void Mlp::Train(const double* next, const double* const* history, const double* phase)
 {
 	Impl& I = *impl_;
 	tfi_ctx* ctx = I.ctx;
 	const int N = I.mesh->nodes;
-	I.Scales(un, um, umm);
+	Hist H;
+	for (int j = 0; j <= kOrder; ++j) H.h[j] = history[j];
+	I.Scales(H);
 	I.phase.Upload(phase, 8);
 	const int steps = std::max(1, I.opt.train_steps);
 	float loss = 0.0f;
@@ -592,10 +613,10 @@ void Mlp::Train(const double* un1, const double* un, const double* um, const dou
 		TFI_LAUNCH_CHECK("BatchRowsKernel");
 		const long long items = (long long)I.batch*kIn;
 		hipLaunchKernelGGL(FeatureKernel, dim3((items + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch, I.rows.data(),
-			I.stencil.data(), I.pos.data(), I.hscale.data(), un, um, umm, I.scales.data(), I.phase.data(), I.X.data());
+			I.stencil.data(), I.pos.data(), I.hscale.data(), H, I.scales.data(), I.phase.data(), I.X.data());
 		TFI_LAUNCH_CHECK("FeatureKernel");
 		hipLaunchKernelGGL(TargetKernel, dim3((I.batch*kOut + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch,
-			I.rows.data(), un1, un, um, umm, I.scales.data(), I.T.data());
+			I.rows.data(), next, H, I.scales.data(), I.T.data());
 		TFI_LAUNCH_CHECK("TargetKernel");
 		I.Forward(I.batch, I.X.data());
 		loss = I.Backward(I.batch, I.X.data(), step + 1 == steps);
@@ -613,12 +634,14 @@ void Mlp::Train(const double* un1, const double* un, const double* um, const dou
 	++snapshots_;
 }
 
-void Mlp::Predict(const double* un, const double* um, const double* umm, const double* phase, double* state)
+void Mlp::Predict(const double* const* history, const double* phase, double* state)
 {
 	Impl& I = *impl_;
 	tfi_ctx* ctx = I.ctx;
 	const int N = I.mesh->nodes;
-	I.Scales(un, um, umm);
+	Hist H;
+	for (int j = 0; j <= kOrder; ++j) H.h[j] = history[j];
+	I.Scales(H);
 	I.phase.Upload(phase, 8);
 	if (I.Xb.size() < (size_t)N*kIn) {
 		I.Xb.Allocate(ctx, (size_t)N*kIn);
@@ -629,7 +652,7 @@ void Mlp::Predict(const double* un, const double* um, const double* umm, const d
 	DevBuf<float> X(ctx, (size_t)N*kIn), Y(ctx, (size_t)N*kOut);
 	const long long items = (long long)N*kIn;
 	hipLaunchKernelGGL(FeatureKernel, dim3((items + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, nullptr,
-		I.stencil.data(), I.pos.data(), I.hscale.data(), un, um, umm, I.scales.data(), I.phase.data(), X.data());
+		I.stencil.data(), I.pos.data(), I.hscale.data(), H, I.scales.data(), I.phase.data(), X.data());
 	TFI_LAUNCH_CHECK("FeatureKernel");
 	hipLaunchKernelGGL(ToBf16Kernel, dim3(4096), dim3(kT), 0, ctx->stream, items, X.data(), I.Xb.data());
 	hipLaunchKernelGGL(TransposeBf16Kernel, dim3((kIn*I.H + kT - 1)/kT), dim3(kT), 0, ctx->stream, kIn, I.H, I.P(0),
@@ -650,8 +673,8 @@ void Mlp::Predict(const double* un, const double* um, const double* umm, const d
 		throw Error("BF16 inference kernels are built for widths 64 and 128");
 	}
 	TFI_LAUNCH_CHECK("MlpInferKernel");
-	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, un, um, umm,
-		Y.data(), I.scales.data(), state);
+	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, H, Y.data(),
+		I.scales.data(), state);
 	TFI_LAUNCH_CHECK("ApplyPredictionKernel");
 	Sync(ctx);
 }

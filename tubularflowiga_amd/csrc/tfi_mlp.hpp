@@ -5,28 +5,30 @@
 
 namespace tfi {
 
-// Feature layout per target node i (kIn = 416 columns, fp32):
-//   for each of the 27 stencil nodes s (i itself, then its 26 nearest pattern
-//   neighbours by control-point distance): u_n(s)/su (4), d1_n(s)/s1 (4),
-//   d2_n(s)/s2 (4), (x_s - x_i)/h_i (3)  -> 405 columns;  then up to 8 phase
-//   features;  zeros.  d1_n = u_n - u_{n-1}, d2_n = d1_n - d1_{n-1}.
-// Output (4): d3_{n+1}(i)/s2 with d3_{n+1} = u_{n+1} - (3 u_n - 3 u_{n-1} + u_{n-2}),
-// so the prediction is the quadratic-in-time extrapolation plus a learned
-// correction, u_{n+1} ~ 3 u_n - 3 u_{n-1} + u_{n-2} + s2 * MLP(x),
-// with su, s1, s2 the per-field RMS of u_n, d1_n, d2_n.
+// Feature layout per target node i (kIn columns, fp32), for history order
+// P = kOrder: for each of the 27 stencil nodes s (i itself, then its 26 nearest
+// pattern neighbours by control-point distance): u_n(s)/s0 (4), then the
+// backward differences d_k(s)/s_k for k = 1..P (4 each), then
+// (x_s - x_i)/h_i (3); then up to 8 phase features; zeros to kIn.
+//   d_k = sum_j (-1)^j C(k, j) u_{n-j}   (d1 = u_n - u_{n-1}, ...)
+// Output (4): d_{P+1} of the next state scaled by s_P, i.e. the error of the
+// order-P polynomial extrapolation in time, so the prediction is
+//   u_{n+1} ~ sum_{j=0..P} (-1)^j C(P+1, j+1) u_{n-j} + s_P * MLP(x),
+// with s_0..s_P the per-field RMS of u_n, d_1, .., d_P.
+constexpr int kOrder = 3;
 constexpr int kStencil = 27;
-constexpr int kPerNode = 15;
-constexpr int kIn = 416;
+constexpr int kPerNode = 4*(kOrder + 1) + 3;
+constexpr int kIn = (kStencil*kPerNode + 8 + 15)/16*16;
 constexpr int kOut = 4;
 
 class Mlp {
 public:
 	Mlp(tfi_mesh* mesh, const tfi_warm_options& options);
 	~Mlp();
-	// One snapshot: (u_{n-2}, u_{n-1}, u_n) -> u_{n+1}.
-	void Train(const double* un1, const double* un, const double* um, const double* umm, const double* phase);
-	// state <- 3 u_n - 3 u_{n-1} + u_{n-2} + s2 * MLP(features(u_n, u_{n-1}, u_{n-2}, phase)).
-	void Predict(const double* un, const double* um, const double* umm, const double* phase, double* state);
+	// One snapshot: history h[j] = u_{n-j} (j = 0..kOrder) -> next = u_{n+1}.
+	void Train(const double* next, const double* const* history, const double* phase);
+	// state <- order-kOrder extrapolation + s_P * MLP(features(history, phase)).
+	void Predict(const double* const* history, const double* phase, double* state);
 	int snapshots() const { return snapshots_; }
 	double last_loss() const { return last_loss_; }
 

@@ -4,8 +4,8 @@
 //   TFI_WARM_EXTRAPOLATE u_n + (u_n - u_{n-1});
 //   TFI_WARM_EXTRAPOLATE2 3 u_n - 3 u_{n-1} + u_{n-2} (quadratic in time);
 //   TFI_WARM_EXTRAPOLATE3 4 u_n - 6 u_{n-1} + 4 u_{n-2} - u_{n-3} (cubic in time);
-//   TFI_WARM_LEARNED     3 u_n - 3 u_{n-1} + u_{n-2} + MLP(stencil features),
-//                        see tfi_mlp.hip.
+//   TFI_WARM_LEARNED     polynomial extrapolation of order kOrder + MLP(stencil
+//                        features), see tfi_mlp.hpp.
 // A prediction is only a guess: the Newton loop accepts it when it lowers the
 // nonlinear residual and always converges to the same tolerance.
 #include "tfi_solver.hpp"
@@ -41,14 +41,15 @@ __global__ void Extrapolate3Kernel(long long n, const double* __restrict__ un, c
 struct WarmStart::Impl {
 	tfi_mesh* mesh;
 	tfi_ctx* ctx;
-	DevBuf<double> hist[4];   // u_{n}, u_{n-1}, u_{n-2}, u_{n-3} (ring)
+	static constexpr int kRing = kOrder + 2;
+	DevBuf<double> hist[kRing];   // u_{n}, u_{n-1}, ... (ring)
 	int count = 0;
-	int head = 0;             // index of u_n
-	double phase_hist[4][8] = {};
+	int head = 0;                 // index of u_n
+	double phase_hist[kRing][8] = {};
 	int accepted = 0, rejected = 0;
 	std::unique_ptr<Mlp> mlp;
 	tfi_warm_options opt{128, 27, 40, 1e-3, 3};
-	const double* at(int back) const { return hist[(head - back + 4)%4].data(); }
+	const double* at(int back) const { return hist[(head - back + kRing)%kRing].data(); }
 };
 
 WarmStart::WarmStart(tfi_mesh* mesh) : impl_(std::make_unique<Impl>())
@@ -76,14 +77,16 @@ void WarmStart::Commit(const double* state, const double* phase, int phase_count
 {
 	Impl& I = *impl_;
 	const long long n = 4LL*I.mesh->nodes;
-	I.head = (I.head + 1)%4;
+	I.head = (I.head + 1)%Impl::kRing;
 	Copy(I.ctx, state, I.hist[I.head].data(), n);
 	for (int i = 0; i < 8; ++i) I.phase_hist[I.head][i] = i < phase_count ? phase[i] : 0.0;
 	++I.count;
-	// learned mode: a new (u_{n-3}, u_{n-2}, u_{n-1}) -> u_n snapshot trains the network
-	if (train && I.count >= 4) {
+	// learned mode: a new (u_{n-1-kOrder}, .., u_{n-1}) -> u_n snapshot trains the network
+	if (train && I.count >= kOrder + 2) {
 		if (!I.mlp) I.mlp = std::make_unique<Mlp>(I.mesh, I.opt);
-		I.mlp->Train(I.at(0), I.at(1), I.at(2), I.at(3), I.phase_hist[I.head]);
+		const double* h[kOrder + 1];
+		for (int j = 0; j <= kOrder; ++j) h[j] = I.at(j + 1);
+		I.mlp->Train(I.at(0), h, I.phase_hist[I.head]);
 	}
 }
 
@@ -93,7 +96,7 @@ bool WarmStart::Ready(int mode) const
 	if (mode == TFI_WARM_EXTRAPOLATE) return I.count >= 2;
 	if (mode == TFI_WARM_EXTRAPOLATE2) return I.count >= 3;
 	if (mode == TFI_WARM_EXTRAPOLATE3) return I.count >= 4;
-	if (mode == TFI_WARM_LEARNED) return I.count >= 3 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
+	if (mode == TFI_WARM_LEARNED) return I.count >= kOrder + 1 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
 	return false;
 }
 
@@ -115,7 +118,9 @@ void WarmStart::Predict(int mode, double* state, const double* phase, int phase_
 	} else if (mode == TFI_WARM_LEARNED) {
 		double ph[8] = {};
 		for (int i = 0; i < 8 && i < phase_count; ++i) ph[i] = phase[i];
-		I.mlp->Predict(I.at(0), I.at(1), I.at(2), ph, state);
+		const double* h[kOrder + 1];
+		for (int j = 0; j <= kOrder; ++j) h[j] = I.at(j);
+		I.mlp->Predict(h, ph, state);
 	}
 }
 
