@@ -33,7 +33,7 @@ enum { TFI_OK = 0, TFI_ERROR = 1 };
 enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
 enum { TFI_FP64 = 0, TFI_MIXED = 1 };
 enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2, TFI_WARM_EXTRAPOLATE2 = 3,
-	TFI_WARM_EXTRAPOLATE3 = 4 };
+	TFI_WARM_EXTRAPOLATE3 = 4, TFI_WARM_EXTRAPOLATE4 = 5 };
 enum { TFI_SMOOTH_JACOBI = 0, TFI_SMOOTH_GAUSS_SEIDEL = 1 };
 enum { TFI_TANGENT_LEGACY = 0, TFI_TANGENT_EXACT = 1 };
 
@@ -213,7 +213,7 @@ int tfi_timers_reset(tfi_ctx* ctx);
  * No reference counterpart: kernel-level entry points of the learned warm
  * start, used by the parity tests against the CPU oracle.
  * tfi_mlp_reference: FP32 forward + backward (MSE loss) of the stencil MLP on a
- * host batch x [batch][528], target [batch][4]; weights w1 [528][H], b1 [H],
+ * host batch x [batch][640], target [batch][4]; weights w1 [640][H], b1 [H],
  * w2 [H][H], b2 [H], w3 [H][4], b3 [4] (row-major, x W + b, tanh hidden).
  * tfi_mlp_infer_bf16: the BF16-MFMA inference kernel on the same layout. */
 int tfi_mlp_reference(tfi_ctx* ctx, int hidden, int batch, const float* x, const float* target, const float* w1,

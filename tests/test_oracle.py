@@ -23,7 +23,7 @@ from oracle import mlp as MO
 from oracle import oracle as O
 from tubularflowiga_amd import casegen
 
-KIN = 528   # stencil-MLP input width (27 nodes x 19 features + 8 phase, padded to 16)
+KIN = 640   # stencil-MLP input width (27 nodes x 23 features + 8 phase, padded to 16)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.json")
 

@@ -16,6 +16,7 @@ LIB_PATH = PKG / "libtfi_hip.so"
 TFI_PC_BJACOBI, TFI_PC_AMG = 0, 1
 TFI_FP64, TFI_MIXED = 0, 1
 TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3 = 0, 1, 2, 3, 4
+TFI_WARM_EXTRAPOLATE4 = 5
 TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL = 0, 1
 TFI_TANGENT_LEGACY, TFI_TANGENT_EXACT = 0, 1
 MAX_NEWTON_RECORDS = 64

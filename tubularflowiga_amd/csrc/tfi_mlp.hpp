@@ -15,7 +15,7 @@ namespace tfi {
 // order-P polynomial extrapolation in time, so the prediction is
 //   u_{n+1} ~ sum_{j=0..P} (-1)^j C(P+1, j+1) u_{n-j} + s_P * MLP(x),
 // with s_0..s_P the per-field RMS of u_n, d_1, .., d_P.
-constexpr int kOrder = 3;
+constexpr int kOrder = 4;
 constexpr int kStencil = 27;
 constexpr int kPerNode = 4*(kOrder + 1) + 3;
 constexpr int kIn = (kStencil*kPerNode + 8 + 15)/16*16;

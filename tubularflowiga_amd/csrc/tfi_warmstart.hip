@@ -4,6 +4,7 @@
 //   TFI_WARM_EXTRAPOLATE u_n + (u_n - u_{n-1});
 //   TFI_WARM_EXTRAPOLATE2 3 u_n - 3 u_{n-1} + u_{n-2} (quadratic in time);
 //   TFI_WARM_EXTRAPOLATE3 4 u_n - 6 u_{n-1} + 4 u_{n-2} - u_{n-3} (cubic in time);
+//   TFI_WARM_EXTRAPOLATE4 quartic in time (the learned mode's base);
 //   TFI_WARM_LEARNED     polynomial extrapolation of order kOrder + MLP(stencil
 //                        features), see tfi_mlp.hpp.
 // A prediction is only a guess: the Newton loop accepts it when it lowers the
@@ -15,25 +16,22 @@ namespace tfi {
 
 namespace {
 
-__global__ void ExtrapolateKernel(long long n, const double* __restrict__ un, const double* __restrict__ um,
-	double* __restrict__ out)
-{
-	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
-		out[i] = 2.0*un[i] - um[i];
-}
+// order-P polynomial extrapolation in time from u_n, .., u_{n-P}:
+// sum_{j=0..P} (-1)^j C(P+1, j+1) u_{n-j}
+struct History {
+	const double* h[5];
+};
 
-__global__ void Extrapolate2Kernel(long long n, const double* __restrict__ un, const double* __restrict__ um,
-	const double* __restrict__ umm, double* __restrict__ out)
+__global__ void ExtrapolateKernel(long long n, int order, History H, double* __restrict__ out)
 {
-	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
-		out[i] = 3.0*(un[i] - um[i]) + umm[i];
-}
-
-__global__ void Extrapolate3Kernel(long long n, const double* __restrict__ un, const double* __restrict__ um,
-	const double* __restrict__ umm, const double* __restrict__ um3, double* __restrict__ out)
-{
-	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
-		out[i] = 4.0*(un[i] + umm[i]) - 6.0*um[i] - um3[i];
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x) {
+		double c = static_cast<double>(order + 1), s = 0.0;
+		for (int j = 0; j <= order; ++j) {
+			s += c*H.h[j][i];
+			c = -c*static_cast<double>(order - j)/static_cast<double>(j + 2);
+		}
+		out[i] = s;
+	}
 }
 
 }  // namespace
@@ -96,6 +94,7 @@ bool WarmStart::Ready(int mode) const
 	if (mode == TFI_WARM_EXTRAPOLATE) return I.count >= 2;
 	if (mode == TFI_WARM_EXTRAPOLATE2) return I.count >= 3;
 	if (mode == TFI_WARM_EXTRAPOLATE3) return I.count >= 4;
+	if (mode == TFI_WARM_EXTRAPOLATE4) return I.count >= 5;
 	if (mode == TFI_WARM_LEARNED) return I.count >= kOrder + 1 && I.mlp && I.mlp->snapshots() >= I.opt.min_snapshots;
 	return false;
 }
@@ -104,17 +103,13 @@ void WarmStart::Predict(int mode, double* state, const double* phase, int phase_
 {
 	Impl& I = *impl_;
 	const long long n = 4LL*I.mesh->nodes;
-	if (mode == TFI_WARM_EXTRAPOLATE) {
-		hipLaunchKernelGGL(ExtrapolateKernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), state);
+	const int order = mode == TFI_WARM_EXTRAPOLATE ? 1 : mode == TFI_WARM_EXTRAPOLATE2 ? 2
+		: mode == TFI_WARM_EXTRAPOLATE3 ? 3 : mode == TFI_WARM_EXTRAPOLATE4 ? 4 : 0;
+	if (order > 0) {
+		History H{};
+		for (int j = 0; j <= order; ++j) H.h[j] = I.at(j);
+		hipLaunchKernelGGL(ExtrapolateKernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, order, H, state);
 		TFI_LAUNCH_CHECK("ExtrapolateKernel");
-	} else if (mode == TFI_WARM_EXTRAPOLATE2) {
-		hipLaunchKernelGGL(Extrapolate2Kernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), I.at(2),
-			state);
-		TFI_LAUNCH_CHECK("Extrapolate2Kernel");
-	} else if (mode == TFI_WARM_EXTRAPOLATE3) {
-		hipLaunchKernelGGL(Extrapolate3Kernel, dim3(1024), dim3(256), 0, I.ctx->stream, n, I.at(0), I.at(1), I.at(2),
-			I.at(3), state);
-		TFI_LAUNCH_CHECK("Extrapolate3Kernel");
 	} else if (mode == TFI_WARM_LEARNED) {
 		double ph[8] = {};
 		for (int i = 0; i < 8 && i < phase_count; ++i) ph[i] = phase[i];

@@ -14,12 +14,14 @@ import numpy as np
 from . import _abi
 from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, WarmOptions, TFI_FP64, TFI_MIXED,
                    TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3,
+                   TFI_WARM_EXTRAPOLATE4,
                    TFI_WARM_LEARNED,
                    TFI_WARM_PREVIOUS)
 
 __all__ = ["Context", "DeviceMesh", "FlowSolver", "TFI_FP64", "TFI_MIXED", "TFI_PC_AMG", "TFI_PC_BJACOBI",
            "TFI_WARM_PREVIOUS", "TFI_WARM_LEARNED", "TFI_WARM_EXTRAPOLATE",
-           "TFI_WARM_EXTRAPOLATE2", "TFI_WARM_EXTRAPOLATE3"]
+           "TFI_WARM_EXTRAPOLATE2", "TFI_WARM_EXTRAPOLATE3",
+           "TFI_WARM_EXTRAPOLATE4"]
 
 
 class TfiError(RuntimeError):

@@ -142,7 +142,7 @@ const char* kUsage =
 	"[--checkpoint PREFIX] [--checkpoint-every N] [--restart PREFIX] "
 	"[--stop-after-step N] [--nonlinear-rtol R] [--nonlinear-atol A] "
 	"[--mass-rtol R] [--visualization-format auto|vtu] "
-	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|learned] "
+	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|learned] "
 	"[--tangent exact|legacy] [--amg-smoother gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
 	"[--linear-max-iterations N]";
 
@@ -186,6 +186,7 @@ Options ParseOptions(int argc, char** argv)
 		else if (arg == "--warm-start")
 			o.warm_start = Choice(v, arg, {{"previous", TFI_WARM_PREVIOUS}, {"extrapolate", TFI_WARM_EXTRAPOLATE},
 				{"extrapolate2", TFI_WARM_EXTRAPOLATE2}, {"extrapolate3", TFI_WARM_EXTRAPOLATE3},
+				{"extrapolate4", TFI_WARM_EXTRAPOLATE4},
 				{"learned", TFI_WARM_LEARNED}});
 		else if (arg == "--tangent") o.tangent = Choice(v, arg, {{"exact", TFI_TANGENT_EXACT}, {"legacy", TFI_TANGENT_LEGACY}});
 		else if (arg == "--amg-smoother")
