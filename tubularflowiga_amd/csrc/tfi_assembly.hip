@@ -63,9 +63,36 @@ struct AsmArgs {
 	double* jac;
 	double* negres;
 	int with_jacobian;
+	int qstride;           // doubles between the basis records of consecutive points (odd)
 };
 
-template <bool kExact>
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// basis record of function a at slab point ql: [N, dN/dx, dN/dy, dN/dz, lap N, u.grad N, w.grad N]
+__device__ inline double* Brec(double* basis, int qstride, int a, int ql)
+{
+	return basis + (size_t)ql*qstride + (size_t)a*kBasisRec;
+}
+
+// C[ti][tj] += A_ti B_tj for the two row tiles and two column tiles of one
+// term (v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16], B[l/16][l%16]).
+__device__ inline void Term(doublex4 (&c)[4], double a0, double a1, double b0, double b1)
+{
+	c[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c[0], 0, 0, 0);
+	c[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c[1], 0, 0, 0);
+	c[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c[2], 0, 0, 0);
+	c[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c[3], 0, 0, 0);
+}
+
+// kMfma: elements with at most 32 basis functions, one workgroup per element.
+// The tangent is a sum over quadrature points of (test factor) x (trial
+// factor) products, so each of the 16 block components is a small GEMM,
+// T_c[a][b] = sum_(q, term) A_term(a, q) B_term(q, b), run on FP64 MFMA with
+// K = 4 quadrature points per instruction: wave w owns the components of test
+// field w (row w of the 4x4 block), lanes build the operands of their
+// (basis function, point) from the LDS records, and the 2 x 2 output tiles of
+// 16 x 16 cover the element's (padded) 32 x 32 basis pairs.
+template <bool kExact, bool kMfma>
 __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 {
 	extern __shared__ double lds[];
@@ -83,10 +110,11 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	const double idt = dt > 0.0 ? 1.0/dt : 0.0;
 
 	// LDS carve-up
+	const int qstride = args.qstride;
 	double* nodal = lds;                                   // [nen][8]
 	double* C = nodal + kMaxBasis*8;                        // [nen][64]
-	double* basis = C + (size_t)nen*64;                     // [nen][kSlab][5]
-	double* qd = basis + (size_t)nen*kSlab*kBasisRec;       // [kSlab][kQd]
+	double* basis = C + (size_t)nen*64;                     // [kSlab][qstride] records
+	double* qd = basis + (size_t)kSlab*qstride;             // [kSlab][kQd]
 	double* resid = qd + kSlab*kQd;                         // [a_tile][4]
 	int* lconn = reinterpret_cast<int*>(resid + 4*kMaxBasis);
 
@@ -98,6 +126,11 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	}
 	for (int i = tid; i < nen*64; i += kThreads) C[i] = 0.0;
 	for (int i = tid; i < 4*kMaxBasis; i += kThreads) resid[i] = 0.0;
+	if (kMfma)  // padding functions nen..31 stay zero in every slab
+		for (int i = tid; i < kSlab*(32 - nen)*kBasisRec; i += kThreads) {
+			const int ql = i/((32 - nen)*kBasisRec), rest = i - ql*(32 - nen)*kBasisRec;
+			Brec(basis, qstride, nen, ql)[rest] = 0.0;
+		}
 	__syncthreads();
 	for (int a = tid; a < nen; a += kThreads)
 		for (int k = args.xoff[b0 + a]; k < args.xoff[b0 + a + 1]; ++k) C[64*a + args.xcol[k]] = args.xval[k];
@@ -108,12 +141,15 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	// pair tile of this thread: rows ra0, ra0+1 ; columns rb0, rb0+1
 	const int tiles_b = (nen + 1)/2;
 	const int tiles_a = (a1 - a0 + 1)/2;
-	const bool pair_active = args.with_jacobian && tid < tiles_a*tiles_b;
+	const bool pair_active = !kMfma && args.with_jacobian && tid < tiles_a*tiles_b;
 	const int ta = pair_active ? tid/tiles_b : 0;
 	const int tb = pair_active ? tid - ta*tiles_b : 0;
 	const int ra[2] = {a0 + 2*ta, min(a0 + 2*ta + 1, a1 - 1)};
 	const int rb[2] = {2*tb, min(2*tb + 1, nen - 1)};
 	double dg[4] = {}, uu[4][9] = {}, i3[4][3] = {}, t3[4][3] = {}, k33[4] = {};
+	doublex4 acc[4][4];  // MFMA path: [component of this wave][tile ti*2 + tj]
+	for (int c = 0; c < 4; ++c)
+		for (int t = 0; t < 4; ++t) acc[c][t] = doublex4{0.0, 0.0, 0.0, 0.0};
 
 	for (int slab = 0; slab < kQ/kSlab; ++slab) {
 		const int q0 = slab*kSlab;
@@ -144,7 +180,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 			double inv[9];
 #pragma unroll
 			for (int i = 0; i < 9; ++i) inv[i] = gq[(1 + i)*kQ];
-			double* out = basis + ((size_t)a*kSlab + ql)*kBasisRec;
+			double* out = Brec(basis, qstride, a, ql);
 			out[0] = v;
 			out[1] = g0*inv[0] + g1*inv[3] + g2*inv[6];
 			out[2] = g0*inv[1] + g1*inv[4] + g2*inv[7];
@@ -159,7 +195,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 			const int ql = tid >> 3, part = tid & 7, q = q0 + ql;
 			double s[22] = {};
 			for (int a = part; a < nen; a += 8) {
-				const double* bb = basis + ((size_t)a*kSlab + ql)*kBasisRec;
+				const double* bb = Brec(basis, qstride, a, ql);
 				const double* un = nodal + 8*a;
 				for (int f = 0; f < 4; ++f) {
 					s[f] += un[f]*bb[0];
@@ -259,7 +295,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 			const double* r = qd + ql*kQd;
 			double R[4] = {0, 0, 0, 0};
 			if (valid) {
-				double* bb = basis + ((size_t)a*kSlab + ql)*kBasisRec;
+				double* bb = Brec(basis, qstride, a, ql);
 				const double N = bb[0], gx = bb[1], gy = bb[2], gz = bb[3];
 				bb[5] = r[3]*gx + r[4]*gy + r[5]*gz;
 				bb[6] = r[22]*gx + r[23]*gy + r[24]*gz;
@@ -278,6 +314,124 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				for (int c = 0; c < 4; ++c) resid[4*(a - a0) + c] += R[c];
 		}
 		__syncthreads();
+		// ---- P4 (MFMA): 4 points per instruction, wave w = test field w
+		if (kMfma && args.with_jacobian) {
+			const int wave = tid >> 6, lane = tid & 63;
+			const int fl = lane & 15, kq = lane >> 4;
+			const int x = wave, y1 = (wave + 1)%3, y2 = (wave + 2)%3;
+			for (int chunk = 0; chunk < kSlab/4; ++chunk) {
+				const int ql = chunk*4 + kq;
+				const double* r = qd + ql*kQd;
+				const double m = r[0], tm = r[1], tc = r[2];
+				// the lane's two basis functions fl, fl + 16 at point ql
+				double N[2], G[2][3], U[2], W[2], Lp[2];
+#pragma unroll
+				for (int i = 0; i < 2; ++i) {
+					const double* bb = Brec(basis, qstride, fl + 16*i, ql);
+					N[i] = bb[0];
+					G[i][0] = bb[1]; G[i][1] = bb[2]; G[i][2] = bb[3];
+					Lp[i] = bb[4];
+					U[i] = bb[5];
+					W[i] = bb[6];
+				}
+				double mN[2], mG[2][3];
+#pragma unroll
+				for (int i = 0; i < 2; ++i) {
+					mN[i] = m*N[i];
+					for (int d = 0; d < 3; ++d) mG[i][d] = m*G[i][d];
+				}
+				if (kExact) {
+					const double* up = r + 19;
+					const double* gu = r + 25;
+					const double* K = r + 34;
+					const double* R = r + 43;
+					const double* c1 = r + 52;
+					double mA[2], sb[2];
+#pragma unroll
+					for (int i = 0; i < 2; ++i) {
+						mA[i] = m*W[i];
+						sb[i] = N[i]*idt + U[i] - nu*Lp[i];
+					}
+					if (wave < 3) {
+						double Zx[2][3], Xbx[2], Yb[2][3];
+#pragma unroll
+						for (int i = 0; i < 2; ++i) {
+#pragma unroll
+							for (int y = 0; y < 3; ++y) {
+								const double mP = K[y]*mG[i][0] + K[3 + y]*mG[i][1] + K[6 + y]*mG[i][2];
+								Zx[i][y] = rho*(mN[i]*R[3*x + y] - mA[i]*K[3*x + y] - up[x]*(mG[i][y] + mP));
+								Yb[i][y] = rho*tc*G[i][y] + c1[y]*N[i];
+							}
+							Xbx[i] = mu*G[i][x] + rho*tm*up[x]*sb[i];
+						}
+						// (x, x)
+						Term(acc[0], mG[0][x], mG[1][x], Xbx[0] + Yb[0][x] + mu*G[0][x], Xbx[1] + Yb[1][x] + mu*G[1][x]);
+						Term(acc[0], mu*mG[0][y1], mu*mG[1][y1], G[0][y1], G[1][y1]);
+						Term(acc[0], mu*mG[0][y2], mu*mG[1][y2], G[0][y2], G[1][y2]);
+						Term(acc[0], Zx[0][x], Zx[1][x], N[0], N[1]);
+						Term(acc[0], rho*tm*(mA[0] - gu[4*x]*mN[0]), rho*tm*(mA[1] - gu[4*x]*mN[1]), sb[0], sb[1]);
+						Term(acc[0], rho*mN[0], rho*mN[1], N[0]*idt + W[0], N[1]*idt + W[1]);
+						// (x, y1), (x, y2)
+						Term(acc[1], mG[0][y1], mG[1][y1], Xbx[0], Xbx[1]);
+						Term(acc[1], mG[0][x], mG[1][x], Yb[0][y1], Yb[1][y1]);
+						Term(acc[1], Zx[0][y1], Zx[1][y1], N[0], N[1]);
+						Term(acc[1], -rho*tm*gu[3*x + y1]*mN[0], -rho*tm*gu[3*x + y1]*mN[1], sb[0], sb[1]);
+						Term(acc[2], mG[0][y2], mG[1][y2], Xbx[0], Xbx[1]);
+						Term(acc[2], mG[0][x], mG[1][x], Yb[0][y2], Yb[1][y2]);
+						Term(acc[2], Zx[0][y2], Zx[1][y2], N[0], N[1]);
+						Term(acc[2], -rho*tm*gu[3*x + y2]*mN[0], -rho*tm*gu[3*x + y2]*mN[1], sb[0], sb[1]);
+						// (x, 3)
+						Term(acc[3], -mG[0][x], -mG[1][x], N[0], N[1]);
+#pragma unroll
+						for (int d = 0; d < 3; ++d) {
+							const double e0 = tm*(up[x]*mG[0][d] - mN[0]*gu[3*x + d] + (d == x ? mA[0] : 0.0));
+							const double e1 = tm*(up[x]*mG[1][d] - mN[1]*gu[3*x + d] + (d == x ? mA[1] : 0.0));
+							Term(acc[3], e0, e1, G[0][d], G[1][d]);
+						}
+					} else {
+						// (3, y) and (3, 3)
+#pragma unroll
+						for (int y = 0; y < 3; ++y) {
+							const double mP0 = K[y]*mG[0][0] + K[3 + y]*mG[0][1] + K[6 + y]*mG[0][2];
+							const double mP1 = K[y]*mG[1][0] + K[3 + y]*mG[1][1] + K[6 + y]*mG[1][2];
+							Term(acc[y], mN[0], mN[1], G[0][y], G[1][y]);
+							Term(acc[y], -mP0, -mP1, N[0], N[1]);
+							Term(acc[y], tm*mG[0][y], tm*mG[1][y], sb[0], sb[1]);
+						}
+#pragma unroll
+						for (int d = 0; d < 3; ++d)
+							Term(acc[3], (tm/rho)*mG[0][d], (tm/rho)*mG[1][d], G[0][d], G[1][d]);
+					}
+				} else {
+					// legacy tau-frozen tangent
+					double sb[2];
+#pragma unroll
+					for (int i = 0; i < 2; ++i) sb[i] = N[i]*idt + U[i];
+					if (wave < 3) {
+						const double al0 = m*rho*(N[0] + tm*U[0]), al1 = m*rho*(N[1] + tm*U[1]);
+						Term(acc[0], al0, al1, sb[0], sb[1]);
+						Term(acc[0], (2.0*mu + rho*tc)*mG[0][x], (2.0*mu + rho*tc)*mG[1][x], G[0][x], G[1][x]);
+						Term(acc[0], mu*mG[0][y1], mu*mG[1][y1], G[0][y1], G[1][y1]);
+						Term(acc[0], mu*mG[0][y2], mu*mG[1][y2], G[0][y2], G[1][y2]);
+						Term(acc[1], mu*mG[0][y1], mu*mG[1][y1], G[0][x], G[1][x]);
+						Term(acc[1], rho*tc*mG[0][x], rho*tc*mG[1][x], G[0][y1], G[1][y1]);
+						Term(acc[2], mu*mG[0][y2], mu*mG[1][y2], G[0][x], G[1][x]);
+						Term(acc[2], rho*tc*mG[0][x], rho*tc*mG[1][x], G[0][y2], G[1][y2]);
+						Term(acc[3], m*tm*U[0], m*tm*U[1], G[0][x], G[1][x]);
+						Term(acc[3], -mG[0][x], -mG[1][x], N[0], N[1]);
+					} else {
+#pragma unroll
+						for (int y = 0; y < 3; ++y) {
+							Term(acc[y], mN[0], mN[1], G[0][y], G[1][y]);
+							Term(acc[y], tm*mG[0][y], tm*mG[1][y], sb[0], sb[1]);
+						}
+#pragma unroll
+						for (int d = 0; d < 3; ++d)
+							Term(acc[3], (tm/rho)*mG[0][d], (tm/rho)*mG[1][d], G[0][d], G[1][d]);
+					}
+				}
+			}
+		}
 		// ---- P4: tangent on 2x2 (test, trial) tiles
 		if (kExact && pair_active) {
 			// consistent tangent, NavierStokesElement.hpp:244-313, in outer-product form
@@ -293,7 +447,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				double mN[2], mG[2][3], mA[2], mP[2][3], Z[2][9];
 #pragma unroll
 				for (int i = 0; i < 2; ++i) {
-					const double* bb = basis + ((size_t)ra[i]*kSlab + ql)*kBasisRec;
+					const double* bb = Brec(basis, qstride, ra[i], ql);
 					mN[i] = m*bb[0];
 					for (int d = 0; d < 3; ++d) mG[i][d] = m*bb[1 + d];
 					mA[i] = m*bb[6];
@@ -305,7 +459,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				double Nb[2], B[2][3], sb[2], wb[2], Qb[2][3], Xb[2][3], Yb[2][3];
 #pragma unroll
 				for (int j = 0; j < 2; ++j) {
-					const double* bb = basis + ((size_t)rb[j]*kSlab + ql)*kBasisRec;
+					const double* bb = Brec(basis, qstride, rb[j], ql);
 					Nb[j] = bb[0];
 					B[j][0] = bb[1]; B[j][1] = bb[2]; B[j][2] = bb[3];
 					sb[j] = Nb[j]*idt + bb[5] - nu_*bb[4];
@@ -344,7 +498,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				double mA[2][3], al[2], be[2], ga[2], P[2][3], Qv[2][3], T[2][3];
 #pragma unroll
 				for (int i = 0; i < 2; ++i) {
-					const double* bb = basis + ((size_t)ra[i]*kSlab + ql)*kBasisRec;
+					const double* bb = Brec(basis, qstride, ra[i], ql);
 					const double N = bb[0], gu = bb[5];
 					for (int d = 0; d < 3; ++d) {
 						mA[i][d] = m*bb[1 + d];
@@ -359,7 +513,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				double Nb[2], B[2][3], sb[2];
 #pragma unroll
 				for (int j = 0; j < 2; ++j) {
-					const double* bb = basis + ((size_t)rb[j]*kSlab + ql)*kBasisRec;
+					const double* bb = Brec(basis, qstride, rb[j], ql);
 					Nb[j] = bb[0];
 					B[j][0] = bb[1]; B[j][1] = bb[2]; B[j][2] = bb[3];
 					sb[j] = Nb[j]*idt + bb[5];
@@ -386,6 +540,26 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	}
 
 	// ---- scatter (exclusive by colour: plain read-modify-write)
+	if (kMfma && args.with_jacobian) {
+		const int wave = tid >> 6, lane = tid & 63;
+		const long long pbase = args.pair_offsets[e];
+		const int x = wave, y1 = (wave + 1)%3, y2 = (wave + 2)%3;
+		const int comp[4] = {wave < 3 ? 5*x : 12, wave < 3 ? 4*x + y1 : 13, wave < 3 ? 4*x + y2 : 14,
+			wave < 3 ? 4*x + 3 : 15};
+#pragma unroll
+		for (int c = 0; c < 4; ++c)
+#pragma unroll
+			for (int t = 0; t < 4; ++t) {
+				const int ti = t >> 1, tj = t & 1;
+				const int b = 16*tj + (lane & 15);
+#pragma unroll
+				for (int i = 0; i < 4; ++i) {
+					const int a = 16*ti + 4*i + (lane >> 4);
+					if (a < nen && b < nen)
+						args.jac[(size_t)args.pair_block[pbase + (long long)a*nen + b]*16 + comp[c]] += acc[c][t][i];
+				}
+			}
+	}
 	if (pair_active) {
 		const long long pbase = args.pair_offsets[e];
 #pragma unroll
@@ -610,18 +784,28 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	double* negative_residual)
 {
 	const bool exact = p.tangent == TFI_TANGENT_EXACT;
-	const void* kernel = exact ? reinterpret_cast<const void*>(AssembleNsKernel<true>)
-		: reinterpret_cast<const void*>(AssembleNsKernel<false>);
+	const int nen_max = m->max_nen;
+	// elements of at most 32 basis functions take the FP64-MFMA tangent
+	const bool mfma = nen_max <= 32;
+	const int variant = (exact ? 1 : 0) + (mfma ? 2 : 0);
+	static const void* const kKernels[4] = {
+		reinterpret_cast<const void*>(AssembleNsKernel<false, false>),
+		reinterpret_cast<const void*>(AssembleNsKernel<true, false>),
+		reinterpret_cast<const void*>(AssembleNsKernel<false, true>),
+		reinterpret_cast<const void*>(AssembleNsKernel<true, true>)};
 	tfi_ctx* ctx = m->ctx;
 	StreamTimer timer(ctx, &ctx->timers.assembly_ms);
 	if (jacobian) Fill(ctx, jacobian, 0.0, m->nnzb*16);
 	Fill(ctx, negative_residual, 0.0, 4LL*m->nodes);
-	const int nen_max = m->max_nen;
 	const int tiles_b = (nen_max + 1)/2;
-	int a_tile = std::min(nen_max, 2*(kThreads/tiles_b));
+	int a_tile = mfma ? nen_max : std::min(nen_max, 2*(kThreads/tiles_b));
 	if (a_tile < 2) a_tile = 2;
 	const int tiles = (nen_max + a_tile - 1)/a_tile;
-	const size_t lds = sizeof(double)*(kMaxBasis*8 + (size_t)nen_max*64 + (size_t)nen_max*kSlab*kBasisRec
+	// per-point stride of the basis records, odd so that lanes reading the
+	// same field of different points hit different LDS banks
+	const int records = mfma ? 32 : nen_max;
+	const int qstride = records*kBasisRec | 1;
+	const size_t lds = sizeof(double)*(kMaxBasis*8 + (size_t)nen_max*64 + (size_t)kSlab*qstride
 		+ kSlab*kQd + 4*kMaxBasis) + sizeof(int)*kMaxBasis;
 	AsmArgs a{};
 	a.eoff = m->eoff.data();
@@ -642,20 +826,25 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	a.jac = jacobian;
 	a.negres = negative_residual;
 	a.with_jacobian = jacobian != nullptr;
+	a.qstride = qstride;
 	if (lds > 65536) {
-		static size_t raised[2] = {0, 0};
-		if (lds > raised[exact]) {
-			Check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
-				"hipFuncSetAttribute");
-			raised[exact] = lds;
+		static size_t raised[4] = {0, 0, 0, 0};
+		if (lds > raised[variant]) {
+			Check(hipFuncSetAttribute(kKernels[variant], hipFuncAttributeMaxDynamicSharedMemorySize,
+				static_cast<int>(lds)), "hipFuncSetAttribute");
+			raised[variant] = lds;
 		}
 	}
 	for (int c = 0; c < m->colors; ++c) {
 		a.elements = m->color_elements.data() + m->color_offsets[c];
 		a.count = m->color_offsets[c + 1] - m->color_offsets[c];
 		if (a.count == 0) continue;
-		if (exact) hipLaunchKernelGGL(AssembleNsKernel<true>, dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a);
-		else hipLaunchKernelGGL(AssembleNsKernel<false>, dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a);
+		switch (variant) {
+		case 0: hipLaunchKernelGGL((AssembleNsKernel<false, false>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
+		case 1: hipLaunchKernelGGL((AssembleNsKernel<true, false>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
+		case 2: hipLaunchKernelGGL((AssembleNsKernel<false, true>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
+		default: hipLaunchKernelGGL((AssembleNsKernel<true, true>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
+		}
 		TFI_LAUNCH_CHECK("AssembleNsKernel");
 	}
 	ctx->timers.assembly_calls += 1;
