@@ -54,6 +54,7 @@ typedef struct {
 	int colors;               /* element colours of the deterministic scatter */
 	double minimum_determinant; /* min over (element, q) of 0.125 det J */
 	long long bad_samples;    /* quadrature points with det <= 0 or non-finite */
+	long long basis_table_bytes; /* device bytes of the precomputed basis table (0: none) */
 } tfi_mesh_stats;
 
 /* Uploads a FlatMesh (DeviceMesh.hpp:16-68, 70-118), builds the node-block
@@ -69,6 +70,15 @@ int tfi_mesh_stats_get(tfi_mesh* mesh, tfi_mesh_stats* out);
 int tfi_mesh_pattern(tfi_mesh* mesh, int32_t* row_offsets, int32_t* columns, int32_t* diagonal);
 /* Element colouring: colour of each element (int32[E]). */
 int tfi_mesh_colors(tfi_mesh* mesh, int32_t* element_color);
+/* Basis table: N, grad N and lap N of every (element, quadrature point, basis
+ * function), 5 doubles each, evaluated once from the extraction operator and
+ * geometry so that every later assembly reads them instead of re-evaluating
+ * the Bernstein sum-factorisation (the reference evaluates them at every
+ * point of every assembly: EvaluateBasisDevice, IgaCudaKernels.cuh:113, called
+ * at :303).  enable = 1 builds it, 0 frees it,
+ * -1 builds it when it takes at most half of the free device memory (what
+ * tfi_mesh_create does).  Assembly results are the same either way. */
+int tfi_mesh_basis_table(tfi_mesh* mesh, int enable);
 
 /* Pressure-traction load of labelled faces (IntegratePressureTractionForces,
  * solvers/cpu/include/PressureTractionFlow.hpp:41-64, with the face integral of
@@ -135,6 +145,9 @@ typedef struct {
 	int amg_coarse_nodes;             /* block rows of the coarsest dense level (0: 256, at most 256) */
 	int amg_gs_min_rows;              /* levels below this many rows use block Jacobi (0: 8192) */
 	int amg_gs_sweeps;                /* 2 (0): symmetric Gauss-Seidel pre and post; 1: forward pre, backward post */
+	int chord_polish;                 /* 1 (default): a residual check that misses the tolerance by at most
+	                                     1000x is closed by a chord step (a solve with the last tangent and
+	                                     preconditioner, to 0.1 x tolerance) before a new tangent is built */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);
@@ -151,6 +164,8 @@ typedef struct {
 	double initial_residual, final_residual, tolerance, assembly_s, linear_s, total_s;
 	double warm_start_initial_ratio;  /* ||r(x0)|| / ||b|| of the first linear solve */
 	int warm_start_used;              /* 1 if the learned guess was accepted */
+	int chord_steps;                  /* chord steps taken (see tfi_flow_options.chord_polish) */
+	double warm_start_blend;          /* learned mode: line-search scale t of the network's correction */
 	tfi_newton_record record[TFI_MAX_NEWTON_RECORDS];
 } tfi_newton_report;
 
@@ -198,12 +213,18 @@ typedef struct {
 int tfi_flow_warm_configure(tfi_flow* flow, const tfi_warm_options* options);
 /* Training statistics: snapshots seen, last training loss, accepted guesses. */
 int tfi_flow_warm_stats(tfi_flow* flow, int* snapshots, double* last_loss, int* accepted, int* rejected);
+/* Scale the learned correction is applied with: the least-squares fit of the
+ * network's prediction to the latest snapshot before training on it, in
+ * [0, 1] (0: the prediction is the extrapolation base alone). */
+int tfi_flow_warm_gate(tfi_flow* flow, double* gate);
 
 /* Kernel timers accumulated on the context stream (HIP events). */
 typedef struct {
 	double spmv_ms, spmv_calls, spmv_bytes;           /* fine-level fp64 SpMV */
 	double spmv32_ms, spmv32_calls, spmv32_bytes;     /* fine-level fp32 SpMV */
-	double assembly_ms, assembly_calls, assembly_flops;
+	double assembly_ms, assembly_calls, assembly_flops;  /* residual + tangent: AssembleNsKernel launches */
+	double assembly_launches;                         /* (one launch per element colour) */
+	double residual_ms, residual_calls, residual_flops;  /* residual-only assemblies */
 } tfi_kernel_timers;
 int tfi_timers_get(tfi_ctx* ctx, tfi_kernel_timers* out);
 int tfi_timers_enable(tfi_ctx* ctx, int enabled);

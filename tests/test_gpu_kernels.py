@@ -64,17 +64,39 @@ def test_geometry_valid(dmesh):
     assert dmesh.stats["minimum_determinant"] > 0.0
 
 
+@pytest.mark.parametrize("table", [1, 0])
 @pytest.mark.parametrize("tangent", ["legacy", "exact"])
 @pytest.mark.parametrize("dt", [1e-3, 0.0])
-def test_assembly_matches_oracle(dmesh, case, dt, tangent):
+def test_assembly_matches_oracle(dmesh, case, dt, tangent, table):
     state = _state(case, 1)
     prev = _state(case, 2)
     pattern = O.block_pattern(case.mesh)
-    jac, negres = dmesh.assemble(state, prev, RHO, MU, dt, tangent=tangent)
+    try:
+        assert (dmesh.basis_table(table) > 0) == bool(table)
+        jac, negres = dmesh.assemble(state, prev, RHO, MU, dt, tangent=tangent)
+        _, negres_only = dmesh.assemble(state, prev, RHO, MU, dt, jacobian=False, tangent=tangent)
+    finally:
+        dmesh.basis_table(1)
     ojac, onegres = O.ns_assemble(case.mesh, pattern, state, prev, RHO, MU, dt, tangent)
     # tolerance: FP64 summation-order differences only
     assert np.max(np.abs(negres - onegres)) <= 1e-11 * np.max(np.abs(onegres))
     assert np.max(np.abs(jac - ojac)) <= 1e-11 * np.max(np.abs(ojac))
+    # the residual-only kernel gives the same residual as the full assembly
+    assert np.max(np.abs(negres_only - negres)) <= 1e-13 * np.max(np.abs(onegres))
+
+
+def test_basis_table_layout(dmesh, case):
+    """The table holds 64 x nen x 5 doubles per element and its removal and
+    rebuild leave the assembly unchanged to rounding."""
+    state = _state(case, 4)
+    nbytes = dmesh.basis_table(1)
+    assert nbytes == 8 * 64 * 5 * int(case.mesh.element_offsets[-1])
+    j1, r1 = dmesh.assemble(state, state, RHO, MU, 1e-3, tangent="exact")
+    assert dmesh.basis_table(0) == 0
+    j0, r0 = dmesh.assemble(state, state, RHO, MU, 1e-3, tangent="exact")
+    assert dmesh.basis_table(-1) == nbytes
+    assert np.max(np.abs(j1 - j0)) <= 1e-13 * np.max(np.abs(j0))
+    assert np.max(np.abs(r1 - r0)) <= 1e-13 * np.max(np.abs(r0))
 
 
 def test_assembly_deterministic(dmesh, case):

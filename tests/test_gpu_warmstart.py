@@ -1,0 +1,130 @@
+"""Initial-state prediction for the time step on the GPU.
+
+Every warm-start mode must end each step at the plain method's solution (the
+converged residual is the judge, so a prediction only changes the work), the
+learned guess may only be used when it lowers the step's initial residual,
+and the trained network must beat the method without it: its own quartic
+extrapolation base and the previous-step start of iga_cuda.cu:917-920.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tubularflowiga_amd import boundary, casegen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tubularflowiga_amd import device
+    c = device.Context(0)
+    print(c.device_info(), flush=True)
+    return c
+
+
+@pytest.fixture(scope="module")
+def ybif():
+    return casegen.workload("ybif_small")
+
+
+def _pulsatile(t):
+    """Inflow scale Q(t)/Q0 = 1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t (period 1 s)."""
+    return 1.0 + 0.6 * math.sin(2 * math.pi * t) + 0.2 * math.sin(4 * math.pi * t)
+
+
+def _run(ctx, case, mode, precision, steps, dt, rtol, waveform=None, dm=None):
+    from tubularflowiga_amd import device
+    rho, mu = casegen.flow_parameters(case)
+    dm = dm or device.DeviceMesh(ctx, case.mesh)
+    vmask, pmask, vt, pt = boundary.resolve_flow_boundaries(case)
+    scale0 = waveform(0.0) if waveform else 1.0
+    flow = device.FlowSolver(dm, density=rho, viscosity=mu, dt=dt, maximum_newton=20, nonlinear_rtol=rtol,
+                             nonlinear_atol=1e-16, mass_rtol=1e30, linear_rtol=1e-8, precision=precision,
+                             warm_start=mode)
+    if mode == device.TFI_WARM_LEARNED:
+        flow.configure_warm_start(hidden=128, train_steps=40, learning_rate=1e-3, min_snapshots=2)
+    flow.set_state(boundary.initial_state(vmask, pmask, vt * scale0, pt))
+    flow.set_boundary_faces(*boundary.boundary_faces(case.mesh))
+    reports = []
+    for k in range(steps):
+        t = (k + 1) * dt
+        flow.begin_step(np.array([math.sin(2 * math.pi * t), math.cos(2 * math.pi * t)]))
+        flow.set_boundary(vmask, pmask, vt * (waveform(t) if waveform else 1.0), pt)
+        reports.append(flow.newton())
+        flow.commit_step()
+    state = flow.state()
+    stats = flow.warm_stats()
+    flow.close()
+    return state, reports, stats
+
+
+def _rel(a, b):
+    a, b = a.reshape(-1, 4), b.reshape(-1, 4)
+    v = np.linalg.norm(a[:, :3] - b[:, :3]) / np.linalg.norm(b[:, :3])
+    p = np.linalg.norm(a[:, 3] - b[:, 3]) / max(np.linalg.norm(b[:, 3]), 1e-300)
+    return v, p
+
+
+MODES = ["extrapolate", "extrapolate2", "extrapolate3", "extrapolate4", "learned"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_every_mode_reaches_the_plain_solution(ctx, ybif, precision):
+    from tubularflowiga_amd import device
+    dm = device.DeviceMesh(ctx, ybif.mesh)
+    steps, dt, rtol = 12, 1e-3, 1e-10
+    plain, plain_reps, _ = _run(ctx, ybif, device.TFI_WARM_PREVIOUS, 0, steps, dt, rtol, dm=dm)
+    for name in MODES:
+        mode = getattr(device, "TFI_WARM_" + name.upper())
+        state, reps, stats = _run(ctx, ybif, mode, precision, steps, dt, rtol, dm=dm)
+        v, p = _rel(state, plain)
+        used = sum(r["warm_start_used"] for r in reps)
+        print(f"precision={precision} {name}: used={used} newton={sum(r['newton_iterations'] for r in reps)} "
+              f"(plain {sum(r['newton_iterations'] for r in plain_reps)}) rel_l2 v={v:.2e} p={p:.2e}", flush=True)
+        assert all(r["converged"] == 1 for r in reps)
+        assert v <= 1e-10 and p <= 1e-10
+        assert used > 0
+
+
+def test_learned_guess_only_used_when_it_lowers_the_residual(ctx, ybif):
+    from tubularflowiga_amd import device
+    _, reps, stats = _run(ctx, ybif, device.TFI_WARM_LEARNED, 1, 16, 5e-3, 1e-8, waveform=_pulsatile)
+    predicted = [r for r in reps if r["warm_start_initial_ratio"] > 0.0]
+    assert predicted, "the learned predictor never became ready"
+    for r in predicted:
+        assert r["warm_start_used"] == (1 if r["warm_start_initial_ratio"] < 1.0 else 0)
+    assert stats["accepted"] == sum(r["warm_start_used"] for r in reps)
+    assert stats["accepted"] + stats["rejected"] == len(predicted)
+    assert stats["snapshots"] > 0 and np.isfinite(stats["last_loss"])
+
+
+@pytest.mark.parametrize("inflow", ["steady", "pulsatile"])
+def test_trained_network_gains_over_the_method_without_it(ctx, ybif, inflow):
+    """Initial residual of the step, relative to the previous-step start (1.0),
+    over the last 10 of 40 steps: the learned predictor against its quartic
+    extrapolation base, same steps, same Newton tolerance.  From rest under a
+    steady inflow (a developing flow) the network must cut the base's initial
+    residual; under the smooth pulsatile inflow at dt = 5e-3 s the base is
+    already within 1e-4 of the next state and the online gate must keep the
+    correction from making it worse."""
+    from tubularflowiga_amd import device
+    dm = device.DeviceMesh(ctx, ybif.mesh)
+    dt, wf = (1e-3, None) if inflow == "steady" else (5e-3, _pulsatile)
+    out = {}
+    for name in ("extrapolate4", "learned"):
+        mode = getattr(device, "TFI_WARM_" + name.upper())
+        _, reps, stats = _run(ctx, ybif, mode, 0, 40, dt, 1e-8, waveform=wf, dm=dm)
+        tail = reps[-10:]
+        out[name] = {"ratio": float(np.exp(np.mean([np.log(r["warm_start_initial_ratio"]) for r in tail]))),
+                     "newton": sum(r["newton_iterations"] for r in tail),
+                     "krylov": sum(r["linear_iterations"] for r in tail), "gate": stats.get("gate"),
+                     "blend": float(np.mean([r["warm_start_blend"] for r in tail]))}
+        assert all(r["warm_start_used"] == 1 for r in tail)
+    print(f"{inflow}: {out}", flush=True)
+    if inflow == "steady":
+        assert out["learned"]["ratio"] < 0.8 * out["extrapolate4"]["ratio"]
+    else:
+        assert out["learned"]["ratio"] <= 1.1 * out["extrapolate4"]["ratio"]
+    assert out["learned"]["newton"] <= out["extrapolate4"]["newton"]

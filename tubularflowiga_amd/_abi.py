@@ -28,7 +28,7 @@ class MeshStats(ctypes.Structure):
     _fields_ = [("nodes", c_int), ("elements", c_int), ("maximum_basis", c_int),
                 ("basis_entries", c_ll), ("extraction_nonzeros", c_ll), ("block_nonzeros", c_ll),
                 ("element_pairs", c_ll), ("colors", c_int), ("minimum_determinant", c_double),
-                ("bad_samples", c_ll)]
+                ("bad_samples", c_ll), ("basis_table_bytes", c_ll)]
 
 
 class FlowOptions(ctypes.Structure):
@@ -37,7 +37,7 @@ class FlowOptions(ctypes.Structure):
                 ("mass_rtol", c_double), ("linear_maximum_iterations", c_int), ("linear_rtol", c_double),
                 ("restart", c_int), ("preconditioner", c_int), ("precision", c_int), ("warm_start", c_int),
                 ("verbose", c_int), ("amg_smoother", c_int), ("tangent", c_int), ("amg_coarse_nodes", c_int),
-                ("amg_gs_min_rows", c_int), ("amg_gs_sweeps", c_int)]
+                ("amg_gs_min_rows", c_int), ("amg_gs_sweeps", c_int), ("chord_polish", c_int)]
 
 
 class NewtonRecord(ctypes.Structure):
@@ -52,6 +52,7 @@ class NewtonReport(ctypes.Structure):
                 ("records", c_int), ("initial_residual", c_double), ("final_residual", c_double),
                 ("tolerance", c_double), ("assembly_s", c_double), ("linear_s", c_double),
                 ("total_s", c_double), ("warm_start_initial_ratio", c_double), ("warm_start_used", c_int),
+                ("chord_steps", c_int), ("warm_start_blend", c_double),
                 ("record", NewtonRecord * MAX_NEWTON_RECORDS)]
 
 
@@ -63,7 +64,9 @@ class WarmOptions(ctypes.Structure):
 class KernelTimers(ctypes.Structure):
     _fields_ = [("spmv_ms", c_double), ("spmv_calls", c_double), ("spmv_bytes", c_double),
                 ("spmv32_ms", c_double), ("spmv32_calls", c_double), ("spmv32_bytes", c_double),
-                ("assembly_ms", c_double), ("assembly_calls", c_double), ("assembly_flops", c_double)]
+                ("assembly_ms", c_double), ("assembly_calls", c_double), ("assembly_flops", c_double),
+                ("assembly_launches", c_double), ("residual_ms", c_double), ("residual_calls", c_double),
+                ("residual_flops", c_double)]
 
 
 def _np(dtype):
@@ -85,6 +88,7 @@ SIGNATURES = {
     "tfi_mesh_stats_get": (c_int, [c_void_p, P(MeshStats)]),
     "tfi_mesh_pattern": (c_int, [c_void_p, _i32, _i32, _i32]),
     "tfi_mesh_colors": (c_int, [c_void_p, _i32]),
+    "tfi_mesh_basis_table": (c_int, [c_void_p, c_int]),
     "tfi_ns_assemble": (c_int, [c_void_p, _d, _d, c_double, c_double, c_double, c_void_p, _d, c_int]),
     "tfi_bsr_spmv": (c_int, [c_void_p, _d, _d, _d, c_int]),
     "tfi_block_inverse": (c_int, [c_void_p, _d, _d, P(c_int)]),
@@ -109,6 +113,7 @@ SIGNATURES = {
     "tfi_flow_warm_configure": (c_int, [c_void_p, P(WarmOptions)]),
     "tfi_flow_amg_levels": (c_int, [c_void_p, c_int, P(c_int), P(c_int), P(c_ll), P(c_int)]),
     "tfi_flow_warm_stats": (c_int, [c_void_p, P(c_int), P(c_double), P(c_int), P(c_int)]),
+    "tfi_flow_warm_gate": (c_int, [c_void_p, P(c_double)]),
     "tfi_timers_get": (c_int, [c_void_p, P(KernelTimers)]),
     "tfi_timers_enable": (c_int, [c_void_p, c_int]),
     "tfi_timers_reset": (c_int, [c_void_p]),

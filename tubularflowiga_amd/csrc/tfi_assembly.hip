@@ -44,6 +44,124 @@ __device__ inline void Bern3(double x, double b[4], double db[4], double d2b[4])
 	d2b[0] = 6.0*y; d2b[1] = -12.0 + 18.0*x; d2b[2] = 6.0 - 18.0*x; d2b[3] = 6.0*x;
 }
 
+constexpr int kTableRec = 5;  // N, dN/dx, dN/dy, dN/dz, lap N
+
+// Value, physical gradient and Laplacian of the basis function with Bezier
+// extraction row c[64] at element point q (Bernstein sum-factorisation in
+// xi, eta, zeta; gq = geometry record of the point, field stride kQ).
+__device__ inline void EvalBasis(const double* __restrict__ c, const double* __restrict__ gq, int q,
+	double out[kTableRec])
+{
+	double bx[4], dbx[4], d2bx[4], by[4], dby[4], d2by[4], bz[4], dbz[4], d2bz[4];
+	Bern3(kGp[q & 3], bx, dbx, d2bx);
+	Bern3(kGp[(q >> 2) & 3], by, dby, d2by);
+	Bern3(kGp[q >> 4], bz, dbz, d2bz);
+	double v = 0, g0 = 0, g1 = 0, g2 = 0, h00 = 0, h11 = 0, h22 = 0, h01 = 0, h02 = 0, h12 = 0;
+	for (int k = 0; k < 4; ++k) {
+		double s00 = 0, s10 = 0, s01 = 0, s20 = 0, s11 = 0, s02 = 0;
+		for (int j = 0; j < 4; ++j) {
+			const double* cr = c + 16*k + 4*j;
+			const double x0 = cr[0]*bx[0] + cr[1]*bx[1] + cr[2]*bx[2] + cr[3]*bx[3];
+			const double x1 = cr[0]*dbx[0] + cr[1]*dbx[1] + cr[2]*dbx[2] + cr[3]*dbx[3];
+			const double x2 = cr[0]*d2bx[0] + cr[1]*d2bx[1] + cr[2]*d2bx[2] + cr[3]*d2bx[3];
+			s00 += x0*by[j]; s10 += x1*by[j]; s20 += x2*by[j];
+			s01 += x0*dby[j]; s11 += x1*dby[j]; s02 += x0*d2by[j];
+		}
+		v += s00*bz[k]; g0 += s10*bz[k]; g1 += s01*bz[k]; g2 += s00*dbz[k];
+		h00 += s20*bz[k]; h11 += s02*bz[k]; h22 += s00*d2bz[k];
+		h01 += s11*bz[k]; h02 += s10*dbz[k]; h12 += s01*dbz[k];
+	}
+	double inv[9];
+#pragma unroll
+	for (int i = 0; i < 9; ++i) inv[i] = gq[(1 + i)*kQ];
+	out[0] = v;
+	out[1] = g0*inv[0] + g1*inv[3] + g2*inv[6];
+	out[2] = g0*inv[1] + g1*inv[4] + g2*inv[7];
+	out[3] = g0*inv[2] + g1*inv[5] + g2*inv[8];
+	out[4] = h00*gq[10*kQ] + h11*gq[11*kQ] + h22*gq[12*kQ]
+		+ 2.0*(h01*gq[13*kQ] + h02*gq[14*kQ] + h12*gq[15*kQ])
+		+ g0*gq[16*kQ] + g1*gq[17*kQ] + g2*gq[18*kQ];
+}
+
+// Per-point record from the field sums s (u, p, grad u, grad p, lap u, u_prev
+// at the point; 22 values) and the point's geometry record gq (field stride
+// kQ): r[0] measure, r[1..2] tau_M, tau_C, r[3..5] u, r[6..8] rho (du/dt +
+// w.grad u), r[9..17] stress rows, r[18] div u, r[19..21] u', r[22..24] w =
+// u + u', r[25..33] grad u and, with exact_tangent, r[34..54] the
+// linearisation of u' (K, R, c1) the consistent tangent needs.
+__device__ inline void PointRecord(const double* __restrict__ s, const double* __restrict__ gq, double rho, double mu,
+	double nu, double dt, bool exact_tangent, double* __restrict__ r)
+{
+	double inv[3][3];
+	for (int a = 0; a < 3; ++a)
+		for (int x = 0; x < 3; ++x) inv[a][x] = gq[(1 + 3*a + x)*kQ];
+	const double u[3] = {s[0], s[1], s[2]};
+	const double p = s[3];
+	double grad[4][3];
+	for (int f = 0; f < 4; ++f)
+		for (int d = 0; d < 3; ++d) grad[f][d] = s[4 + 3*f + d];
+	double dudt[3];
+	for (int c = 0; c < 3; ++c) dudt[c] = dt > 0.0 ? (u[c] - s[19 + c])/dt : 0.0;
+	double G[3][3] = {}, gv[3] = {};
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j) {
+			for (int k = 0; k < 3; ++k) G[i][j] += inv[k][i]*inv[k][j];
+			gv[i] += inv[j][i];
+		}
+	double GG = 0.0, gg = 0.0, uGu = 0.0;
+	for (int i = 0; i < 3; ++i) {
+		gg += gv[i]*gv[i];
+		for (int j = 0; j < 3; ++j) {
+			GG += G[i][j]*G[i][j];
+			uGu += u[i]*G[i][j]*u[j];
+		}
+	}
+	const double temporal = dt > 0.0 ? 4.0/(dt*dt) : 0.0;
+	const double tau_m = 1.0/sqrt(temporal + uGu + (1.0/12.0)*nu*nu*GG);
+	const double tau_c = 1.0/(tau_m*gg);
+	double fu[3];
+	for (int c = 0; c < 3; ++c) {
+		const double conv = u[0]*grad[c][0] + u[1]*grad[c][1] + u[2]*grad[c][2];
+		fu[c] = -tau_m*(dudt[c] + conv + grad[3][c]/rho - nu*s[16 + c]);
+	}
+	const double div = grad[0][0] + grad[1][1] + grad[2][2];
+	const double fp = -rho*tau_c*div;
+	r[0] = gq[0];
+	r[1] = tau_m;
+	r[2] = tau_c;
+	for (int c = 0; c < 3; ++c) r[3 + c] = u[c];
+	const double w[3] = {u[0] + fu[0], u[1] + fu[1], u[2] + fu[2]};
+	for (int c = 0; c < 3; ++c) {
+		r[6 + c] = rho*(dudt[c] + w[0]*grad[c][0] + w[1]*grad[c][1] + w[2]*grad[c][2]);
+		for (int d = 0; d < 3; ++d)
+			r[9 + 3*c + d] = mu*(grad[c][d] + grad[d][c]) - rho*fu[c]*w[d] - (c == d ? p + fp : 0.0);
+	}
+	r[18] = div;
+	for (int c = 0; c < 3; ++c) r[19 + c] = fu[c];
+	for (int c = 0; c < 3; ++c) r[22 + c] = w[c];
+	for (int c = 0; c < 3; ++c)
+		for (int d = 0; d < 3; ++d) r[25 + 3*c + d] = grad[c][d];
+	if (exact_tangent) {
+		// d(u')/d(u_f) = N_b K[.][f] - delta tau_M s_b with
+		// K_cf = tau_M^3 strong_c (G u)_f - tau_M du_c/dx_f, and
+		// R = grad u + grad u K, c1_f = rho tau_C tau_M^2 div (G u)_f.
+		double Gu[3];
+		for (int i = 0; i < 3; ++i) Gu[i] = G[i][0]*u[0] + G[i][1]*u[1] + G[i][2]*u[2];
+		const double tm3 = tau_m*tau_m*tau_m;
+		double K[3][3];
+		for (int c = 0; c < 3; ++c) {
+			const double strong = -fu[c]/tau_m;
+			for (int f = 0; f < 3; ++f) K[c][f] = tm3*strong*Gu[f] - tau_m*grad[c][f];
+		}
+		for (int c = 0; c < 3; ++c)
+			for (int f = 0; f < 3; ++f) {
+				r[34 + 3*c + f] = K[c][f];
+				r[43 + 3*c + f] = grad[c][f] + grad[c][0]*K[0][f] + grad[c][1]*K[1][f] + grad[c][2]*K[2][f];
+			}
+		for (int f = 0; f < 3; ++f) r[52 + f] = rho*tau_c*tau_m*tau_m*div*Gu[f];
+	}
+}
+
 struct AsmArgs {
 	const int* eoff;
 	const int* conn;
@@ -62,7 +180,7 @@ struct AsmArgs {
 	double rho, mu, dt;
 	double* jac;
 	double* negres;
-	int with_jacobian;
+	const double* table;   // per-element basis records [q][a][kTableRec] at eoff[e]*kQ*kTableRec, or null
 	int qstride;           // doubles between the basis records of consecutive points (odd)
 };
 
@@ -92,7 +210,8 @@ __device__ inline void Term(doublex4 (&c)[4], double a0, double a1, double b0, d
 // field w (row w of the 4x4 block), lanes build the operands of their
 // (basis function, point) from the LDS records, and the 2 x 2 output tiles of
 // 16 x 16 cover the element's (padded) 32 x 32 basis pairs.
-template <bool kExact, bool kMfma>
+// kJac: residual and tangent; otherwise the residual alone.
+template <bool kExact, bool kMfma, bool kJac>
 __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 {
 	extern __shared__ double lds[];
@@ -124,7 +243,9 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 		for (int f = 0; f < 4; ++f) nodal[8*a + f] = args.state[4*(size_t)node + f];
 		for (int f = 0; f < 3; ++f) nodal[8*a + 4 + f] = args.previous[4*(size_t)node + f];
 	}
-	for (int i = tid; i < nen*64; i += kThreads) C[i] = 0.0;
+	const double* table = args.table;
+	if (!table)
+		for (int i = tid; i < nen*64; i += kThreads) C[i] = 0.0;
 	for (int i = tid; i < 4*kMaxBasis; i += kThreads) resid[i] = 0.0;
 	if (kMfma)  // padding functions nen..31 stay zero in every slab
 		for (int i = tid; i < kSlab*(32 - nen)*kBasisRec; i += kThreads) {
@@ -132,8 +253,9 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 			Brec(basis, qstride, nen, ql)[rest] = 0.0;
 		}
 	__syncthreads();
-	for (int a = tid; a < nen; a += kThreads)
-		for (int k = args.xoff[b0 + a]; k < args.xoff[b0 + a + 1]; ++k) C[64*a + args.xcol[k]] = args.xval[k];
+	if (!table)
+		for (int a = tid; a < nen; a += kThreads)
+			for (int k = args.xoff[b0 + a]; k < args.xoff[b0 + a + 1]; ++k) C[64*a + args.xcol[k]] = args.xval[k];
 	__syncthreads();
 
 	const double* geo = args.geometry + (size_t)e*kGeomFields*kQ;
@@ -141,7 +263,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	// pair tile of this thread: rows ra0, ra0+1 ; columns rb0, rb0+1
 	const int tiles_b = (nen + 1)/2;
 	const int tiles_a = (a1 - a0 + 1)/2;
-	const bool pair_active = !kMfma && args.with_jacobian && tid < tiles_a*tiles_b;
+	const bool pair_active = !kMfma && kJac && tid < tiles_a*tiles_b;
 	const int ta = pair_active ? tid/tiles_b : 0;
 	const int tb = pair_active ? tid - ta*tiles_b : 0;
 	const int ra[2] = {a0 + 2*ta, min(a0 + 2*ta + 1, a1 - 1)};
@@ -153,41 +275,20 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 
 	for (int slab = 0; slab < kQ/kSlab; ++slab) {
 		const int q0 = slab*kSlab;
-		// ---- P1: basis values, gradients and Laplacians per (a, point)
-		for (int it = tid; it < nen*kSlab; it += kThreads) {
-			const int a = it/kSlab, ql = it - a*kSlab, q = q0 + ql;
-			double bx[4], dbx[4], d2bx[4], by[4], dby[4], d2by[4], bz[4], dbz[4], d2bz[4];
-			Bern3(kGp[q & 3], bx, dbx, d2bx);
-			Bern3(kGp[(q >> 2) & 3], by, dby, d2by);
-			Bern3(kGp[q >> 4], bz, dbz, d2bz);
-			const double* c = C + 64*a;
-			double v = 0, g0 = 0, g1 = 0, g2 = 0, h00 = 0, h11 = 0, h22 = 0, h01 = 0, h02 = 0, h12 = 0;
-			for (int k = 0; k < 4; ++k) {
-				double s00 = 0, s10 = 0, s01 = 0, s20 = 0, s11 = 0, s02 = 0;
-				for (int j = 0; j < 4; ++j) {
-					const double* cr = c + 16*k + 4*j;
-					const double x0 = cr[0]*bx[0] + cr[1]*bx[1] + cr[2]*bx[2] + cr[3]*bx[3];
-					const double x1 = cr[0]*dbx[0] + cr[1]*dbx[1] + cr[2]*dbx[2] + cr[3]*dbx[3];
-					const double x2 = cr[0]*d2bx[0] + cr[1]*d2bx[1] + cr[2]*d2bx[2] + cr[3]*d2bx[3];
-					s00 += x0*by[j]; s10 += x1*by[j]; s20 += x2*by[j];
-					s01 += x0*dby[j]; s11 += x1*dby[j]; s02 += x0*d2by[j];
-				}
-				v += s00*bz[k]; g0 += s10*bz[k]; g1 += s01*bz[k]; g2 += s00*dbz[k];
-				h00 += s20*bz[k]; h11 += s02*bz[k]; h22 += s00*d2bz[k];
-				h01 += s11*bz[k]; h02 += s10*dbz[k]; h12 += s01*dbz[k];
+		// ---- P1: basis values, gradients and Laplacians per (a, point): read
+		// from the precomputed table when there is one, else evaluated here
+		if (table) {
+			const int rec = nen*kTableRec;
+			const double* src = table + ((size_t)b0*kQ + (size_t)q0*nen)*kTableRec;
+			for (int i = tid; i < kSlab*rec; i += kThreads) {
+				const int ql = i/rec, r = i - ql*rec, a = r/kTableRec, f = r - a*kTableRec;
+				Brec(basis, qstride, a, ql)[f] = src[i];
 			}
-			const double* gq = geo + q;
-			double inv[9];
-#pragma unroll
-			for (int i = 0; i < 9; ++i) inv[i] = gq[(1 + i)*kQ];
-			double* out = Brec(basis, qstride, a, ql);
-			out[0] = v;
-			out[1] = g0*inv[0] + g1*inv[3] + g2*inv[6];
-			out[2] = g0*inv[1] + g1*inv[4] + g2*inv[7];
-			out[3] = g0*inv[2] + g1*inv[5] + g2*inv[8];
-			out[4] = h00*gq[10*kQ] + h11*gq[11*kQ] + h22*gq[12*kQ]
-				+ 2.0*(h01*gq[13*kQ] + h02*gq[14*kQ] + h12*gq[15*kQ])
-				+ g0*gq[16*kQ] + g1*gq[17*kQ] + g2*gq[18*kQ];
+		} else {
+			for (int it = tid; it < nen*kSlab; it += kThreads) {
+				const int a = it/kSlab, ql = it - a*kSlab;
+				EvalBasis(C + 64*a, geo + q0 + ql, q0 + ql, Brec(basis, qstride, a, ql));
+			}
 		}
 		__syncthreads();
 		// ---- P2: fields at the points (8 lanes per point), tau, fine scales
@@ -214,78 +315,8 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				s[i] += __shfl_xor(s[i], 2, 64);
 				s[i] += __shfl_xor(s[i], 4, 64);
 			}
-			if (part == 0) {
-				const double* gq = geo + q;
-				double inv[3][3];
-				for (int a = 0; a < 3; ++a)
-					for (int x = 0; x < 3; ++x) inv[a][x] = gq[(1 + 3*a + x)*kQ];
-				const double u[3] = {s[0], s[1], s[2]};
-				const double p = s[3];
-				double grad[4][3];
-				for (int f = 0; f < 4; ++f)
-					for (int d = 0; d < 3; ++d) grad[f][d] = s[4 + 3*f + d];
-				double dudt[3];
-				for (int c = 0; c < 3; ++c) dudt[c] = dt > 0.0 ? (u[c] - s[19 + c])/dt : 0.0;
-				double G[3][3] = {}, gv[3] = {};
-				for (int i = 0; i < 3; ++i)
-					for (int j = 0; j < 3; ++j) {
-						for (int k = 0; k < 3; ++k) G[i][j] += inv[k][i]*inv[k][j];
-						gv[i] += inv[j][i];
-					}
-				double GG = 0.0, gg = 0.0, uGu = 0.0;
-				for (int i = 0; i < 3; ++i) {
-					gg += gv[i]*gv[i];
-					for (int j = 0; j < 3; ++j) {
-						GG += G[i][j]*G[i][j];
-						uGu += u[i]*G[i][j]*u[j];
-					}
-				}
-				const double temporal = dt > 0.0 ? 4.0/(dt*dt) : 0.0;
-				const double tau_m = 1.0/sqrt(temporal + uGu + (1.0/12.0)*nu*nu*GG);
-				const double tau_c = 1.0/(tau_m*gg);
-				double fu[3];
-				for (int c = 0; c < 3; ++c) {
-					const double conv = u[0]*grad[c][0] + u[1]*grad[c][1] + u[2]*grad[c][2];
-					fu[c] = -tau_m*(dudt[c] + conv + grad[3][c]/rho - nu*s[16 + c]);
-				}
-				const double div = grad[0][0] + grad[1][1] + grad[2][2];
-				const double fp = -rho*tau_c*div;
-				double* r = qd + ql*kQd;
-				r[0] = gq[0];
-				r[1] = tau_m;
-				r[2] = tau_c;
-				for (int c = 0; c < 3; ++c) r[3 + c] = u[c];
-				const double w[3] = {u[0] + fu[0], u[1] + fu[1], u[2] + fu[2]};
-				for (int c = 0; c < 3; ++c) {
-					r[6 + c] = rho*(dudt[c] + w[0]*grad[c][0] + w[1]*grad[c][1] + w[2]*grad[c][2]);
-					for (int d = 0; d < 3; ++d)
-						r[9 + 3*c + d] = mu*(grad[c][d] + grad[d][c]) - rho*fu[c]*w[d] - (c == d ? p + fp : 0.0);
-				}
-				r[18] = div;
-				for (int c = 0; c < 3; ++c) r[19 + c] = fu[c];
-				for (int c = 0; c < 3; ++c) r[22 + c] = w[c];
-				for (int c = 0; c < 3; ++c)
-					for (int d = 0; d < 3; ++d) r[25 + 3*c + d] = grad[c][d];
-				if (kExact) {
-					// d(u')/d(u_f) = N_b K[.][f] - delta tau_M s_b with
-					// K_cf = tau_M^3 strong_c (G u)_f - tau_M du_c/dx_f, and
-					// R = grad u + grad u K, c1_f = rho tau_C tau_M^2 div (G u)_f.
-					double Gu[3];
-					for (int i = 0; i < 3; ++i) Gu[i] = G[i][0]*u[0] + G[i][1]*u[1] + G[i][2]*u[2];
-					const double tm3 = tau_m*tau_m*tau_m;
-					double K[3][3];
-					for (int c = 0; c < 3; ++c) {
-						const double strong = -fu[c]/tau_m;
-						for (int f = 0; f < 3; ++f) K[c][f] = tm3*strong*Gu[f] - tau_m*grad[c][f];
-					}
-					for (int c = 0; c < 3; ++c)
-						for (int f = 0; f < 3; ++f) {
-							r[34 + 3*c + f] = K[c][f];
-							r[43 + 3*c + f] = grad[c][f] + grad[c][0]*K[0][f] + grad[c][1]*K[1][f] + grad[c][2]*K[2][f];
-						}
-					for (int f = 0; f < 3; ++f) r[52 + f] = rho*tau_c*tau_m*tau_m*div*Gu[f];
-				}
-			}
+			if (part == 0)
+				PointRecord(s, geo + q, rho, mu, nu, dt, kExact && kJac, qd + ql*kQd);
 		}
 		__syncthreads();
 		// ---- P3: u.grad N per (a, point); residual rows of this tile
@@ -315,7 +346,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 		}
 		__syncthreads();
 		// ---- P4 (MFMA): 4 points per instruction, wave w = test field w
-		if (kMfma && args.with_jacobian) {
+		if (kMfma && kJac) {
 			const int wave = tid >> 6, lane = tid & 63;
 			const int fl = lane & 15, kq = lane >> 4;
 			const int x = wave, y1 = (wave + 1)%3, y2 = (wave + 2)%3;
@@ -540,25 +571,43 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	}
 
 	// ---- scatter (exclusive by colour: plain read-modify-write)
-	if (kMfma && args.with_jacobian) {
+	if (kMfma && kJac) {
+		// The 16 components of a (test, trial) block sit in four waves: they are
+		// gathered in LDS (over the basis and point-record area, free after the
+		// last slab), 16 test functions at a time, so that every block is then
+		// updated as one contiguous 128-byte row by 16 consecutive lanes.
+		constexpr int kBs = 17;  // padded block stride of the staging area [16][32][kBs]
+		static_assert(16*32*kBs <= kMaxBasis*8 + kSlab*(32*kBasisRec + 1) + kSlab*kQd,
+			"staging area must fit below the residual accumulators");
+		double* stage = lds;
 		const int wave = tid >> 6, lane = tid & 63;
 		const long long pbase = args.pair_offsets[e];
 		const int x = wave, y1 = (wave + 1)%3, y2 = (wave + 2)%3;
 		const int comp[4] = {wave < 3 ? 5*x : 12, wave < 3 ? 4*x + y1 : 13, wave < 3 ? 4*x + y2 : 14,
 			wave < 3 ? 4*x + 3 : 15};
 #pragma unroll
-		for (int c = 0; c < 4; ++c)
+		for (int ti = 0; ti < 2; ++ti) {
+			const int rows = min(16, nen - 16*ti);
+			if (rows > 0) {
 #pragma unroll
-			for (int t = 0; t < 4; ++t) {
-				const int ti = t >> 1, tj = t & 1;
-				const int b = 16*tj + (lane & 15);
+				for (int c = 0; c < 4; ++c)
 #pragma unroll
-				for (int i = 0; i < 4; ++i) {
-					const int a = 16*ti + 4*i + (lane >> 4);
-					if (a < nen && b < nen)
-						args.jac[(size_t)args.pair_block[pbase + (long long)a*nen + b]*16 + comp[c]] += acc[c][t][i];
+					for (int tj = 0; tj < 2; ++tj) {
+						const int b = 16*tj + (lane & 15);
+#pragma unroll
+						for (int i = 0; i < 4; ++i)
+							stage[((4*i + (lane >> 4))*32 + b)*kBs + comp[c]] = acc[c][2*ti + tj][i];
+					}
+				__syncthreads();
+				for (int idx = tid; idx < rows*nen*16; idx += kThreads) {
+					const int blk = idx >> 4, cmp = idx & 15;
+					const int al = blk/nen, b = blk - al*nen;
+					const long long pair = pbase + (long long)(16*ti + al)*nen + b;
+					args.jac[(size_t)args.pair_block[pair]*16 + cmp] += stage[(al*32 + b)*kBs + cmp];
 				}
+				__syncthreads();
 			}
+		}
 	}
 	if (pair_active) {
 		const long long pbase = args.pair_offsets[e];
@@ -778,23 +827,154 @@ __global__ void FaceTractionKernel(int count, const int* __restrict__ felem, con
 	}
 }
 
+// Residual-only assembly from the basis table, for elements of at most 32
+// basis functions: one workgroup of 256 threads per element and no slabs.
+// Phase A: four lanes per point sum the fields over the basis functions and
+// the first of them stores the point record; phase B: eight lanes per basis
+// function sum its residual rows over the 64 points in a fixed order, so the
+// result is deterministic.  Without the LDS basis staging and tangent
+// registers of AssembleNsKernel, several elements share a compute unit.
+constexpr int kQr = 35;  // point record r[0..33] in LDS, odd stride
+
+__global__ void __launch_bounds__(kThreads, 2) ResidualKernel(AsmArgs args)
+{
+	__shared__ double nodal[32*8];
+	__shared__ double qd[kQ*kQr];
+	__shared__ int lconn[32];
+	const int tid = threadIdx.x;
+	const int item = blockIdx.x;
+	if (item >= args.count) return;
+	const int e = args.elements[item];
+	const int b0 = args.eoff[e];
+	const int nen = args.eoff[e + 1] - b0;
+	const double rho = args.rho, mu = args.mu, dt = args.dt;
+	const double nu = mu/rho;
+	for (int a = tid; a < nen; a += kThreads) {
+		const int node = args.conn[b0 + a];
+		lconn[a] = node;
+		for (int f = 0; f < 4; ++f) nodal[8*a + f] = args.state[4*(size_t)node + f];
+		for (int f = 0; f < 3; ++f) nodal[8*a + 4 + f] = args.previous[4*(size_t)node + f];
+	}
+	__syncthreads();
+	const double* tab = args.table + (size_t)b0*kQ*kTableRec;
+	{
+		const int q = tid >> 2, part = tid & 3;
+		double s[22] = {};
+		for (int a = part; a < nen; a += 4) {
+			const double* bb = tab + ((size_t)q*nen + a)*kTableRec;
+			const double N = bb[0], gx = bb[1], gy = bb[2], gz = bb[3], L = bb[4];
+			const double* un = nodal + 8*a;
+#pragma unroll
+			for (int f = 0; f < 4; ++f) {
+				s[f] += un[f]*N;
+				s[4 + 3*f] += un[f]*gx;
+				s[5 + 3*f] += un[f]*gy;
+				s[6 + 3*f] += un[f]*gz;
+			}
+#pragma unroll
+			for (int c = 0; c < 3; ++c) {
+				s[16 + c] += un[c]*L;
+				s[19 + c] += un[4 + c]*N;
+			}
+		}
+#pragma unroll
+		for (int i = 0; i < 22; ++i) {
+			s[i] += __shfl_xor(s[i], 1, 64);
+			s[i] += __shfl_xor(s[i], 2, 64);
+		}
+		if (part == 0)
+			PointRecord(s, args.geometry + (size_t)e*kGeomFields*kQ + q, rho, mu, nu, dt, false, qd + q*kQr);
+	}
+	__syncthreads();
+	if (tid < nen*8) {
+		const int a = tid >> 3, sub = tid & 7;
+		double R[4] = {0.0, 0.0, 0.0, 0.0};
+		for (int q = sub; q < kQ; q += 8) {
+			const double* bb = tab + ((size_t)q*nen + a)*kTableRec;
+			const double* r = qd + q*kQr;
+			const double N = bb[0], gx = bb[1], gy = bb[2], gz = bb[3], m = r[0];
+#pragma unroll
+			for (int c = 0; c < 3; ++c) R[c] -= (N*r[6 + c] + gx*r[9 + 3*c] + gy*r[10 + 3*c] + gz*r[11 + 3*c])*m;
+			R[3] -= (N*r[18] - gx*r[19] - gy*r[20] - gz*r[21])*m;
+		}
+#pragma unroll
+		for (int c = 0; c < 4; ++c) {
+			R[c] += __shfl_xor(R[c], 1, 64);
+			R[c] += __shfl_xor(R[c], 2, 64);
+			R[c] += __shfl_xor(R[c], 4, 64);
+		}
+		if (sub == 0) {
+			const size_t node = lconn[a];
+			for (int c = 0; c < 4; ++c) args.negres[4*node + c] += R[c];
+		}
+	}
+}
+
+// Basis records of every (point, function) of one element per workgroup, in
+// the table layout [q][a][kTableRec] the assembly reads slab by slab.
+__global__ void __launch_bounds__(256) BasisTableKernel(const int* __restrict__ eoff, const int* __restrict__ xoff,
+	const uint8_t* __restrict__ xcol, const double* __restrict__ xval, const double* __restrict__ geometry,
+	double* __restrict__ table)
+{
+	extern __shared__ double C[];  // [nen][64] extraction rows
+	const int e = blockIdx.x;
+	const int b0 = eoff[e], nen = eoff[e + 1] - b0;
+	for (int i = threadIdx.x; i < nen*64; i += blockDim.x) C[i] = 0.0;
+	__syncthreads();
+	for (int a = threadIdx.x; a < nen; a += blockDim.x)
+		for (int k = xoff[b0 + a]; k < xoff[b0 + a + 1]; ++k) C[64*a + xcol[k]] = xval[k];
+	__syncthreads();
+	const double* geo = geometry + (size_t)e*kGeomFields*kQ;
+	double* dst = table + (size_t)b0*kQ*kTableRec;
+	for (int it = threadIdx.x; it < kQ*nen; it += blockDim.x) {
+		const int q = it/nen, a = it - q*nen;
+		double rec[kTableRec];
+		EvalBasis(C + 64*a, geo + q, q, rec);
+#pragma unroll
+		for (int f = 0; f < kTableRec; ++f) dst[(size_t)it*kTableRec + f] = rec[f];
+	}
+}
+
 }  // namespace
+
+void SetBasisTable(tfi_mesh* m, int enable)
+{
+	tfi_ctx* ctx = m->ctx;
+	const size_t count = (size_t)m->basis_entries*kQ*kTableRec;
+	if (enable < 0) {
+		size_t free_bytes = 0, total_bytes = 0;
+		Check(hipMemGetInfo(&free_bytes, &total_bytes), "hipMemGetInfo");
+		enable = count*sizeof(double) <= free_bytes/2 ? 1 : 0;
+	}
+	if (!enable) {
+		Sync(ctx);
+		m->basis_table.Release();
+		return;
+	}
+	if (m->basis_table.size() == count) return;
+	m->basis_table.Allocate(ctx, count);
+	hipLaunchKernelGGL(BasisTableKernel, dim3(m->elements), dim3(256), sizeof(double)*64*m->max_nen, ctx->stream,
+		m->eoff.data(), m->xoff.data(), m->xcol.data(), m->xval.data(), m->geometry.data(), m->basis_table.data());
+	TFI_LAUNCH_CHECK("BasisTableKernel");
+	Sync(ctx);
+}
 
 void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const NsParams& p, double* jacobian,
 	double* negative_residual)
 {
 	const bool exact = p.tangent == TFI_TANGENT_EXACT;
+	const bool jac = jacobian != nullptr;
 	const int nen_max = m->max_nen;
 	// elements of at most 32 basis functions take the FP64-MFMA tangent
-	const bool mfma = nen_max <= 32;
-	const int variant = (exact ? 1 : 0) + (mfma ? 2 : 0);
-	static const void* const kKernels[4] = {
-		reinterpret_cast<const void*>(AssembleNsKernel<false, false>),
-		reinterpret_cast<const void*>(AssembleNsKernel<true, false>),
-		reinterpret_cast<const void*>(AssembleNsKernel<false, true>),
-		reinterpret_cast<const void*>(AssembleNsKernel<true, true>)};
+	const bool mfma = jac && nen_max <= 32;
+	const int variant = jac ? (exact ? 1 : 0) + (mfma ? 2 : 0) : 4;
+	static const void* const kKernels[5] = {
+		reinterpret_cast<const void*>(AssembleNsKernel<false, false, true>),
+		reinterpret_cast<const void*>(AssembleNsKernel<true, false, true>),
+		reinterpret_cast<const void*>(AssembleNsKernel<false, true, true>),
+		reinterpret_cast<const void*>(AssembleNsKernel<true, true, true>),
+		reinterpret_cast<const void*>(AssembleNsKernel<false, false, false>)};
 	tfi_ctx* ctx = m->ctx;
-	StreamTimer timer(ctx, &ctx->timers.assembly_ms);
 	if (jacobian) Fill(ctx, jacobian, 0.0, m->nnzb*16);
 	Fill(ctx, negative_residual, 0.0, 4LL*m->nodes);
 	const int tiles_b = (nen_max + 1)/2;
@@ -825,45 +1005,64 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	a.dt = p.dt;
 	a.jac = jacobian;
 	a.negres = negative_residual;
-	a.with_jacobian = jacobian != nullptr;
+	a.table = m->basis_table.size() ? m->basis_table.data() : nullptr;
 	a.qstride = qstride;
 	if (lds > 65536) {
-		static size_t raised[4] = {0, 0, 0, 0};
+		static size_t raised[5] = {0, 0, 0, 0, 0};
 		if (lds > raised[variant]) {
 			Check(hipFuncSetAttribute(kKernels[variant], hipFuncAttributeMaxDynamicSharedMemorySize,
 				static_cast<int>(lds)), "hipFuncSetAttribute");
 			raised[variant] = lds;
 		}
 	}
-	for (int c = 0; c < m->colors; ++c) {
+	StreamTimer timer(ctx, jac ? &ctx->timers.assembly_ms : &ctx->timers.residual_ms);
+	if (!jac && a.table && nen_max <= 32) {
+		for (int c = 0; c < m->colors; ++c) {
+			a.elements = m->color_elements.data() + m->color_offsets[c];
+			a.count = m->color_offsets[c + 1] - m->color_offsets[c];
+			if (a.count == 0) continue;
+			hipLaunchKernelGGL(ResidualKernel, dim3(a.count), dim3(kThreads), 0, ctx->stream, a);
+			TFI_LAUNCH_CHECK("ResidualKernel");
+		}
+	} else for (int c = 0; c < m->colors; ++c) {
 		a.elements = m->color_elements.data() + m->color_offsets[c];
 		a.count = m->color_offsets[c + 1] - m->color_offsets[c];
 		if (a.count == 0) continue;
+		const dim3 grid(a.count*tiles), block(kThreads);
 		switch (variant) {
-		case 0: hipLaunchKernelGGL((AssembleNsKernel<false, false>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
-		case 1: hipLaunchKernelGGL((AssembleNsKernel<true, false>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
-		case 2: hipLaunchKernelGGL((AssembleNsKernel<false, true>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
-		default: hipLaunchKernelGGL((AssembleNsKernel<true, true>), dim3(a.count*tiles), dim3(kThreads), lds, ctx->stream, a); break;
+		case 0: hipLaunchKernelGGL((AssembleNsKernel<false, false, true>), grid, block, lds, ctx->stream, a); break;
+		case 1: hipLaunchKernelGGL((AssembleNsKernel<true, false, true>), grid, block, lds, ctx->stream, a); break;
+		case 2: hipLaunchKernelGGL((AssembleNsKernel<false, true, true>), grid, block, lds, ctx->stream, a); break;
+		case 3: hipLaunchKernelGGL((AssembleNsKernel<true, true, true>), grid, block, lds, ctx->stream, a); break;
+		default: hipLaunchKernelGGL((AssembleNsKernel<false, false, false>), grid, block, lds, ctx->stream, a); break;
 		}
 		TFI_LAUNCH_CHECK("AssembleNsKernel");
 	}
-	ctx->timers.assembly_calls += 1;
-	ctx->timers.assembly_flops += AssemblyFlops(m, jacobian != nullptr, exact);
+	if (jac) {
+		ctx->timers.assembly_calls += 1;
+		ctx->timers.assembly_launches += m->colors;
+		ctx->timers.assembly_flops += AssemblyFlops(m, true, exact);
+	} else {
+		ctx->timers.residual_calls += 1;
+		ctx->timers.residual_flops += AssemblyFlops(m, false, exact);
+	}
 }
 
 // Counted arithmetic of AssembleNsKernel per element with nen basis functions
-// and Q = 64 points: sum-factorised basis (656 flops per (a, q)), push-forward
-// (42), field interpolation (44), u.grad N and residual row (38), per-point
-// scalars (~300 per q) and the 2x2-tiled tangent (72 flops per (a, b, q) for
-// the legacy tangent; 132 per (a, b, q) plus ~100 per (a, q) for the exact one).
+// and Q = 64 points: sum-factorised basis and push-forward (656 + 42 flops per
+// (a, q), none when the basis table supplies them), field interpolation (44),
+// u.grad N and residual row (38), per-point scalars (~300 per q) and the
+// tangent (72 flops per (a, b, q) for the legacy tangent; 132 per (a, b, q)
+// plus ~100 per (a, q) for the exact one).
 double AssemblyFlops(const tfi_mesh* m, bool with_jacobian, bool exact)
 {
+	const double basis = m->basis_table.size() ? 0.0 : 656.0 + 42.0;
 	double total = 0.0;
 	for (int e = 0; e < m->elements; ++e) {
 		const double nen = m->h_eoff[e + 1] - m->h_eoff[e];
 		double pair = 0.0;
 		if (with_jacobian) pair = exact ? 132.0*nen*nen + 100.0*nen : 72.0*nen*nen;
-		total += kQ*(nen*(656.0 + 42.0 + 44.0 + 38.0) + 300.0 + pair);
+		total += kQ*(nen*(basis + 44.0 + 38.0) + 300.0 + pair);
 	}
 	return total;
 }

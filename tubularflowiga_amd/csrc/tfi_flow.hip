@@ -138,6 +138,7 @@ struct tfi_flow {
 	tfi_ctx* ctx = nullptr;
 	tfi_flow_options opt{};
 	DevBuf<double> state, previous, update, rhs, traction, jac, vtarget, ptarget, face_flow, cont_partial;
+	DevBuf<double> guess_base, resid_base;  // learned mode: extrapolation base and its residual
 	DevBuf<int> vmask, pmask, face_elem, face_id;
 	std::vector<int> face_label;
 	bool has_traction = false, has_boundary = false;
@@ -234,6 +235,7 @@ void tfi_flow_default_options(tfi_flow_options* o)
 	o->amg_coarse_nodes = 0;
 	o->amg_gs_min_rows = 0;
 	o->amg_gs_sweeps = 0;
+	o->chord_polish = 1;
 }
 
 int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** out)
@@ -252,6 +254,10 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 		f->state.Allocate(ctx, n);
 		f->previous.Allocate(ctx, n);
 		f->update.Allocate(ctx, n);
+		if (f->opt.warm_start == TFI_WARM_LEARNED) {
+			f->guess_base.Allocate(ctx, n);
+			f->resid_base.Allocate(ctx, n);
+		}
 		f->rhs.Allocate(ctx, n);
 		f->traction.Allocate(ctx, n);
 		f->jac.Allocate(ctx, mesh->nnzb*16);
@@ -350,6 +356,11 @@ int tfi_flow_commit_step(tfi_flow* f)
 	});
 }
 
+// A residual check that misses the Newton tolerance by at most this factor is
+// closed by a chord step (a solve with the tangent and preconditioner already
+// set up, to a tenth of the tolerance) instead of a new tangent.
+constexpr double kChordWindow = 1000.0;
+
 int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 {
 	return Guard(f->ctx, [&] {
@@ -383,9 +394,32 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			const auto a0 = Clock::now();
 			const double base = assemble(false, nullptr);
 			Copy(ctx, f->state.data(), f->update.data(), n);  // keep the plain guess
-			f->warm->Predict(f->opt.warm_start, f->state.data(), f->phase, f->phase_count);
-			SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
-				f->state.data());
+			double blend = 1.0;
+			if (f->opt.warm_start == TFI_WARM_LEARNED) {
+				// Residual line search along the learned correction: with u_b the
+				// extrapolation base and u_L = u_b + g c the network's guess, take
+				// u = u_b + t (u_L - u_b) with t minimising the linearised
+				// ||r_b + t (r_L - r_b)||, t in [0, 2].  t = 0 is the base alone.
+				f->warm->Predict(TFI_WARM_EXTRAPOLATE4, f->state.data(), f->phase, f->phase_count);
+				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
+					f->state.data());
+				const double nb = assemble(false, nullptr);
+				Copy(ctx, f->rhs.data(), f->resid_base.data(), n);
+				Copy(ctx, f->state.data(), f->guess_base.data(), n);
+				f->warm->Predict(TFI_WARM_LEARNED, f->state.data(), f->phase, f->phase_count);
+				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
+					f->state.data());
+				const double nl = assemble(false, nullptr);
+				const double bl = Dot(ctx, f->resid_base.data(), f->rhs.data(), n);
+				const double dd = nl*nl - 2.0*bl + nb*nb;
+				blend = dd > 1e-24*nb*nb ? std::min(2.0, std::max(0.0, (nb*nb - bl)/dd)) : 1.0;
+				Axpby(ctx, 1.0 - blend, f->guess_base.data(), blend, f->state.data(), n);
+			} else {
+				f->warm->Predict(f->opt.warm_start, f->state.data(), f->phase, f->phase_count);
+				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
+					f->state.data());
+			}
+			rep.warm_start_blend = blend;
 			const double guessed = assemble(true, &pre_mt);
 			pre_asm = std::chrono::duration<double>(Clock::now() - a0).count();
 			rep.assembly_s += pre_asm;
@@ -417,19 +451,43 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				// When quadratic convergence predicts this iterate converged, check
 				// it with a residual-only assembly first and build the tangent only
 				// if it did not.
+				// (With chord polish, "converged" includes a miss a chord step closes.)
 				const double c = std::max(contraction, f->newton_contraction);
+				const double reach = f->opt.chord_polish ? kChordWindow : 0.25;
 				const bool predict = prev_norm > 0.0 && initial > 0.0 && c > 0.0
-					&& c*prev_norm*prev_norm <= 0.25*std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*initial);
+					&& c*prev_norm*prev_norm <= reach*std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*initial);
 				have_jacobian = !predict;
 				norm = assemble(have_jacobian, &mt);
 			}
 			const double tol_now = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*(initial < 0.0 ? norm : initial));
-			if (!have_jacobian && !(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol))
-				norm = assemble(true, &mt);  // prediction missed: full assembly at the same state
+			double chord_s = 0.0;  // linear-solve time of a chord step inside this assembly phase
+			if (!have_jacobian && !(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol)) {
+				// The prediction missed.  A small miss is closed by a chord step: one
+				// more solve with the tangent and preconditioner of the last iteration
+				// (still set up), to a tenth of the tolerance, then a residual check.
+				if (f->opt.chord_polish && norm <= kChordWindow*tol_now) {
+					const auto c0 = Clock::now();
+					const double eta = std::min(0.1, std::max(f->opt.linear_rtol, 0.1*tol_now/norm));
+					Fill(ctx, f->update.data(), 0.0, n);
+					const auto r = f->solver->Solve(f->rhs.data(), f->update.data(), f->opt.linear_maximum_iterations,
+						eta);
+					Sync(ctx);
+					chord_s = std::chrono::duration<double>(Clock::now() - c0).count();
+					rep.linear_s += chord_s;
+					rep.linear_iterations += r.iterations;
+					if (r.converged) {
+						Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
+						++rep.chord_steps;
+						norm = assemble(false, &mt);
+					}
+				}
+				if (!(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol))
+					norm = assemble(true, &mt);  // full assembly at the current state
+			}
 			Sync(ctx);
 			const auto a1 = Clock::now();
-			rec.assembly_s = std::chrono::duration<double>(a1 - a0).count() + (it == 0 ? pre_asm : 0.0);
-			rep.assembly_s += std::chrono::duration<double>(a1 - a0).count();
+			rec.assembly_s = std::chrono::duration<double>(a1 - a0).count() - chord_s + (it == 0 ? pre_asm : 0.0);
+			rep.assembly_s += std::chrono::duration<double>(a1 - a0).count() - chord_s;
 			rec.residual_l2 = norm;
 			rec.continuity_l2 = mt.continuity_l2;
 			rec.continuity_sum = mt.continuity_sum;
@@ -437,7 +495,10 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			rec.relative_mass_imbalance = mt.imbalance;
 			if (initial < 0.0) initial = norm;
 			if (it == 0) rep.initial_residual = norm;
-			if (prev_norm > 0.0) contraction = std::max(contraction, norm/(prev_norm*prev_norm));
+			// quadratic-convergence constant ||r_k|| / ||r_{k-1}||^2, from residuals
+			// clear of the rounding floor (a floor-level ||r_k|| says nothing about it)
+			if (prev_norm > 0.0 && norm > 1e-11*initial)
+				contraction = std::max(contraction, norm/(prev_norm*prev_norm));
 			const double tol = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*initial);
 			rep.tolerance = tol;
 			rep.final_residual = norm;
@@ -532,6 +593,14 @@ int tfi_flow_warm_configure(tfi_flow* f, const tfi_warm_options* o)
 int tfi_flow_warm_stats(tfi_flow* f, int* snapshots, double* last_loss, int* accepted, int* rejected)
 {
 	return Guard(f->ctx, [&] { f->warm->Stats(snapshots, last_loss, accepted, rejected); });
+}
+
+int tfi_flow_warm_gate(tfi_flow* f, double* gate)
+{
+	return Guard(f->ctx, [&] {
+		if (!gate) throw Error("tfi_flow_warm_gate needs an output pointer");
+		*gate = f->warm->Gate();
+	});
 }
 
 // ---------------------------------------------------------------- host in/out kernels

@@ -148,6 +148,9 @@ struct tfi_mesh {
 	tfi::DevBuf<int> pair_block;             // [sum nen^2]
 	// geometry [E][kGeomFields][64]
 	tfi::DevBuf<double> geometry;
+	// optional precomputed basis records: element e at eoff[e]*64*5, [q][a][5]
+	// (N, grad N, lap N); empty when assembly evaluates the basis itself
+	tfi::DevBuf<double> basis_table;
 	double min_det = 0.0;
 	long long bad_samples = 0;
 	// host copies
@@ -211,6 +214,9 @@ struct NsParams {
 void AssembleNs(tfi_mesh* mesh, const double* state, const double* previous, const NsParams& p, double* jacobian,
 	double* negative_residual);
 double AssemblyFlops(const tfi_mesh* mesh, bool with_jacobian, bool exact);
+// Builds (enable) or frees the per-element basis table; enable < 0 builds it
+// when it takes at most half of the free device memory.
+void SetBasisTable(tfi_mesh* mesh, int enable);
 void ApplyDirichlet(tfi_mesh* mesh, const int* vmask, const int* pmask, const double* vtarget, const double* ptarget,
 	const double* state, double* jacobian, double* rhs);
 // state[constrained] = target

@@ -334,6 +334,7 @@ void BuildMesh(tfi_mesh* m)
 	bad.Download(&hb, 1);
 	m->min_det = *std::min_element(hmin.begin(), hmin.end());
 	m->bad_samples = static_cast<long long>(hb);
+	SetBasisTable(m, -1);
 }
 
 }  // namespace tfi
@@ -422,7 +423,20 @@ int tfi_mesh_stats_get(tfi_mesh* m, tfi_mesh_stats* out)
 	out->colors = m->colors;
 	out->minimum_determinant = m->min_det;
 	out->bad_samples = m->bad_samples;
+	out->basis_table_bytes = static_cast<long long>(m->basis_table.size()*sizeof(double));
 	return TFI_OK;
+}
+
+int tfi_mesh_basis_table(tfi_mesh* m, int enable)
+{
+	if (!m) return TFI_ERROR;
+	try {
+		SetBasisTable(m, enable);
+		return TFI_OK;
+	} catch (const std::exception& e) {
+		m->ctx->error = e.what();
+		return TFI_ERROR;
+	}
 }
 
 int tfi_mesh_pattern(tfi_mesh* m, int32_t* row_offsets, int32_t* columns, int32_t* diagonal)

@@ -192,6 +192,31 @@ __global__ void LossKernel(int B, const float* __restrict__ Y, const float* __re
 	if (threadIdx.x == 0) partial[blockIdx.x] = lds[0]/static_cast<float>(total);
 }
 
+// <Y, T> and <Y, Y> partial sums over B rows (one pair per block)
+__global__ void GateKernel(int B, const float* __restrict__ Y, const float* __restrict__ T, double* __restrict__ partial)
+{
+	__shared__ double lds[2][kT];
+	double yt = 0.0, yy = 0.0;
+	for (int i = blockIdx.x*blockDim.x + threadIdx.x; i < B*kOut; i += gridDim.x*blockDim.x) {
+		yt += static_cast<double>(Y[i])*T[i];
+		yy += static_cast<double>(Y[i])*Y[i];
+	}
+	lds[0][threadIdx.x] = yt;
+	lds[1][threadIdx.x] = yy;
+	__syncthreads();
+	for (int o = kT/2; o > 0; o >>= 1) {
+		if (threadIdx.x < o) {
+			lds[0][threadIdx.x] += lds[0][threadIdx.x + o];
+			lds[1][threadIdx.x] += lds[1][threadIdx.x + o];
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		partial[2*blockIdx.x] = lds[0][0];
+		partial[2*blockIdx.x + 1] = lds[1][0];
+	}
+}
+
 __global__ void AdamKernel(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
 	float* __restrict__ v, float lr, float b1, float b2, float c1, float c2, float eps)
 {
@@ -270,12 +295,12 @@ __global__ void TargetKernel(int count, const int* __restrict__ rows, const doub
 }
 
 __global__ void ApplyPredictionKernel(int nodes, Hist H, const float* __restrict__ Y, const double* __restrict__ scales,
-	double* __restrict__ state)
+	double gate, double* __restrict__ state)
 {
 	const int t = blockIdx.x*blockDim.x + threadIdx.x;
 	if (t >= nodes*4) return;
 	const int f = t & 3;
-	state[t] = Extrapolate(H, t) + scales[4*kOrder + f]*static_cast<double>(Y[t]);
+	state[t] = Extrapolate(H, t) + gate*scales[4*kOrder + f]*static_cast<double>(Y[t]);
 }
 
 constexpr int kScales = 4*(kOrder + 1);
@@ -422,8 +447,10 @@ struct Mlp::Impl {
 	DevBuf<float> params, grads, adam_m, adam_v;
 	long long off[7] = {};
 	DevBuf<float> X, H1, H2, Y, T, D3, D2, D1, loss_partial, split;
-	DevBuf<double> scales, scale_partial, phase;
+	DevBuf<double> scales, scale_partial, phase, gate_partial;
 	DevBuf<short> Xb, W1t, W2t, W3t;
+	DevBuf<float> Xall, Yall;  // inference features / outputs of every node
+	double gate = 0.0;         // least-squares scale of the correction on the latest unseen snapshot
 	int adam_t = 0;
 	uint64_t rng = 20261202ULL;
 
@@ -596,6 +623,11 @@ Mlp::Mlp(tfi_mesh* mesh, const tfi_warm_options& options) : impl_(std::make_uniq
 
 Mlp::~Mlp() = default;
 
+double Mlp::gate() const
+{
+	return impl_->gate;
+}
+
 void Mlp::Train(const double* next, const double* const* history, const double* phase)
 {
 	Impl& I = *impl_;
@@ -605,6 +637,35 @@ void Mlp::Train(const double* next, const double* const* history, const double* 
 	for (int j = 0; j <= kOrder; ++j) H.h[j] = history[j];
 	I.Scales(H);
 	I.phase.Upload(phase, 8);
+	// Gate: before it trains on this snapshot, the network predicts a batch of
+	// it; the scale g minimising ||T - g Y|| (clamped to [0, 1]) is what the
+	// next prediction applies, so a correction that does not help is dropped
+	// and the prediction falls back to the extrapolation base.
+	{
+		hipLaunchKernelGGL(BatchRowsKernel, dim3((I.batch + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch, N,
+			static_cast<unsigned long long>(SplitMix(I.rng)), I.rows.data());
+		TFI_LAUNCH_CHECK("BatchRowsKernel");
+		const long long items = (long long)I.batch*kIn;
+		hipLaunchKernelGGL(FeatureKernel, dim3((items + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch, I.rows.data(),
+			I.stencil.data(), I.pos.data(), I.hscale.data(), H, I.scales.data(), I.phase.data(), I.X.data());
+		TFI_LAUNCH_CHECK("FeatureKernel");
+		hipLaunchKernelGGL(TargetKernel, dim3((I.batch*kOut + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch,
+			I.rows.data(), next, H, I.scales.data(), I.T.data());
+		TFI_LAUNCH_CHECK("TargetKernel");
+		I.Forward(I.batch, I.X.data());
+		constexpr int kGateBlocks = 32;
+		if (I.gate_partial.size() < 2*kGateBlocks) I.gate_partial.Allocate(ctx, 2*kGateBlocks);
+		hipLaunchKernelGGL(GateKernel, dim3(kGateBlocks), dim3(kT), 0, ctx->stream, I.batch, I.Y.data(), I.T.data(),
+			I.gate_partial.data());
+		TFI_LAUNCH_CHECK("GateKernel");
+		double part[2*kGateBlocks], yt = 0.0, yy = 0.0;
+		I.gate_partial.Download(part, 2*kGateBlocks);
+		for (int b = 0; b < kGateBlocks; ++b) {
+			yt += part[2*b];
+			yy += part[2*b + 1];
+		}
+		I.gate = yy > 0.0 ? std::min(1.0, std::max(0.0, yt/yy)) : 0.0;
+	}
 	const int steps = std::max(1, I.opt.train_steps);
 	float loss = 0.0f;
 	for (int step = 0; step < steps; ++step) {
@@ -649,12 +710,17 @@ void Mlp::Predict(const double* const* history, const double* phase, double* sta
 		I.W2t.Allocate(ctx, (size_t)I.H*I.H);
 		I.W3t.Allocate(ctx, (size_t)I.H*kOut);
 	}
-	DevBuf<float> X(ctx, (size_t)N*kIn), Y(ctx, (size_t)N*kOut);
+	if (I.Xall.size() < (size_t)N*kIn) {
+		I.Xall.Allocate(ctx, (size_t)N*kIn);
+		I.Yall.Allocate(ctx, (size_t)N*kOut);
+	}
+	float* X = I.Xall.data();
+	float* Y = I.Yall.data();
 	const long long items = (long long)N*kIn;
 	hipLaunchKernelGGL(FeatureKernel, dim3((items + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, nullptr,
-		I.stencil.data(), I.pos.data(), I.hscale.data(), H, I.scales.data(), I.phase.data(), X.data());
+		I.stencil.data(), I.pos.data(), I.hscale.data(), H, I.scales.data(), I.phase.data(), X);
 	TFI_LAUNCH_CHECK("FeatureKernel");
-	hipLaunchKernelGGL(ToBf16Kernel, dim3(4096), dim3(kT), 0, ctx->stream, items, X.data(), I.Xb.data());
+	hipLaunchKernelGGL(ToBf16Kernel, dim3(4096), dim3(kT), 0, ctx->stream, items, X, I.Xb.data());
 	hipLaunchKernelGGL(TransposeBf16Kernel, dim3((kIn*I.H + kT - 1)/kT), dim3(kT), 0, ctx->stream, kIn, I.H, I.P(0),
 		I.W1t.data());
 	hipLaunchKernelGGL(TransposeBf16Kernel, dim3((I.H*I.H + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.H, I.H, I.P(2),
@@ -665,16 +731,16 @@ void Mlp::Predict(const double* const* history, const double* phase, double* sta
 	const int blocks = (N + 63)/64;
 	if (I.H == 128) {
 		hipLaunchKernelGGL(MlpInferKernel<128>, dim3(blocks), dim3(256), 0, ctx->stream, N, I.Xb.data(), I.W1t.data(),
-			I.P(1), I.W2t.data(), I.P(3), I.W3t.data(), I.P(5), Y.data());
+			I.P(1), I.W2t.data(), I.P(3), I.W3t.data(), I.P(5), Y);
 	} else if (I.H == 64) {
 		hipLaunchKernelGGL(MlpInferKernel<64>, dim3(blocks), dim3(256), 0, ctx->stream, N, I.Xb.data(), I.W1t.data(),
-			I.P(1), I.W2t.data(), I.P(3), I.W3t.data(), I.P(5), Y.data());
+			I.P(1), I.W2t.data(), I.P(3), I.W3t.data(), I.P(5), Y);
 	} else {
 		throw Error("BF16 inference kernels are built for widths 64 and 128");
 	}
 	TFI_LAUNCH_CHECK("MlpInferKernel");
-	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, H, Y.data(),
-		I.scales.data(), state);
+	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, H, Y,
+		I.scales.data(), I.gate, state);
 	TFI_LAUNCH_CHECK("ApplyPredictionKernel");
 	Sync(ctx);
 }

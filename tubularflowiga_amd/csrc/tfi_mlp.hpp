@@ -31,6 +31,7 @@ public:
 	void Predict(const double* const* history, const double* phase, double* state);
 	int snapshots() const { return snapshots_; }
 	double last_loss() const { return last_loss_; }
+	double gate() const;
 
 	// Test hooks: forward + backward of a host batch (x [B][kIn], target [B][4])
 	// with the given fp32 weights; returns loss and gradients.

@@ -70,6 +70,7 @@ public:
 	void Predict(int mode, double* state, const double* phase, int phase_count);
 	void Accepted(bool used);
 	void Stats(int* snapshots, double* last_loss, int* accepted, int* rejected) const;
+	double Gate() const;
 	struct Impl;
 
 private:

@@ -134,4 +134,9 @@ void WarmStart::Stats(int* snapshots, double* last_loss, int* accepted, int* rej
 	if (rejected) *rejected = I.rejected;
 }
 
+double WarmStart::Gate() const
+{
+	return impl_->mlp ? impl_->mlp->gate() : 0.0;
+}
+
 }  // namespace tfi

@@ -125,6 +125,15 @@ class DeviceMesh:
         self.ctx.check(self.lib.tfi_mesh_pattern(self.handle, ro, cols, diag))
         return ro, cols, diag
 
+    def basis_table(self, enable: int):
+        """Build (1) or free (0) the precomputed basis table; -1 builds it if it fits
+        in half of the free device memory.  Returns its size in bytes."""
+        self.ctx.check(self.lib.tfi_mesh_basis_table(self.handle, int(enable)))
+        s = MeshStats()
+        self.ctx.check(self.lib.tfi_mesh_stats_get(self.handle, ctypes.byref(s)))
+        self.stats["basis_table_bytes"] = s.basis_table_bytes
+        return s.basis_table_bytes
+
     def colors(self):
         c = np.zeros(self.host.elements, np.int32)
         self.ctx.check(self.lib.tfi_mesh_colors(self.handle, c))
@@ -294,7 +303,10 @@ class FlowSolver:
         s, l, a, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
         self.ctx.check(self.lib.tfi_flow_warm_stats(self.handle, ctypes.byref(s), ctypes.byref(l), ctypes.byref(a),
                                                     ctypes.byref(r)))
-        return {"snapshots": s.value, "last_loss": l.value, "accepted": a.value, "rejected": r.value}
+        g = ctypes.c_double()
+        self.ctx.check(self.lib.tfi_flow_warm_gate(self.handle, ctypes.byref(g)))
+        return {"snapshots": s.value, "last_loss": l.value, "accepted": a.value, "rejected": r.value,
+                "gate": g.value}
 
     def close(self):
         if self.handle:
