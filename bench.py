@@ -38,7 +38,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol):
+def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol, train_steps=20):
     from tubularflowiga_amd import boundary, device
     vmask, pmask, vt, pt = bc
     flow = device.FlowSolver(dm, density=rho, viscosity=mu, dt=dt, nonlinear_rtol=tol["rtol"],
@@ -46,7 +46,7 @@ def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm,
                              preconditioner=device.TFI_PC_AMG, precision=precision, warm_start=warm,
                              restart=100, linear_maximum_iterations=4000)
     if warm == device.TFI_WARM_LEARNED:
-        flow.configure_warm_start(hidden=128, train_steps=40, learning_rate=1e-3, min_snapshots=2)
+        flow.configure_warm_start(hidden=128, train_steps=train_steps, learning_rate=1e-3, min_snapshots=2)
     flow.set_state(boundary.initial_state(vmask, pmask, vt, pt))
     flow.set_boundary(vmask, pmask, vt, pt)
     flow.set_boundary_faces(*faces)
@@ -165,6 +165,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="ybif_c1")
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=20, help="Adam steps of the predictor per time step")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernel on the workload's pattern plus a 2 GiB copy (for counter passes)")
     ap.add_argument("--assembly-only", action="store_true",
@@ -202,7 +203,8 @@ def main(argv=None):
         dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol)
     log("mixed precision + learned warm start")
     fast, fast_all, fast_timers, fast_state, warm_stats = run_method(
-        dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_MIXED, device.TFI_WARM_LEARNED, tol)
+        dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_MIXED, device.TFI_WARM_LEARNED, tol,
+        a.train_steps)
 
     ms_plain = 1e3 * float(np.mean([r["wall_s"] for r in plain]))
     ms_fast = 1e3 * float(np.mean([r["wall_s"] for r in fast]))

@@ -43,6 +43,46 @@ __device__ inline T WaveSumT(T v)
 	return v;
 }
 
+// This lane's share of row `row` of A x for one block-row lane r: the wave
+// splits the row's blocks over 16 groups of 4 lanes (lane r of a group takes
+// block row r).  In FP32 each lane keeps four blocks in flight (column
+// indices, then block rows and x segments as 16-byte vector loads) so that a
+// row costs about one memory latency instead of one per block; in FP64 the
+// plain loop measured faster (the four 32-byte loads in flight cost
+// occupancy).
+template <class T>
+__device__ inline T RowPartial(const int* __restrict__ ro, const int* __restrict__ cols, const T* __restrict__ vals,
+	const T* __restrict__ x, int row, int sub, int r)
+{
+	typedef T T4 __attribute__((ext_vector_type(4)));
+	const int end = ro[row + 1];
+	T acc = 0;
+	if constexpr (sizeof(T) == 8) {
+		for (int k = ro[row] + sub; k < end; k += 16) {
+			const T* a = vals + (size_t)k*16 + 4*r;
+			const T* xv = x + 4*(size_t)cols[k];
+			acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
+		}
+		return acc;
+	}
+	for (int k0 = ro[row] + sub; k0 < end; k0 += 64) {
+		int c[4];
+#pragma unroll
+		for (int u = 0; u < 4; ++u) c[u] = k0 + 16*u < end ? cols[k0 + 16*u] : -1;
+		T4 a[4], xv[4];
+#pragma unroll
+		for (int u = 0; u < 4; ++u)
+			if (c[u] >= 0) {
+				a[u] = *reinterpret_cast<const T4*>(vals + (size_t)(k0 + 16*u)*16 + 4*r);
+				xv[u] = *reinterpret_cast<const T4*>(x + 4*(size_t)c[u]);
+			}
+#pragma unroll
+		for (int u = 0; u < 4; ++u)
+			if (c[u] >= 0) acc += a[u].x*xv[u].x + a[u].y*xv[u].y + a[u].z*xv[u].z + a[u].w*xv[u].w;
+	}
+	return acc;
+}
+
 // x_out = x_in + omega D^-1 (b - A x_in); x_in == nullptr means x_in = 0.
 template <class T>
 __global__ void __launch_bounds__(256) SmoothKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
@@ -55,11 +95,7 @@ __global__ void __launch_bounds__(256) SmoothKernel(int rows, const int* __restr
 	const int sub = lane >> 2, r = lane & 3;
 	T acc = 0;
 	if (xin) {
-		for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
-			const T* a = vals + (size_t)k*16 + 4*r;
-			const T* xv = xin + 4*(size_t)cols[k];
-			acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
-		}
+		acc = RowPartial(ro, cols, vals, xin, row, sub, r);
 		acc += __shfl_xor(acc, 4, 64);
 		acc += __shfl_xor(acc, 8, 64);
 		acc += __shfl_xor(acc, 16, 64);
@@ -85,11 +121,7 @@ __global__ void __launch_bounds__(256) ResidualKernel(int rows, const int* __res
 	if (row >= rows) return;
 	const int sub = lane >> 2, rr = lane & 3;
 	T acc = 0;
-	for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
-		const T* a = vals + (size_t)k*16 + 4*rr;
-		const T* xv = x + 4*(size_t)cols[k];
-		acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
-	}
+	acc = RowPartial(ro, cols, vals, x, row, sub, rr);
 	acc += __shfl_xor(acc, 4, 64);
 	acc += __shfl_xor(acc, 8, 64);
 	acc += __shfl_xor(acc, 16, 64);
@@ -148,11 +180,7 @@ __global__ void __launch_bounds__(256) GsColorKernel(int count, const int* __res
 	const int row = list[item];
 	const int sub = lane >> 2, r = lane & 3;
 	T acc = 0;
-	for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
-		const T* a = vals + (size_t)k*16 + 4*r;
-		const T* xv = x + 4*(size_t)cols[k];
-		acc += a[0]*xv[0] + a[1]*xv[1] + a[2]*xv[2] + a[3]*xv[3];
-	}
+	acc = RowPartial(ro, cols, vals, static_cast<const T*>(x), row, sub, r);
 	acc += __shfl_xor(acc, 4, 64);
 	acc += __shfl_xor(acc, 8, 64);
 	acc += __shfl_xor(acc, 16, 64);
@@ -717,11 +745,19 @@ void Amg::Impl::Cycle(int l)
 	const T omega = static_cast<T>(kOmegaScale/v.lambda_max);
 	const bool gs = v.gs;
 	const int ncol = static_cast<int>(v.color_offsets.size()) - 1;
-	auto gs_sweep = [&](bool forward) {
+	// after_forward: this backward sweep directly follows a forward one, whose
+	// last colour it would repeat with nothing changed in between (a colour's
+	// update depends only on the other colours), so that colour is skipped
+	auto gs_sweep = [&](bool forward, bool after_forward = false) {
+		bool skip = after_forward;
 		for (int cc = 0; cc < ncol; ++cc) {
 			const int c = forward ? cc : ncol - 1 - cc;
 			const int count = v.color_offsets[c + 1] - v.color_offsets[c];
 			if (count == 0) continue;
+			if (skip) {
+				skip = false;
+				continue;
+			}
 			hipLaunchKernelGGL(GsColorKernel<T>, dim3((count + 3)/4), dim3(256), 0, ctx->stream, count,
 				v.color_rows.data() + v.color_offsets[c], v.ro, v.cols, vals, inv, b, x);
 		}
@@ -733,7 +769,7 @@ void Amg::Impl::Cycle(int l)
 		if (f32) Check(hipMemsetAsync(x, 0, n*sizeof(T), ctx->stream), "memset");
 		else Fill(ctx, reinterpret_cast<double*>(x), 0.0, n);
 		gs_sweep(true);
-		if (opt.gs_sweeps > 1) gs_sweep(false);
+		if (opt.gs_sweeps > 1) gs_sweep(false, true);
 	} else {
 		// damped block Jacobi from x = 0
 		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
@@ -760,7 +796,7 @@ void Amg::Impl::Cycle(int l)
 	TFI_LAUNCH_CHECK("ProlongKernel");
 	if (gs) {
 		if (opt.gs_sweeps > 1) gs_sweep(true);
-		gs_sweep(false);
+		gs_sweep(false, opt.gs_sweeps > 1);
 	} else {
 		for (int s = 0; s < kPost; ++s) {
 			hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv,

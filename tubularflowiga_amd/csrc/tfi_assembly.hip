@@ -570,44 +570,28 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 		__syncthreads();
 	}
 
-	// ---- scatter (exclusive by colour: plain read-modify-write)
+	// ---- scatter (exclusive by colour: plain read-modify-write), straight
+	// from the MFMA output layout (staging through LDS into 128-byte block rows
+	// measured slower: the four waves' updates of a block meet in L2)
 	if (kMfma && kJac) {
-		// The 16 components of a (test, trial) block sit in four waves: they are
-		// gathered in LDS (over the basis and point-record area, free after the
-		// last slab), 16 test functions at a time, so that every block is then
-		// updated as one contiguous 128-byte row by 16 consecutive lanes.
-		constexpr int kBs = 17;  // padded block stride of the staging area [16][32][kBs]
-		static_assert(16*32*kBs <= kMaxBasis*8 + kSlab*(32*kBasisRec + 1) + kSlab*kQd,
-			"staging area must fit below the residual accumulators");
-		double* stage = lds;
 		const int wave = tid >> 6, lane = tid & 63;
 		const long long pbase = args.pair_offsets[e];
 		const int x = wave, y1 = (wave + 1)%3, y2 = (wave + 2)%3;
 		const int comp[4] = {wave < 3 ? 5*x : 12, wave < 3 ? 4*x + y1 : 13, wave < 3 ? 4*x + y2 : 14,
 			wave < 3 ? 4*x + 3 : 15};
 #pragma unroll
-		for (int ti = 0; ti < 2; ++ti) {
-			const int rows = min(16, nen - 16*ti);
-			if (rows > 0) {
+		for (int c = 0; c < 4; ++c)
 #pragma unroll
-				for (int c = 0; c < 4; ++c)
+			for (int t = 0; t < 4; ++t) {
+				const int ti = t >> 1, tj = t & 1;
+				const int b = 16*tj + (lane & 15);
 #pragma unroll
-					for (int tj = 0; tj < 2; ++tj) {
-						const int b = 16*tj + (lane & 15);
-#pragma unroll
-						for (int i = 0; i < 4; ++i)
-							stage[((4*i + (lane >> 4))*32 + b)*kBs + comp[c]] = acc[c][2*ti + tj][i];
-					}
-				__syncthreads();
-				for (int idx = tid; idx < rows*nen*16; idx += kThreads) {
-					const int blk = idx >> 4, cmp = idx & 15;
-					const int al = blk/nen, b = blk - al*nen;
-					const long long pair = pbase + (long long)(16*ti + al)*nen + b;
-					args.jac[(size_t)args.pair_block[pair]*16 + cmp] += stage[(al*32 + b)*kBs + cmp];
+				for (int i = 0; i < 4; ++i) {
+					const int a = 16*ti + 4*i + (lane >> 4);
+					if (a < nen && b < nen)
+						args.jac[(size_t)args.pair_block[pbase + (long long)a*nen + b]*16 + comp[c]] += acc[c][t][i];
 				}
-				__syncthreads();
 			}
-		}
 	}
 	if (pair_active) {
 		const long long pbase = args.pair_offsets[e];
@@ -674,84 +658,26 @@ __global__ void DirichletStateKernel(int nodes, const int* __restrict__ vmask, c
 }
 
 // Volumetric flow through labelled faces (IntegrateBoundaryFlow,
-// solvers/cpu/include/BoundaryFlow.hpp:164-200): one 64-thread group per
-// face, 16 lanes per surface point sum the basis, 4x4 points reduced in order.
-__global__ void FaceFlowKernel(int count, const int* __restrict__ felem, const int* __restrict__ fface,
-	const int* __restrict__ eoff, const int* __restrict__ conn, const int* __restrict__ xoff,
-	const uint8_t* __restrict__ xcol, const double* __restrict__ xval, const double* __restrict__ bezier,
-	const double* __restrict__ state, double* __restrict__ flows)
+// solvers/cpu/include/BoundaryFlow.hpp:164-200).  The flow is linear in the
+// state, so it is taken from the face's area-weighted basis vectors
+// int_face N_a n dA (integrated once, with the same 4x4 Gauss rule, by
+// FaceTractionKernel at unit pressure): one 64-lane block per face,
+// fixed-order wave sum.
+__global__ void FaceFlowVecKernel(int count, const int* __restrict__ felem, const int* __restrict__ eoff,
+	const int* __restrict__ conn, const double* __restrict__ vec, const double* __restrict__ state,
+	double* __restrict__ flows)
 {
 	const int f = blockIdx.x;
 	if (f >= count) return;
-	__shared__ double bern[16][64];
-	__shared__ double contrib[16];
-	const int e = felem[f], face = fface[f];
-	const int fixed_axis[6] = {2, 1, 0, 1, 0, 2};
-	const int var0[6] = {0, 0, 1, 0, 1, 0}, var1[6] = {1, 2, 2, 2, 2, 1};
-	const double fixed_value[6] = {0.0, 0.0, 1.0, 1.0, 0.0, 1.0};
-	const double sign[6] = {-1.0, -1.0, 1.0, 1.0, -1.0, 1.0};
-	const double gw[4] = {0.3478548451374539, 0.6521451548625461, 0.6521451548625461, 0.3478548451374539};
-	const int lane = threadIdx.x;  // 64 threads
-	// Bernstein values at the 16 face points: lane -> (point = lane/4, chunk)
-	for (int idx = lane; idx < 16*64; idx += 64) {
-		const int pt = idx >> 6, p = idx & 63;
-		const int qi = pt >> 2, qj = pt & 3;
-		double c[3];
-		c[fixed_axis[face]] = fixed_value[face];
-		c[var0[face]] = kGp[qi];
-		c[var1[face]] = kGp[qj];
-		double b[3][4], db[3][4], d2b[3][4];
-		for (int d = 0; d < 3; ++d) Bern3(c[d], b[d], db[d], d2b[d]);
-		bern[pt][p] = b[0][p & 3]*b[1][(p >> 2) & 3]*b[2][p >> 4];
+	const int e = felem[f], b0 = eoff[e], nen = eoff[e + 1] - b0;
+	double s = 0.0;
+	for (int a = threadIdx.x; a < nen; a += 64) {
+		const double* v = vec + ((size_t)f*kMaxBasis + a)*3;
+		const size_t node = conn[b0 + a];
+		s += state[4*node]*v[0] + state[4*node + 1]*v[1] + state[4*node + 2]*v[2];
 	}
-	__syncthreads();
-	if (lane < 16) {
-		const int pt = lane, qi = pt >> 2, qj = pt & 3;
-		double c[3];
-		c[fixed_axis[face]] = fixed_value[face];
-		c[var0[face]] = kGp[qi];
-		c[var1[face]] = kGp[qj];
-		double b[3][4], db[3][4], d2b[3][4];
-		for (int d = 0; d < 3; ++d) Bern3(c[d], b[d], db[d], d2b[d]);
-		const double* X = bezier + (size_t)e*192;
-		double J[3][3] = {};
-		for (int k = 0; k < 4; ++k)
-			for (int j = 0; j < 4; ++j)
-				for (int i = 0; i < 4; ++i) {
-					const int p = i + 4*j + 16*k;
-					const double g[3] = {db[0][i]*b[1][j]*b[2][k], b[0][i]*db[1][j]*b[2][k], b[0][i]*b[1][j]*db[2][k]};
-					for (int x = 0; x < 3; ++x)
-						for (int d = 0; d < 3; ++d) J[x][d] += X[3*p + x]*g[d];
-				}
-		const double det = J[0][0]*(J[1][1]*J[2][2] - J[1][2]*J[2][1]) - J[0][1]*(J[1][0]*J[2][2] - J[1][2]*J[2][0])
-			+ J[0][2]*(J[1][0]*J[2][1] - J[1][1]*J[2][0]);
-		// row `fixed_axis` of J^-1 times det = cofactor row
-		const int a = fixed_axis[face];
-		const int a1 = (a + 1)%3, a2 = (a + 2)%3;
-		double cof[3];
-		for (int x = 0; x < 3; ++x) {
-			const int x1 = (x + 1)%3, x2 = (x + 2)%3;
-			// inv[a][x] = C[x][a]/det with C the cofactor of J
-			cof[x] = J[x1][a1]*J[x2][a2] - J[x1][a2]*J[x2][a1];
-		}
-		const int b0 = eoff[e], nen = eoff[e + 1] - b0;
-		double vel[3] = {0, 0, 0};
-		for (int aa = 0; aa < nen; ++aa) {
-			double N = 0.0;
-			for (int k = xoff[b0 + aa]; k < xoff[b0 + aa + 1]; ++k) N += xval[k]*bern[pt][xcol[k]];
-			const size_t node = conn[b0 + aa];
-			for (int c = 0; c < 3; ++c) vel[c] += N*state[4*node + c];
-		}
-		const double un_det = vel[0]*cof[0] + vel[1]*cof[1] + vel[2]*cof[2];  // = det * (u . inv[a][:])
-		(void)det;
-		contrib[pt] = sign[face]*gw[qi]*gw[qj]*2.0*0.125*un_det;
-	}
-	__syncthreads();
-	if (lane == 0) {
-		double s = 0.0;
-		for (int pt = 0; pt < 16; ++pt) s += contrib[pt];
-		flows[f] = s;
-	}
+	for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+	if (threadIdx.x == 0) flows[f] = -s;
 }
 
 // Pressure-traction load of one labelled face (IntegrateBoundaryPressureTraction,
@@ -1092,14 +1018,13 @@ void SetDirichletState(tfi_mesh* m, const int* vmask, const int* pmask, const do
 	TFI_LAUNCH_CHECK("DirichletStateKernel");
 }
 
-void BoundaryFaceFlows(tfi_mesh* m, int count, const int* element, const int* face, const double* state,
-	double* flows_out)
+void FaceFlowsFromVectors(tfi_mesh* m, int count, const int* element, const double* vec, const double* state,
+	double* flows)
 {
 	if (count <= 0) return;
-	tfi_ctx* ctx = m->ctx;
-	hipLaunchKernelGGL(FaceFlowKernel, dim3(count), dim3(64), 0, ctx->stream, count, element, face, m->eoff.data(),
-		m->conn.data(), m->xoff.data(), m->xcol.data(), m->xval.data(), m->bezier.data(), state, flows_out);
-	TFI_LAUNCH_CHECK("FaceFlowKernel");
+	hipLaunchKernelGGL(FaceFlowVecKernel, dim3(count), dim3(64), 0, m->ctx->stream, count, element, m->eoff.data(),
+		m->conn.data(), vec, state, flows);
+	TFI_LAUNCH_CHECK("FaceFlowVecKernel");
 }
 
 void FaceTraction(tfi_mesh* m, int count, const int* element, const int* face, const double* pressure,

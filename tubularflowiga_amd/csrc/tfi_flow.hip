@@ -139,6 +139,7 @@ struct tfi_flow {
 	tfi_flow_options opt{};
 	DevBuf<double> state, previous, update, rhs, traction, jac, vtarget, ptarget, face_flow, cont_partial;
 	DevBuf<double> guess_base, resid_base;  // learned mode: extrapolation base and its residual
+	DevBuf<double> face_vec;                 // per labelled face: int N_a n dA, [face][kMaxBasis][3] (negated)
 	DevBuf<int> vmask, pmask, face_elem, face_id;
 	std::vector<int> face_label;
 	bool has_traction = false, has_boundary = false;
@@ -172,7 +173,8 @@ Metrics Measure(tfi_flow* f)
 	m.continuity_l2 = std::sqrt(m.continuity_l2);
 	if (!f->face_label.empty()) {
 		const int count = static_cast<int>(f->face_label.size());
-		BoundaryFaceFlows(f->mesh, count, f->face_elem.data(), f->face_id.data(), f->state.data(), f->face_flow.data());
+		FaceFlowsFromVectors(f->mesh, count, f->face_elem.data(), f->face_vec.data(), f->state.data(),
+			f->face_flow.data());
 		std::vector<double> flows(count);
 		f->face_flow.Download(flows.data(), count);
 		std::map<int, double> per_label;
@@ -331,9 +333,17 @@ int tfi_flow_set_boundary_faces(tfi_flow* f, int count, const int32_t* element, 
 		f->face_elem.Allocate(f->ctx, count);
 		f->face_id.Allocate(f->ctx, count);
 		f->face_flow.Allocate(f->ctx, count);
+		f->face_vec.Allocate(f->ctx, (size_t)count*kMaxBasis*3);
 		if (count) {
 			f->face_elem.Upload(element, count);
 			f->face_id.Upload(face, count);
+			// the face flows are linear in the state: their area-weighted basis
+			// vectors are integrated once here (unit-pressure traction vectors)
+			std::vector<double> ones(count, 1.0);
+			DevBuf<double> unit(f->ctx, count);
+			unit.Upload(ones.data(), count);
+			FaceTraction(f->mesh, count, f->face_elem.data(), f->face_id.data(), unit.data(), f->face_vec.data());
+			Sync(f->ctx);
 		}
 	});
 }
@@ -557,7 +567,7 @@ int tfi_flow_boundary_flows(tfi_flow* f, int count, const int32_t* labels, doubl
 		const int faces = static_cast<int>(f->face_label.size());
 		std::vector<double> per(faces, 0.0);
 		if (faces) {
-			BoundaryFaceFlows(f->mesh, faces, f->face_elem.data(), f->face_id.data(), f->state.data(),
+			FaceFlowsFromVectors(f->mesh, faces, f->face_elem.data(), f->face_vec.data(), f->state.data(),
 				f->face_flow.data());
 			f->face_flow.Download(per.data(), faces);
 		}

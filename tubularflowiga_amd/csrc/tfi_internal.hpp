@@ -222,10 +222,11 @@ void ApplyDirichlet(tfi_mesh* mesh, const int* vmask, const int* pmask, const do
 // state[constrained] = target
 void SetDirichletState(tfi_mesh* mesh, const int* vmask, const int* pmask, const double* vtarget,
 	const double* ptarget, double* state);
-// Boundary faces flows: per face flow into flows_out[face] (deterministic).
-void BoundaryFaceFlows(tfi_mesh* mesh, int count, const int* element, const int* face, const double* state,
-	double* flows_out);
 // Per-face pressure-traction vectors local[count][kMaxBasis][3] (device).
+// flows[f] = -sum_(a, c) state[4 conn(a) + c] vec[f][a][c] with vec the
+// per-face vectors FaceTraction gives for unit pressure (= int_face N_a n dA).
+void FaceFlowsFromVectors(tfi_mesh* mesh, int count, const int* element, const double* vec, const double* state,
+	double* flows);
 void FaceTraction(tfi_mesh* mesh, int count, const int* element, const int* face, const double* pressure,
 	double* local);
 
