@@ -192,28 +192,29 @@ __device__ inline double* Brec(double* basis, int qstride, int a, int ql)
 	return basis + (size_t)ql*qstride + (size_t)a*kBasisRec;
 }
 
-// C[ti][tj] += A_ti B_tj for the two row tiles and two column tiles of one
-// term (v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16], B[l/16][l%16]).
-__device__ inline void Term(doublex4 (&c)[4], double a0, double a1, double b0, double b1)
+// C[tj] += A B_tj for one test-function tile and the two trial tiles.
+__device__ inline void Term2(doublex4 (&c)[2], double a, double b0, double b1)
 {
-	c[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c[0], 0, 0, 0);
-	c[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c[1], 0, 0, 0);
-	c[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c[2], 0, 0, 0);
-	c[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c[3], 0, 0, 0);
+	c[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, c[0], 0, 0, 0);
+	c[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, c[1], 0, 0, 0);
 }
 
 // kMfma: elements with at most 32 basis functions, one workgroup per element.
 // The tangent is a sum over quadrature points of (test factor) x (trial
 // factor) products, so each of the 16 block components is a small GEMM,
 // T_c[a][b] = sum_(q, term) A_term(a, q) B_term(q, b), run on FP64 MFMA with
-// K = 4 quadrature points per instruction: wave w owns the components of test
-// field w (row w of the 4x4 block), lanes build the operands of their
-// (basis function, point) from the LDS records, and the 2 x 2 output tiles of
-// 16 x 16 cover the element's (padded) 32 x 32 basis pairs.
+// K = 4 quadrature points per instruction (v_mfma_f64_16x16x4f64: lane l
+// holds A[l%16][l/16] and B[l/16][l%16]).  The workgroup has 8 waves, two per
+// SIMD, so that one wave's operand building overlaps the other's MFMAs: wave
+// w owns test field w%4 (row w%4 of the 4x4 block) for the 16 test functions
+// of tile w/4, lanes build the operands of their (basis function, point) from
+// the LDS records, and two 16 x 16 output tiles per wave cover the trial
+// functions of the element's (padded) 32 x 32 basis pairs.
 // kJac: residual and tangent; otherwise the residual alone.
 template <bool kExact, bool kMfma, bool kJac>
-__global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
+__global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs args)
 {
+	constexpr int kT = kMfma ? 512 : 256;  // threads of the workgroup
 	extern __shared__ double lds[];
 	const int tid = threadIdx.x;
 	const int item = blockIdx.x/args.tiles;
@@ -237,24 +238,24 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	double* resid = qd + kSlab*kQd;                         // [a_tile][4]
 	int* lconn = reinterpret_cast<int*>(resid + 4*kMaxBasis);
 
-	for (int a = tid; a < nen; a += kThreads) {
+	for (int a = tid; a < nen; a += kT) {
 		const int node = args.conn[b0 + a];
 		lconn[a] = node;
 		for (int f = 0; f < 4; ++f) nodal[8*a + f] = args.state[4*(size_t)node + f];
 		for (int f = 0; f < 3; ++f) nodal[8*a + 4 + f] = args.previous[4*(size_t)node + f];
 	}
 	const double* table = args.table;
-	if (!table)
-		for (int i = tid; i < nen*64; i += kThreads) C[i] = 0.0;
-	for (int i = tid; i < 4*kMaxBasis; i += kThreads) resid[i] = 0.0;
+	if (!kMfma && !table)
+		for (int i = tid; i < nen*64; i += kT) C[i] = 0.0;
+	for (int i = tid; i < 4*kMaxBasis; i += kT) resid[i] = 0.0;
 	if (kMfma)  // padding functions nen..31 stay zero in every slab
-		for (int i = tid; i < kSlab*(32 - nen)*kBasisRec; i += kThreads) {
+		for (int i = tid; i < kSlab*(32 - nen)*kBasisRec; i += kT) {
 			const int ql = i/((32 - nen)*kBasisRec), rest = i - ql*(32 - nen)*kBasisRec;
 			Brec(basis, qstride, nen, ql)[rest] = 0.0;
 		}
 	__syncthreads();
-	if (!table)
-		for (int a = tid; a < nen; a += kThreads)
+	if (!kMfma && !table)
+		for (int a = tid; a < nen; a += kT)
 			for (int k = args.xoff[b0 + a]; k < args.xoff[b0 + a + 1]; ++k) C[64*a + args.xcol[k]] = args.xval[k];
 	__syncthreads();
 
@@ -269,33 +270,34 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	const int ra[2] = {a0 + 2*ta, min(a0 + 2*ta + 1, a1 - 1)};
 	const int rb[2] = {2*tb, min(2*tb + 1, nen - 1)};
 	double dg[4] = {}, uu[4][9] = {}, i3[4][3] = {}, t3[4][3] = {}, k33[4] = {};
-	doublex4 acc[4][4];  // MFMA path: [component of this wave][tile ti*2 + tj]
+	doublex4 acc[4][2];  // MFMA path: [component of this wave][trial tile tj]
 	for (int c = 0; c < 4; ++c)
-		for (int t = 0; t < 4; ++t) acc[c][t] = doublex4{0.0, 0.0, 0.0, 0.0};
+		for (int t = 0; t < 2; ++t) acc[c][t] = doublex4{0.0, 0.0, 0.0, 0.0};
 
 	for (int slab = 0; slab < kQ/kSlab; ++slab) {
 		const int q0 = slab*kSlab;
 		// ---- P1: basis values, gradients and Laplacians per (a, point): read
 		// from the precomputed table when there is one, else evaluated here
-		if (table) {
+		if (kMfma || table) {  // the MFMA variants run only with the table
 			const int rec = nen*kTableRec;
 			const double* src = table + ((size_t)b0*kQ + (size_t)q0*nen)*kTableRec;
-			for (int i = tid; i < kSlab*rec; i += kThreads) {
+			for (int i = tid; i < kSlab*rec; i += kT) {
 				const int ql = i/rec, r = i - ql*rec, a = r/kTableRec, f = r - a*kTableRec;
 				Brec(basis, qstride, a, ql)[f] = src[i];
 			}
 		} else {
-			for (int it = tid; it < nen*kSlab; it += kThreads) {
+			for (int it = tid; it < nen*kSlab; it += kT) {
 				const int a = it/kSlab, ql = it - a*kSlab;
 				EvalBasis(C + 64*a, geo + q0 + ql, q0 + ql, Brec(basis, qstride, a, ql));
 			}
 		}
 		__syncthreads();
-		// ---- P2: fields at the points (8 lanes per point), tau, fine scales
+		// ---- P2: fields at the points (kT/kSlab lanes per point), tau, fine scales
 		{
-			const int ql = tid >> 3, part = tid & 7, q = q0 + ql;
+			constexpr int kL = kT/kSlab;
+			const int ql = tid/kL, part = tid%kL, q = q0 + ql;
 			double s[22] = {};
-			for (int a = part; a < nen; a += 8) {
+			for (int a = part; a < nen; a += kL) {
 				const double* bb = Brec(basis, qstride, a, ql);
 				const double* un = nodal + 8*a;
 				for (int f = 0; f < 4; ++f) {
@@ -310,17 +312,15 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				}
 			}
 #pragma unroll
-			for (int i = 0; i < 22; ++i) {
-				s[i] += __shfl_xor(s[i], 1, 64);
-				s[i] += __shfl_xor(s[i], 2, 64);
-				s[i] += __shfl_xor(s[i], 4, 64);
-			}
+			for (int i = 0; i < 22; ++i)
+#pragma unroll
+				for (int o = 1; o < kL; o <<= 1) s[i] += __shfl_xor(s[i], o, 64);
 			if (part == 0)
 				PointRecord(s, geo + q, rho, mu, nu, dt, kExact && kJac, qd + ql*kQd);
 		}
 		__syncthreads();
 		// ---- P3: u.grad N per (a, point); residual rows of this tile
-		for (int it = tid; it < ((nen + 1) & ~1)*kSlab; it += kThreads) {
+		for (int it = tid; it < ((nen + 1) & ~1)*kSlab; it += kT) {
 			const int a = it/kSlab, ql = it - a*kSlab;
 			const bool valid = a < nen;
 			const double* r = qd + ql*kQd;
@@ -345,16 +345,17 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				for (int c = 0; c < 4; ++c) resid[4*(a - a0) + c] += R[c];
 		}
 		__syncthreads();
-		// ---- P4 (MFMA): 4 points per instruction, wave w = test field w
+		// ---- P4 (MFMA): 4 points per instruction; wave w: test field w%4, test tile w/4
 		if (kMfma && kJac) {
 			const int wave = tid >> 6, lane = tid & 63;
 			const int fl = lane & 15, kq = lane >> 4;
-			const int x = wave, y1 = (wave + 1)%3, y2 = (wave + 2)%3;
+			const int x = wave & 3, ti = wave >> 2;
+			const int y1 = (x + 1)%3, y2 = (x + 2)%3;
 			for (int chunk = 0; chunk < kSlab/4; ++chunk) {
 				const int ql = chunk*4 + kq;
 				const double* r = qd + ql*kQd;
 				const double m = r[0], tm = r[1], tc = r[2];
-				// the lane's two basis functions fl, fl + 16 at point ql
+				// trial side: the lane's basis functions fl, fl + 16 at point ql
 				double N[2], G[2][3], U[2], W[2], Lp[2];
 #pragma unroll
 				for (int i = 0; i < 2; ++i) {
@@ -365,100 +366,92 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 					U[i] = bb[5];
 					W[i] = bb[6];
 				}
-				double mN[2], mG[2][3];
+				// test side: function 16 ti + fl, one of the two
+				const double Na = ti ? N[1] : N[0], Ua = ti ? U[1] : U[0], Wa = ti ? W[1] : W[0];
+				double mG[3];
 #pragma unroll
-				for (int i = 0; i < 2; ++i) {
-					mN[i] = m*N[i];
-					for (int d = 0; d < 3; ++d) mG[i][d] = m*G[i][d];
-				}
+				for (int d = 0; d < 3; ++d) mG[d] = m*(ti ? G[1][d] : G[0][d]);
+				const double mN = m*Na;
 				if (kExact) {
 					const double* up = r + 19;
 					const double* gu = r + 25;
 					const double* K = r + 34;
 					const double* R = r + 43;
 					const double* c1 = r + 52;
-					double mA[2], sb[2];
+					const double mA = m*Wa;
+					double sb[2];
 #pragma unroll
-					for (int i = 0; i < 2; ++i) {
-						mA[i] = m*W[i];
-						sb[i] = N[i]*idt + U[i] - nu*Lp[i];
-					}
-					if (wave < 3) {
-						double Zx[2][3], Xbx[2], Yb[2][3];
+					for (int j = 0; j < 2; ++j) sb[j] = N[j]*idt + U[j] - nu*Lp[j];
+					if (x < 3) {
+						double Zx[3], Xbx[2], Yb[2][3];
 #pragma unroll
-						for (int i = 0; i < 2; ++i) {
+						for (int y = 0; y < 3; ++y) {
+							const double mP = K[y]*mG[0] + K[3 + y]*mG[1] + K[6 + y]*mG[2];
+							Zx[y] = rho*(mN*R[3*x + y] - mA*K[3*x + y] - up[x]*(mG[y] + mP));
+						}
 #pragma unroll
-							for (int y = 0; y < 3; ++y) {
-								const double mP = K[y]*mG[i][0] + K[3 + y]*mG[i][1] + K[6 + y]*mG[i][2];
-								Zx[i][y] = rho*(mN[i]*R[3*x + y] - mA[i]*K[3*x + y] - up[x]*(mG[i][y] + mP));
-								Yb[i][y] = rho*tc*G[i][y] + c1[y]*N[i];
-							}
-							Xbx[i] = mu*G[i][x] + rho*tm*up[x]*sb[i];
+						for (int j = 0; j < 2; ++j) {
+#pragma unroll
+							for (int y = 0; y < 3; ++y) Yb[j][y] = rho*tc*G[j][y] + c1[y]*N[j];
+							Xbx[j] = mu*G[j][x] + rho*tm*up[x]*sb[j];
 						}
 						// (x, x)
-						Term(acc[0], mG[0][x], mG[1][x], Xbx[0] + Yb[0][x] + mu*G[0][x], Xbx[1] + Yb[1][x] + mu*G[1][x]);
-						Term(acc[0], mu*mG[0][y1], mu*mG[1][y1], G[0][y1], G[1][y1]);
-						Term(acc[0], mu*mG[0][y2], mu*mG[1][y2], G[0][y2], G[1][y2]);
-						Term(acc[0], Zx[0][x], Zx[1][x], N[0], N[1]);
-						Term(acc[0], rho*tm*(mA[0] - gu[4*x]*mN[0]), rho*tm*(mA[1] - gu[4*x]*mN[1]), sb[0], sb[1]);
-						Term(acc[0], rho*mN[0], rho*mN[1], N[0]*idt + W[0], N[1]*idt + W[1]);
+						Term2(acc[0], mG[x], Xbx[0] + Yb[0][x] + mu*G[0][x], Xbx[1] + Yb[1][x] + mu*G[1][x]);
+						Term2(acc[0], mu*mG[y1], G[0][y1], G[1][y1]);
+						Term2(acc[0], mu*mG[y2], G[0][y2], G[1][y2]);
+						Term2(acc[0], Zx[x], N[0], N[1]);
+						Term2(acc[0], rho*tm*(mA - gu[4*x]*mN), sb[0], sb[1]);
+						Term2(acc[0], rho*mN, N[0]*idt + W[0], N[1]*idt + W[1]);
 						// (x, y1), (x, y2)
-						Term(acc[1], mG[0][y1], mG[1][y1], Xbx[0], Xbx[1]);
-						Term(acc[1], mG[0][x], mG[1][x], Yb[0][y1], Yb[1][y1]);
-						Term(acc[1], Zx[0][y1], Zx[1][y1], N[0], N[1]);
-						Term(acc[1], -rho*tm*gu[3*x + y1]*mN[0], -rho*tm*gu[3*x + y1]*mN[1], sb[0], sb[1]);
-						Term(acc[2], mG[0][y2], mG[1][y2], Xbx[0], Xbx[1]);
-						Term(acc[2], mG[0][x], mG[1][x], Yb[0][y2], Yb[1][y2]);
-						Term(acc[2], Zx[0][y2], Zx[1][y2], N[0], N[1]);
-						Term(acc[2], -rho*tm*gu[3*x + y2]*mN[0], -rho*tm*gu[3*x + y2]*mN[1], sb[0], sb[1]);
+						Term2(acc[1], mG[y1], Xbx[0], Xbx[1]);
+						Term2(acc[1], mG[x], Yb[0][y1], Yb[1][y1]);
+						Term2(acc[1], Zx[y1], N[0], N[1]);
+						Term2(acc[1], -rho*tm*gu[3*x + y1]*mN, sb[0], sb[1]);
+						Term2(acc[2], mG[y2], Xbx[0], Xbx[1]);
+						Term2(acc[2], mG[x], Yb[0][y2], Yb[1][y2]);
+						Term2(acc[2], Zx[y2], N[0], N[1]);
+						Term2(acc[2], -rho*tm*gu[3*x + y2]*mN, sb[0], sb[1]);
 						// (x, 3)
-						Term(acc[3], -mG[0][x], -mG[1][x], N[0], N[1]);
+						Term2(acc[3], -mG[x], N[0], N[1]);
 #pragma unroll
-						for (int d = 0; d < 3; ++d) {
-							const double e0 = tm*(up[x]*mG[0][d] - mN[0]*gu[3*x + d] + (d == x ? mA[0] : 0.0));
-							const double e1 = tm*(up[x]*mG[1][d] - mN[1]*gu[3*x + d] + (d == x ? mA[1] : 0.0));
-							Term(acc[3], e0, e1, G[0][d], G[1][d]);
-						}
+						for (int d = 0; d < 3; ++d)
+							Term2(acc[3], tm*(up[x]*mG[d] - mN*gu[3*x + d] + (d == x ? mA : 0.0)), G[0][d], G[1][d]);
 					} else {
 						// (3, y) and (3, 3)
 #pragma unroll
 						for (int y = 0; y < 3; ++y) {
-							const double mP0 = K[y]*mG[0][0] + K[3 + y]*mG[0][1] + K[6 + y]*mG[0][2];
-							const double mP1 = K[y]*mG[1][0] + K[3 + y]*mG[1][1] + K[6 + y]*mG[1][2];
-							Term(acc[y], mN[0], mN[1], G[0][y], G[1][y]);
-							Term(acc[y], -mP0, -mP1, N[0], N[1]);
-							Term(acc[y], tm*mG[0][y], tm*mG[1][y], sb[0], sb[1]);
+							const double mP = K[y]*mG[0] + K[3 + y]*mG[1] + K[6 + y]*mG[2];
+							Term2(acc[y], mN, G[0][y], G[1][y]);
+							Term2(acc[y], -mP, N[0], N[1]);
+							Term2(acc[y], tm*mG[y], sb[0], sb[1]);
 						}
 #pragma unroll
-						for (int d = 0; d < 3; ++d)
-							Term(acc[3], (tm/rho)*mG[0][d], (tm/rho)*mG[1][d], G[0][d], G[1][d]);
+						for (int d = 0; d < 3; ++d) Term2(acc[3], (tm/rho)*mG[d], G[0][d], G[1][d]);
 					}
 				} else {
 					// legacy tau-frozen tangent
 					double sb[2];
 #pragma unroll
-					for (int i = 0; i < 2; ++i) sb[i] = N[i]*idt + U[i];
-					if (wave < 3) {
-						const double al0 = m*rho*(N[0] + tm*U[0]), al1 = m*rho*(N[1] + tm*U[1]);
-						Term(acc[0], al0, al1, sb[0], sb[1]);
-						Term(acc[0], (2.0*mu + rho*tc)*mG[0][x], (2.0*mu + rho*tc)*mG[1][x], G[0][x], G[1][x]);
-						Term(acc[0], mu*mG[0][y1], mu*mG[1][y1], G[0][y1], G[1][y1]);
-						Term(acc[0], mu*mG[0][y2], mu*mG[1][y2], G[0][y2], G[1][y2]);
-						Term(acc[1], mu*mG[0][y1], mu*mG[1][y1], G[0][x], G[1][x]);
-						Term(acc[1], rho*tc*mG[0][x], rho*tc*mG[1][x], G[0][y1], G[1][y1]);
-						Term(acc[2], mu*mG[0][y2], mu*mG[1][y2], G[0][x], G[1][x]);
-						Term(acc[2], rho*tc*mG[0][x], rho*tc*mG[1][x], G[0][y2], G[1][y2]);
-						Term(acc[3], m*tm*U[0], m*tm*U[1], G[0][x], G[1][x]);
-						Term(acc[3], -mG[0][x], -mG[1][x], N[0], N[1]);
+					for (int j = 0; j < 2; ++j) sb[j] = N[j]*idt + U[j];
+					if (x < 3) {
+						Term2(acc[0], m*rho*(Na + tm*Ua), sb[0], sb[1]);
+						Term2(acc[0], (2.0*mu + rho*tc)*mG[x], G[0][x], G[1][x]);
+						Term2(acc[0], mu*mG[y1], G[0][y1], G[1][y1]);
+						Term2(acc[0], mu*mG[y2], G[0][y2], G[1][y2]);
+						Term2(acc[1], mu*mG[y1], G[0][x], G[1][x]);
+						Term2(acc[1], rho*tc*mG[x], G[0][y1], G[1][y1]);
+						Term2(acc[2], mu*mG[y2], G[0][x], G[1][x]);
+						Term2(acc[2], rho*tc*mG[x], G[0][y2], G[1][y2]);
+						Term2(acc[3], m*tm*Ua, G[0][x], G[1][x]);
+						Term2(acc[3], -mG[x], N[0], N[1]);
 					} else {
 #pragma unroll
 						for (int y = 0; y < 3; ++y) {
-							Term(acc[y], mN[0], mN[1], G[0][y], G[1][y]);
-							Term(acc[y], tm*mG[0][y], tm*mG[1][y], sb[0], sb[1]);
+							Term2(acc[y], mN, G[0][y], G[1][y]);
+							Term2(acc[y], tm*mG[y], sb[0], sb[1]);
 						}
 #pragma unroll
-						for (int d = 0; d < 3; ++d)
-							Term(acc[3], (tm/rho)*mG[0][d], (tm/rho)*mG[1][d], G[0][d], G[1][d]);
+						for (int d = 0; d < 3; ++d) Term2(acc[3], (tm/rho)*mG[d], G[0][d], G[1][d]);
 					}
 				}
 			}
@@ -576,20 +569,19 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 	if (kMfma && kJac) {
 		const int wave = tid >> 6, lane = tid & 63;
 		const long long pbase = args.pair_offsets[e];
-		const int x = wave, y1 = (wave + 1)%3, y2 = (wave + 2)%3;
-		const int comp[4] = {wave < 3 ? 5*x : 12, wave < 3 ? 4*x + y1 : 13, wave < 3 ? 4*x + y2 : 14,
-			wave < 3 ? 4*x + 3 : 15};
+		const int x = wave & 3, ti = wave >> 2;
+		const int y1 = (x + 1)%3, y2 = (x + 2)%3;
+		const int comp[4] = {x < 3 ? 5*x : 12, x < 3 ? 4*x + y1 : 13, x < 3 ? 4*x + y2 : 14, x < 3 ? 4*x + 3 : 15};
 #pragma unroll
 		for (int c = 0; c < 4; ++c)
 #pragma unroll
-			for (int t = 0; t < 4; ++t) {
-				const int ti = t >> 1, tj = t & 1;
+			for (int tj = 0; tj < 2; ++tj) {
 				const int b = 16*tj + (lane & 15);
 #pragma unroll
 				for (int i = 0; i < 4; ++i) {
 					const int a = 16*ti + 4*i + (lane >> 4);
 					if (a < nen && b < nen)
-						args.jac[(size_t)args.pair_block[pbase + (long long)a*nen + b]*16 + comp[c]] += acc[c][t][i];
+						args.jac[(size_t)args.pair_block[pbase + (long long)a*nen + b]*16 + comp[c]] += acc[c][tj][i];
 				}
 			}
 	}
@@ -611,7 +603,7 @@ __global__ void __launch_bounds__(kThreads) AssembleNsKernel(AsmArgs args)
 				J[15] += k33[pr];
 			}
 	}
-	for (int a = a0 + tid; a < a1; a += kThreads) {
+	for (int a = a0 + tid; a < a1; a += kT) {
 		const size_t node = lconn[a];
 		for (int c = 0; c < 4; ++c) args.negres[4*node + c] += resid[4*(a - a0) + c];
 	}
@@ -762,7 +754,7 @@ __global__ void FaceTractionKernel(int count, const int* __restrict__ felem, con
 // registers of AssembleNsKernel, several elements share a compute unit.
 constexpr int kQr = 35;  // point record r[0..33] in LDS, odd stride
 
-__global__ void __launch_bounds__(kThreads, 2) ResidualKernel(AsmArgs args)
+__global__ void __launch_bounds__(kThreads, 2) NsResidualKernel(AsmArgs args)
 {
 	__shared__ double nodal[32*8];
 	__shared__ double qd[kQ*kQr];
@@ -891,8 +883,9 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	const bool exact = p.tangent == TFI_TANGENT_EXACT;
 	const bool jac = jacobian != nullptr;
 	const int nen_max = m->max_nen;
-	// elements of at most 32 basis functions take the FP64-MFMA tangent
-	const bool mfma = jac && nen_max <= 32;
+	// elements of at most 32 basis functions take the FP64-MFMA tangent (which
+	// reads the basis table)
+	const bool mfma = jac && nen_max <= 32 && m->basis_table.size() > 0;
 	const int variant = jac ? (exact ? 1 : 0) + (mfma ? 2 : 0) : 4;
 	static const void* const kKernels[5] = {
 		reinterpret_cast<const void*>(AssembleNsKernel<false, false, true>),
@@ -947,14 +940,14 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 			a.elements = m->color_elements.data() + m->color_offsets[c];
 			a.count = m->color_offsets[c + 1] - m->color_offsets[c];
 			if (a.count == 0) continue;
-			hipLaunchKernelGGL(ResidualKernel, dim3(a.count), dim3(kThreads), 0, ctx->stream, a);
-			TFI_LAUNCH_CHECK("ResidualKernel");
+			hipLaunchKernelGGL(NsResidualKernel, dim3(a.count), dim3(kThreads), 0, ctx->stream, a);
+			TFI_LAUNCH_CHECK("NsResidualKernel");
 		}
 	} else for (int c = 0; c < m->colors; ++c) {
 		a.elements = m->color_elements.data() + m->color_offsets[c];
 		a.count = m->color_offsets[c + 1] - m->color_offsets[c];
 		if (a.count == 0) continue;
-		const dim3 grid(a.count*tiles), block(kThreads);
+		const dim3 grid(a.count*tiles), block(mfma ? 512 : kThreads);
 		switch (variant) {
 		case 0: hipLaunchKernelGGL((AssembleNsKernel<false, false, true>), grid, block, lds, ctx->stream, a); break;
 		case 1: hipLaunchKernelGGL((AssembleNsKernel<true, false, true>), grid, block, lds, ctx->stream, a); break;
