@@ -38,7 +38,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol, train_steps=20):
+def pulsatile(t):
+    """Inflow scale of the learned-warm-start workload: Q(t)/Q0 = 1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t."""
+    return 1.0 + 0.6 * math.sin(2 * math.pi * t) + 0.2 * math.sin(4 * math.pi * t)
+
+
+def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol, train_steps=20,
+               waveform=None):
     from tubularflowiga_amd import boundary, device
     vmask, pmask, vt, pt = bc
     flow = device.FlowSolver(dm, density=rho, viscosity=mu, dt=dt, nonlinear_rtol=tol["rtol"],
@@ -47,8 +53,9 @@ def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm,
                              restart=100, linear_maximum_iterations=4000)
     if warm == device.TFI_WARM_LEARNED:
         flow.configure_warm_start(hidden=128, train_steps=train_steps, learning_rate=1e-3, min_snapshots=2)
-    flow.set_state(boundary.initial_state(vmask, pmask, vt, pt))
-    flow.set_boundary(vmask, pmask, vt, pt)
+    scale0 = waveform(0.0) if waveform else 1.0
+    flow.set_state(boundary.initial_state(vmask, pmask, vt * scale0, pt))
+    flow.set_boundary(vmask, pmask, vt * scale0, pt)
     flow.set_boundary_faces(*faces)
     ctx = dm.ctx
     records = []
@@ -58,7 +65,13 @@ def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm,
         if is_timed and step == warmup:
             ctx.timers(enable=True, reset=True)
         t0 = time.perf_counter()
-        flow.begin_step()
+        if waveform:
+            t = (step + 1) * dt
+            flow.begin_step(np.array([math.sin(2 * math.pi * t), math.cos(2 * math.pi * t),
+                                      math.sin(4 * math.pi * t), math.cos(4 * math.pi * t)]))
+            flow.set_boundary(vmask, pmask, vt * waveform(t), pt)
+        else:
+            flow.begin_step()
         rep = flow.newton()
         flow.commit_step()
         t1 = time.perf_counter()
@@ -166,6 +179,8 @@ def main(argv=None):
     ap.add_argument("--dt", type=float, default=1e-3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20, help="Adam steps of the predictor per time step")
+    ap.add_argument("--inflow", choices=["steady", "pulsatile"], default="steady",
+                    help="steady Q = 5 ml/s, or Q(t) = 5 (1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t) ml/s")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernel on the workload's pattern plus a 2 GiB copy (for counter passes)")
     ap.add_argument("--assembly-only", action="store_true",
@@ -197,14 +212,17 @@ def main(argv=None):
     # state itself is within ~1e-10 of the converged solution; every Newton
     # linear solve to a true relative residual of 1e-8
     tol = {"rtol": 1e-10, "atol": 1e-14, "mass": 1e-6}
+    wf = pulsatile if a.inflow == "pulsatile" else None
+    inflow_text = ("steady Q=5 ml/s" if wf is None else "pulsatile Q(t)=5(1+0.6 sin 2 pi t+0.2 sin 4 pi t) ml/s")
 
     log("plain FP64 (AMG, previous-step start)")
     plain, plain_all, plain_timers, plain_state, _ = run_method(
-        dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol)
+        dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol,
+        waveform=wf)
     log("mixed precision + learned warm start")
     fast, fast_all, fast_timers, fast_state, warm_stats = run_method(
         dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_MIXED, device.TFI_WARM_LEARNED, tol,
-        a.train_steps)
+        a.train_steps, waveform=wf)
 
     ms_plain = 1e3 * float(np.mean([r["wall_s"] for r in plain]))
     ms_fast = 1e3 * float(np.mean([r["wall_s"] for r in fast]))
@@ -240,7 +258,7 @@ def main(argv=None):
         "dtype": "f64 (plain) / f32+f64 refinement (mixed); mlp bf16 inference",
         "data": "synthetic (generated Y-bifurcation case; no external data)",
         "config": {"workload": f"{a.workload}: symmetric Y-bifurcation, quadratic IGA, {4*case.mesh.nodes} DOFs, "
-                               f"{case.mesh.elements} elements, steady Q=5 ml/s, backward Euler dt={a.dt}",
+                               f"{case.mesh.elements} elements, {inflow_text}, backward Euler dt={a.dt}",
                    "nodes": case.mesh.nodes, "elements": case.mesh.elements, "dofs": 4 * case.mesh.nodes,
                    "block_nonzeros": int(dm.nnzb), "newton_rtol": tol["rtol"], "linear_rtol": 1e-8},
         "fp64_ms_per_step": round(ms_plain, 3),

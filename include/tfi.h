@@ -146,8 +146,11 @@ typedef struct {
 	int amg_gs_min_rows;              /* levels below this many rows use block Jacobi (0: 8192) */
 	int amg_gs_sweeps;                /* 2 (0): symmetric Gauss-Seidel pre and post; 1: forward pre, backward post */
 	int chord_polish;                 /* 1 (default): a residual check that misses the tolerance by at most
-	                                     1000x is closed by a chord step (a solve with the last tangent and
-	                                     preconditioner, to 0.1 x tolerance) before a new tangent is built */
+	                                     1000x is closed by chord steps (solves with the tangent and
+	                                     preconditioner already set up, to 0.1 x tolerance; up to 3, each
+	                                     cutting the residual 4x) before a new tangent is built.  A learned
+	                                     guess predicted within that window starts this way, with the
+	                                     tangent of an earlier step. */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);

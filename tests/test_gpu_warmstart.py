@@ -128,3 +128,22 @@ def test_trained_network_gains_over_the_method_without_it(ctx, ybif, inflow):
     else:
         assert out["learned"]["ratio"] <= 1.1 * out["extrapolate4"]["ratio"]
     assert out["learned"]["newton"] <= out["extrapolate4"]["newton"]
+
+
+def test_chord_with_earlier_tangent_reaches_the_plain_solution(ctx, ybif):
+    """Once the learned guess is within the chord window of the tolerance, the
+    step is closed by chord steps with the tangent set up at an earlier state
+    (no new tangent): the state must still be the plain method's."""
+    from tubularflowiga_amd import device
+    dm = device.DeviceMesh(ctx, ybif.mesh)
+    steps, dt, rtol = 100, 1e-3, 1e-9
+    plain, _, _ = _run(ctx, ybif, device.TFI_WARM_PREVIOUS, 0, steps, dt, rtol, waveform=_pulsatile, dm=dm)
+    state, reps, _ = _run(ctx, ybif, device.TFI_WARM_LEARNED, 0, steps, dt, rtol, waveform=_pulsatile, dm=dm)
+    tangent_free = [k for k, r in enumerate(reps) if r["newton_iterations"] == 0 and r["chord_steps"] >= 1]
+    v, p = _rel(state, plain)
+    print(f"steps closed by a chord with an earlier tangent: {len(tangent_free)} of {steps}; "
+          f"rel_l2 v={v:.2e} p={p:.2e}", flush=True)
+    assert all(r["converged"] == 1 for r in reps)
+    assert tangent_free
+    # both runs stop somewhere below the same Newton tolerance
+    assert v <= rtol and p <= rtol

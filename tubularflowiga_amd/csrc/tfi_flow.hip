@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <limits>
 #include <string>
 #include <map>
 
@@ -148,6 +149,7 @@ struct tfi_flow {
 	double phase[8] = {};
 	int phase_count = 0;
 	double newton_contraction = 0.0;  // largest ||r_k|| / ||r_{k-1}||^2 of the last step
+	bool tangent_ready = false;       // the linear solver holds a set-up tangent (possibly of an earlier state)
 };
 
 namespace {
@@ -370,6 +372,8 @@ int tfi_flow_commit_step(tfi_flow* f)
 // closed by a chord step (a solve with the tangent and preconditioner already
 // set up, to a tenth of the tolerance) instead of a new tangent.
 constexpr double kChordWindow = 1000.0;
+constexpr int kMaxChords = 3;
+constexpr double kChordContraction = 4.0;
 
 int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 {
@@ -397,7 +401,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		// learned / extrapolated initial state, accepted only if it lowers the
 		// residual; the tangent assembled at an accepted guess is reused by the
 		// first Newton iteration
-		bool preassembled = false;
+		bool preassembled = false, pre_jacobian = true;
 		Metrics pre_mt;
 		double pre_norm = 0.0, pre_asm = 0.0;
 		if (f->opt.warm_start != TFI_WARM_PREVIOUS && f->warm->Ready(f->opt.warm_start)) {
@@ -405,6 +409,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			const double base = assemble(false, nullptr);
 			Copy(ctx, f->state.data(), f->update.data(), n);  // keep the plain guess
 			double blend = 1.0;
+			double predicted = std::numeric_limits<double>::infinity();  // linearised ||r|| of the guess
 			if (f->opt.warm_start == TFI_WARM_LEARNED) {
 				// Residual line search along the learned correction: with u_b the
 				// extrapolation base and u_L = u_b + g c the network's guess, take
@@ -423,6 +428,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				const double bl = Dot(ctx, f->resid_base.data(), f->rhs.data(), n);
 				const double dd = nl*nl - 2.0*bl + nb*nb;
 				blend = dd > 1e-24*nb*nb ? std::min(2.0, std::max(0.0, (nb*nb - bl)/dd)) : 1.0;
+				predicted = std::sqrt(std::max(0.0, nb*nb + 2.0*blend*(bl - nb*nb) + blend*blend*dd));
 				Axpby(ctx, 1.0 - blend, f->guess_base.data(), blend, f->state.data(), n);
 			} else {
 				f->warm->Predict(f->opt.warm_start, f->state.data(), f->phase, f->phase_count);
@@ -430,7 +436,13 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 					f->state.data());
 			}
 			rep.warm_start_blend = blend;
-			const double guessed = assemble(true, &pre_mt);
+			// A guess predicted within the chord window of the tolerance is checked
+			// with a residual-only assembly and, if it is not converged already,
+			// closed by a chord step with the tangent already set up (from an
+			// earlier state); a new tangent is assembled only if that fails.
+			const double tol_guess = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*base);
+			pre_jacobian = !(f->opt.chord_polish && f->tangent_ready && predicted <= kChordWindow*tol_guess);
+			const double guessed = assemble(pre_jacobian, &pre_mt);
 			pre_asm = std::chrono::duration<double>(Clock::now() - a0).count();
 			rep.assembly_s += pre_asm;
 			rep.warm_start_initial_ratio = base > 0.0 ? guessed/base : 0.0;
@@ -456,6 +468,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			if (preassembled) {
 				mt = pre_mt;
 				norm = pre_norm;
+				have_jacobian = pre_jacobian;
 				preassembled = false;
 			} else {
 				// When quadratic convergence predicts this iterate converged, check
@@ -475,22 +488,26 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				// The prediction missed.  A small miss is closed by a chord step: one
 				// more solve with the tangent and preconditioner of the last iteration
 				// (still set up), to a tenth of the tolerance, then a residual check.
-				if (f->opt.chord_polish && norm <= kChordWindow*tol_now) {
+				// Further chord steps follow while each cuts the residual at least
+				// kChordContraction-fold (a tangent of an earlier state may need two).
+				for (int chord = 0; chord < kMaxChords && f->opt.chord_polish && norm <= kChordWindow*tol_now
+					&& !(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol); ++chord) {
 					const auto c0 = Clock::now();
 					const double eta = std::min(0.1, std::max(f->opt.linear_rtol, 0.1*tol_now/norm));
 					Fill(ctx, f->update.data(), 0.0, n);
 					const auto r = f->solver->Solve(f->rhs.data(), f->update.data(), f->opt.linear_maximum_iterations,
 						eta);
 					Sync(ctx);
-					chord_s = std::chrono::duration<double>(Clock::now() - c0).count();
-					rep.linear_s += chord_s;
+					chord_s += std::chrono::duration<double>(Clock::now() - c0).count();
 					rep.linear_iterations += r.iterations;
-					if (r.converged) {
-						Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
-						++rep.chord_steps;
-						norm = assemble(false, &mt);
-					}
+					if (!r.converged) break;
+					Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
+					++rep.chord_steps;
+					const double before = norm;
+					norm = assemble(false, &mt);
+					if (!(norm*kChordContraction <= before)) break;
 				}
+				rep.linear_s += chord_s;
 				if (!(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol))
 					norm = assemble(true, &mt);  // full assembly at the current state
 			}
@@ -521,6 +538,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			const auto l0 = Clock::now();
 			Fill(ctx, f->update.data(), 0.0, n);
 			f->solver->Setup(f->jac.data());
+			f->tangent_ready = true;
 			const auto r = f->solver->Solve(f->rhs.data(), f->update.data(), f->opt.linear_maximum_iterations,
 				f->opt.linear_rtol);
 			Sync(ctx);

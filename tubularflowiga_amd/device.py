@@ -273,7 +273,9 @@ class FlowSolver:
         rep = {k: getattr(r, k) for k, _ in NewtonReport._fields_ if k != "record"}
         rep["records"] = [{k: getattr(r.record[i], k) for k, _ in r.record[i]._fields_} for i in range(r.records)]
         if status != 0:
-            raise TfiError(self.lib.tfi_last_error(self.ctx.handle).decode())
+            err = TfiError(self.lib.tfi_last_error(self.ctx.handle).decode())
+            err.report = rep  # the iterations done before the failure
+            raise err
         return rep
 
     def commit_step(self):
