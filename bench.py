@@ -12,7 +12,9 @@ initial state:
     refinement, Newton from the online-trained MLP prediction (residual-checked).
 `value` is the mixed+learned wall time per time step (lower is better); the
 plain FP64 time, the speedup, the assembly kernel's FP64-MFMA roofline and the
-SpMV HBM roofline sit beside it.
+SpMV HBM roofline sit beside it.  A secondary comparison (`pulsatile`) runs the
+learned-warm-start workload on the same mesh: inflow Q(t) = 5 (1 + 0.6 sin 2 pi t
++ 0.2 sin 4 pi t) ml/s, 100 untimed steps, then 10 timed steps of each method.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -109,6 +111,45 @@ def measured_traffic():
     return out
 
 
+def method_summary(timed):
+    return {"ms_per_step": round(1e3 * float(np.mean([r["wall_s"] for r in timed])), 3),
+            "assembly_ms_per_step": round(1e3 * float(np.mean([r["assembly_s"] for r in timed])), 3),
+            "linear_ms_per_step": round(1e3 * float(np.mean([r["linear_s"] for r in timed])), 3),
+            "newton_per_step": float(np.mean([r["newton"] for r in timed])),
+            "krylov_per_step": float(np.mean([r["linear"] for r in timed])),
+            "chord_steps": int(sum(r["chord"] for r in timed))}
+
+
+def rel_l2(a, b):
+    a, b = a.reshape(-1, 4), b.reshape(-1, 4)
+    return {"velocity": float(np.linalg.norm(a[:, :3] - b[:, :3]) / np.linalg.norm(b[:, :3])),
+            "pressure": float(np.linalg.norm(a[:, 3] - b[:, 3]) / max(np.linalg.norm(b[:, 3]), 1e-300))}
+
+
+def pulsatile_comparison(dm, case, bc, faces, rho, mu, dt, tol, train_steps, warmup=100, steps=10):
+    """The learned-warm-start workload on the same mesh: pulsatile inflow,
+    the first `warmup` steps untimed (they also train the predictor), then
+    `steps` timed steps of both methods."""
+    from tubularflowiga_amd import device
+    log(f"pulsatile inflow: plain FP64, {warmup} + {steps} steps")
+    plain, plain_all, _, plain_state, _ = run_method(
+        dm, case, bc, faces, rho, mu, dt, steps, warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol,
+        waveform=pulsatile)
+    log(f"pulsatile inflow: mixed precision + learned warm start, {warmup} + {steps} steps")
+    fast, fast_all, _, fast_state, warm_stats = run_method(
+        dm, case, bc, faces, rho, mu, dt, steps, warmup, device.TFI_MIXED, device.TFI_WARM_LEARNED, tol,
+        train_steps, waveform=pulsatile)
+    p, f = method_summary(plain), method_summary(fast)
+    return {"workload": f"same mesh, pulsatile Q(t)=5(1+0.6 sin 2 pi t+0.2 sin 4 pi t) ml/s, backward Euler dt={dt}",
+            "warmup": warmup, "steps": steps, "fp64_ms_per_step": p["ms_per_step"],
+            "mixed_learned_ms_per_step": f["ms_per_step"],
+            "speedup_vs_fp64": round(p["ms_per_step"] / f["ms_per_step"], 3), "plain_fp64": p, "mixed_learned": f,
+            "whole_run_ms_per_step": {
+                "plain_fp64": round(1e3 * float(np.mean([r["wall_s"] for r in plain_all])), 3),
+                "mixed_learned": round(1e3 * float(np.mean([r["wall_s"] for r in fast_all])), 3)},
+            "rel_l2_mixed_vs_fp64": rel_l2(fast_state, plain_state), "warm_start": warm_stats}
+
+
 def cpu_baseline(case, dm, rho, mu, dt, plain_timed, budget_s=20.0):
     """CPU oracle (FP64 C++ restatement, OpenMP) on a bounded sample: element
     assembly of a slice of the mesh and FP64 SpMVs on the full pattern, scaled
@@ -181,6 +222,8 @@ def main(argv=None):
     ap.add_argument("--train-steps", type=int, default=20, help="Adam steps of the predictor per time step")
     ap.add_argument("--inflow", choices=["steady", "pulsatile"], default="steady",
                     help="steady Q = 5 ml/s, or Q(t) = 5 (1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t) ml/s")
+    ap.add_argument("--no-pulsatile", action="store_true",
+                    help="skip the secondary pulsatile-inflow comparison (100 + 10 steps per method)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernel on the workload's pattern plus a 2 GiB copy (for counter passes)")
     ap.add_argument("--assembly-only", action="store_true",
@@ -296,6 +339,8 @@ def main(argv=None):
                           "kernel": "BsrSpmvKernel<double> (FGMRES operator, FP64 values)",
                           "avg_ms": round(spmv_ms, 4), "algorithmic_bytes": int(spmv_bytes)},
     }
+    if a.inflow == "steady" and not a.no_pulsatile:
+        out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, a.dt, tol, a.train_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, dm, rho, mu, a.dt, plain)
     print(json.dumps(out), flush=True)
