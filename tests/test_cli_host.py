@@ -113,3 +113,7 @@ def test_corrupt_database_and_usage_errors(tube_case, tmp_path):
     assert r.returncode == 1 and "--output-every requires --output" in r.stderr
     r = run("navier-stokes", d / "case.ntiga", d, "--precision", "fp16")
     assert r.returncode == 1 and "--precision must be fp64|mixed" in r.stderr
+    r = run("navier-stokes", d / "case.ntiga", d, "--chord-polish", "maybe")
+    assert r.returncode == 1 and "--chord-polish must be on|off" in r.stderr
+    r = run("navier-stokes", d / "case.ntiga", d, "--train-steps", "-3")
+    assert r.returncode == 1 and "--train-steps requires a positive integer" in r.stderr

@@ -135,3 +135,25 @@ def test_cli_non_convergence_is_an_error(tube, tmp_path):
             "--nonlinear-atol", "1e-20")
     assert r.returncode == 1
     assert "iga_hip: " in r.stderr and "MAX_NEWTON" in r.stderr and "at physical step 1" in r.stderr
+
+
+def test_cli_learned_mixed_reaches_the_plain_solution(tmp_path):
+    """`--warm-start learned --precision mixed` through the drop-in, under a
+    sinusoidal inflow (whose phase the driver feeds to the predictor): the
+    learned guess is used once the predictor is ready, and the states match
+    the plain FP64 run of the same steps."""
+    case = casegen.workload("tube_c0")
+    wave = {"name": "pulse", "kind": "sinusoid", "units": "1", "mean": 1.0, "amplitude": 0.5, "period": 0.05}
+    d = casegen.write_case(case, tmp_path / "case", steps=12, dt=1e-3, waveform=wave)
+    plain, fast = tmp_path / "plain.txt", tmp_path / "fast.txt"
+    tol = ["--nonlinear-rtol", "1e-11", "--nonlinear-atol", "1e-16", "--mass-rtol", "1e30"]
+    r = run("navier-stokes", d / "case.ntiga", d, "--output", plain, *tol)
+    assert r.returncode == 0, r.stderr
+    r = run("navier-stokes", d / "case.ntiga", d, "--output", fast, *tol, "--warm-start", "learned",
+            "--precision", "mixed", "--train-steps", "20")
+    assert r.returncode == 0, r.stderr
+    used = re.findall(r"^warm_start step=(\d+) used=(\d) ", r.stdout, re.M)
+    assert len(used) == 12 and any(u == "1" for _, u in used)
+    v, p = rel(read_state(fast), read_state(plain))
+    print(f"learned+mixed vs plain rel_l2 velocity={v:.3e} pressure={p:.3e}; used={[u for _, u in used]}", flush=True)
+    assert v <= 1e-10 and p <= 1e-10

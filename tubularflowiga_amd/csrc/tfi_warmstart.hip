@@ -46,7 +46,7 @@ struct WarmStart::Impl {
 	double phase_hist[kRing][8] = {};
 	int accepted = 0, rejected = 0;
 	std::unique_ptr<Mlp> mlp;
-	tfi_warm_options opt{128, 27, 40, 1e-3, 3};
+	tfi_warm_options opt{128, 27, 20, 1e-3, 2};
 	const double* at(int back) const { return hist[(head - back + kRing)%kRing].data(); }
 };
 

@@ -98,6 +98,7 @@ struct Options {
 	int tangent = TFI_TANGENT_EXACT, smoother = TFI_SMOOTH_GAUSS_SEIDEL, restart_length = 200;
 	int linear_maximum_iterations = 5000;
 	double linear_rtol = 1e-8;
+	int chord_polish = 1, train_steps = 0;
 };
 
 int PositiveInt(const std::string& t, const std::string& opt)
@@ -144,7 +145,7 @@ const char* kUsage =
 	"[--mass-rtol R] [--visualization-format auto|vtu] "
 	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|learned] "
 	"[--tangent exact|legacy] [--amg-smoother gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
-	"[--linear-max-iterations N]";
+	"[--linear-max-iterations N] [--chord-polish on|off] [--train-steps N]";
 
 // ParseCudaFlowOptions (iga_cuda.cu:447-498) plus the solver-path options
 Options ParseOptions(int argc, char** argv)
@@ -194,6 +195,8 @@ Options ParseOptions(int argc, char** argv)
 		else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
 		else if (arg == "--restart-length") o.restart_length = PositiveInt(v, arg);
 		else if (arg == "--linear-max-iterations") o.linear_maximum_iterations = PositiveInt(v, arg);
+		else if (arg == "--chord-polish") o.chord_polish = Choice(v, arg, {{"on", 1}, {"off", 0}});
+		else if (arg == "--train-steps") o.train_steps = PositiveInt(v, arg);
 		else throw std::runtime_error("unknown option: " + arg);
 	}
 	if (o.output_every > 0 && o.output.empty()) throw std::runtime_error("--output-every requires --output or legacy OUTPUT");
@@ -287,8 +290,21 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	opt.warm_start = transient ? o.warm_start : TFI_WARM_PREVIOUS;
 	opt.amg_smoother = o.smoother;
 	opt.tangent = o.tangent;
+	opt.chord_polish = o.chord_polish;
 	Flow flow;
 	c.Check(tfi_flow_create(m.mesh, &opt, &flow.flow), "flow solver");
+	if (opt.warm_start == TFI_WARM_LEARNED && o.train_steps > 0) {
+		tfi_warm_options w{128, 27, o.train_steps, 1e-3, 2};
+		c.Check(tfi_flow_warm_configure(flow.flow, &w), "warm start");
+	}
+	// Inflow phase for the learned predictor: sin / cos of 2 pi t / T and
+	// 4 pi t / T, T the period of the first periodic temporal function.
+	double period = 0.0;
+	for (const auto& fn : config.functions)
+		if (fn.period > 0.0) {
+			period = fn.period;
+			break;
+		}
 	c.Check(tfi_flow_set_boundary_faces(flow.flow, static_cast<int>(face_el.size()), face_el.data(), face_id.data(),
 		face_lab.data()), "boundary faces");
 
@@ -340,7 +356,13 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		c.Check(tfi_pressure_traction(m.mesh, static_cast<int>(te.size()), te.data(), tf.data(), tp.data(),
 			traction.data()), "pressure traction");
 		const bool any_traction = std::any_of(traction.begin(), traction.end(), [](double v) { return v != 0.0; });
-		c.Check(tfi_flow_begin_step(flow.flow, nullptr, 0), "begin step");
+		if (period > 0.0) {
+			const double w = 2.0*M_PI*time/period;
+			const double phase[4] = {std::sin(w), std::cos(w), std::sin(2.0*w), std::cos(2.0*w)};
+			c.Check(tfi_flow_begin_step(flow.flow, phase, 4), "begin step");
+		} else {
+			c.Check(tfi_flow_begin_step(flow.flow, nullptr, 0), "begin step");
+		}
 		c.Check(tfi_flow_set_boundary(flow.flow, bc.velocity_mask.data(), bc.pressure_mask.data(), bc.velocity.data(),
 			bc.pressure.data(), any_traction ? traction.data() : nullptr), "boundary data");
 		tfi_newton_report rep{};
