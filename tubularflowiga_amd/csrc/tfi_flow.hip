@@ -109,6 +109,7 @@ KrylovResult LinearSolver::Solve(const double* b, double* x, int maxit, double r
 	KrylovResult total;
 	total.rhs_norm = Norm2(ctx, b, n);
 	const double target = rtol*total.rhs_norm;
+	double rn0 = -1.0;
 	for (int refinement = 0; refinement < 30; ++refinement) {
 		Residual64(ctx, a, values_, x, b, residual_.data());
 		const double rn = Norm2(ctx, residual_.data(), n);
@@ -117,7 +118,8 @@ KrylovResult LinearSolver::Solve(const double* b, double* x, int maxit, double r
 			total.converged = true;
 			break;
 		}
-		if (total.iterations >= maxit) break;
+		if (rn0 < 0.0) rn0 = rn;
+		if (total.iterations >= maxit || !std::isfinite(rn) || rn > 10.0*rn0) break;
 		Fill(ctx, correction_.data(), 0.0, n);
 		// ask the inner solve for what FP32 storage can deliver, never more
 		// than the remaining reduction
@@ -141,6 +143,7 @@ struct tfi_flow {
 	DevBuf<double> state, previous, update, rhs, traction, jac, vtarget, ptarget, face_flow, cont_partial;
 	DevBuf<double> guess_base, resid_base;  // learned mode: extrapolation base and its residual
 	DevBuf<double> face_vec;                 // per labelled face: int N_a n dA, [face][kMaxBasis][3] (negated)
+	DevBuf<double> last_update;              // the last Newton update, for backtracking after a failed solve
 	DevBuf<int> vmask, pmask, face_elem, face_id;
 	std::vector<int> face_label;
 	bool has_traction = false, has_boundary = false;
@@ -258,6 +261,7 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 		f->state.Allocate(ctx, n);
 		f->previous.Allocate(ctx, n);
 		f->update.Allocate(ctx, n);
+		f->last_update.Allocate(ctx, n);
 		if (f->opt.warm_start == TFI_WARM_LEARNED) {
 			f->guess_base.Allocate(ctx, n);
 			f->resid_base.Allocate(ctx, n);
@@ -373,6 +377,9 @@ int tfi_flow_commit_step(tfi_flow* f)
 // set up, to a tenth of the tolerance) instead of a new tangent.
 constexpr double kChordWindow = 1000.0;
 constexpr int kMaxChords = 3;
+// A Newton update that raises the residual, or after which the next solve
+// fails, is backtracked by halves (at most this many times in a step).
+constexpr int kMaxBacktracks = 6;
 constexpr double kChordContraction = 4.0;
 
 int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
@@ -459,6 +466,9 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		}
 		bool converged = false;
 		double prev_norm = -1.0, contraction = 0.0;  // observed ||r_k|| / ||r_{k-1}||^2 this step
+		double applied = 0.0;                         // fraction of last_update in the state (0: none)
+		double norm_before = 0.0;                     // residual of the state last_update was computed at
+		int backtracks = 0;
 		for (int it = 0; it < f->opt.maximum_newton; ++it) {
 			tfi_newton_record rec{};
 			const auto a0 = Clock::now();
@@ -535,6 +545,17 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				rep.newton_iterations = it;
 				break;
 			}
+			// Damped Newton: an update that raised the residual is halved (back
+			// toward the state it started from) and the residual assembled again.
+			if (applied > 0.0 && norm > norm_before && backtracks < kMaxBacktracks) {
+				Axpy(ctx, -0.5*applied, f->last_update.data(), f->state.data(), n);
+				applied *= 0.5;
+				++backtracks;
+				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				rep.newton_iterations = it + 1;
+				prev_norm = -1.0;  // the quadratic model does not hold across a backtrack
+				continue;
+			}
 			const auto l0 = Clock::now();
 			Fill(ctx, f->update.data(), 0.0, n);
 			f->solver->Setup(f->jac.data());
@@ -549,6 +570,17 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			rec.linear_residual = r.residual;
 			rec.singular_diagonal_blocks = f->solver->singular_blocks();
 			rep.linear_iterations += r.iterations;
+			if (!r.converged && applied > 0.0 && backtracks < kMaxBacktracks) {
+				// the tangent at this iterate defeated the solver: step back to the
+				// middle of the last update and assemble there
+				Axpy(ctx, -0.5*applied, f->last_update.data(), f->state.data(), n);
+				applied *= 0.5;
+				++backtracks;
+				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				rep.newton_iterations = it + 1;
+				prev_norm = -1.0;  // the quadratic model does not hold across a backtrack
+				continue;
+			}
 			if (!r.converged) {
 				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
 				rep.total_s = std::chrono::duration<double>(Clock::now() - t_start).count();
@@ -560,6 +592,9 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			}
 			rec.update_l2 = Norm2(ctx, f->update.data(), n);
 			Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
+			Copy(ctx, f->update.data(), f->last_update.data(), n);
+			applied = 1.0;
+			norm_before = norm;
 			if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
 			rep.newton_iterations = it + 1;
 			prev_norm = norm;

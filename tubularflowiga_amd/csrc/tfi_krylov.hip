@@ -171,7 +171,9 @@ KrylovResult Fgmres::Solve(const Operator& A, const Precond& M, const double* b,
 		return res;
 	}
 	double* hcol = ctx->host_pinned;  // pinned staging
+	const double beta0 = beta;
 	while (res.iterations < maxit) {
+		const double beta_cycle = beta;
 		Copy(ctx, r_.data(), V, n);
 		Scale(ctx, 1.0/beta, V, n);
 		std::fill(H_.begin(), H_.end(), 0.0);
@@ -254,6 +256,10 @@ KrylovResult Fgmres::Solve(const Operator& A, const Precond& M, const double* b,
 			break;
 		}
 		if (breakdown) break;
+		// a restart cycle that made no progress, or a residual far above the
+		// initial one, means the preconditioned operator defeats the method:
+		// stop early instead of spending the remaining iterations
+		if (!std::isfinite(beta) || beta >= beta_cycle || beta > 10.0*beta0) break;
 	}
 	return res;
 }
