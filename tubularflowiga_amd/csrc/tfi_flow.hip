@@ -376,7 +376,11 @@ int tfi_flow_commit_step(tfi_flow* f)
 // closed by a chord step (a solve with the tangent and preconditioner already
 // set up, to a tenth of the tolerance) instead of a new tangent.
 constexpr double kChordWindow = 1000.0;
-constexpr int kMaxChords = 3;
+constexpr int kMaxChords = 6;
+// A learned guess whose linearised residual is below this fraction of the
+// previous-step state's starts with chord steps on the tangent of an earlier
+// step, each solved only as far as the observed contraction makes useful.
+constexpr double kStaleStart = 1e-3;
 // A Newton update that raises the residual, or after which the next solve
 // fails, is backtracked by halves (at most this many times in a step).
 constexpr int kMaxBacktracks = 6;
@@ -448,7 +452,8 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			// closed by a chord step with the tangent already set up (from an
 			// earlier state); a new tangent is assembled only if that fails.
 			const double tol_guess = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*base);
-			pre_jacobian = !(f->opt.chord_polish && f->tangent_ready && predicted <= kChordWindow*tol_guess);
+			pre_jacobian = !(f->opt.chord_polish && f->tangent_ready
+				&& predicted <= std::max(kChordWindow*tol_guess, kStaleStart*base));
 			const double guessed = assemble(pre_jacobian, &pre_mt);
 			pre_asm = std::chrono::duration<double>(Clock::now() - a0).count();
 			rep.assembly_s += pre_asm;
@@ -500,10 +505,19 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				// (still set up), to a tenth of the tolerance, then a residual check.
 				// Further chord steps follow while each cuts the residual at least
 				// kChordContraction-fold (a tangent of an earlier state may need two).
-				for (int chord = 0; chord < kMaxChords && f->opt.chord_polish && norm <= kChordWindow*tol_now
+				// A step that starts on an earlier tangent (it == 0) has a wider
+				// window, and each of its chord solves stops at a quarter of the last
+				// observed contraction (the stale tangent limits what a more accurate
+				// solve could gain); after a fresh tangent they go to 0.1 x tolerance.
+				const bool stale = it == 0;
+				const double window = stale ? std::max(kChordWindow*tol_now, kStaleStart*initial)
+					: kChordWindow*tol_now;
+				double rho = 0.04;  // assumed contraction of the first chord on an earlier tangent
+				for (int chord = 0; chord < kMaxChords && f->opt.chord_polish && norm <= window
 					&& !(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol); ++chord) {
 					const auto c0 = Clock::now();
-					const double eta = std::min(0.1, std::max(f->opt.linear_rtol, 0.1*tol_now/norm));
+					const double floor_eta = std::max(f->opt.linear_rtol, 0.1*tol_now/norm);
+					const double eta = std::min(0.1, std::max(floor_eta, stale ? 0.25*rho : 0.0));
 					Fill(ctx, f->update.data(), 0.0, n);
 					const auto r = f->solver->Solve(f->rhs.data(), f->update.data(), f->opt.linear_maximum_iterations,
 						eta);
@@ -515,7 +529,12 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 					++rep.chord_steps;
 					const double before = norm;
 					norm = assemble(false, &mt);
+					rho = norm/before;
 					if (!(norm*kChordContraction <= before)) break;
+					// give up early when the observed contraction cannot reach the
+					// tolerance within the chord budget
+					if (norm > tol_now && rho > 0.0
+						&& chord + 1 + std::ceil(std::log(norm/tol_now)/std::log(1.0/rho)) > kMaxChords) break;
 				}
 				rep.linear_s += chord_s;
 				if (!(norm <= tol_now && mt.imbalance <= f->opt.mass_rtol))
