@@ -572,18 +572,32 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 		const int x = wave & 3, ti = wave >> 2;
 		const int y1 = (x + 1)%3, y2 = (x + 2)%3;
 		const int comp[4] = {x < 3 ? 5*x : 12, x < 3 ? 4*x + y1 : 13, x < 3 ? 4*x + y2 : 14, x < 3 ? 4*x + 3 : 15};
+		// block indices of this lane's 8 pairs, then per component all 8 loads
+		// before the 8 stores, so that the read latencies overlap
+		int slot[2][4];
 #pragma unroll
-		for (int c = 0; c < 4; ++c)
+		for (int tj = 0; tj < 2; ++tj) {
+			const int b = 16*tj + (lane & 15);
 #pragma unroll
-			for (int tj = 0; tj < 2; ++tj) {
-				const int b = 16*tj + (lane & 15);
-#pragma unroll
-				for (int i = 0; i < 4; ++i) {
-					const int a = 16*ti + 4*i + (lane >> 4);
-					if (a < nen && b < nen)
-						args.jac[(size_t)args.pair_block[pbase + (long long)a*nen + b]*16 + comp[c]] += acc[c][tj][i];
-				}
+			for (int i = 0; i < 4; ++i) {
+				const int a = 16*ti + 4*i + (lane >> 4);
+				slot[tj][i] = (a < nen && b < nen) ? args.pair_block[pbase + (long long)a*nen + b] : -1;
 			}
+		}
+#pragma unroll
+		for (int c = 0; c < 4; ++c) {
+			double v[2][4];
+#pragma unroll
+			for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+				for (int i = 0; i < 4; ++i)
+					v[tj][i] = slot[tj][i] >= 0 ? args.jac[(size_t)slot[tj][i]*16 + comp[c]] : 0.0;
+#pragma unroll
+			for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+				for (int i = 0; i < 4; ++i)
+					if (slot[tj][i] >= 0) args.jac[(size_t)slot[tj][i]*16 + comp[c]] = v[tj][i] + acc[c][tj][i];
+		}
 	}
 	if (pair_active) {
 		const long long pbase = args.pair_offsets[e];
