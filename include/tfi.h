@@ -186,11 +186,13 @@ int tfi_flow_set_boundary(tfi_flow* flow, const int32_t* velocity_mask, const in
  * iga_cuda.cu:520-557): count faces given as (element, local face, label). */
 int tfi_flow_set_boundary_faces(tfi_flow* flow, int count, const int32_t* element, const int32_t* face,
 	const int32_t* label);
-/* Start of a physical step: previous <- state (iga_cuda.cu:918-920), then the
- * warm-start prediction of the step increment (learned mode). phase = inflow
- * phase features for the learned predictor (may be NULL). */
+/* Start of a physical step: previous <- state (iga_cuda.cu:918-920). phase =
+ * inflow phase features for the learned predictor (may be NULL). */
 int tfi_flow_begin_step(tfi_flow* flow, const double* phase, int phase_count);
-/* Newton loop at the current boundary data (iga_cuda.cu:956-1046). */
+/* Newton loop at the current boundary data (iga_cuda.cu:956-1046). The first
+ * call after tfi_flow_begin_step starts from the warm-start prediction (when
+ * the mode predicts); later calls in the same step (outlet fixed-point passes,
+ * iga_cuda.cu:933-1084) start from the state the previous call left. */
 int tfi_flow_newton(tfi_flow* flow, tfi_newton_report* report);
 /* End of a physical step: history for the learned warm start. */
 int tfi_flow_commit_step(tfi_flow* flow);

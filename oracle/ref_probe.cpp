@@ -11,12 +11,27 @@
 //   ref_probe surface ELEMENT face       -> SurfaceJacobianMeasure at the 16 face points
 //   ref_probe ntiga   DATABASE           -> header + every element record as text
 //   ref_probe walltrace DATABASE VTK     -> WallTraceBasis node ids
+//   ref_probe outlet-config CONFIG.json  -> InitializeOutletModels of the first
+//                                          system: per model label, kind index
+//                                          (0 resistance, 1 RC, 2 RCR), pressure,
+//                                          capacitor pressure; then the traction
+//                                          value MaterializeOutletPressures gives
+//                                          each outlet label
+//   ref_probe outlet KIND R RP RD C PREF PCAP PRESSURE DT FLOW
+//                                        -> EvaluateOutletModel (pressure, capacitor
+//                                          pressure); EvaluateOutletCoupling with
+//                                          PCAP as the step's start (maximum change,
+//                                          tolerance); pressure after
+//                                          RelaxOutletCoupling; pressure and capacitor
+//                                          pressure after CommitOutletCoupling
+//                                          (include/OutletModel.hpp)
 // ELEMENT file: nen, then nen connectivity ints, nen x 64 extraction values,
 // 64 x 3 Bezier points, nen x 4 nodal state values (whitespace separated).
 #include "BoundaryFlow.hpp"
 #include "BoundarySupport.hpp"
 #include "IgaDatabase.hpp"
 #include "Quadrature.hpp"
+#include "OutletModel.hpp"
 
 #include <cstdio>
 #include <fstream>
@@ -127,6 +142,45 @@ int main(int argc, char** argv)
 			iga::Database db(argv[2]);
 			const auto mesh = iga::ReadLabeledHexMesh(argv[3], db.header().nodes, db.header().elements);
 			for (auto n : iga::WallTraceBasis(db, mesh)) std::printf("%d\n", n);
+		} else if (cmd == "outlet-config" && argc == 3) {
+			const auto config = iga::ReadSimulationConfiguration(argv[2]);
+			const auto models = iga::InitializeOutletModels(config, config.equation_systems.at(0));
+			auto kind_index = [](iga::FieldBoundaryKind k) {
+				return k == iga::FieldBoundaryKind::Resistance ? 0 : k == iga::FieldBoundaryKind::WindkesselRC ? 1 : 2;
+			};
+			std::printf("%zu\n", models.size());
+			for (const auto& m : models)
+				std::printf("%d %d %.17g %.17g\n", m.label, kind_index(m.kind), m.pressure, m.capacitor_pressure);
+			const auto applied = iga::MaterializeOutletPressures(config, models);
+			for (const auto& b : applied.boundaries)
+				for (const auto& c : b.conditions)
+					if (c.kind == iga::FieldBoundaryKind::PressureTraction)
+						std::printf("%d %.17g\n", b.label, c.value.at(0));
+		} else if (cmd == "outlet" && argc == 12) {
+			iga::OutletModelState m;
+			m.label = 2;
+			m.kind = iga::simulation_detail::ParseFieldBoundaryKind(argv[2]);
+			m.resistance = std::stod(argv[3]);
+			m.proximal_resistance = std::stod(argv[4]);
+			m.distal_resistance = std::stod(argv[5]);
+			m.capacitance = std::stod(argv[6]);
+			m.reference_pressure = std::stod(argv[7]);
+			m.capacitor_pressure = std::stod(argv[8]);
+			m.pressure = std::stod(argv[9]);
+			const double dt = std::stod(argv[10]), flow = std::stod(argv[11]);
+			const auto e = iga::EvaluateOutletModel(m, flow, dt);
+			Print(e.pressure);
+			Print(e.capacitor_pressure);
+			const auto c = iga::EvaluateOutletCoupling({m}, {m.capacitor_pressure}, {flow}, dt);
+			Print(c.maximum_pressure_change);
+			Print(iga::OutletCouplingTolerance(c));
+			auto relaxed = std::vector<iga::OutletModelState>{m};
+			iga::RelaxOutletCoupling(relaxed, c);
+			Print(relaxed[0].pressure);
+			auto committed = std::vector<iga::OutletModelState>{m};
+			iga::CommitOutletCoupling(committed, c);
+			Print(committed[0].pressure);
+			Print(committed[0].capacitor_pressure);
 		} else {
 			throw std::runtime_error("unknown command or wrong argument count");
 		}

@@ -10,7 +10,13 @@ outputs as fixtures:
     (IntegrateBoundaryPressureTraction) with a seeded nodal state;
   * surface Jacobian measures (SurfaceJacobianMeasure);
   * the SHA-256 of the reference reader's text dump of a generated `.ntiga`
-    database (IgaDatabase.hpp) and its wall-trace basis (WallTraceBasis).
+    database (IgaDatabase.hpp) and its wall-trace basis (WallTraceBasis);
+  * into outlet_vectors.json: lumped outlet model evaluations, coupling
+    tolerance, relaxation and commit (include/OutletModel.hpp) over a grid of
+    kinds, parameters, flows and time steps, and the outlet models, applied
+    outlet tractions or configuration errors of the tube case with each outlet
+    condition (InitializeOutletModels / MaterializeOutletPressures /
+    ReadSimulationConfiguration, include/SimulationConfig.hpp).
 
 Run in the build container only (the reference tree is not shipped):
     python tests/golden/make_ref_vectors.py
@@ -30,6 +36,29 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 PROBE = os.path.join(ROOT, "oracle", "_ref", "ref_probe")
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.json")
+OUTLETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "outlet_vectors.json")
+
+# (resistance, proximal, distal, capacitance, reference pressure, capacitor
+# pressure, applied pressure): the first row is solvers/cpu/tests/test_outlet_model.cpp's
+OUTLET_PARAMETERS = [(4.0, 2.0, 4.0, 2.0, 1.0, 5.0, 8.0), (3.0, 0.5, 1.5, 0.25, 4.0, -2.0, 0.0),
+                     (0.0125, 1e-3, 0.02, 40.0, 0.1, 0.2, 0.15)]
+OUTLET_FLOWS = [3.0, 2.0, -1.25, 0.0, 7.5e-4]
+OUTLET_STEPS = [0.5, 1e-3, 0.0]
+# outlet conditions placed on the tube case's outlet label; the last ones are
+# rejected by the reference's configuration reader
+OUTLET_CONDITIONS = [
+    {"field": "pressure", "type": "resistance", "resistance": 3.0, "reference_pressure": 4.0},
+    {"field": "pressure", "type": "windkessel_rc", "resistance": 4.0, "capacitance": 2.0,
+     "reference_pressure": 1.0, "initial_pressure": 5.0},
+    {"field": "pressure", "type": "windkessel_rcr", "proximal_resistance": 2.0, "distal_resistance": 4.0,
+     "capacitance": 2.0, "reference_pressure": 1.0, "initial_pressure": 5.0},
+    {"field": "pressure", "type": "resistance", "resistance": 0.0},
+    {"field": "pressure", "type": "windkessel_rc", "resistance": 4.0},
+    {"field": "pressure", "type": "windkessel_rcr", "proximal_resistance": -1.0, "distal_resistance": 4.0,
+     "capacitance": 2.0},
+    {"field": "pressure", "type": "windkessel_rcr", "proximal_resistance": 1.0, "distal_resistance": 0.0,
+     "capacitance": 2.0},
+]
 
 
 def element_file(path, conn, extraction, bezier, state):
@@ -66,6 +95,47 @@ def elements():
     for e in (0, 37):
         out[f"tube_c0_e{e}"] = (m.element_connectivity(e), m.dense_extraction(e), m.element_bezier(e))
     return out
+
+
+def outlet_vectors():
+    from tubularflowiga_amd import casegen
+    out = {"evaluations": [], "configs": []}
+    for kind in ("resistance", "windkessel_rc", "windkessel_rcr"):
+        for params in OUTLET_PARAMETERS:
+            for dt in OUTLET_STEPS:
+                for q in OUTLET_FLOWS:
+                    args = [kind, *params, dt, q]
+                    r = subprocess.run([PROBE, "outlet", kind, *map(repr, args[1:])], capture_output=True, text=True)
+                    rec = {"args": args}
+                    if r.returncode == 0:
+                        rec["out"] = [float(v) for v in r.stdout.split()]
+                    else:
+                        rec["error"] = r.stderr.strip().removeprefix("ref_probe: ")
+                    out["evaluations"].append(rec)
+    with tempfile.TemporaryDirectory() as tmp:
+        casegen.write_case(casegen.workload("tube_c0"), tmp, steps=2, dt=1e-3)
+        path = os.path.join(tmp, "simulation_config.json")
+        base = json.loads(open(path).read())
+        for cond in OUTLET_CONDITIONS:
+            cfg = json.loads(json.dumps(base))
+            for b in cfg["boundaries"]:
+                if b["label"] == 2:
+                    b["conditions"] = [cond]
+            with open(path, "w") as f:
+                json.dump(cfg, f)
+            r = subprocess.run([PROBE, "outlet-config", path], capture_output=True, text=True)
+            rec = {"condition": cond}
+            if r.returncode == 0:
+                lines = r.stdout.split("\n")
+                count = int(lines[0])
+                rec["models"] = [[float(v) for v in line.split()] for line in lines[1:1 + count]]
+                rec["tractions"] = [[float(v) for v in line.split()] for line in lines[1 + count:] if line]
+            else:
+                rec["error"] = r.stderr.strip().removeprefix("ref_probe: ")
+            out["configs"].append(rec)
+    with open(OUTLETS, "w") as f:
+        json.dump(out, f, indent=0)
+    print("wrote", OUTLETS, os.path.getsize(OUTLETS), "bytes")
 
 
 def main():
@@ -108,6 +178,7 @@ def main():
     with open(OUT, "w") as f:
         json.dump(result, f)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    outlet_vectors()
 
 
 if __name__ == "__main__":

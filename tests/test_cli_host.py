@@ -87,7 +87,6 @@ def _edit_config(d, tmp_path, fn):
     (lambda c: c.update(schema_version=7), "supported schema versions"),
     (lambda c: c.update(bogus=1), "unknown key 'bogus'"),
     (lambda c: c["boundaries"].pop(), "has no simulation_config.json boundary definition"),
-    (lambda c: c["boundaries"][2]["conditions"][0].update(type="resistance", resistance=1.0), "outlet models"),
     (lambda c: c["boundaries"][1]["conditions"][0].update(profile="other.txt"), "unsupported velocity profile"),
     (lambda c: c["equation_systems"][0].update(viscosity=-1.0), "viscosity must be positive"),
 ])

@@ -157,3 +157,116 @@ def test_cli_learned_mixed_reaches_the_plain_solution(tmp_path):
     v, p = rel(read_state(fast), read_state(plain))
     print(f"learned+mixed vs plain rel_l2 velocity={v:.3e} pressure={p:.3e}; used={[u for _, u in used]}", flush=True)
     assert v <= 1e-10 and p <= 1e-10
+
+
+RCR = {"field": "pressure", "type": "windkessel_rcr", "proximal_resistance": 1e-5, "distal_resistance": 1e-2,
+       "capacitance": 10.0, "reference_pressure": 0.0, "initial_pressure": 0.14}
+OUTLET_TOL = ["--nonlinear-rtol", "1e-12", "--nonlinear-atol", "1e-14", "--mass-rtol", "1e30", "--linear-rtol", "1e-12",
+              "--max-newton", "40"]
+
+
+def oracle_outlet_steps(case, steps, dt, condition):
+    """The reference driver's per-step outlet fixed point (iga_cuda.cu:927-1084)
+    on the oracle: every pass solves the step from where the last one ended with
+    the models' pressures as outlet tractions, then evaluates the models on the
+    outlet flows.  Returns the final state, the passes per step and the
+    committed (flow, pressure, capacitor pressure) per step."""
+    from oracle import outlets as OL
+    m = case.mesh
+    rho, mu = casegen.flow_parameters(case)
+    pattern = O.block_pattern(m)
+    vmask, pmask, vt, pt = boundary.resolve_flow_boundaries(case)
+    faces = [(int(e), int(f), int(lab)) for e, f, lab in zip(*boundary.boundary_faces(m))]
+    unit = np.zeros(4 * m.nodes)
+    for e, f, lab in faces:
+        if lab == 2:
+            loc = O.pressure_traction(m.element_bezier(e), m.dense_extraction(e), f, 1.0).reshape(-1, 4)
+            for a, node in enumerate(m.element_connectivity(e)):
+                unit[4 * node:4 * node + 3] += loc[a, :3]
+    models = OL.init_models([{"label": 2, "conditions": [condition]}])
+    cur = {"state": boundary.initial_state(vmask, pmask, vt, pt)}
+
+    def solve(pressures):
+        cur["state"], _, _, _ = O.ns_newton_step(m, pattern, cur["state"], cur["previous"], rho, mu, dt, vmask, pmask,
+                                                 vt, pt, traction=pressures[2] * unit, max_newton=40, rtol=1e-12,
+                                                 atol=1e-14, linear_rtol=1e-12, tangent="exact")
+
+    def flows(labels):
+        s = cur["state"].reshape(-1, 4)
+        return [sum(O.boundary_flow(m.element_bezier(e), m.dense_extraction(e), s[m.element_connectivity(e)], f)
+                    for e, f, lab in faces if lab == want) for want in labels]
+
+    passes, committed = [], []
+    for _ in range(steps):
+        cur["previous"] = cur["state"].copy()
+        passes.append(len(OL.coupled_step(models, dt, solve, flows)))
+        committed.append((models[0].flow, models[0].pressure, models[0].capacitor_pressure))
+    return cur["state"].reshape(-1, 4), passes, committed
+
+
+def _outlet_case(tmp_path, name, steps):
+    case = casegen.workload("tube_c0")
+    d = casegen.write_case(case, tmp_path / name, steps=steps, dt=1e-3)
+    cfg = json.loads((d / "simulation_config.json").read_text())
+    for b in cfg["boundaries"]:
+        if b["label"] == 2:
+            b["conditions"] = [RCR]
+    (d / "simulation_config.json").write_text(json.dumps(cfg))
+    return case, d
+
+
+def _outlet_records(stdout):
+    passes = {}
+    for s, k in re.findall(r"^step=(\d+) time=\S+ outlet_iteration=(\d+) ", stdout, re.M):
+        passes[int(s)] = int(k) + 1
+    committed = [tuple(float(v) for v in rec) for rec in
+                 re.findall(r"^outlet label=2 flow=(\S+) pressure=(\S+) capacitor_pressure=(\S+)$", stdout, re.M)]
+    return [passes[k] for k in sorted(passes)], committed
+
+
+def test_cli_rcr_outlet_coupling_matches_the_oracle(tmp_path):
+    """An RCR Windkessel outlet: the driver's fixed point takes the oracle's
+    number of passes per step, commits the same outlet flow, pressure and
+    capacitor pressure (stdout records carry the reference's default 6
+    significant digits; the checkpoint written after the last step carries 17),
+    and ends at the oracle's state."""
+    case, d = _outlet_case(tmp_path, "rcr", 2)
+    out, ck = tmp_path / "rcr.txt", tmp_path / "rcr_ck"
+    r = run("navier-stokes", d / "case.ntiga", d, "--output", out, "--checkpoint", ck, *OUTLET_TOL)
+    assert r.returncode == 0, r.stderr
+    passes, committed = _outlet_records(r.stdout)
+    ref_state, ref_passes, ref_committed = oracle_outlet_steps(case, 2, 1e-3, RCR)
+    saved = json.loads(Path(str(ck) + ".json").read_text())["outlets"][0]
+    last = (saved["flow"], saved["pressure"], saved["capacitor_pressure"])
+    v, p = rel(read_state(out), ref_state)
+    print(f"rcr outlet: passes {passes} (oracle {ref_passes}); committed {committed} (oracle {ref_committed}); "
+          f"checkpointed {last}; rel_l2 velocity={v:.3e} pressure={p:.3e}", flush=True)
+    assert passes == ref_passes and len(committed) == 2
+    for got, want in zip(committed, ref_committed):
+        assert np.allclose(got, want, rtol=5e-6, atol=0.0)
+    assert np.allclose(last, ref_committed[-1], rtol=1e-10, atol=0.0)
+    assert v <= 1e-10 and p <= 1e-10
+
+
+def test_cli_outlet_state_survives_restart(tmp_path):
+    """Outlet models are checkpointed with the flow state
+    (include/OutletCheckpoint.hpp): a run resumed after step 1 commits the same
+    outlet values at step 2 as the uninterrupted run."""
+    case, d = _outlet_case(tmp_path, "restart", 2)
+
+    def outlet(prefix):
+        o = json.loads(Path(str(prefix) + ".json").read_text())["outlets"][0]
+        assert o["label"] == 2 and o["kind"] == "windkessel_rcr"
+        return o["flow"], o["pressure"], o["capacitor_pressure"]
+
+    full, ck, resumed = tmp_path / "full", tmp_path / "ck", tmp_path / "resumed"
+    r = run("navier-stokes", d / "case.ntiga", d, "--checkpoint", full, *OUTLET_TOL)
+    assert r.returncode == 0, r.stderr
+    _, committed = _outlet_records(r.stdout)
+    r = run("navier-stokes", d / "case.ntiga", d, "--checkpoint", ck, "--stop-after-step", "1", *OUTLET_TOL)
+    assert r.returncode == 0, r.stderr
+    assert np.allclose(outlet(ck), committed[0], rtol=5e-6, atol=0.0)
+    r = run("navier-stokes", d / "case.ntiga", d, "--restart", ck, "--checkpoint", resumed, *OUTLET_TOL)
+    assert r.returncode == 0, r.stderr
+    print(f"outlet restart: uninterrupted {outlet(full)} resumed {outlet(resumed)}", flush=True)
+    assert np.allclose(outlet(resumed), outlet(full), rtol=1e-10, atol=0.0)

@@ -153,6 +153,10 @@ struct tfi_flow {
 	int phase_count = 0;
 	double newton_contraction = 0.0;  // largest ||r_k|| / ||r_{k-1}||^2 of the last step
 	bool tangent_ready = false;       // the linear solver holds a set-up tangent (possibly of an earlier state)
+	// no Newton solve has run since tfi_flow_begin_step: only the first solve of
+	// a step starts from a prediction; later ones (outlet fixed-point passes)
+	// start where the previous one ended
+	bool fresh_step = true;
 };
 
 namespace {
@@ -358,6 +362,7 @@ int tfi_flow_begin_step(tfi_flow* f, const double* phase, int phase_count)
 {
 	return Guard(f->ctx, [&] {
 		Copy(f->ctx, f->state.data(), f->previous.data(), 4LL*f->mesh->nodes);
+		f->fresh_step = true;
 		f->phase_count = std::min(phase_count, 8);
 		for (int i = 0; i < f->phase_count; ++i) f->phase[i] = phase[i];
 	});
@@ -415,7 +420,9 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		bool preassembled = false, pre_jacobian = true;
 		Metrics pre_mt;
 		double pre_norm = 0.0, pre_asm = 0.0;
-		if (f->opt.warm_start != TFI_WARM_PREVIOUS && f->warm->Ready(f->opt.warm_start)) {
+		const bool fresh = f->fresh_step;
+		f->fresh_step = false;
+		if (fresh && f->opt.warm_start != TFI_WARM_PREVIOUS && f->warm->Ready(f->opt.warm_start)) {
 			const auto a0 = Clock::now();
 			const double base = assemble(false, nullptr);
 			Copy(ctx, f->state.data(), f->update.data(), n);  // keep the plain guess

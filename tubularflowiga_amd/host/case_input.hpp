@@ -288,6 +288,9 @@ struct Condition {
 	std::vector<double> value;
 	double scale = 1.0;
 	bool pressure_gauge = false;
+	// lumped outlet models (resistance, windkessel_rc, windkessel_rcr)
+	double resistance = 0.0, proximal_resistance = 0.0, distal_resistance = 0.0, capacitance = 0.0;
+	double reference_pressure = 0.0, initial_pressure = 0.0;
 };
 
 struct Boundary {
@@ -465,6 +468,21 @@ inline SimulationConfig ReadSimulationConfig(const std::string& path)
 			if (const Json* v = co.Find("profile")) cond.profile = RequireString(*v, cctx + ".profile");
 			if (const Json* v = co.Find("scale")) cond.scale = RequireNumber(*v, cctx + ".scale");
 			if (const Json* v = co.Find("waveform")) cond.waveform = RequireString(*v, cctx + ".waveform");
+			const std::pair<const char*, double*> model_fields[] = {{"resistance", &cond.resistance},
+				{"proximal_resistance", &cond.proximal_resistance}, {"distal_resistance", &cond.distal_resistance},
+				{"capacitance", &cond.capacitance}, {"reference_pressure", &cond.reference_pressure},
+				{"initial_pressure", &cond.initial_pressure}};
+			for (const auto& mf : model_fields)
+				if (const Json* v = co.Find(mf.first)) *mf.second = RequireNumber(*v, cctx + "." + mf.first);
+			// SimulationConfig.hpp:576-584
+			if (cond.type == "resistance" && !(cond.resistance > 0.0))
+				throw std::runtime_error("simulation_config.json: resistance outlet requires positive resistance");
+			if (cond.type == "windkessel_rc" && (!(cond.resistance > 0.0) || !(cond.capacitance > 0.0)))
+				throw std::runtime_error("simulation_config.json: windkessel_rc requires positive resistance and capacitance");
+			if (cond.type == "windkessel_rcr" && (cond.proximal_resistance < 0.0 || !(cond.distal_resistance > 0.0)
+				|| !(cond.capacitance > 0.0)))
+				throw std::runtime_error("simulation_config.json: windkessel_rcr requires nonnegative proximal resistance "
+					"and positive distal resistance and capacitance");
 			if (const Json* v = co.Find("pressure_gauge")) {
 				if (v->kind != Json::Kind::Bool) throw std::runtime_error(cctx + ".pressure_gauge must be a boolean");
 				cond.pressure_gauge = v->boolean;
@@ -599,8 +617,11 @@ struct ResolvedFlowBoundaries {
 	std::map<int, double> tractions;
 };
 
+// outlet_pressures: the current pressures of the lumped outlet models by label
+// (MaterializeOutletPressures, OutletModel.hpp:150-170), applied as tractions.
 inline ResolvedFlowBoundaries ResolveFlowBoundaries(const SimulationConfig& c, const FlowSystem& s,
-	const std::vector<int>& labels, const std::vector<std::array<double, 3>>& profile)
+	const std::vector<int>& labels, const std::vector<std::array<double, 3>>& profile,
+	const std::map<int, double>* outlet_pressures = nullptr)
 {
 	const std::size_t N = labels.size();
 	if (profile.size() != N) throw std::runtime_error("velocity profile and boundary labels have different node counts");
@@ -623,16 +644,24 @@ inline ResolvedFlowBoundaries ResolveFlowBoundaries(const SimulationConfig& c, c
 			if (cond.field != s.velocity && cond.field != s.pressure) continue;
 			if (!cond.waveform.empty())
 				throw std::runtime_error("time-dependent Navier-Stokes boundary execution is not implemented");
-			if (cond.type == "resistance" || cond.type == "windkessel_rc" || cond.type == "windkessel_rcr")
-				throw std::runtime_error("outlet models (" + cond.type + ") are not supported by iga_hip navier-stokes");
-			if (cond.type == "pressure_traction") {
-				if (cond.field != s.pressure || cond.value.size() != 1)
+			const bool outlet = cond.type == "resistance" || cond.type == "windkessel_rc" || cond.type == "windkessel_rcr";
+			if (cond.type == "pressure_traction" || outlet) {
+				if (outlet && cond.field != s.pressure)
+					throw std::runtime_error("outlet models require a Navier-Stokes velocity-pressure system");
+				if (!outlet && (cond.field != s.pressure || cond.value.size() != 1))
 					throw std::runtime_error("pressure_traction requires the Navier-Stokes pressure field");
+				double value = outlet ? 0.0 : cond.value[0];
+				if (outlet) {
+					const auto it = outlet_pressures ? outlet_pressures->find(b.label) : std::map<int, double>::const_iterator{};
+					if (!outlet_pressures || it == outlet_pressures->end())
+						throw std::runtime_error("outlet model on label " + std::to_string(b.label) + " has no pressure");
+					value = it->second;
+				}
 				if (cond.pressure_gauge)
 					throw std::runtime_error("pressure_traction already fixes the pressure reference; remove pressure_gauge "
 						"to preserve every continuity equation");
 				traction = true;
-				if (!r.tractions.emplace(b.label, cond.value[0]).second)
+				if (!r.tractions.emplace(b.label, value).second)
 					throw std::runtime_error("duplicate pressure traction boundary label");
 				continue;
 			}
@@ -673,12 +702,20 @@ inline ResolvedFlowBoundaries ResolveFlowBoundaries(const SimulationConfig& c, c
 
 // ---------------------------------------------------------------- checkpoints
 // FlowCheckpointMetadata (include/FlowCheckpoint.hpp:19-215), raw_float64 state.
+// FlowCheckpointOutlet (FlowCheckpoint.hpp:33-41)
+struct CheckpointOutlet {
+	int label = -1;
+	std::string kind;
+	double flow = 0.0, pressure = 0.0, capacitor_pressure = 0.0;
+};
+
 struct CheckpointMeta {
 	int schema_version = 1;
 	std::uint64_t nodes = 0;
 	int fields = 4, completed_step = 0;
 	double physical_time = 0.0, dt = 0.0, density = 0.0, viscosity = 0.0;
 	std::string state_file, state_format = "raw_float64";
+	std::vector<CheckpointOutlet> outlets;
 };
 
 inline std::string JsonEscape(const std::string& v)
@@ -720,7 +757,15 @@ inline void WriteCheckpoint(const fs::path& prefix, CheckpointMeta meta, const s
 		<< "  \"viscosity\": " << meta.viscosity << ",\n"
 		<< "  \"state_file\": \"" << JsonEscape(meta.state_file) << "\",\n"
 		<< "  \"state_format\": \"" << JsonEscape(meta.state_format) << "\",\n"
-		<< "  \"outlets\": [  ]\n}\n";
+		<< "  \"outlets\": [";
+	for (std::size_t i = 0; i < meta.outlets.size(); ++i) {
+		const auto& o = meta.outlets[i];
+		out << (i ? ",\n" : "\n") << "    {\"label\": " << o.label << ", \"kind\": \"" << JsonEscape(o.kind)
+			<< "\", \"flow\": " << o.flow << ", \"pressure\": " << o.pressure
+			<< ", \"capacitor_pressure\": " << o.capacitor_pressure << "}";
+	}
+	if (!meta.outlets.empty()) out << '\n';
+	out << "  ]\n}\n";
 	if (!out) throw std::runtime_error("cannot write flow checkpoint metadata: " + json_path.string());
 }
 
@@ -749,9 +794,27 @@ inline CheckpointMeta ReadCheckpointMeta(const fs::path& prefix)
 	m.state_file = RequireString(Required(root, "state_file", "flow checkpoint"), "flow checkpoint.state_file");
 	m.state_format = "petsc_binary";
 	if (const Json* f = root.Find("state_format")) m.state_format = RequireString(*f, "flow checkpoint.state_format");
-	if (const Json* o = root.Find("outlets"))
-		if (!RequireArray(*o, "flow checkpoint.outlets").array.empty())
-			throw std::runtime_error("flow checkpoint carries outlet-model state, which iga_hip navier-stokes does not support");
+	if (const Json* o = root.Find("outlets")) {
+		// FlowCheckpoint.hpp:137-164
+		const auto& arr = RequireArray(*o, "flow checkpoint.outlets").array;
+		std::set<int> seen;
+		for (std::size_t i = 0; i < arr.size(); ++i) {
+			const std::string ctx = "flow checkpoint.outlets[" + std::to_string(i) + "]";
+			const Json& item = RequireObject(arr[i], ctx);
+			RequireKnownKeys(item, {"label", "kind", "flow", "pressure", "capacitor_pressure"}, ctx);
+			CheckpointOutlet out;
+			out.label = RequireInteger(Required(item, "label", ctx), ctx + ".label");
+			out.kind = RequireString(Required(item, "kind", ctx), ctx + ".kind");
+			out.flow = RequireNumber(Required(item, "flow", ctx), ctx + ".flow");
+			out.pressure = RequireNumber(Required(item, "pressure", ctx), ctx + ".pressure");
+			out.capacitor_pressure = RequireNumber(Required(item, "capacitor_pressure", ctx), ctx + ".capacitor_pressure");
+			if (!seen.insert(out.label).second
+				|| (out.kind != "resistance" && out.kind != "windkessel_rc" && out.kind != "windkessel_rcr")
+				|| !std::isfinite(out.flow) || !std::isfinite(out.pressure) || !std::isfinite(out.capacitor_pressure))
+				throw std::runtime_error(ctx + " contains invalid outlet state");
+			m.outlets.push_back(out);
+		}
+	}
 	if (m.schema_version != 1) throw std::runtime_error("unsupported flow checkpoint schema_version");
 	if (m.nodes == 0 || m.fields != 4 || m.completed_step < 0 || m.physical_time < 0.0 || !(m.dt > 0.0)
 		|| !(m.density > 0.0) || !(m.viscosity > 0.0) || m.state_file.empty()

@@ -5,6 +5,7 @@
 // reference does not have select the MI355X solver path (preconditioner,
 // precision, warm start, tangent) and default to the reproducing settings.
 #include "case_input.hpp"
+#include "outlets.hpp"
 #include "tfi.h"
 
 #include <chrono>
@@ -238,7 +239,9 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	const double dt = transient ? config.dt : 0.0;
 	const int physical_steps = transient ? config.steps : 1;
 	auto step_config = [&](double time) { return transient ? MaterializeWaveforms(config, o.case_dir, time) : config; };
-	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(step_config(0.0), sys, labeled.labels, profile);
+	std::vector<OutletModel> outlets = InitOutletModels(config, sys);
+	auto outlet_p = OutletPressures(outlets);
+	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(step_config(0.0), sys, labeled.labels, profile, &outlet_p);
 	ApplyWallTrace(bc, wall_trace);
 	if (o.stop_after_step > physical_steps) throw std::runtime_error("--stop-after-step exceeds configured physical steps");
 	const int run_end = o.stop_after_step > 0 ? o.stop_after_step : physical_steps;
@@ -316,6 +319,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		ValidateCheckpoint(meta, db.nodes, physical_steps, dt, sys.density, sys.viscosity);
 		state = ReadCheckpointState(o.restart, meta);
 		start_step = meta.completed_step;
+		RestoreOutlets(meta.outlets, outlets);
 		std::cout << "restart=" << o.restart.string() << " completed_step=" << start_step
 			<< " physical_time=" << meta.physical_time << '\n';
 	} else if (transient) {
@@ -340,22 +344,11 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	long long total_linear_iterations = 0;
 	double total_assembly = 0.0, total_linear = 0.0;
 	std::vector<double> traction(4*N, 0.0);
+	std::vector<std::int32_t> outlet_labels;
+	for (const auto& om : outlets) outlet_labels.push_back(om.label);
 	for (int step = start_step; step < run_end; ++step) {
 		const double time = transient ? (step + 1)*dt : 0.0;
-		bc = ResolveFlowBoundaries(step_config(time), sys, labeled.labels, profile);
-		ApplyWallTrace(bc, wall_trace);
-		std::vector<std::int32_t> te, tf;
-		std::vector<double> tp;
-		for (std::size_t k = 0; k < face_el.size(); ++k) {
-			const auto it = bc.tractions.find(face_lab[k]);
-			if (it == bc.tractions.end()) continue;
-			te.push_back(face_el[k]);
-			tf.push_back(face_id[k]);
-			tp.push_back(it->second);
-		}
-		c.Check(tfi_pressure_traction(m.mesh, static_cast<int>(te.size()), te.data(), tf.data(), tp.data(),
-			traction.data()), "pressure traction");
-		const bool any_traction = std::any_of(traction.begin(), traction.end(), [](double v) { return v != 0.0; });
+		const SimulationConfig cfg = step_config(time);
 		if (period > 0.0) {
 			const double w = 2.0*M_PI*time/period;
 			const double phase[4] = {std::sin(w), std::cos(w), std::sin(2.0*w), std::cos(2.0*w)};
@@ -363,42 +356,89 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		} else {
 			c.Check(tfi_flow_begin_step(flow.flow, nullptr, 0), "begin step");
 		}
-		c.Check(tfi_flow_set_boundary(flow.flow, bc.velocity_mask.data(), bc.pressure_mask.data(), bc.velocity.data(),
-			bc.pressure.data(), any_traction ? traction.data() : nullptr), "boundary data");
+		// Outlet fixed point (iga_cuda.cu:927-1084): each pass solves the step
+		// with the outlet models' pressures as tractions, from where the last
+		// pass ended; the models then turn the outlet flows into new pressures.
+		std::vector<double> capacitor_start;
+		for (const auto& om : outlets) capacitor_start.push_back(om.capacitor_pressure);
+		const int passes = outlets.empty() ? 1 : 12;
+		bool outlets_converged = false;
 		tfi_newton_report rep{};
-		const int status = tfi_flow_newton(flow.flow, &rep);
-		const double scale = std::sqrt(static_cast<double>(4*N));
-		for (int r = 0; r < rep.records; ++r) {
-			const auto& rec = rep.record[r];
-			const bool last_converged = status == TFI_OK && r == rep.records - 1;
-			if (last_converged) {
-				std::cout << "step=" << step + 1 << " time=" << time << " converged newton=" << r
-					<< " residual_l2=" << rec.residual_l2 << " residual_rms=" << rec.residual_l2/scale
-					<< " tolerance=" << rep.tolerance << " absolute_rms_tolerance=" << o.nonlinear_atol
-					<< " continuity_l2=" << rec.continuity_l2 << " continuity_sum=" << rec.continuity_sum
-					<< " net_boundary_flow=" << rec.net_boundary_flow
-					<< " relative_mass_imbalance=" << rec.relative_mass_imbalance
-					<< " mass_tolerance=" << o.mass_rtol << '\n';
-			} else {
-				std::cout << "step=" << step + 1 << " time=" << time << " newton=" << r
-					<< " residual_l2=" << rec.residual_l2 << " residual_rms=" << rec.residual_l2/scale
-					<< " continuity_l2=" << rec.continuity_l2 << " continuity_sum=" << rec.continuity_sum
-					<< " net_boundary_flow=" << rec.net_boundary_flow
-					<< " relative_mass_imbalance=" << rec.relative_mass_imbalance << " update_l2=" << rec.update_l2
-					<< " linear_iterations=" << rec.linear_iterations << " linear_residual=" << rec.linear_residual
-					<< " singular_diagonal_blocks=" << rec.singular_diagonal_blocks << " assembly_s=" << rec.assembly_s
-					<< " linear_s=" << rec.linear_s << '\n';
+		for (int coupling = 0; coupling < passes; ++coupling) {
+			outlet_p = OutletPressures(outlets);
+			bc = ResolveFlowBoundaries(cfg, sys, labeled.labels, profile, &outlet_p);
+			ApplyWallTrace(bc, wall_trace);
+			std::vector<std::int32_t> te, tf;
+			std::vector<double> tp;
+			for (std::size_t k = 0; k < face_el.size(); ++k) {
+				const auto it = bc.tractions.find(face_lab[k]);
+				if (it == bc.tractions.end()) continue;
+				te.push_back(face_el[k]);
+				tf.push_back(face_id[k]);
+				tp.push_back(it->second);
 			}
+			c.Check(tfi_pressure_traction(m.mesh, static_cast<int>(te.size()), te.data(), tf.data(), tp.data(),
+				traction.data()), "pressure traction");
+			const bool any_traction = std::any_of(traction.begin(), traction.end(), [](double v) { return v != 0.0; });
+			c.Check(tfi_flow_set_boundary(flow.flow, bc.velocity_mask.data(), bc.pressure_mask.data(), bc.velocity.data(),
+				bc.pressure.data(), any_traction ? traction.data() : nullptr), "boundary data");
+			rep = tfi_newton_report{};
+			const int status = tfi_flow_newton(flow.flow, &rep);
+			const double scale = std::sqrt(static_cast<double>(4*N));
+			for (int r = 0; r < rep.records; ++r) {
+				const auto& rec = rep.record[r];
+				const bool last_converged = status == TFI_OK && r == rep.records - 1;
+				if (last_converged) {
+					std::cout << "step=" << step + 1 << " time=" << time << " converged newton=" << r
+						<< " residual_l2=" << rec.residual_l2 << " residual_rms=" << rec.residual_l2/scale
+						<< " tolerance=" << rep.tolerance << " absolute_rms_tolerance=" << o.nonlinear_atol
+						<< " continuity_l2=" << rec.continuity_l2 << " continuity_sum=" << rec.continuity_sum
+						<< " net_boundary_flow=" << rec.net_boundary_flow
+						<< " relative_mass_imbalance=" << rec.relative_mass_imbalance
+						<< " mass_tolerance=" << o.mass_rtol << '\n';
+				} else {
+					std::cout << "step=" << step + 1 << " time=" << time << " newton=" << r
+						<< " residual_l2=" << rec.residual_l2 << " residual_rms=" << rec.residual_l2/scale
+						<< " continuity_l2=" << rec.continuity_l2 << " continuity_sum=" << rec.continuity_sum
+						<< " net_boundary_flow=" << rec.net_boundary_flow
+						<< " relative_mass_imbalance=" << rec.relative_mass_imbalance << " update_l2=" << rec.update_l2
+						<< " linear_iterations=" << rec.linear_iterations << " linear_residual=" << rec.linear_residual
+						<< " singular_diagonal_blocks=" << rec.singular_diagonal_blocks << " assembly_s=" << rec.assembly_s
+						<< " linear_s=" << rec.linear_s << '\n';
+				}
+			}
+			if (status != TFI_OK)
+				throw std::runtime_error(std::string(tfi_last_error(c.ctx)) + " at physical step " + std::to_string(step + 1));
+			total_assembly += rep.assembly_s;
+			total_linear += rep.linear_s;
+			total_linear_iterations += rep.linear_iterations;
+			if (outlets.empty()) {
+				outlets_converged = true;
+				break;
+			}
+			std::vector<double> flows(outlets.size());
+			c.Check(tfi_flow_boundary_flows(flow.flow, static_cast<int>(outlet_labels.size()), outlet_labels.data(),
+				flows.data()), "outlet flows");
+			const OutletCoupling e = EvaluateOutlets(outlets, capacitor_start, flows, dt);
+			std::cout << "step=" << step + 1 << " time=" << time << " outlet_iteration=" << coupling
+				<< " pressure_change=" << e.maximum_pressure_change << " tolerance=" << e.Tolerance() << '\n';
+			if (e.maximum_pressure_change <= e.Tolerance()) {
+				CommitOutlets(outlets, e);
+				for (const auto& om : outlets)
+					std::cout << "outlet label=" << om.label << " flow=" << om.flow << " pressure=" << om.pressure
+						<< " capacitor_pressure=" << om.capacitor_pressure << '\n';
+				outlets_converged = true;
+				break;
+			}
+			RelaxOutlets(outlets, e);
 		}
-		if (status != TFI_OK)
-			throw std::runtime_error(std::string(tfi_last_error(c.ctx)) + " at physical step " + std::to_string(step + 1));
+		if (!outlets_converged)
+			throw std::runtime_error("outlet fixed-point iteration did not converge at physical step "
+				+ std::to_string(step + 1));
 		if (opt.warm_start != TFI_WARM_PREVIOUS)
 			std::cout << "warm_start step=" << step + 1 << " used=" << rep.warm_start_used
 				<< " initial_ratio=" << rep.warm_start_initial_ratio << '\n';
 		c.Check(tfi_flow_commit_step(flow.flow), "commit step");
-		total_assembly += rep.assembly_s;
-		total_linear += rep.linear_s;
-		total_linear_iterations += rep.linear_iterations;
 		const int completed = step + 1;
 		if (o.output_every > 0 && completed % o.output_every == 0) {
 			c.Check(tfi_flow_get_state(flow.flow, state.data()), "state download");
@@ -418,6 +458,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 			meta.dt = dt;
 			meta.density = sys.density;
 			meta.viscosity = sys.viscosity;
+			meta.outlets = OutletCheckpoint(outlets);
 			WriteCheckpoint(o.checkpoint, meta, state);
 			std::cout << "checkpoint=" << o.checkpoint.string() << " completed_step=" << completed << '\n';
 		}
@@ -480,8 +521,10 @@ int CaseCheck(int argc, char** argv)
 	const FlowSystem& sys = FindFlowSystem(config, system);
 	const bool transient = sys.time_integration == "backward_euler";
 	const double time = transient ? config.dt : 0.0;
+	const auto outlets = InitOutletModels(config, sys);
+	const auto outlet_p = OutletPressures(outlets);
 	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(transient ? MaterializeWaveforms(config, case_dir, time) : config,
-		sys, labeled.labels, profile);
+		sys, labeled.labels, profile, &outlet_p);
 	ApplyWallTrace(bc, trace);
 	double vsum = 0.0, psum = 0.0;
 	long long vhash = 0, phash = 0;
@@ -503,6 +546,9 @@ int CaseCheck(int argc, char** argv)
 		first = false;
 	}
 	std::cout << '\n';
+	for (const auto& om : outlets)
+		std::cout << "outlet label=" << om.label << " kind=" << om.kind << " pressure=" << om.pressure
+			<< " capacitor_pressure=" << om.capacitor_pressure << '\n';
 	return 0;
 }
 
