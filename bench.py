@@ -129,7 +129,10 @@ def rel_l2(a, b):
 def pulsatile_comparison(dm, case, bc, faces, rho, mu, dt, tol, train_steps, warmup=100, steps=10):
     """The learned-warm-start workload on the same mesh: pulsatile inflow,
     the first `warmup` steps untimed (they also train the predictor), then
-    `steps` timed steps of both methods."""
+    `steps` timed steps of both methods.  Over the 110 steps of this run the
+    differences between two states that each meet a relative residual of 1e-10
+    accumulate to a few 1e-10 in pressure, so both methods solve each step to
+    3e-11 here, which keeps the final states within 1e-10 of each other."""
     from tubularflowiga_amd import device
     log(f"pulsatile inflow: plain FP64, {warmup} + {steps} steps")
     plain, plain_all, _, plain_state, _ = run_method(
@@ -141,7 +144,7 @@ def pulsatile_comparison(dm, case, bc, faces, rho, mu, dt, tol, train_steps, war
         train_steps, waveform=pulsatile)
     p, f = method_summary(plain), method_summary(fast)
     return {"workload": f"same mesh, pulsatile Q(t)=5(1+0.6 sin 2 pi t+0.2 sin 4 pi t) ml/s, backward Euler dt={dt}",
-            "warmup": warmup, "steps": steps, "fp64_ms_per_step": p["ms_per_step"],
+            "warmup": warmup, "steps": steps, "newton_rtol": tol["rtol"], "fp64_ms_per_step": p["ms_per_step"],
             "mixed_learned_ms_per_step": f["ms_per_step"],
             "speedup_vs_fp64": round(p["ms_per_step"] / f["ms_per_step"], 3), "plain_fp64": p, "mixed_learned": f,
             "whole_run_ms_per_step": {
@@ -340,7 +343,8 @@ def main(argv=None):
                           "avg_ms": round(spmv_ms, 4), "algorithmic_bytes": int(spmv_bytes)},
     }
     if a.inflow == "steady" and not a.no_pulsatile:
-        out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, a.dt, tol, a.train_steps)
+        out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, a.dt, dict(tol, rtol=3e-11),
+                                                a.train_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, dm, rho, mu, a.dt, plain)
     print(json.dumps(out), flush=True)

@@ -33,7 +33,7 @@ enum { TFI_OK = 0, TFI_ERROR = 1 };
 enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
 enum { TFI_FP64 = 0, TFI_MIXED = 1 };
 enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2, TFI_WARM_EXTRAPOLATE2 = 3,
-	TFI_WARM_EXTRAPOLATE3 = 4, TFI_WARM_EXTRAPOLATE4 = 5 };
+	TFI_WARM_EXTRAPOLATE3 = 4, TFI_WARM_EXTRAPOLATE4 = 5, TFI_WARM_MPE = 6 };
 enum { TFI_SMOOTH_JACOBI = 0, TFI_SMOOTH_GAUSS_SEIDEL = 1 };
 enum { TFI_TANGENT_LEGACY = 0, TFI_TANGENT_EXACT = 1 };
 
@@ -147,10 +147,11 @@ typedef struct {
 	int amg_gs_sweeps;                /* 2 (0): symmetric Gauss-Seidel pre and post; 1: forward pre, backward post */
 	int chord_polish;                 /* 1 (default): a residual check that misses the tolerance by at most
 	                                     1000x is closed by chord steps (solves with the tangent and
-	                                     preconditioner already set up, to 0.1 x tolerance; up to 3, each
-	                                     cutting the residual 4x) before a new tangent is built.  A learned
-	                                     guess predicted within that window starts this way, with the
-	                                     tangent of an earlier step. */
+	                                     preconditioner already set up, each solved as far as the observed
+	                                     contraction makes useful; up to 6, each required to cut the
+	                                     residual 4x) before a new tangent is built.  A learned guess
+	                                     predicted within that window starts this way, with the tangent
+	                                     of an earlier step. */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);
@@ -169,6 +170,8 @@ typedef struct {
 	int warm_start_used;              /* 1 if the learned guess was accepted */
 	int chord_steps;                  /* chord steps taken (see tfi_flow_options.chord_polish) */
 	double warm_start_blend;          /* learned mode: line-search scale t of the network's correction */
+	double warm_start_base_ratio;     /* learned mode: residual of its base prediction alone, relative to
+	                                     the previous-step start (compare warm_start_initial_ratio) */
 	tfi_newton_record record[TFI_MAX_NEWTON_RECORDS];
 } tfi_newton_report;
 
@@ -207,13 +210,17 @@ int tfi_flow_boundary_flows(tfi_flow* flow, int count, const int32_t* labels, do
 int tfi_flow_amg_levels(tfi_flow* flow, int capacity, int* levels, int* rows, long long* block_nonzeros,
 	int* colors);
 
-/* Learned warm start (MLP on 27-control-point stencils). */
+/* Learned warm start (MLP on 27-control-point stencils).  The network
+ * predicts the error of a base prediction: TFI_WARM_MPE (minimal polynomial
+ * extrapolation of the step increments, the default) or TFI_WARM_EXTRAPOLATE4
+ * (quartic extrapolation in time). */
 typedef struct {
 	int hidden;            /* width (128) */
 	int stencil;           /* control points per stencil (27) */
 	int train_steps;       /* optimisation steps per snapshot */
 	double learning_rate;
 	int min_snapshots;     /* snapshots before the predictor is used */
+	int base;              /* TFI_WARM_MPE (0: default) or TFI_WARM_EXTRAPOLATE4 */
 } tfi_warm_options;
 int tfi_flow_warm_configure(tfi_flow* flow, const tfi_warm_options* options);
 /* Training statistics: snapshots seen, last training loss, accepted guesses. */
@@ -222,6 +229,16 @@ int tfi_flow_warm_stats(tfi_flow* flow, int* snapshots, double* last_loss, int* 
  * network's prediction to the latest snapshot before training on it, in
  * [0, 1] (0: the prediction is the extrapolation base alone). */
 int tfi_flow_warm_gate(tfi_flow* flow, double* gate);
+/* The state a predicting mode would start the next step from, given the
+ * states recorded by tfi_flow_commit_step so far (host array [4N]).  For
+ * TFI_WARM_MPE, *order receives the number of increments the extrapolation
+ * combined (may be NULL).  No reference counterpart. */
+int tfi_flow_warm_predict(tfi_flow* flow, int mode, double* state, int* order);
+/* Append a state (host array [4N]) to the warm-start history, as
+ * tfi_flow_commit_step appends the converged state (without training), e.g.
+ * to resume a restarted run with the states before its checkpoint.
+ * tfi_flow_set_state clears the history.  No reference counterpart. */
+int tfi_flow_warm_record(tfi_flow* flow, const double* state);
 
 /* Kernel timers accumulated on the context stream (HIP events). */
 typedef struct {

@@ -16,7 +16,7 @@ LIB_PATH = PKG / "libtfi_hip.so"
 TFI_PC_BJACOBI, TFI_PC_AMG = 0, 1
 TFI_FP64, TFI_MIXED = 0, 1
 TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3 = 0, 1, 2, 3, 4
-TFI_WARM_EXTRAPOLATE4 = 5
+TFI_WARM_EXTRAPOLATE4, TFI_WARM_MPE = 5, 6
 TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL = 0, 1
 TFI_TANGENT_LEGACY, TFI_TANGENT_EXACT = 0, 1
 MAX_NEWTON_RECORDS = 64
@@ -52,13 +52,13 @@ class NewtonReport(ctypes.Structure):
                 ("records", c_int), ("initial_residual", c_double), ("final_residual", c_double),
                 ("tolerance", c_double), ("assembly_s", c_double), ("linear_s", c_double),
                 ("total_s", c_double), ("warm_start_initial_ratio", c_double), ("warm_start_used", c_int),
-                ("chord_steps", c_int), ("warm_start_blend", c_double),
+                ("chord_steps", c_int), ("warm_start_blend", c_double), ("warm_start_base_ratio", c_double),
                 ("record", NewtonRecord * MAX_NEWTON_RECORDS)]
 
 
 class WarmOptions(ctypes.Structure):
     _fields_ = [("hidden", c_int), ("stencil", c_int), ("train_steps", c_int), ("learning_rate", c_double),
-                ("min_snapshots", c_int)]
+                ("min_snapshots", c_int), ("base", c_int)]
 
 
 class KernelTimers(ctypes.Structure):
@@ -114,6 +114,8 @@ SIGNATURES = {
     "tfi_flow_amg_levels": (c_int, [c_void_p, c_int, P(c_int), P(c_int), P(c_ll), P(c_int)]),
     "tfi_flow_warm_stats": (c_int, [c_void_p, P(c_int), P(c_double), P(c_int), P(c_int)]),
     "tfi_flow_warm_gate": (c_int, [c_void_p, P(c_double)]),
+    "tfi_flow_warm_predict": (c_int, [c_void_p, c_int, _d, P(c_int)]),
+    "tfi_flow_warm_record": (c_int, [c_void_p, _d]),
     "tfi_timers_get": (c_int, [c_void_p, P(KernelTimers)]),
     "tfi_timers_enable": (c_int, [c_void_p, c_int]),
     "tfi_timers_reset": (c_int, [c_void_p]),

@@ -256,6 +256,8 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 	return Guard(mesh->ctx, [&] {
 		if (!(options->density > 0.0) || !(options->viscosity > 0.0) || !(options->dt >= 0.0))
 			throw Error("invalid Navier-Stokes density, dynamic viscosity, or time step");
+		if (options->warm_start < TFI_WARM_PREVIOUS || options->warm_start > TFI_WARM_MPE)
+			throw Error("unknown warm-start mode " + std::to_string(options->warm_start));
 		auto f = std::make_unique<tfi_flow>();
 		f->mesh = mesh;
 		f->ctx = mesh->ctx;
@@ -430,13 +432,14 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			double predicted = std::numeric_limits<double>::infinity();  // linearised ||r|| of the guess
 			if (f->opt.warm_start == TFI_WARM_LEARNED) {
 				// Residual line search along the learned correction: with u_b the
-				// extrapolation base and u_L = u_b + g c the network's guess, take
+				// base prediction and u_L = u_b + g c the network's guess, take
 				// u = u_b + t (u_L - u_b) with t minimising the linearised
 				// ||r_b + t (r_L - r_b)||, t in [0, 2].  t = 0 is the base alone.
-				f->warm->Predict(TFI_WARM_EXTRAPOLATE4, f->state.data(), f->phase, f->phase_count);
+				f->warm->PredictBase(f->state.data());
 				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
 					f->state.data());
 				const double nb = assemble(false, nullptr);
+				rep.warm_start_base_ratio = base > 0.0 ? nb/base : 0.0;
 				Copy(ctx, f->rhs.data(), f->resid_base.data(), n);
 				Copy(ctx, f->state.data(), f->guess_base.data(), n);
 				f->warm->Predict(TFI_WARM_LEARNED, f->state.data(), f->phase, f->phase_count);
@@ -689,6 +692,32 @@ int tfi_flow_warm_gate(tfi_flow* f, double* gate)
 	return Guard(f->ctx, [&] {
 		if (!gate) throw Error("tfi_flow_warm_gate needs an output pointer");
 		*gate = f->warm->Gate();
+	});
+}
+
+int tfi_flow_warm_record(tfi_flow* f, const double* state)
+{
+	return Guard(f->ctx, [&] {
+		if (!state) throw Error("tfi_flow_warm_record needs a state");
+		const long long n = 4LL*f->mesh->nodes;
+		f->update.Upload(state, n);
+		const double none[1] = {0.0};
+		f->warm->Commit(f->update.data(), none, 0, false);
+		Sync(f->ctx);
+	});
+}
+
+int tfi_flow_warm_predict(tfi_flow* f, int mode, double* state, int* order)
+{
+	return Guard(f->ctx, [&] {
+		if (!state) throw Error("tfi_flow_warm_predict needs an output array");
+		if (mode == TFI_WARM_PREVIOUS || !f->warm->Ready(mode))
+			throw Error("warm-start mode " + std::to_string(mode) + " has no prediction for the recorded history");
+		const long long n = 4LL*f->mesh->nodes;
+		Copy(f->ctx, f->state.data(), f->update.data(), n);
+		f->warm->Predict(mode, f->update.data(), f->phase, f->phase_count);
+		f->update.Download(state, n);
+		if (order) *order = mode == TFI_WARM_MPE ? f->warm->MpeOrder() : 0;
 	});
 }
 

@@ -283,24 +283,27 @@ __global__ void FeatureKernel(int count, const int* __restrict__ rows, const int
 	X[t] = v;
 }
 
-// extrapolation error of the next state, scaled by s_P
+// error of the base prediction (base, or the extrapolation when base is null)
+// of the next state, scaled by s_P
 __global__ void TargetKernel(int count, const int* __restrict__ rows, const double* __restrict__ next, Hist H,
-	const double* __restrict__ scales, float* __restrict__ T)
+	const double* __restrict__ base, const double* __restrict__ scales, float* __restrict__ T)
 {
 	const int t = blockIdx.x*blockDim.x + threadIdx.x;
 	if (t >= count*kOut) return;
 	const int r = t >> 2, f = t & 3;
 	const long long k = 4LL*rows[r] + f;
-	T[t] = static_cast<float>((next[k] - Extrapolate(H, k))/scales[4*kOrder + f]);
+	const double b = base ? base[k] : Extrapolate(H, k);
+	T[t] = static_cast<float>((next[k] - b)/scales[4*kOrder + f]);
 }
 
-__global__ void ApplyPredictionKernel(int nodes, Hist H, const float* __restrict__ Y, const double* __restrict__ scales,
-	double gate, double* __restrict__ state)
+__global__ void ApplyPredictionKernel(int nodes, Hist H, const double* __restrict__ base, const float* __restrict__ Y,
+	const double* __restrict__ scales, double gate, double* __restrict__ state)
 {
 	const int t = blockIdx.x*blockDim.x + threadIdx.x;
 	if (t >= nodes*4) return;
 	const int f = t & 3;
-	state[t] = Extrapolate(H, t) + gate*scales[4*kOrder + f]*static_cast<double>(Y[t]);
+	const double b = base ? base[t] : Extrapolate(H, t);
+	state[t] = b + gate*scales[4*kOrder + f]*static_cast<double>(Y[t]);
 }
 
 constexpr int kScales = 4*(kOrder + 1);
@@ -628,7 +631,7 @@ double Mlp::gate() const
 	return impl_->gate;
 }
 
-void Mlp::Train(const double* next, const double* const* history, const double* phase)
+void Mlp::Train(const double* next, const double* const* history, const double* base, const double* phase)
 {
 	Impl& I = *impl_;
 	tfi_ctx* ctx = I.ctx;
@@ -650,7 +653,7 @@ void Mlp::Train(const double* next, const double* const* history, const double* 
 			I.stencil.data(), I.pos.data(), I.hscale.data(), H, I.scales.data(), I.phase.data(), I.X.data());
 		TFI_LAUNCH_CHECK("FeatureKernel");
 		hipLaunchKernelGGL(TargetKernel, dim3((I.batch*kOut + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch,
-			I.rows.data(), next, H, I.scales.data(), I.T.data());
+			I.rows.data(), next, H, base, I.scales.data(), I.T.data());
 		TFI_LAUNCH_CHECK("TargetKernel");
 		I.Forward(I.batch, I.X.data());
 		constexpr int kGateBlocks = 32;
@@ -677,7 +680,7 @@ void Mlp::Train(const double* next, const double* const* history, const double* 
 			I.stencil.data(), I.pos.data(), I.hscale.data(), H, I.scales.data(), I.phase.data(), I.X.data());
 		TFI_LAUNCH_CHECK("FeatureKernel");
 		hipLaunchKernelGGL(TargetKernel, dim3((I.batch*kOut + kT - 1)/kT), dim3(kT), 0, ctx->stream, I.batch,
-			I.rows.data(), next, H, I.scales.data(), I.T.data());
+			I.rows.data(), next, H, base, I.scales.data(), I.T.data());
 		TFI_LAUNCH_CHECK("TargetKernel");
 		I.Forward(I.batch, I.X.data());
 		loss = I.Backward(I.batch, I.X.data(), step + 1 == steps);
@@ -695,7 +698,7 @@ void Mlp::Train(const double* next, const double* const* history, const double* 
 	++snapshots_;
 }
 
-void Mlp::Predict(const double* const* history, const double* phase, double* state)
+void Mlp::Predict(const double* const* history, const double* base, const double* phase, double* state)
 {
 	Impl& I = *impl_;
 	tfi_ctx* ctx = I.ctx;
@@ -739,7 +742,7 @@ void Mlp::Predict(const double* const* history, const double* phase, double* sta
 		throw Error("BF16 inference kernels are built for widths 64 and 128");
 	}
 	TFI_LAUNCH_CHECK("MlpInferKernel");
-	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, H, Y,
+	hipLaunchKernelGGL(ApplyPredictionKernel, dim3((4*N + kT - 1)/kT), dim3(kT), 0, ctx->stream, N, H, base, Y,
 		I.scales.data(), I.gate, state);
 	TFI_LAUNCH_CHECK("ApplyPredictionKernel");
 	Sync(ctx);

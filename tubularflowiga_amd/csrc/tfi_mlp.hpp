@@ -11,9 +11,11 @@ namespace tfi {
 // backward differences d_k(s)/s_k for k = 1..P (4 each), then
 // (x_s - x_i)/h_i (3); then up to 8 phase features; zeros to kIn.
 //   d_k = sum_j (-1)^j C(k, j) u_{n-j}   (d1 = u_n - u_{n-1}, ...)
-// Output (4): d_{P+1} of the next state scaled by s_P, i.e. the error of the
-// order-P polynomial extrapolation in time, so the prediction is
-//   u_{n+1} ~ sum_{j=0..P} (-1)^j C(P+1, j+1) u_{n-j} + s_P * MLP(x),
+// Output (4): the error of a base prediction b of the next state (minimal
+// polynomial extrapolation of the increments, or the order-P polynomial
+// extrapolation in time sum_{j=0..P} (-1)^j C(P+1, j+1) u_{n-j}) scaled by
+// s_P, so the prediction is
+//   u_{n+1} ~ b + s_P * MLP(x),
 // with s_0..s_P the per-field RMS of u_n, d_1, .., d_P.
 constexpr int kOrder = 4;
 constexpr int kStencil = 27;
@@ -25,10 +27,11 @@ class Mlp {
 public:
 	Mlp(tfi_mesh* mesh, const tfi_warm_options& options);
 	~Mlp();
-	// One snapshot: history h[j] = u_{n-j} (j = 0..kOrder) -> next = u_{n+1}.
-	void Train(const double* next, const double* const* history, const double* phase);
-	// state <- order-kOrder extrapolation + s_P * MLP(features(history, phase)).
-	void Predict(const double* const* history, const double* phase, double* state);
+	// One snapshot: history h[j] = u_{n-j} (j = 0..kOrder), base prediction of
+	// u_{n+1} (nullptr: the order-kOrder extrapolation) -> next = u_{n+1}.
+	void Train(const double* next, const double* const* history, const double* base, const double* phase);
+	// state <- base (nullptr: order-kOrder extrapolation) + s_P * MLP(features(history, phase)).
+	void Predict(const double* const* history, const double* base, const double* phase, double* state);
 	int snapshots() const { return snapshots_; }
 	double last_loss() const { return last_loss_; }
 	double gate() const;

@@ -68,6 +68,10 @@ public:
 	bool Ready(int mode) const;
 	// state <- predicted state of the next step (state holds u_n on entry).
 	void Predict(int mode, double* state, const double* phase, int phase_count);
+	// state <- the learned mode's base prediction (its correction is added to it)
+	void PredictBase(double* state);
+	// increments combined by the last TFI_WARM_MPE prediction
+	int MpeOrder() const;
 	void Accepted(bool used);
 	void Stats(int* snapshots, double* last_loss, int* accepted, int* rejected) const;
 	double Gate() const;

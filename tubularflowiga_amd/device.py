@@ -15,13 +15,13 @@ from . import _abi
 from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, WarmOptions, TFI_FP64, TFI_MIXED,
                    TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3,
                    TFI_WARM_EXTRAPOLATE4,
-                   TFI_WARM_LEARNED,
+                   TFI_WARM_LEARNED, TFI_WARM_MPE,
                    TFI_WARM_PREVIOUS)
 
 __all__ = ["Context", "DeviceMesh", "FlowSolver", "TFI_FP64", "TFI_MIXED", "TFI_PC_AMG", "TFI_PC_BJACOBI",
            "TFI_WARM_PREVIOUS", "TFI_WARM_LEARNED", "TFI_WARM_EXTRAPOLATE",
            "TFI_WARM_EXTRAPOLATE2", "TFI_WARM_EXTRAPOLATE3",
-           "TFI_WARM_EXTRAPOLATE4"]
+           "TFI_WARM_EXTRAPOLATE4", "TFI_WARM_MPE"]
 
 
 class TfiError(RuntimeError):
@@ -289,8 +289,10 @@ class FlowSolver:
         self.ctx.check(self.lib.tfi_flow_boundary_flows(self.handle, len(labels), labels, out))
         return out
 
-    def configure_warm_start(self, hidden=128, train_steps=40, learning_rate=1e-3, min_snapshots=3):
-        o = WarmOptions(hidden, 27, train_steps, learning_rate, min_snapshots)
+    def configure_warm_start(self, hidden=128, train_steps=40, learning_rate=1e-3, min_snapshots=3, base=0):
+        """base: TFI_WARM_MPE (0: default) or TFI_WARM_EXTRAPOLATE4, the
+        prediction whose error the network learns."""
+        o = WarmOptions(hidden, 27, train_steps, learning_rate, min_snapshots, base)
         self.ctx.check(self.lib.tfi_flow_warm_configure(self.handle, ctypes.byref(o)))
 
     def amg_levels(self):
@@ -302,6 +304,18 @@ class FlowSolver:
         cols = (ctypes.c_int * cap)()
         self.ctx.check(self.lib.tfi_flow_amg_levels(self.handle, cap, ctypes.byref(n), rows, nnzb, cols))
         return [(rows[i], nnzb[i], cols[i]) for i in range(min(n.value, cap))]
+
+    def warm_record(self, state):
+        """Append a state to the warm-start history (no training)."""
+        self.ctx.check(self.lib.tfi_flow_warm_record(self.handle, np.ascontiguousarray(state, np.float64)))
+
+    def warm_predict(self, mode):
+        """(state the predicting `mode` would start the next step from, order
+        of the increment extrapolation for TFI_WARM_MPE)."""
+        out = np.zeros(4 * self.nodes)
+        order = ctypes.c_int()
+        self.ctx.check(self.lib.tfi_flow_warm_predict(self.handle, mode, out, ctypes.byref(order)))
+        return out, order.value
 
     def warm_stats(self):
         s, l, a, r = ctypes.c_int(), ctypes.c_double(), ctypes.c_int(), ctypes.c_int()

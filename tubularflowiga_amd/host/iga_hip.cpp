@@ -144,7 +144,7 @@ const char* kUsage =
 	"[--checkpoint PREFIX] [--checkpoint-every N] [--restart PREFIX] "
 	"[--stop-after-step N] [--nonlinear-rtol R] [--nonlinear-atol A] "
 	"[--mass-rtol R] [--visualization-format auto|vtu] "
-	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|learned] "
+	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|mpe|learned] "
 	"[--tangent exact|legacy] [--amg-smoother gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
 	"[--linear-max-iterations N] [--chord-polish on|off] [--train-steps N]";
 
@@ -188,7 +188,7 @@ Options ParseOptions(int argc, char** argv)
 		else if (arg == "--warm-start")
 			o.warm_start = Choice(v, arg, {{"previous", TFI_WARM_PREVIOUS}, {"extrapolate", TFI_WARM_EXTRAPOLATE},
 				{"extrapolate2", TFI_WARM_EXTRAPOLATE2}, {"extrapolate3", TFI_WARM_EXTRAPOLATE3},
-				{"extrapolate4", TFI_WARM_EXTRAPOLATE4},
+				{"extrapolate4", TFI_WARM_EXTRAPOLATE4}, {"mpe", TFI_WARM_MPE},
 				{"learned", TFI_WARM_LEARNED}});
 		else if (arg == "--tangent") o.tangent = Choice(v, arg, {{"exact", TFI_TANGENT_EXACT}, {"legacy", TFI_TANGENT_LEGACY}});
 		else if (arg == "--amg-smoother")
@@ -297,7 +297,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	Flow flow;
 	c.Check(tfi_flow_create(m.mesh, &opt, &flow.flow), "flow solver");
 	if (opt.warm_start == TFI_WARM_LEARNED && o.train_steps > 0) {
-		tfi_warm_options w{128, 27, o.train_steps, 1e-3, 2};
+		tfi_warm_options w{128, 27, o.train_steps, 1e-3, 2, TFI_WARM_MPE};
 		c.Check(tfi_flow_warm_configure(flow.flow, &w), "warm start");
 	}
 	// Inflow phase for the learned predictor: sin / cos of 2 pi t / T and
