@@ -392,6 +392,9 @@ constexpr double kStaleStart = 1e-3;
 // fails, is backtracked by halves (at most this many times in a step).
 constexpr int kMaxBacktracks = 6;
 constexpr double kChordContraction = 4.0;
+// A learned correction whose gate (its least-squares scale on the latest unseen
+// snapshot) is below this is not evaluated: the guess is the base prediction.
+constexpr double kGateOff = 0.05;
 
 int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 {
@@ -430,31 +433,44 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			Copy(ctx, f->state.data(), f->update.data(), n);  // keep the plain guess
 			double blend = 1.0;
 			double predicted = std::numeric_limits<double>::infinity();  // linearised ||r|| of the guess
+			bool guess_assembled = false;  // rhs and pre_mt already hold the guess's residual
 			if (f->opt.warm_start == TFI_WARM_LEARNED) {
 				// Residual line search along the learned correction: with u_b the
 				// base prediction and u_L = u_b + g c the network's guess, take
 				// u = u_b + t (u_L - u_b) with t minimising the linearised
 				// ||r_b + t (r_L - r_b)||, t in [0, 2].  t = 0 is the base alone.
+				// A correction the gate has switched off is not evaluated: the
+				// guess is u_b, whose residual the base assembly leaves in rhs.
+				const bool network = f->warm->Gate() >= kGateOff;
 				f->warm->PredictBase(f->state.data());
 				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
 					f->state.data());
-				const double nb = assemble(false, nullptr);
+				const double nb = assemble(false, network ? nullptr : &pre_mt);
 				rep.warm_start_base_ratio = base > 0.0 ? nb/base : 0.0;
-				Copy(ctx, f->rhs.data(), f->resid_base.data(), n);
-				Copy(ctx, f->state.data(), f->guess_base.data(), n);
-				f->warm->Predict(TFI_WARM_LEARNED, f->state.data(), f->phase, f->phase_count);
-				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
-					f->state.data());
-				const double nl = assemble(false, nullptr);
-				const double bl = Dot(ctx, f->resid_base.data(), f->rhs.data(), n);
-				const double dd = nl*nl - 2.0*bl + nb*nb;
-				blend = dd > 1e-24*nb*nb ? std::min(2.0, std::max(0.0, (nb*nb - bl)/dd)) : 1.0;
-				predicted = std::sqrt(std::max(0.0, nb*nb + 2.0*blend*(bl - nb*nb) + blend*blend*dd));
-				Axpby(ctx, 1.0 - blend, f->guess_base.data(), blend, f->state.data(), n);
+				if (!network) {
+					blend = 0.0;
+					predicted = nb;
+					guess_assembled = true;
+				} else {
+					Copy(ctx, f->rhs.data(), f->resid_base.data(), n);
+					Copy(ctx, f->state.data(), f->guess_base.data(), n);
+					f->warm->Predict(TFI_WARM_LEARNED, f->state.data(), f->phase, f->phase_count);
+					SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
+						f->state.data());
+					const double nl = assemble(false, nullptr);
+					const double bl = Dot(ctx, f->resid_base.data(), f->rhs.data(), n);
+					const double dd = nl*nl - 2.0*bl + nb*nb;
+					blend = dd > 1e-24*nb*nb ? std::min(2.0, std::max(0.0, (nb*nb - bl)/dd)) : 1.0;
+					predicted = std::sqrt(std::max(0.0, nb*nb + 2.0*blend*(bl - nb*nb) + blend*blend*dd));
+					Axpby(ctx, 1.0 - blend, f->guess_base.data(), blend, f->state.data(), n);
+				}
 			} else {
+				// a plain prediction: its residual decides between a chord start and a new tangent
 				f->warm->Predict(f->opt.warm_start, f->state.data(), f->phase, f->phase_count);
 				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
 					f->state.data());
+				predicted = assemble(false, &pre_mt);
+				guess_assembled = true;
 			}
 			rep.warm_start_blend = blend;
 			// A guess predicted within the chord window of the tolerance is checked
@@ -464,7 +480,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			const double tol_guess = std::max(f->opt.nonlinear_atol*scale, f->opt.nonlinear_rtol*base);
 			pre_jacobian = !(f->opt.chord_polish && f->tangent_ready
 				&& predicted <= std::max(kChordWindow*tol_guess, kStaleStart*base));
-			const double guessed = assemble(pre_jacobian, &pre_mt);
+			const double guessed = guess_assembled && !pre_jacobian ? predicted : assemble(pre_jacobian, &pre_mt);
 			pre_asm = std::chrono::duration<double>(Clock::now() - a0).count();
 			rep.assembly_s += pre_asm;
 			rep.warm_start_initial_ratio = base > 0.0 ? guessed/base : 0.0;
