@@ -240,6 +240,43 @@ int tfi_flow_warm_predict(tfi_flow* flow, int mode, double* state, int* order);
  * tfi_flow_set_state clears the history.  No reference counterpart. */
 int tfi_flow_warm_record(tfi_flow* flow, const double* state);
 
+/* ------------------------------------------------------------------ transport */
+/* Two-species neuron transport on the mesh pattern with 2x2 node blocks
+ * [N0, Nplus] (`iga_cuda transport`, solvers/cuda/src/iga_cuda.cu:621-733).
+ * Parameters as TransportParameters (solvers/cpu/include/CaseInput.hpp:19-33)
+ * without the step count and boundary values. */
+typedef struct tfi_transport tfi_transport;
+typedef struct {
+	double diffusion, vplus, vminus, kplus, kminus, detach_plus, detach_minus, dt;
+	double artificial_diffusion;
+} tfi_transport_params;
+/* Assembles the backward-Euler matrices L and P for the prescribed nodal
+ * velocity [3N] (AssembleTransportKernel, IgaCudaKernels.cuh:155-237) and
+ * applies the constrained rows (ApplyTransportBoundaryKernel, :411-424):
+ * constrained [N] = 1 for a node whose N0 and Nplus are prescribed. */
+int tfi_transport_create(tfi_mesh* mesh, const tfi_transport_params* params, const double* velocity,
+	const int32_t* constrained, tfi_transport** out);
+void tfi_transport_destroy(tfi_transport* transport);
+/* Prescribed N0 / Nplus values [N] of the constrained nodes
+ * (SetTransportBoundaryVectorKernel, IgaCudaKernels.cuh:426-434). */
+int tfi_transport_set_boundary(tfi_transport* transport, const double* n0, const double* nplus);
+/* State [2N] (N0, Nplus per node). */
+int tfi_transport_set_state(tfi_transport* transport, const double* state);
+int tfi_transport_get_state(tfi_transport* transport, double* state);
+/* state <- prescribed values at the constrained nodes (iga_cuda.cu:674-677). */
+int tfi_transport_apply_boundary(tfi_transport* transport);
+/* One backward-Euler step (iga_cuda.cu:685-705): b = P c, constrained entries
+ * of b set to their values, L c' = b solved as D^-1 L c' = D^-1 b (D the 2x2
+ * diagonal blocks, the reference's left block-Jacobi preconditioning) by
+ * FGMRES(50) from c' = 0 (zero_guess) or c, to ||D^-1 (b - L c')|| <= rtol
+ * ||D^-1 b||; c <- c'.  A solve that does not converge is an error.  Destroy
+ * the transport before its context. */
+int tfi_transport_step(tfi_transport* transport, int zero_guess, int maximum_iterations, double relative_tolerance,
+	int* iterations, double* relative_residual);
+/* Host copies of the assembled 2x2 blocks of L and P [nnzb*4] and the count of
+ * diagonal blocks inverted by their diagonal (either pointer may be NULL). */
+int tfi_transport_matrices(tfi_transport* transport, double* left, double* previous, int* singular_blocks);
+
 /* Kernel timers accumulated on the context stream (HIP events). */
 typedef struct {
 	double spmv_ms, spmv_calls, spmv_bytes;           /* fine-level fp64 SpMV */

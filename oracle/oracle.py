@@ -46,6 +46,7 @@ def lib():
                                             ctypes.POINTER(c_dbl)]
         L.oracle_ns_element.argtypes = [c_int, _dp, _dp, _dp, _dp, c_dbl, c_dbl, c_dbl, c_int, c_int,
                                         vp, vp, vp, _dp]
+        L.oracle_transport_element.argtypes = [c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_block_pattern.argtypes = [c_int, c_int, _ip, _ip, vp, vp, vp, ctypes.POINTER(c_ll)]
         L.oracle_ns_assemble.argtypes = [c_int, c_int, _ip, _ip, _ip, _u8p, _dp, _dp, _ip, _ip, _dp, _dp,
                                          c_dbl, c_dbl, c_dbl, c_int, vp, _dp]

@@ -25,6 +25,16 @@
 //                                          RelaxOutletCoupling; pressure and capacitor
 //                                          pressure after CommitOutletCoupling
 //                                          (include/OutletModel.hpp)
+//   ref_probe transport-case CASE_DIR NODES -> ReadTransportParameters of
+//                                          simulation_parameter.txt, then
+//                                          ResolveBoundaryConditions with
+//                                          ReadCaseConfiguration (case_config.json,
+//                                          optional) and ReadPointLabels: the
+//                                          twelve parameters, the transport node
+//                                          count, the sum of the constrained node
+//                                          indices, the sums of N0 and Nplus and
+//                                          the velocity / pressure node counts
+//                                          (solvers/cpu/include/CaseInput.hpp)
 // ELEMENT file: nen, then nen connectivity ints, nen x 64 extraction values,
 // 64 x 3 Bezier points, nen x 4 nodal state values (whitespace separated).
 #include "BoundaryFlow.hpp"
@@ -32,6 +42,7 @@
 #include "IgaDatabase.hpp"
 #include "Quadrature.hpp"
 #include "OutletModel.hpp"
+#include "CaseInput.hpp"
 
 #include <cstdio>
 #include <fstream>
@@ -156,6 +167,28 @@ int main(int argc, char** argv)
 				for (const auto& c : b.conditions)
 					if (c.kind == iga::FieldBoundaryKind::PressureTraction)
 						std::printf("%d %.17g\n", b.label, c.value.at(0));
+		} else if (cmd == "transport-case" && argc == 4) {
+			const std::string dir = argv[2];
+			const auto nodes = static_cast<std::uint64_t>(std::stoull(argv[3]));
+			const auto p = iga::ReadTransportParameters(dir + "/simulation_parameter.txt");
+			const auto labels = iga::ReadPointLabels(dir + "/controlmesh.vtk", nodes);
+			const auto velocity = iga::ReadVelocity(dir + "/initial_velocityfield.txt", nodes);
+			const auto config = iga::ReadCaseConfiguration(dir + "/case_config.json");
+			const auto bc = iga::ResolveBoundaryConditions(config, labels, velocity, p);
+			for (double v : {p.diffusion, p.vplus, p.vminus, p.kplus, p.kminus, p.detach_plus, p.detach_minus, p.dt,
+				static_cast<double>(p.steps), p.n0_bc, p.nplus_bc, p.nminus_bc, p.artificial_diffusion})
+				Print(v);
+			long long index_sum = 0;
+			double n0 = 0.0, nplus = 0.0;
+			for (std::size_t i = 0; i < bc.transport_constrained.size(); ++i) {
+				if (bc.transport_constrained[i]) index_sum += static_cast<long long>(i);
+				n0 += bc.n0[i];
+				nplus += bc.nplus[i];
+			}
+			std::printf("%zu %lld\n", bc.transport_nodes, index_sum);
+			Print(n0);
+			Print(nplus);
+			std::printf("%zu %zu\n", bc.velocity_nodes, bc.pressure_nodes);
 		} else if (cmd == "outlet" && argc == 12) {
 			iga::OutletModelState m;
 			m.label = 2;

@@ -333,6 +333,54 @@ void NavierStokesElement(const ElementData& e, const double* state, const double
 	}
 }
 
+// Two-species neuron transport element (BuildTransportElement,
+// solvers/cpu/include/TransportElement.hpp:213-253; the GPU reference builds the
+// same matrices per (a, b) pair, solvers/cuda/include/IgaCudaKernels.cuh:155-237).
+// p = {D, v+, v-, k+, k-, k'+, k'-, dt, artificial diffusion}; velocity [nen][3]
+// nodal; left / previous [2nen][2nen] with dof 2a + field.
+void TransportElement(const ElementData& e, const double* velocity, const double* p, int nq, const double* qpts,
+	const double* qw, double* left, double* previous)
+{
+	const int nen = e.nen;
+	const int ndof = 2*nen;
+	const double D = p[0], vplus = p[1], kplus = p[3], kminus = p[4], detach_plus = p[5], dt = p[7], eps = p[8];
+	std::fill(left, left + static_cast<size_t>(ndof)*ndof, 0.0);
+	std::fill(previous, previous + static_cast<size_t>(ndof)*ndof, 0.0);
+	Basis basis;
+	std::vector<double> test(nen);
+	for (int q = 0; q < nq; ++q) {
+		EvaluateBasis(e, qpts[3*q], qpts[3*q + 1], qpts[3*q + 2], false, basis);
+		const double measure = qw[q]*basis.raw_det;
+		double u[3] = {0.0, 0.0, 0.0};
+		for (int a = 0; a < nen; ++a)
+			for (int d = 0; d < 3; ++d) u[d] += basis.value[a]*velocity[3*a + d];
+		double inverse_length = 0.0;
+		for (int a = 0; a < nen; ++a)
+			inverse_length += std::abs(u[0]*basis.grad[3*a] + u[1]*basis.grad[3*a + 1] + u[2]*basis.grad[3*a + 2]);
+		const double tau_space = inverse_length > 0.0 ? 1.0/inverse_length : 0.0;
+		const double tau_time = dt/2.0;
+		const double tau = tau_space > 0.0 ? 1.0/std::sqrt(1.0/(tau_space*tau_space) + 1.0/(tau_time*tau_time)) : 0.0;
+		for (int a = 0; a < nen; ++a)
+			test[a] = basis.value[a] + tau*(u[0]*basis.grad[3*a] + u[1]*basis.grad[3*a + 1] + u[2]*basis.grad[3*a + 2]);
+		for (int a = 0; a < nen; ++a)
+			for (int b = 0; b < nen; ++b) {
+				const double* ga = &basis.grad[3*a];
+				const double* gb = &basis.grad[3*b];
+				const double gd = ga[0]*gb[0] + ga[1]*gb[1] + ga[2]*gb[2];
+				const double adv = u[0]*gb[0] + u[1]*gb[1] + u[2]*gb[2];
+				const double nn = basis.value[a]*basis.value[b];
+				const size_t r0 = 2*a, rp = 2*a + 1, c0 = 2*b, cp = 2*b + 1;
+				left[r0*ndof + c0] += ((1.0 + dt*(kplus + kminus))*nn + dt*D*gd)*measure;
+				left[r0*ndof + cp] += -detach_plus*dt*nn*measure;
+				left[rp*ndof + c0] += -kplus*dt*test[a]*basis.value[b]*measure;
+				left[rp*ndof + cp] += ((1.0 + dt*detach_plus)*test[a]*basis.value[b] + dt*test[a]*adv
+					+ dt*vplus*eps*test[a]*gd)*measure;
+				previous[r0*ndof + c0] += nn*measure;
+				previous[rp*ndof + cp] += test[a]*basis.value[b]*measure;
+			}
+	}
+}
+
 void FullRule(std::vector<double>& pts, std::vector<double>& w)
 {
 	pts.resize(192);
@@ -532,6 +580,21 @@ int oracle_ns_element(int nen, const double* bezier, const double* extraction, c
 			qw = w.data();
 		}
 		NavierStokesElement(e, state, previous, rho, mu, dt, tangent, nq, qpts, qw, jac, negres);
+	});
+}
+
+int oracle_transport_element(int nen, const double* bezier, const double* extraction, const double* velocity,
+	const double* params, double* left, double* previous)
+{
+	return Guard([&] {
+		if (!(params[7] > 0.0)) throw std::runtime_error("transport dt must be positive");
+		ElementData e;
+		e.nen = nen;
+		e.bezier = bezier;
+		e.extraction.assign(extraction, extraction + static_cast<size_t>(nen)*64);
+		std::vector<double> pts, w;
+		FullRule(pts, w);
+		TransportElement(e, velocity, params, 64, pts.data(), w.data(), left, previous);
 	});
 }
 

@@ -61,6 +61,12 @@ class WarmOptions(ctypes.Structure):
                 ("min_snapshots", c_int), ("base", c_int)]
 
 
+class TransportParams(ctypes.Structure):
+    _fields_ = [("diffusion", c_double), ("vplus", c_double), ("vminus", c_double), ("kplus", c_double),
+                ("kminus", c_double), ("detach_plus", c_double), ("detach_minus", c_double), ("dt", c_double),
+                ("artificial_diffusion", c_double)]
+
+
 class KernelTimers(ctypes.Structure):
     _fields_ = [("spmv_ms", c_double), ("spmv_calls", c_double), ("spmv_bytes", c_double),
                 ("spmv32_ms", c_double), ("spmv32_calls", c_double), ("spmv32_bytes", c_double),
@@ -116,6 +122,14 @@ SIGNATURES = {
     "tfi_flow_warm_gate": (c_int, [c_void_p, P(c_double)]),
     "tfi_flow_warm_predict": (c_int, [c_void_p, c_int, _d, P(c_int)]),
     "tfi_flow_warm_record": (c_int, [c_void_p, _d]),
+    "tfi_transport_create": (c_int, [c_void_p, P(TransportParams), _d, _i32, P(c_void_p)]),
+    "tfi_transport_destroy": (None, [c_void_p]),
+    "tfi_transport_set_boundary": (c_int, [c_void_p, _d, _d]),
+    "tfi_transport_set_state": (c_int, [c_void_p, _d]),
+    "tfi_transport_get_state": (c_int, [c_void_p, _d]),
+    "tfi_transport_apply_boundary": (c_int, [c_void_p]),
+    "tfi_transport_step": (c_int, [c_void_p, c_int, c_int, c_double, P(c_int), P(c_double)]),
+    "tfi_transport_matrices": (c_int, [c_void_p, c_void_p, c_void_p, P(c_int)]),
     "tfi_timers_get": (c_int, [c_void_p, P(KernelTimers)]),
     "tfi_timers_enable": (c_int, [c_void_p, c_int]),
     "tfi_timers_reset": (c_int, [c_void_p]),

@@ -533,6 +533,38 @@ def write_case(case: Case, directory, *, rho=1060.0, mu=3.5e-3, dt=1e-3, steps=5
     return d
 
 
+TRANSPORT_KEYS = {"diffusion": "D", "vplus": "vplus", "vminus": "vminus", "kplus": "kplus", "kminus": "kminus",
+                  "detach_plus": "k'plus", "detach_minus": "k'minus", "dt": "dt", "n0_bc": "N0bc",
+                  "nplus_bc": "Nplusbc", "nminus_bc": "Nminusbc", "artificial_diffusion": "artificial_diffusion"}
+
+
+def tube_velocity(case: Case) -> np.ndarray:
+    """[N,3] velocity over a straight tube along z: every cross-section carries
+    the inlet profile (nodes matched by their (x, y) control-point position)."""
+    P = np.asarray(case.control_points)
+    inlet = np.nonzero(np.abs(case.inlet_velocity).sum(1) > 0)[0]
+    lookup = {(round(float(P[i, 0]), 9), round(float(P[i, 1]), 9)): case.inlet_velocity[i] for i in inlet}
+    out = np.zeros((case.mesh.nodes, 3))
+    for n in range(case.mesh.nodes):
+        out[n] = lookup.get((round(float(P[n, 0]), 9), round(float(P[n, 1]), 9)), 0.0)
+    return out
+
+
+def write_transport_case(case: Case, directory, params: dict, steps: int, velocity=None, case_config=None):
+    """A case directory for `transport`: the flow case files plus the legacy
+    simulation_parameter.txt ("key value" lines, solvers/cpu/include/CaseInput.hpp:85-119),
+    optionally a velocity file transport_velocity.txt and a case_config.json."""
+    d = write_case(case, directory)
+    lines = [f"{TRANSPORT_KEYS[k]} {float(v)!r}" for k, v in params.items()]
+    lines.append(f"nstep {int(steps)}")
+    (d / "simulation_parameter.txt").write_text("\n".join(lines) + "\n")
+    if velocity is not None:
+        np.savetxt(d / "transport_velocity.txt", np.asarray(velocity).reshape(-1, 3), fmt="%.17g")
+    if case_config is not None:
+        (d / "case_config.json").write_text(json.dumps(case_config, indent=2) + "\n")
+    return d
+
+
 # Named workloads (sizes chosen so the DOF counts match the workload text).
 WORKLOADS = {
     # straight tube R=1 mm, L=10 mm, 5 hexes per section x 16 axial, Poiseuille 0.1 m/s

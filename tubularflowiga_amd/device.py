@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, WarmOptions, TFI_FP64, TFI_MIXED,
+from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, TransportParams, WarmOptions, TFI_FP64, TFI_MIXED,
                    TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3,
                    TFI_WARM_EXTRAPOLATE4,
                    TFI_WARM_LEARNED, TFI_WARM_MPE,
@@ -329,4 +329,56 @@ class FlowSolver:
     def close(self):
         if self.handle:
             self.lib.tfi_flow_destroy(self.handle)
+            self.handle = None
+
+
+class Transport:
+    """Two-species neuron transport on the mesh pattern (2x2 node blocks
+    [N0, Nplus]); see tfi_transport_* in include/tfi.h."""
+
+    def __init__(self, dmesh: DeviceMesh, params: dict, velocity, constrained):
+        self.dmesh = dmesh
+        self.ctx = dmesh.ctx
+        self.lib = dmesh.lib
+        self.nodes = dmesh.nodes
+        p = TransportParams(*[float(params.get(k, 0.0)) for k, _ in TransportParams._fields_])
+        h = ctypes.c_void_p()
+        self.ctx.check(self.lib.tfi_transport_create(
+            dmesh.handle, ctypes.byref(p), np.ascontiguousarray(velocity, np.float64).ravel(),
+            np.ascontiguousarray(constrained, np.int32), ctypes.byref(h)))
+        self.handle = h
+
+    def set_boundary(self, n0, nplus):
+        self.ctx.check(self.lib.tfi_transport_set_boundary(self.handle, np.ascontiguousarray(n0, np.float64),
+                                                           np.ascontiguousarray(nplus, np.float64)))
+
+    def apply_boundary(self):
+        self.ctx.check(self.lib.tfi_transport_apply_boundary(self.handle))
+
+    def set_state(self, state):
+        self.ctx.check(self.lib.tfi_transport_set_state(self.handle, np.ascontiguousarray(state, np.float64).ravel()))
+
+    def state(self):
+        out = np.zeros(2 * self.nodes)
+        self.ctx.check(self.lib.tfi_transport_get_state(self.handle, out))
+        return out.reshape(-1, 2)
+
+    def step(self, zero_guess=False, maximum_iterations=10000, rtol=1e-8):
+        it, res = ctypes.c_int(), ctypes.c_double()
+        self.ctx.check(self.lib.tfi_transport_step(self.handle, int(zero_guess), maximum_iterations, rtol,
+                                                   ctypes.byref(it), ctypes.byref(res)))
+        return it.value, res.value
+
+    def matrices(self):
+        nnzb = self.dmesh.nnzb
+        left, prev = np.zeros(4 * nnzb), np.zeros(4 * nnzb)
+        s = ctypes.c_int()
+        self.ctx.check(self.lib.tfi_transport_matrices(self.handle, left.ctypes.data_as(ctypes.c_void_p),
+                                                       prev.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s)))
+        return left.reshape(-1, 2, 2), prev.reshape(-1, 2, 2), s.value
+
+    def close(self):
+        """Release the device buffers (before the context is closed)."""
+        if self.handle:
+            self.lib.tfi_transport_destroy(self.handle)
             self.handle = None

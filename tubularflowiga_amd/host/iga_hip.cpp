@@ -6,6 +6,7 @@
 // precision, warm start, tangent) and default to the reproducing settings.
 #include "case_input.hpp"
 #include "outlets.hpp"
+#include "transport_input.hpp"
 #include "tfi.h"
 
 #include <chrono>
@@ -552,14 +553,179 @@ int CaseCheck(int argc, char** argv)
 	return 0;
 }
 
+struct TransportOptions {
+	fs::path database, case_dir, output, velocity;
+	int steps = -1;
+	double linear_rtol = 1e-8;
+};
+
+// iga_cuda transport DATABASE CASE_DIR [STEPS] [OUTPUT] [VELOCITY] (iga_cuda.cu:623-625),
+// plus --linear-rtol R (the FGMRES relative tolerance, 1e-8 as in iga_cuda.cu:697)
+TransportOptions ParseTransportOptions(int argc, char** argv)
+{
+	const char* usage = "usage: iga_hip transport DATABASE.ntiga CASE_DIR [STEPS] [OUTPUT] [VELOCITY] [--linear-rtol R]";
+	std::vector<std::string> pos;
+	TransportOptions o;
+	for (int i = 2; i < argc; ++i) {
+		const std::string arg(argv[i]);
+		if (arg == "--linear-rtol") {
+			if (i + 1 >= argc) throw std::runtime_error(usage);
+			char* end = nullptr;
+			o.linear_rtol = std::strtod(argv[++i], &end);
+			if (!end || *end || !(o.linear_rtol > 0.0)) throw std::runtime_error("--linear-rtol requires a positive number");
+		} else if (arg.rfind("--", 0) == 0) {
+			throw std::runtime_error("unknown option: " + arg);
+		} else {
+			pos.push_back(arg);
+		}
+	}
+	if (pos.size() < 2 || pos.size() > 5) throw std::runtime_error(usage);
+	o.database = pos[0];
+	o.case_dir = pos[1];
+	if (pos.size() >= 3) {
+		char* end = nullptr;
+		const long v = std::strtol(pos[2].c_str(), &end, 10);
+		if (!end || *end || v < 0) throw std::runtime_error("STEPS must be a non-negative integer");
+		o.steps = static_cast<int>(v);
+	}
+	if (pos.size() >= 4) o.output = pos[3];
+	o.velocity = pos.size() >= 5 ? fs::path(pos[4]) : o.case_dir/"initial_velocityfield.txt";
+	return o;
+}
+
+// Inputs of the transport command, read (and validated) before the device is touched.
+struct TransportCase {
+	TransportOptions o;
+	Clock::time_point start;
+	FlatDatabase db;
+	TransportParameters p;
+	std::vector<std::array<double, 3>> velocity;
+	CaseRules rules;
+	TransportBoundaries bc;
+};
+
+TransportCase LoadTransportCase(int argc, char** argv)
+{
+	TransportCase t;
+	t.o = ParseTransportOptions(argc, argv);
+	t.start = Clock::now();
+	t.db = ReadDatabase(t.o.database.string());
+	t.p = ReadTransportParameters((t.o.case_dir/"simulation_parameter.txt").string());
+	if (t.o.steps >= 0) t.p.steps = t.o.steps;
+	const auto labeled = ReadLabeledHexMesh((t.o.case_dir/"controlmesh.vtk").string(), t.db.nodes,
+		static_cast<std::uint64_t>(t.db.elements()));
+	t.velocity = ReadVelocity(t.o.velocity.string(), t.db.nodes);
+	t.rules = ReadCaseRules((t.o.case_dir/"case_config.json").string());
+	t.bc = ResolveTransportBoundaries(t.rules, labeled.labels, t.p);
+	return t;
+}
+
+// transport-check DATABASE CASE_DIR [STEPS] [OUTPUT] [VELOCITY]: the transport
+// inputs resolved without a GPU (no reference counterpart).
+int TransportCheck(int argc, char** argv)
+{
+	std::vector<char*> args(argv, argv + argc);
+	const TransportCase t = LoadTransportCase(argc, args.data());
+	long long index_sum = 0;
+	double n0 = 0.0, nplus = 0.0;
+	for (std::size_t i = 0; i < t.bc.constrained.size(); ++i) {
+		if (t.bc.constrained[i]) index_sum += static_cast<long long>(i);
+		n0 += t.bc.n0[i];
+		nplus += t.bc.nplus[i];
+	}
+	const TransportParameters& p = t.p;
+	std::cout << std::setprecision(17) << "transport_check nodes=" << t.db.nodes << " elements=" << t.db.elements()
+		<< " boundary_config=" << (t.rules.present ? "case_config.json" : "legacy-defaults")
+		<< " transport_nodes=" << t.bc.transport_nodes << " velocity_nodes=" << t.bc.velocity_nodes
+		<< " pressure_nodes=" << t.bc.pressure_nodes << " constrained_index_sum=" << index_sum << " n0_sum=" << n0
+		<< " nplus_sum=" << nplus << " D=" << p.diffusion << " vplus=" << p.vplus << " vminus=" << p.vminus
+		<< " kplus=" << p.kplus << " kminus=" << p.kminus << " detach_plus=" << p.detach_plus
+		<< " detach_minus=" << p.detach_minus << " dt=" << p.dt << " nstep=" << p.steps << " N0bc=" << p.n0_bc
+		<< " Nplusbc=" << p.nplus_bc << " Nminusbc=" << p.nminus_bc << " artificial_diffusion=" << p.artificial_diffusion
+		<< '\n';
+	return 0;
+}
+
+int Transport(Ctx& c, const TransportCase& t)
+{
+	const TransportOptions& o = t.o;
+	const auto total_start = t.start;
+	const FlatDatabase& db = t.db;
+	const TransportParameters& p = t.p;
+	const auto& velocity = t.velocity;
+	const TransportBoundaries& bc = t.bc;
+	std::cout << "boundary_config=" << (t.rules.present ? "case_config.json" : "legacy-defaults")
+		<< " transport_nodes=" << bc.transport_nodes << '\n';
+	Mesh m;
+	CreateMesh(c, db, m);
+	tfi_mesh_stats st{};
+	c.Check(tfi_mesh_stats_get(m.mesh, &st), "mesh statistics");
+	PrintMesh(db, &st);
+	if (st.bad_samples)
+		throw std::runtime_error("mesh has " + std::to_string(st.bad_samples)
+			+ " non-positive quadrature Jacobians; minimum detJ " + std::to_string(st.minimum_determinant));
+	const auto preprocess_end = Clock::now();
+	std::vector<double> vel(3*db.nodes);
+	for (std::size_t n = 0; n < db.nodes; ++n)
+		for (int d = 0; d < 3; ++d) vel[3*n + d] = velocity[n][d];
+	const tfi_transport_params tp{p.diffusion, p.vplus, p.vminus, p.kplus, p.kminus, p.detach_plus, p.detach_minus,
+		p.dt, p.artificial_diffusion};
+	struct Handle {
+		tfi_transport* t = nullptr;
+		~Handle() { tfi_transport_destroy(t); }
+	} tr;
+	const auto assembly_start = Clock::now();
+	c.Check(tfi_transport_create(m.mesh, &tp, vel.data(), bc.constrained.data(), &tr.t), "transport assembly");
+	c.Check(tfi_transport_set_boundary(tr.t, bc.n0.data(), bc.nplus.data()), "transport boundary values");
+	c.Check(tfi_transport_apply_boundary(tr.t), "transport boundary state");
+	const auto assembly_end = Clock::now();
+	long long total_iterations = 0;
+	int singular = 0;
+	c.Check(tfi_transport_matrices(tr.t, nullptr, nullptr, &singular), "transport block inverses");
+	const auto solve_start = Clock::now();
+	for (int step = 0; step < p.steps; ++step) {
+		int it = 0;
+		double res = 0.0;
+		if (tfi_transport_step(tr.t, step == 0 ? 1 : 0, 10000, o.linear_rtol, &it, &res) != TFI_OK)
+			throw std::runtime_error(std::string(tfi_last_error(c.ctx)) + " at step " + std::to_string(step));
+		total_iterations += it;
+		if ((step + 1) % 25 == 0 || step + 1 == p.steps)
+			std::cout << "transport step=" << step + 1 << '/' << p.steps << " iterations=" << it << " residual=" << res
+				<< '\n';
+	}
+	const auto solve_end = Clock::now();
+	std::vector<double> state(2*db.nodes);
+	c.Check(tfi_transport_get_state(tr.t, state.data()), "transport state download");
+	double norm = 0.0;
+	for (double v : state) norm += v*v;
+	norm = std::sqrt(norm);
+	if (!o.output.empty()) {
+		WriteTransportText(o.output, state);
+		WriteTransportVtk(o.case_dir/"controlmesh.vtk", o.output.string() + ".vtk", state);
+	}
+	long long peak = 0, live = 0;
+	c.Check(tfi_allocation_stats(c.ctx, &peak, &live), "allocation statistics");
+	std::cout << "transport_cuda nodes=" << db.nodes << " elements=" << db.elements() << " steps=" << p.steps
+		<< " preprocess_s=" << Seconds(total_start, preprocess_end) << " assembly_s=" << Seconds(assembly_start, assembly_end)
+		<< " solve_s=" << Seconds(solve_start, solve_end) << " total_iterations=" << total_iterations
+		<< " final_l2=" << norm << " singular_diagonal_blocks=" << singular << " gpu_used_gib=" << Gib(peak)
+		<< " backend=hip\n";
+	return 0;
+}
+
 int main(int argc, char** argv)
 {
 	std::unique_ptr<Ctx> ctx;
 	try {
-		if (argc < 2) throw std::runtime_error("usage: iga_hip device-info|mesh-check|navier-stokes|case-check ...");
+		if (argc < 2)
+			throw std::runtime_error("usage: iga_hip device-info|mesh-check|transport|navier-stokes|case-check ...");
 		const std::string command(argv[1]);
 		if (command == "case-check") return CaseCheck(argc, argv);
-		if (command == "navier-stokes") ParseOptions(argc, argv);  // usage errors before the device is touched
+		if (command == "transport-check") return TransportCheck(argc, argv);
+		// usage errors before the device is touched
+		if (command == "navier-stokes") ParseOptions(argc, argv);
+		std::unique_ptr<TransportCase> transport_case;
+		if (command == "transport") transport_case = std::make_unique<TransportCase>(LoadTransportCase(argc, argv));
 		ctx = std::make_unique<Ctx>(0);
 		PrintDevice(*ctx);
 		if (command == "device-info") return 0;
@@ -569,6 +735,7 @@ int main(int argc, char** argv)
 		}
 		int status = 0;
 		if (command == "navier-stokes") status = NavierStokes(*ctx, argc, argv);
+		else if (command == "transport") status = Transport(*ctx, *transport_case);
 		else throw std::runtime_error("unknown command: " + command);
 		PrintAllocations(std::cout, ctx->ctx);
 		return status;
