@@ -153,6 +153,43 @@ def pulsatile_comparison(dm, case, bc, faces, rho, mu, dt, tol, train_steps, war
             "rel_l2_mixed_vs_fp64": rel_l2(fast_state, plain_state), "warm_start": warm_stats}
 
 
+def transport_measurement(dm, case, state, steps=20, warmup=2):
+    """Two-species neuron transport on the same mesh, advected by the
+    converged flow (`iga_hip transport`): assembly of L and P once, then
+    `steps` timed backward-Euler steps, and the HBM roofline of the 2x2 SpMV
+    (FP64 values and column indices once, x and y once per launch)."""
+    from tubularflowiga_amd import device
+    ctx = dm.ctx
+    velocity = np.asarray(state).reshape(-1, 4)[:, :3]
+    constrained = (np.asarray(case.point_labels) == 1).astype(np.int32)
+    params = dict(diffusion=1e-3, vplus=1.0, kplus=0.5, kminus=0.1, detach_plus=0.3, dt=1e-3, artificial_diffusion=0.1)
+    ctx.timers(enable=True, reset=True)
+    tr = device.Transport(dm, params, velocity, constrained)
+    try:
+        assembly_ms = ctx.timers()["transport_assembly_ms"]
+        tr.set_boundary(np.where(constrained == 1, 1.0, 0.0), np.where(constrained == 1, 0.5, 0.0))
+        tr.apply_boundary()
+        for k in range(warmup):
+            tr.step(zero_guess=(k == 0))
+        ctx.timers(reset=True)
+        t0 = time.perf_counter()
+        its = [tr.step()[0] for _ in range(steps)]
+        wall = time.perf_counter() - t0
+        tm = ctx.timers(enable=False)
+    finally:
+        tr.close()
+    gbs = tm["spmv2_bytes"] / (tm["spmv2_ms"] * 1e-3) / 1e9 if tm["spmv2_ms"] > 0 else 0.0
+    return {"workload": "two-species neuron transport on the same mesh, advected by the converged steady flow, "
+                        "dt=0.001, FGMRES(50) + 2x2 block Jacobi to 1e-8",
+            "steps": steps, "ms_per_step": round(1e3 * wall / steps, 3),
+            "krylov_per_step": float(np.mean(its)), "assembly_ms": round(assembly_ms, 3),
+            "spmv_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                              "kernel": "Bsr2SpmvKernel (2x2 FP64 values)",
+                              "avg_ms": round(tm["spmv2_ms"] / max(tm["spmv2_calls"], 1), 4),
+                              "algorithmic_bytes": int(tm["spmv2_bytes"] / max(tm["spmv2_calls"], 1))}}
+
+
 def cpu_baseline(case, dm, rho, mu, dt, plain_timed, budget_s=20.0):
     """CPU oracle (FP64 C++ restatement, OpenMP) on a bounded sample: element
     assembly of a slice of the mesh and FP64 SpMVs on the full pattern, scaled
@@ -342,6 +379,7 @@ def main(argv=None):
                           "kernel": "BsrSpmvKernel<double> (FGMRES operator, FP64 values)",
                           "avg_ms": round(spmv_ms, 4), "algorithmic_bytes": int(spmv_bytes)},
     }
+    out["transport"] = transport_measurement(dm, case, plain_state)
     if a.inflow == "steady" and not a.no_pulsatile:
         out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, a.dt, dict(tol, rtol=3e-11),
                                                 a.train_steps)

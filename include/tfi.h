@@ -284,6 +284,8 @@ typedef struct {
 	double assembly_ms, assembly_calls, assembly_flops;  /* residual + tangent: AssembleNsKernel launches */
 	double assembly_launches;                         /* (one launch per element colour) */
 	double residual_ms, residual_calls, residual_flops;  /* residual-only assemblies */
+	double spmv2_ms, spmv2_calls, spmv2_bytes;        /* transport 2x2 fp64 SpMV */
+	double transport_assembly_ms;                     /* transport L and P assembly (all colours) */
 } tfi_kernel_timers;
 int tfi_timers_get(tfi_ctx* ctx, tfi_kernel_timers* out);
 int tfi_timers_enable(tfi_ctx* ctx, int enabled);

@@ -72,7 +72,8 @@ class KernelTimers(ctypes.Structure):
                 ("spmv32_ms", c_double), ("spmv32_calls", c_double), ("spmv32_bytes", c_double),
                 ("assembly_ms", c_double), ("assembly_calls", c_double), ("assembly_flops", c_double),
                 ("assembly_launches", c_double), ("residual_ms", c_double), ("residual_calls", c_double),
-                ("residual_flops", c_double)]
+                ("residual_flops", c_double), ("spmv2_ms", c_double), ("spmv2_calls", c_double),
+                ("spmv2_bytes", c_double), ("transport_assembly_ms", c_double)]
 
 
 def _np(dtype):

@@ -163,29 +163,32 @@ __global__ void TransportValuesKernel(int nodes, const int* __restrict__ constra
 	}
 }
 
-// y = A x with 2x2 blocks: one wave per block row, lanes over the row's blocks
+// y = A x with 2x2 blocks: kRowLanes lanes per block row (one row per
+// wave; a row of the quadratic meshes has about 100 blocks), each lane over
+// every kRowLanes-th block of its row, reduced within the lane group
+constexpr int kRowLanes = 64;
+
 __global__ void __launch_bounds__(kThreads) Bsr2SpmvKernel(int rows, const int* __restrict__ ro,
 	const int* __restrict__ cols, const double* __restrict__ vals, const double* __restrict__ x, double* __restrict__ y)
 {
-	const int lane = threadIdx.x & 63;
-	const int row = blockIdx.x*(kThreads/64) + (threadIdx.x >> 6);
-	if (row >= rows) return;
+	const int sub = threadIdx.x & (kRowLanes - 1);
+	const int row = (blockIdx.x*kThreads + threadIdx.x)/kRowLanes;
 	double y0 = 0.0, y1 = 0.0;
-	for (int k = ro[row] + lane; k < ro[row + 1]; k += 64) {
-		const double* v = vals + 4*(size_t)k;
-		const size_t c = cols[k];
-		const double x0 = x[2*c], x1 = x[2*c + 1];
-		y0 += v[0]*x0 + v[1]*x1;
-		y1 += v[2]*x0 + v[3]*x1;
+	if (row < rows) {
+		for (int k = ro[row] + sub; k < ro[row + 1]; k += kRowLanes) {
+			const double2* v = reinterpret_cast<const double2*>(vals + 4*(size_t)k);
+			const double2 top = v[0], bottom = v[1];
+			const double2 xc = reinterpret_cast<const double2*>(x)[cols[k]];
+			y0 += top.x*xc.x + top.y*xc.y;
+			y1 += bottom.x*xc.x + bottom.y*xc.y;
+		}
 	}
-	for (int o = 32; o > 0; o >>= 1) {
+	for (int o = kRowLanes/2; o > 0; o >>= 1) {
 		y0 += __shfl_xor(y0, o, 64);
 		y1 += __shfl_xor(y1, o, 64);
 	}
-	if (lane == 0) {
-		y[2*(size_t)row] = y0;
-		y[2*(size_t)row + 1] = y1;
-	}
+	if (row < rows && sub == 0)
+		reinterpret_cast<double2*>(y)[row] = make_double2(y0, y1);
 }
 
 // 2x2 diagonal-block inverses; a block with |det| <= 1e-14 |a d| (or a
@@ -237,12 +240,24 @@ int Guard(tfi_ctx* ctx, F&& fn)
 	}
 }
 
+// algorithmic bytes of one 2x2 SpMV: values and column indices once, row
+// offsets, x and y once each
+double Spmv2Bytes(const BsrView& a)
+{
+	return static_cast<double>(a.nnzb)*(4*8 + 4) + (a.rows + 1)*4.0 + 2.0*a.rows*2*8;
+}
+
 void Spmv2(tfi_transport* t, const double* vals, const double* x, double* y)
 {
+	tfi_ctx* ctx = t->ctx;
 	const BsrView a = t->mesh->view();
-	hipLaunchKernelGGL(Bsr2SpmvKernel, dim3((a.rows + 3)/4), dim3(kThreads), 0, t->ctx->stream, a.rows,
+	StreamTimer timer(ctx, &ctx->timers.spmv2_ms);
+	hipLaunchKernelGGL(Bsr2SpmvKernel, dim3((a.rows + kThreads/kRowLanes - 1)/(kThreads/kRowLanes)), dim3(kThreads), 0,
+		ctx->stream, a.rows,
 		a.row_offsets, a.columns, vals, x, y);
 	TFI_LAUNCH_CHECK("Bsr2SpmvKernel");
+	ctx->timers.spmv2_calls += 1;
+	ctx->timers.spmv2_bytes += Spmv2Bytes(a);
 }
 
 }  // namespace
@@ -298,6 +313,8 @@ int tfi_transport_create(tfi_mesh* mesh, const tfi_transport_params* params, con
 		if (lds > 65536)
 			Check(hipFuncSetAttribute(reinterpret_cast<const void*>(&TransportAssembleKernel<true>),
 				hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), "hipFuncSetAttribute");
+		{
+		StreamTimer timer(ctx, &ctx->timers.transport_assembly_ms);
 		for (int c = 0; c < mesh->colors; ++c) {
 			a.elements = mesh->color_elements.data() + mesh->color_offsets[c];
 			a.count = mesh->color_offsets[c + 1] - mesh->color_offsets[c];
@@ -307,6 +324,7 @@ int tfi_transport_create(tfi_mesh* mesh, const tfi_transport_params* params, con
 			else
 				hipLaunchKernelGGL(TransportAssembleKernel<false>, dim3(a.count), dim3(kThreads), lds, ctx->stream, a);
 			TFI_LAUNCH_CHECK("TransportAssembleKernel");
+		}
 		}
 		const BsrView v = mesh->view();
 		hipLaunchKernelGGL(TransportBoundaryKernel, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N, v.row_offsets,
