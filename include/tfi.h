@@ -276,6 +276,10 @@ int tfi_transport_step(tfi_transport* transport, int zero_guess, int maximum_ite
 /* Host copies of the assembled 2x2 blocks of L and P [nnzb*4] and the count of
  * diagonal blocks inverted by their diagonal (either pointer may be NULL). */
 int tfi_transport_matrices(tfi_transport* transport, double* left, double* previous, int* singular_blocks);
+/* Device-resident timing of the transport's 2x2 SpMV (y = L x on the current
+ * state) with HIP events around `repetitions` back-to-back launches: average
+ * milliseconds per launch.  No reference counterpart. */
+int tfi_transport_bench_spmv(tfi_transport* transport, int repetitions, double* milliseconds);
 
 /* Kernel timers accumulated on the context stream (HIP events). */
 typedef struct {

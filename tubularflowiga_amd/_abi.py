@@ -131,6 +131,7 @@ SIGNATURES = {
     "tfi_transport_apply_boundary": (c_int, [c_void_p]),
     "tfi_transport_step": (c_int, [c_void_p, c_int, c_int, c_double, P(c_int), P(c_double)]),
     "tfi_transport_matrices": (c_int, [c_void_p, c_void_p, c_void_p, P(c_int)]),
+    "tfi_transport_bench_spmv": (c_int, [c_void_p, c_int, P(c_double)]),
     "tfi_timers_get": (c_int, [c_void_p, P(KernelTimers)]),
     "tfi_timers_enable": (c_int, [c_void_p, c_int]),
     "tfi_timers_reset": (c_int, [c_void_p]),

@@ -422,6 +422,29 @@ int tfi_transport_step(tfi_transport* t, int zero_guess, int maximum_iterations,
 	});
 }
 
+int tfi_transport_bench_spmv(tfi_transport* t, int repetitions, double* milliseconds)
+{
+	return Guard(t->ctx, [&] {
+		tfi_ctx* ctx = t->ctx;
+		const BsrView a = t->mesh->view();
+		auto run = [&] {
+			hipLaunchKernelGGL(Bsr2SpmvKernel, dim3((a.rows + kThreads/kRowLanes - 1)/(kThreads/kRowLanes)),
+				dim3(kThreads), 0, ctx->stream, a.rows, a.row_offsets, a.columns, t->left.data(), t->state.data(),
+				t->work.data());
+			TFI_LAUNCH_CHECK("Bsr2SpmvKernel");
+		};
+		for (int i = 0; i < 3; ++i) run();
+		Sync(ctx);
+		Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event");
+		for (int i = 0; i < repetitions; ++i) run();
+		Check(hipEventRecord(ctx->ev_end, ctx->stream), "event");
+		Check(hipEventSynchronize(ctx->ev_end), "event sync");
+		float ms = 0.0f;
+		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
+		*milliseconds = ms/std::max(1, repetitions);
+	});
+}
+
 int tfi_transport_matrices(tfi_transport* t, double* left, double* previous, int* singular_blocks)
 {
 	return Guard(t->ctx, [&] {

@@ -369,6 +369,12 @@ class Transport:
                                                    ctypes.byref(it), ctypes.byref(res)))
         return it.value, res.value
 
+    def bench_spmv(self, repetitions=50):
+        """Average ms per launch of the 2x2 SpMV, timed back to back."""
+        ms = ctypes.c_double()
+        self.ctx.check(self.lib.tfi_transport_bench_spmv(self.handle, repetitions, ctypes.byref(ms)))
+        return ms.value
+
     def matrices(self):
         nnzb = self.dmesh.nnzb
         left, prev = np.zeros(4 * nnzb), np.zeros(4 * nnzb)
