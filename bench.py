@@ -185,7 +185,7 @@ def transport_measurement(dm, case, state, steps=20, warmup=2):
     spmv_bytes = tm["spmv2_bytes"] / max(tm["spmv2_calls"], 1)
     gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     return {"workload": "two-species neuron transport on the same mesh, advected by the converged steady flow, "
-                        "dt=0.001, FGMRES(50) + 2x2 block Jacobi to 1e-8",
+                        "dt=0.001, FGMRES(50) on the block-Jacobi-scaled system + AMG to 1e-8",
             "steps": steps, "ms_per_step": round(1e3 * wall / steps, 3),
             "krylov_per_step": float(np.mean(its)), "assembly_ms": round(assembly_ms, 3),
             "spmv_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

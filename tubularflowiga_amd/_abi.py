@@ -126,6 +126,7 @@ SIGNATURES = {
     "tfi_transport_create": (c_int, [c_void_p, P(TransportParams), _d, _i32, P(c_void_p)]),
     "tfi_transport_destroy": (None, [c_void_p]),
     "tfi_transport_set_boundary": (c_int, [c_void_p, _d, _d]),
+    "tfi_transport_set_preconditioner": (c_int, [c_void_p, c_int]),
     "tfi_transport_set_state": (c_int, [c_void_p, _d]),
     "tfi_transport_get_state": (c_int, [c_void_p, _d]),
     "tfi_transport_apply_boundary": (c_int, [c_void_p]),

@@ -20,6 +20,7 @@
 // constrained values in the right-hand side (iga_cuda.cu:685-705) by FGMRES
 // with a 2x2 block-Jacobi preconditioner.
 #include "tfi_internal.hpp"
+#include "tfi_solver.hpp"
 
 #include <cmath>
 
@@ -30,6 +31,12 @@ struct tfi_transport {
 	tfi::DevBuf<double> left, previous, inverse, state, rhs, next, scaled, work, n0, nplus;
 	tfi::DevBuf<int> constrained;
 	std::unique_ptr<tfi::Fgmres> krylov;
+	// D^-1 L (the solved operator) and, for the AMG preconditioner, its 2x2
+	// blocks embedded in 4x4 blocks with an identity for the two padding fields
+	tfi::DevBuf<double> scaled_left, padded64, pad_in, pad_out;
+	tfi::DevBuf<float> padded32;
+	std::unique_ptr<tfi::Amg> amg;
+	int preconditioner = TFI_PC_AMG;
 	int restart = 50;
 	int singular = 0;
 };
@@ -215,6 +222,62 @@ __global__ void Block2InverseKernel(int rows, const int* __restrict__ diag, cons
 	}
 }
 
+// S = D^-1 L row by row (D_i^-1 times every block of row i)
+__global__ void ScaleRowsKernel(int rows, const int* __restrict__ ro, const double* __restrict__ inv,
+	const double* __restrict__ left, double* __restrict__ scaled)
+{
+	const int row = blockIdx.x*blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	const double* o = inv + 4*(size_t)row;
+	for (int k = ro[row]; k < ro[row + 1]; ++k) {
+		const double* a = left + 4*(size_t)k;
+		double* s = scaled + 4*(size_t)k;
+		s[0] = o[0]*a[0] + o[1]*a[2];
+		s[1] = o[0]*a[1] + o[1]*a[3];
+		s[2] = o[2]*a[0] + o[3]*a[2];
+		s[3] = o[2]*a[1] + o[3]*a[3];
+	}
+}
+
+// 4x4 embedding of the 2x2 blocks: [[S, 0], [0, I if diagonal block else 0]]
+__global__ void PadBlocksKernel(int rows, const int* __restrict__ ro, const int* __restrict__ diag,
+	const double* __restrict__ s, double* __restrict__ out)
+{
+	const int row = blockIdx.x*blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	for (int k = ro[row]; k < ro[row + 1]; ++k) {
+		const double* a = s + 4*(size_t)k;
+		double* o = out + 16*(size_t)k;
+		const double one = k == diag[row] ? 1.0 : 0.0;
+		for (int i = 0; i < 16; ++i) o[i] = 0.0;
+		o[0] = a[0];
+		o[1] = a[1];
+		o[4] = a[2];
+		o[5] = a[3];
+		o[10] = one;
+		o[15] = one;
+	}
+}
+
+// [N0, Nplus] per node <-> [N0, Nplus, 0, 0] per node
+__global__ void PadVectorKernel(int rows, const double* __restrict__ v, double* __restrict__ p)
+{
+	const int row = blockIdx.x*blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	p[4*(size_t)row] = v[2*(size_t)row];
+	p[4*(size_t)row + 1] = v[2*(size_t)row + 1];
+	p[4*(size_t)row + 2] = 0.0;
+	p[4*(size_t)row + 3] = 0.0;
+}
+
+__global__ void UnpadVectorKernel(int rows, const double* __restrict__ p, double* __restrict__ v)
+{
+	const int row = blockIdx.x*blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	v[2*(size_t)row] = p[4*(size_t)row];
+	v[2*(size_t)row + 1] = p[4*(size_t)row + 1];
+}
+
 __global__ void Block2ApplyKernel(int rows, const double* __restrict__ inv, const double* __restrict__ v,
 	double* __restrict__ z)
 {
@@ -337,6 +400,10 @@ int tfi_transport_create(tfi_mesh* mesh, const tfi_transport_params* params, con
 			t->left.data(), t->inverse.data(), singular.data());
 		TFI_LAUNCH_CHECK("Block2InverseKernel");
 		singular.Download(&t->singular, 1);
+		t->scaled_left.Allocate(ctx, (size_t)mesh->nnzb*4);
+		hipLaunchKernelGGL(ScaleRowsKernel, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N, v.row_offsets,
+			t->inverse.data(), t->left.data(), t->scaled_left.data());
+		TFI_LAUNCH_CHECK("ScaleRowsKernel");
 		t->krylov = std::make_unique<Fgmres>(ctx, 2LL*N, t->restart);
 		Sync(ctx);
 		*out = t.release();
@@ -347,6 +414,15 @@ void tfi_transport_destroy(tfi_transport* t)
 {
 	if (t && t->ctx) (void)hipStreamSynchronize(t->ctx->stream);
 	delete t;
+}
+
+int tfi_transport_set_preconditioner(tfi_transport* t, int preconditioner)
+{
+	return Guard(t->ctx, [&] {
+		if (preconditioner != TFI_PC_AMG && preconditioner != TFI_PC_BJACOBI)
+			throw Error("transport preconditioner must be TFI_PC_AMG or TFI_PC_BJACOBI");
+		t->preconditioner = preconditioner;
+	});
 }
 
 int tfi_transport_set_boundary(tfi_transport* t, const double* n0, const double* nplus)
@@ -405,11 +481,37 @@ int tfi_transport_step(tfi_transport* t, int zero_guess, int maximum_iterations,
 			TFI_LAUNCH_CHECK("Block2ApplyKernel");
 		};
 		block_apply(t->rhs.data(), t->scaled.data());
-		Operator A = [&](const double* x, double* y) {
-			Spmv2(t, t->left.data(), x, t->work.data());
-			block_apply(t->work.data(), y);
-		};
+		Operator A = [&](const double* x, double* y) { Spmv2(t, t->scaled_left.data(), x, y); };
 		Precond M = [&](const double* v, double* z) { Copy(ctx, v, z, n); };
+		if (t->preconditioner == TFI_PC_AMG) {
+			if (!t->amg) {
+				// the operator is fixed for the run: one AMG set-up (FP32 cycles)
+				const BsrView a = t->mesh->view();
+				t->padded64.Allocate(ctx, (size_t)a.nnzb*16);
+				t->padded32.Allocate(ctx, (size_t)a.nnzb*16);
+				t->pad_in.Allocate(ctx, 4LL*N);
+				t->pad_out.Allocate(ctx, 4LL*N);
+				hipLaunchKernelGGL(PadBlocksKernel, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N, a.row_offsets,
+					a.diagonal, t->scaled_left.data(), t->padded64.data());
+				TFI_LAUNCH_CHECK("PadBlocksKernel");
+				ToFloat(ctx, t->padded64.data(), t->padded32.data(), (long long)a.nnzb*16);
+				// symmetric Gauss-Seidel (as the flow solver's); measured on the
+				// 0.6M-DOF Y: 22 iterations per step against 145 with block Jacobi
+				// alone, while damped-Jacobi smoothing (51) and one sweep pre and
+				// post (stagnation) do worse
+				t->amg = std::make_unique<Amg>(t->mesh, TFI_MIXED, AmgOptions());
+				t->amg->Setup(t->padded64.data(), t->padded32.data());
+			}
+			M = [&](const double* v, double* z) {
+				hipLaunchKernelGGL(PadVectorKernel, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N, v,
+					t->pad_in.data());
+				TFI_LAUNCH_CHECK("PadVectorKernel");
+				t->amg->Apply(t->pad_in.data(), t->pad_out.data());
+				hipLaunchKernelGGL(UnpadVectorKernel, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
+					t->pad_out.data(), z);
+				TFI_LAUNCH_CHECK("UnpadVectorKernel");
+			};
+		}
 		const KrylovResult r = t->krylov->Solve(A, M, t->scaled.data(), t->next.data(), maximum_iterations,
 			relative_tolerance);
 		if (iterations) *iterations = r.iterations;

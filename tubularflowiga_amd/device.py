@@ -348,6 +348,10 @@ class Transport:
             np.ascontiguousarray(constrained, np.int32), ctypes.byref(h)))
         self.handle = h
 
+    def set_preconditioner(self, preconditioner):
+        """TFI_PC_AMG (default) or TFI_PC_BJACOBI."""
+        self.ctx.check(self.lib.tfi_transport_set_preconditioner(self.handle, preconditioner))
+
     def set_boundary(self, n0, nplus):
         self.ctx.check(self.lib.tfi_transport_set_boundary(self.handle, np.ascontiguousarray(n0, np.float64),
                                                            np.ascontiguousarray(nplus, np.float64)))

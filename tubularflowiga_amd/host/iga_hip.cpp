@@ -557,13 +557,15 @@ struct TransportOptions {
 	fs::path database, case_dir, output, velocity;
 	int steps = -1;
 	double linear_rtol = 1e-8;
+	int preconditioner = TFI_PC_AMG;
 };
 
 // iga_cuda transport DATABASE CASE_DIR [STEPS] [OUTPUT] [VELOCITY] (iga_cuda.cu:623-625),
 // plus --linear-rtol R (the FGMRES relative tolerance, 1e-8 as in iga_cuda.cu:697)
 TransportOptions ParseTransportOptions(int argc, char** argv)
 {
-	const char* usage = "usage: iga_hip transport DATABASE.ntiga CASE_DIR [STEPS] [OUTPUT] [VELOCITY] [--linear-rtol R]";
+	const char* usage = "usage: iga_hip transport DATABASE.ntiga CASE_DIR [STEPS] [OUTPUT] [VELOCITY] [--linear-rtol R] "
+		"[--preconditioner amg|bjacobi]";
 	std::vector<std::string> pos;
 	TransportOptions o;
 	for (int i = 2; i < argc; ++i) {
@@ -573,6 +575,12 @@ TransportOptions ParseTransportOptions(int argc, char** argv)
 			char* end = nullptr;
 			o.linear_rtol = std::strtod(argv[++i], &end);
 			if (!end || *end || !(o.linear_rtol > 0.0)) throw std::runtime_error("--linear-rtol requires a positive number");
+		} else if (arg == "--preconditioner") {
+			if (i + 1 >= argc) throw std::runtime_error(usage);
+			const std::string v(argv[++i]);
+			if (v == "amg") o.preconditioner = TFI_PC_AMG;
+			else if (v == "bjacobi") o.preconditioner = TFI_PC_BJACOBI;
+			else throw std::runtime_error("--preconditioner must be amg|bjacobi");
 		} else if (arg.rfind("--", 0) == 0) {
 			throw std::runtime_error("unknown option: " + arg);
 		} else {
@@ -676,6 +684,7 @@ int Transport(Ctx& c, const TransportCase& t)
 	} tr;
 	const auto assembly_start = Clock::now();
 	c.Check(tfi_transport_create(m.mesh, &tp, vel.data(), bc.constrained.data(), &tr.t), "transport assembly");
+	c.Check(tfi_transport_set_preconditioner(tr.t, o.preconditioner), "transport preconditioner");
 	c.Check(tfi_transport_set_boundary(tr.t, bc.n0.data(), bc.nplus.data()), "transport boundary values");
 	c.Check(tfi_transport_apply_boundary(tr.t), "transport boundary state");
 	const auto assembly_end = Clock::now();
