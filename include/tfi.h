@@ -253,7 +253,9 @@ typedef struct {
 /* Assembles the backward-Euler matrices L and P for the prescribed nodal
  * velocity [3N] (AssembleTransportKernel, IgaCudaKernels.cuh:155-237) and
  * applies the constrained rows (ApplyTransportBoundaryKernel, :411-424):
- * constrained [N] = 1 for a node whose N0 and Nplus are prescribed. */
+ * constrained [N] = 1 for a node whose N0 and Nplus are prescribed.  The
+ * assembly reads the mesh's basis table and builds it when the mesh has none
+ * (tfi_mesh_basis_table). */
 int tfi_transport_create(tfi_mesh* mesh, const tfi_transport_params* params, const double* velocity,
 	const int32_t* constrained, tfi_transport** out);
 void tfi_transport_destroy(tfi_transport* transport);
