@@ -153,14 +153,19 @@ def pulsatile_comparison(dm, case, bc, faces, rho, mu, dt, tol, train_steps, war
             "rel_l2_mixed_vs_fp64": rel_l2(fast_state, plain_state), "warm_start": warm_stats}
 
 
-def transport_measurement(dm, case, state, steps=20, warmup=2):
+def transport_measurement(dm, case, state, steps=20, warmup=2, cpu=False):
     """Two-species neuron transport on the same mesh, advected by the
     converged flow (`iga_hip transport`): assembly of L and P once, then
     `steps` timed backward-Euler steps, and the HBM roofline of the 2x2 SpMV
     (FP64 values and column indices once, x and y once per launch), timed over
     50 back-to-back launches (an event pair around each short launch of the
     solve adds a few microseconds; that per-launch figure is reported beside
-    it)."""
+    it).  The steps use the reference's preconditioner (the block-Jacobi scaling
+    alone); three more with the AMG give its iteration count and time.  With
+    `cpu`, the CPU
+    oracle is timed on a sample (element matrices of 2000 elements, scipy CSR
+    products of the full 2x2 system) and scaled to the assembly and to one
+    step of that block-Jacobi solve."""
     from tubularflowiga_amd import device
     ctx = dm.ctx
     velocity = np.asarray(state).reshape(-1, 4)[:, :3]
@@ -180,12 +185,20 @@ def transport_measurement(dm, case, state, steps=20, warmup=2):
         wall = time.perf_counter() - t0
         tm = ctx.timers(enable=False)
         spmv_ms = tr.bench_spmv(50)
+        tr.set_preconditioner(device.TFI_PC_AMG)
+        tr.step()  # AMG set-up
+        t0 = time.perf_counter()
+        its_amg = [tr.step()[0] for _ in range(3)]
+        wall_amg = time.perf_counter() - t0
     finally:
         tr.close()
     spmv_bytes = tm["spmv2_bytes"] / max(tm["spmv2_calls"], 1)
     gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    return {"workload": "two-species neuron transport on the same mesh, advected by the converged steady flow, "
-                        "dt=0.001, FGMRES(50) on the block-Jacobi-scaled system + AMG to 1e-8",
+    extra = {"amg": {"ms_per_step": round(1e3 * wall_amg / 3, 3), "krylov_per_step": float(np.mean(its_amg))}}
+    if cpu:
+        extra["cpu_baseline"] = transport_cpu_baseline(dm, case, velocity, params, float(np.mean(its)))
+    out = {"workload": "two-species neuron transport on the same mesh, advected by the converged steady flow, "
+                        "dt=0.001, FGMRES(50) on the block-Jacobi-scaled system to 1e-8",
             "steps": steps, "ms_per_step": round(1e3 * wall / steps, 3),
             "krylov_per_step": float(np.mean(its)), "assembly_ms": round(assembly_ms, 3),
             "spmv_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -193,6 +206,36 @@ def transport_measurement(dm, case, state, steps=20, warmup=2):
                               "kernel": "Bsr2SpmvKernel (2x2 FP64 values)",
                               "avg_ms": round(spmv_ms, 4), "algorithmic_bytes": int(spmv_bytes),
                               "in_solve_avg_ms": round(tm["spmv2_ms"] / max(tm["spmv2_calls"], 1), 4)}}
+    out.update(extra)
+    return out
+
+
+def transport_cpu_baseline(dm, case, velocity, params, iterations, n_el=2000, products=20):
+    """The oracle's transport element (C++, one element per call) on a sample
+    of elements and scipy CSR products of a 2x2 system on the full pattern;
+    one step = `iterations` products (block Jacobi, as the reference solves)."""
+    import scipy.sparse as sp
+    from oracle import transport as T
+    mesh = case.mesh
+    n_el = min(n_el, mesh.elements)
+    t0 = time.perf_counter()
+    for e in range(n_el):
+        conn = mesh.element_connectivity(e)
+        T.element(mesh.element_bezier(e), mesh.dense_extraction(e), velocity[conn], params)
+    per_element = (time.perf_counter() - t0) / n_el
+    ro, cols, _ = dm.pattern()
+    A = sp.bsr_matrix((np.full((len(cols), 2, 2), 1e-3), cols, ro), shape=(2 * mesh.nodes,) * 2).tocsr()
+    x = np.ones(2 * mesh.nodes)
+    A @ x
+    t0 = time.perf_counter()
+    for _ in range(products):
+        A @ x
+    per_product = (time.perf_counter() - t0) / products
+    return {"value": round(1e3 * per_product * iterations, 3), "unit": "ms/step", "cores": 1, "kind": "port",
+            "assembly_ms": round(1e3 * per_element * mesh.elements, 1),
+            "sample": f"oracle transport element on {n_el} elements ({1e3 * per_element:.3f} ms each) and "
+                      f"{products} scipy CSR products of the 2x2 system ({1e3 * per_product:.2f} ms each); one step "
+                      f"= {iterations:.0f} products (block-Jacobi iterations of the GPU run)"}
 
 
 def cpu_baseline(case, dm, rho, mu, dt, plain_timed, budget_s=20.0):
@@ -391,7 +434,7 @@ def main(argv=None):
                           "kernel": "BsrSpmvKernel<double> (FGMRES operator, FP64 values)",
                           "avg_ms": round(spmv_ms, 4), "algorithmic_bytes": int(spmv_bytes)},
     }
-    out["transport"] = transport_measurement(dm, case, plain_state)
+    out["transport"] = transport_measurement(dm, case, plain_state, cpu=not a.no_cpu)
     out["transport"]["spmv_roofline"]["traffic"] = traffic.get("Bsr2SpmvKernel")
     if a.inflow == "steady" and not a.no_pulsatile:
         out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, a.dt, dict(tol, rtol=3e-11),

@@ -259,11 +259,12 @@ typedef struct {
 int tfi_transport_create(tfi_mesh* mesh, const tfi_transport_params* params, const double* velocity,
 	const int32_t* constrained, tfi_transport** out);
 void tfi_transport_destroy(tfi_transport* transport);
-/* Preconditioner of the step's FGMRES on D^-1 L: TFI_PC_AMG (default; the
- * coupled aggregation AMG with FP32 V-cycles, its 2x2 blocks embedded in the
- * 4x4 hierarchy with an identity on the two padding fields) or
- * TFI_PC_BJACOBI (none beyond the block-Jacobi scaling itself, as the
- * reference's GMRES<2>).  The solution is the same either way. */
+/* Preconditioner of the step's FGMRES on D^-1 L: TFI_PC_BJACOBI (default;
+ * none beyond the block-Jacobi scaling itself, as the reference's GMRES<2>)
+ * or TFI_PC_AMG (the coupled aggregation AMG with FP32 V-cycles, its 2x2
+ * blocks embedded in the 4x4 hierarchy with an identity on the two padding
+ * fields: about 5x fewer iterations, each about 7x dearer).  The solution is
+ * the same either way. */
 int tfi_transport_set_preconditioner(tfi_transport* transport, int preconditioner);
 /* Prescribed N0 / Nplus values [N] of the constrained nodes
  * (SetTransportBoundaryVectorKernel, IgaCudaKernels.cuh:426-434). */

@@ -36,7 +36,7 @@ struct tfi_transport {
 	tfi::DevBuf<double> scaled_left, padded64, pad_in, pad_out;
 	tfi::DevBuf<float> padded32;
 	std::unique_ptr<tfi::Amg> amg;
-	int preconditioner = TFI_PC_AMG;
+	int preconditioner = TFI_PC_BJACOBI;
 	int restart = 50;
 	int singular = 0;
 };
@@ -495,10 +495,11 @@ int tfi_transport_step(tfi_transport* t, int zero_guess, int maximum_iterations,
 					a.diagonal, t->scaled_left.data(), t->padded64.data());
 				TFI_LAUNCH_CHECK("PadBlocksKernel");
 				ToFloat(ctx, t->padded64.data(), t->padded32.data(), (long long)a.nnzb*16);
-				// symmetric Gauss-Seidel (as the flow solver's); measured on the
-				// 0.6M-DOF Y: 22 iterations per step against 145 with block Jacobi
-				// alone, while damped-Jacobi smoothing (51) and one sweep pre and
-				// post (stagnation) do worse
+				// symmetric Gauss-Seidel (as the flow solver's); on the 0.6M-DOF Y
+				// 22-26 iterations per step against 120-145 with block Jacobi alone
+				// (damped-Jacobi smoothing: 51; one sweep pre and post stagnates),
+				// but at about 2 ms per V-cycle a step is not faster than block
+				// Jacobi's at 0.3 ms per iteration, hence not the default
 				t->amg = std::make_unique<Amg>(t->mesh, TFI_MIXED, AmgOptions());
 				t->amg->Setup(t->padded64.data(), t->padded32.data());
 			}

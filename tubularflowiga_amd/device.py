@@ -349,7 +349,7 @@ class Transport:
         self.handle = h
 
     def set_preconditioner(self, preconditioner):
-        """TFI_PC_AMG (default) or TFI_PC_BJACOBI."""
+        """TFI_PC_BJACOBI (default) or TFI_PC_AMG."""
         self.ctx.check(self.lib.tfi_transport_set_preconditioner(self.handle, preconditioner))
 
     def set_boundary(self, n0, nplus):

@@ -557,7 +557,7 @@ struct TransportOptions {
 	fs::path database, case_dir, output, velocity;
 	int steps = -1;
 	double linear_rtol = 1e-8;
-	int preconditioner = TFI_PC_AMG;
+	int preconditioner = TFI_PC_BJACOBI;
 };
 
 // iga_cuda transport DATABASE CASE_DIR [STEPS] [OUTPUT] [VELOCITY] (iga_cuda.cu:623-625),
