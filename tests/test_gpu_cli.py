@@ -270,3 +270,29 @@ def test_cli_outlet_state_survives_restart(tmp_path):
     assert r.returncode == 0, r.stderr
     print(f"outlet restart: uninterrupted {outlet(full)} resumed {outlet(resumed)}", flush=True)
     assert np.allclose(outlet(resumed), outlet(full), rtol=1e-10, atol=0.0)
+
+
+def test_cli_outlets_with_predicted_starts_reach_the_plain_solution(tmp_path):
+    """An RCR outlet with increment extrapolation and mixed precision: only the
+    first outlet pass of a step starts from the prediction (later passes start
+    where the previous one ended), and the states and committed outlet values
+    match the plain FP64 run's."""
+    case, d = _outlet_case(tmp_path, "predicted", 8)
+    plain, fast = tmp_path / "plain", tmp_path / "fast"
+    r = run("navier-stokes", d / "case.ntiga", d, "--checkpoint", plain, "--output", str(plain) + ".txt", *OUTLET_TOL)
+    assert r.returncode == 0, r.stderr
+    r = run("navier-stokes", d / "case.ntiga", d, "--checkpoint", fast, "--output", str(fast) + ".txt", *OUTLET_TOL,
+            "--warm-start", "mpe", "--precision", "mixed")
+    assert r.returncode == 0, r.stderr
+    used = re.findall(r"^warm_start step=(\d+) used=(\d) ", r.stdout, re.M)
+    assert len(used) == 8 and any(u == "1" for _, u in used)
+    v, p = rel(read_state(str(fast) + ".txt"), read_state(str(plain) + ".txt"))
+
+    def outlet(prefix):
+        o = json.loads(Path(str(prefix) + ".json").read_text())["outlets"][0]
+        return o["flow"], o["pressure"], o["capacitor_pressure"]
+
+    print(f"predicted starts with an RCR outlet: rel_l2 velocity={v:.3e} pressure={p:.3e}; "
+          f"outlet {outlet(fast)} vs {outlet(plain)}", flush=True)
+    assert v <= 1e-10 and p <= 1e-10
+    assert np.allclose(outlet(fast), outlet(plain), rtol=1e-10, atol=0.0)

@@ -364,7 +364,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		for (const auto& om : outlets) capacitor_start.push_back(om.capacitor_pressure);
 		const int passes = outlets.empty() ? 1 : 12;
 		bool outlets_converged = false;
-		tfi_newton_report rep{};
+		tfi_newton_report rep{}, first_rep{};  // last and first outlet pass of the step
 		for (int coupling = 0; coupling < passes; ++coupling) {
 			outlet_p = OutletPressures(outlets);
 			bc = ResolveFlowBoundaries(cfg, sys, labeled.labels, profile, &outlet_p);
@@ -385,6 +385,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 				bc.pressure.data(), any_traction ? traction.data() : nullptr), "boundary data");
 			rep = tfi_newton_report{};
 			const int status = tfi_flow_newton(flow.flow, &rep);
+			if (coupling == 0) first_rep = rep;
 			const double scale = std::sqrt(static_cast<double>(4*N));
 			for (int r = 0; r < rep.records; ++r) {
 				const auto& rec = rep.record[r];
@@ -436,9 +437,10 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		if (!outlets_converged)
 			throw std::runtime_error("outlet fixed-point iteration did not converge at physical step "
 				+ std::to_string(step + 1));
+		// the prediction is made for the step's first outlet pass only
 		if (opt.warm_start != TFI_WARM_PREVIOUS)
-			std::cout << "warm_start step=" << step + 1 << " used=" << rep.warm_start_used
-				<< " initial_ratio=" << rep.warm_start_initial_ratio << '\n';
+			std::cout << "warm_start step=" << step + 1 << " used=" << first_rep.warm_start_used
+				<< " initial_ratio=" << first_rep.warm_start_initial_ratio << '\n';
 		c.Check(tfi_flow_commit_step(flow.flow), "commit step");
 		const int completed = step + 1;
 		if (o.output_every > 0 && completed % o.output_every == 0) {
