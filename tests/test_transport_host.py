@@ -62,6 +62,10 @@ def test_transport_usage_errors(tmp_path):
     assert r.returncode == 1 and "--linear-rtol requires a positive number" in r.stderr
     r = run("transport", d / "case.ntiga", d, "x")
     assert r.returncode == 1 and "STEPS must be a non-negative integer" in r.stderr
+    r = run("transport", d / "case.ntiga", d, "--preconditioner", "ilu")
+    assert r.returncode == 1 and "--preconditioner must be amg|bjacobi" in r.stderr
+    r = run("transport", d / "case.ntiga", d, "--bogus")
+    assert r.returncode == 1 and "unknown option: --bogus" in r.stderr
     (d / "simulation_parameter.txt").unlink()
     r = run("transport", d / "case.ntiga", d)
     assert r.returncode == 1 and "cannot open simulation parameters" in r.stderr
