@@ -290,6 +290,67 @@ int tfi_transport_matrices(tfi_transport* transport, double* left, double* previ
  * milliseconds per launch.  No reference counterpart. */
 int tfi_transport_bench_spmv(tfi_transport* transport, int repetitions, double* milliseconds);
 
+/* ------------------------------------------------------------------ configured transport */
+/* A linear_transport equation system of 1..8 coupled scalar fields with
+ * F x F node blocks (`iga_cuda solve`, solvers/cuda/include/ConfiguredTransportHost.inc:142-471;
+ * kernels GenericTransportKernels.cuh).  Terms as CompiledTerm
+ * (include/SimulationConfig.hpp:159-165): kind TFI_TERM_*, equation and trial
+ * field indices, coefficient. */
+typedef struct tfi_solve tfi_solve;
+enum {
+	TFI_TERM_TIME_DERIVATIVE = 0, TFI_TERM_DIFFUSION = 1, TFI_TERM_ADVECTION = 2, TFI_TERM_LINEAR_COUPLING = 3,
+	TFI_TERM_VOLUME_SOURCE = 4
+};
+/* SUPG setting per equation (ConfiguredTransportHost.inc:186-193): none, or
+ * SUPG with the diffusion weighted "galerkin", "supg_test" or "consistent". */
+enum { TFI_SUPG_NONE = 0, TFI_SUPG_GALERKIN = 1, TFI_SUPG_TEST = 2, TFI_SUPG_CONSISTENT = 3 };
+typedef struct {
+	int32_t kind, equation, trial;
+	double coefficient;
+} tfi_solve_term;
+typedef struct {
+	int32_t fields;              /* 1..8 */
+	int32_t term_count;          /* 0..64 */
+	const tfi_solve_term* terms;
+	const int32_t* supg;         /* [fields] TFI_SUPG_* */
+	const double* tau_diffusivity;  /* [fields] (SupgTauDiffusivity, SimulationConfig.hpp:728-741) */
+	double dt;
+	/* flux / Robin faces (ConfiguredTransportHost.inc:211-255): element, face
+	 * (0..5) and per-field flux, Robin coefficient and coefficient x exterior
+	 * value [surface_count * fields] */
+	int32_t surface_count;
+	const int32_t* surface_element;
+	const int32_t* surface_face;
+	const double* surface_flux;
+	const double* surface_robin;
+	const double* surface_robin_source;
+} tfi_solve_system;
+/* Assembles the backward-Euler left matrix, previous-state matrix and source
+ * vector for the prescribed nodal velocity [3N] (AssembleGenericTransportKernel,
+ * AssembleGenericSurfaceKernel) and applies the constrained rows
+ * (ApplyGenericBoundaryKernel): constrained [N*fields] = 1 for a prescribed
+ * (node, field).  Elements of at most 80 basis functions. */
+int tfi_solve_create(tfi_mesh* mesh, const tfi_solve_system* system, const double* velocity,
+	const int32_t* constrained, tfi_solve** out);
+void tfi_solve_destroy(tfi_solve* solve);
+/* Prescribed values [N*fields] of the constrained entries
+ * (SetGenericBoundaryVectorKernel). */
+int tfi_solve_set_boundary(tfi_solve* solve, const double* values);
+int tfi_solve_set_state(tfi_solve* solve, const double* state);
+int tfi_solve_get_state(tfi_solve* solve, double* state);
+/* One backward-Euler step (ConfiguredTransportHost.inc:378-400): b = P c + s,
+ * constrained entries of b set to their values, L c' = b solved as
+ * D^-1 L c' = D^-1 b (D the F x F diagonal blocks, the reference's left
+ * block-Jacobi GMRES) by FGMRES(50) from c' = 0 (zero_guess) or c, to
+ * ||D^-1 (b - L c')|| <= rtol ||D^-1 (b - L c'_0)||; c <- c'.  A solve that
+ * does not converge is an error. */
+int tfi_solve_step(tfi_solve* solve, int zero_guess, int maximum_iterations, double relative_tolerance,
+	int* iterations, double* relative_residual);
+/* Host copies of the assembled blocks of L and P [nnzb*fields^2] (row-major
+ * blocks), the source vector [N*fields] and the count of diagonal blocks
+ * inverted by their diagonal (any pointer may be NULL). */
+int tfi_solve_matrices(tfi_solve* solve, double* left, double* previous, double* source, int* singular_blocks);
+
 /* Kernel timers accumulated on the context stream (HIP events). */
 typedef struct {
 	double spmv_ms, spmv_calls, spmv_bytes;           /* fine-level fp64 SpMV */

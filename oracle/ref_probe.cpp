@@ -35,6 +35,15 @@
 //                                          indices, the sums of N0 and Nplus and
 //                                          the velocity / pressure node counts
 //                                          (solvers/cpu/include/CaseInput.hpp)
+//   ref_probe solve-case CASE_DIR DATABASE [SYSTEM]
+//                                        -> CompileLinearSystem of the named (or
+//                                          first) linear_transport system, the
+//                                          per-field SUPG code of
+//                                          ConfiguredTransportHost.inc:186-193,
+//                                          SupgTauDiffusivity, InitialScalarValues,
+//                                          and ResolveScalarBoundaries with
+//                                          NodeFaceLabels: "term", "field" and
+//                                          "boundary" lines as `iga_hip solve-check`
 // ELEMENT file: nen, then nen connectivity ints, nen x 64 extraction values,
 // 64 x 3 Bezier points, nen x 4 nodal state values (whitespace separated).
 #include "BoundaryFlow.hpp"
@@ -43,6 +52,7 @@
 #include "Quadrature.hpp"
 #include "OutletModel.hpp"
 #include "CaseInput.hpp"
+#include "GenericCaseInput.hpp"
 
 #include <cstdio>
 #include <fstream>
@@ -189,6 +199,35 @@ int main(int argc, char** argv)
 			Print(n0);
 			Print(nplus);
 			std::printf("%zu %zu\n", bc.velocity_nodes, bc.pressure_nodes);
+		} else if (cmd == "solve-case" && (argc == 4 || argc == 5)) {
+			const std::string dir = argv[2];
+			iga::Database database(argv[3]);
+			const auto config = iga::ReadSimulationConfiguration(dir + "/simulation_config.json");
+			const auto name = argc == 5 ? std::string(argv[4]) : iga::FirstLinearTransportSystem(config);
+			const auto system = iga::CompileLinearSystem(config, name);
+			for (const auto& t : system.terms)
+				std::printf("term kind=%d equation=%zu trial=%zu coefficient=%.17g\n", static_cast<int>(t.kind),
+					t.equation, t.trial, t.coefficient);
+			std::vector<int> supg(system.fields.size(), 0);
+			for (const auto& st : system.stabilization)
+				supg[system.field_index.at(st.equation)] = st.diffusion_weighting == "supg_test" ? 2
+					: st.diffusion_weighting == "consistent" ? 3 : 1;
+			const auto tau = iga::SupgTauDiffusivity(system);
+			const auto initial = iga::InitialScalarValues(config, system);
+			for (std::size_t f = 0; f < system.fields.size(); ++f)
+				std::printf("field index=%zu supg=%d tau_diffusivity=%.17g initial=%.17g\n", f, supg[f], tau[f],
+					initial[f]);
+			const auto labels = iga::ReadPointLabels(dir + "/controlmesh.vtk", database.header().nodes);
+			const auto faces = iga::NodeFaceLabels(database);
+			const auto bc = iga::ResolveScalarBoundaries(config, system, labels, &faces);
+			long long index_sum = 0;
+			double value_sum = 0.0;
+			for (std::size_t i = 0; i < bc.constrained.size(); ++i)
+				if (bc.constrained[i]) {
+					index_sum += static_cast<long long>(i);
+					value_sum += bc.value[i];
+				}
+			std::printf("boundary constrained_index_sum=%lld value_sum=%.17g\n", index_sum, value_sum);
 		} else if (cmd == "outlet" && argc == 12) {
 			iga::OutletModelState m;
 			m.label = 2;

@@ -1,0 +1,166 @@
+"""The `iga_hip solve` input path on the CPU: the linear_transport system
+compiled from simulation_config.json (CompileLinearSystem,
+include/SimulationConfig.hpp:743-785), the SUPG codes and tau diffusivities
+(ConfiguredTransportHost.inc:186-193, SupgTauDiffusivity), the initial values
+and the scalar boundaries with face-support Dirichlet entries
+(ResolveScalarBoundaries, solvers/cpu/include/GenericCaseInput.hpp:63-141),
+printed by `iga_hip solve-check` and compared with the reference's own
+functions run by oracle/_ref/ref_probe on the same case; then the usage and
+configuration errors."""
+import copy
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from tubularflowiga_amd import casegen
+
+ROOT = Path(__file__).resolve().parent.parent
+CLI = ROOT / "tubularflowiga_amd" / "iga_hip"
+PROBE = ROOT / "oracle" / "_ref" / "ref_probe"
+
+FIELDS = [{"name": "velocity", "kind": "vector3"}, {"name": "pressure", "kind": "pressure"},
+          {"name": "oxygen", "kind": "scalar", "initial_value": 0.2},
+          {"name": "carrier", "kind": "scalar", "initial_value": 0.05},
+          {"name": "tracer", "kind": "scalar"}]
+SYSTEM = {"name": "species", "kind": "linear_transport", "unknowns": ["oxygen", "carrier", "tracer"],
+          "terms": [{"operator": "time_derivative", "equation": "oxygen"},
+                    {"operator": "diffusion", "equation": "oxygen", "coefficient": 0.05},
+                    {"operator": "advection", "equation": "oxygen", "velocity": "prescribed"},
+                    {"operator": "linear_coupling", "equation": "oxygen", "trial": "carrier", "coefficient": -0.2},
+                    {"operator": "volume_source", "equation": "oxygen", "coefficient": 0.3},
+                    {"operator": "time_derivative", "equation": "carrier"},
+                    {"operator": "diffusion", "equation": "carrier", "coefficient": 0.02},
+                    {"operator": "advection", "equation": "carrier", "velocity": "prescribed"},
+                    {"operator": "linear_coupling", "equation": "carrier", "trial": "oxygen", "coefficient": -0.1},
+                    {"operator": "time_derivative", "equation": "tracer", "coefficient": 2.0},
+                    {"operator": "diffusion", "equation": "tracer", "coefficient": 0.01},
+                    {"operator": "diffusion", "equation": "oxygen", "coefficient": 0.01}],
+          "stabilization": [{"equation": "oxygen", "method": "supg", "velocity": "prescribed"},
+                            {"equation": "carrier", "method": "supg", "velocity": "prescribed",
+                             "diffusion_weighting": "supg_test", "tau": "advective"}]}
+BOUNDARIES = [{"label": 0, "name": "wall", "conditions": [
+                  {"field": "velocity", "type": "dirichlet", "value": [0.0, 0.0, 0.0]},
+                  {"field": "tracer", "type": "flux", "value": 0.4},
+                  {"field": "oxygen", "type": "robin", "coefficient": 0.7, "exterior_value": 0.25}]},
+              {"label": 1, "name": "inlet", "conditions": [
+                  {"field": "oxygen", "type": "dirichlet", "value": 1.0},
+                  {"field": "carrier", "type": "dirichlet", "value": 0.25, "dirichlet_dofs": "node_label"}]},
+              {"label": 2, "name": "outlet", "conditions": [{"field": "oxygen", "type": "advective_outflow"}]}]
+
+
+def variants():
+    out = {"three_fields": (FIELDS, [SYSTEM], BOUNDARIES, [])}
+    second = copy.deepcopy(SYSTEM)
+    second["name"] = "tracer_only"
+    second["unknowns"] = ["tracer"]
+    second["terms"] = [t for t in second["terms"] if t["equation"] == "tracer"]
+    second["stabilization"] = []
+    out["named_second_system"] = (FIELDS, [SYSTEM, second], BOUNDARIES, ["--system", "tracer_only"])
+    b = copy.deepcopy(BOUNDARIES)
+    b[1]["conditions"][1]["dirichlet_dofs"] = "face_support"
+    out["face_support"] = (FIELDS, [SYSTEM], b, [])
+    return out
+
+
+def write(tmp_path, fields, systems, boundaries, **kw):
+    return casegen.write_solve_case(casegen.workload("tube_c0"), tmp_path / "case", fields, systems, boundaries,
+                                    dt=2e-3, steps=6, **kw)
+
+
+def run(*args):
+    return subprocess.run([str(CLI), *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def check_lines(out):
+    return [line for line in out.splitlines() if line.split(" ")[0] in ("term", "field", "boundary")]
+
+
+@pytest.mark.parametrize("name", list(variants()))
+def test_solve_check_matches_the_reference_compile(tmp_path, name):
+    if not PROBE.exists():
+        pytest.skip("oracle/_ref/ref_probe is not built (needs the reference sources)")
+    fields, systems, boundaries, opts = variants()[name]
+    d = write(tmp_path, fields, systems, boundaries)
+    ours = run("solve-check", d / "case.ntiga", d, *opts)
+    assert ours.returncode == 0, ours.stderr
+    system = opts[1] if opts else []
+    ref = subprocess.run([str(PROBE), "solve-case", str(d), str(d / "case.ntiga"), *([system] if system else [])],
+                         capture_output=True, text=True, timeout=120)
+    assert ref.returncode == 0, ref.stderr
+    assert check_lines(ours.stdout) == check_lines(ref.stdout)
+    head = ours.stdout.splitlines()[0]
+    assert "velocity_source=prescribed" in head
+    if name != "named_second_system":
+        assert "surfaces=" in head and "surfaces=0" not in head
+
+
+def test_solve_check_counts(tmp_path):
+    d = write(tmp_path, FIELDS, [SYSTEM], BOUNDARIES)
+    out = run("solve-check", d / "case.ntiga", d)
+    assert out.returncode == 0, out.stderr
+    head = out.stdout.splitlines()[0]
+    assert "system=species fields=oxygen,carrier,tracer" in head and "terms=12" in head
+    # wall faces carry the flux / Robin conditions: one record per labelled wall face
+    labels = casegen.workload("tube_c0").mesh.boundary_labels
+    assert f"surfaces={int((labels == 0).sum())}" in head
+
+
+def bad(mutate):
+    fields, systems, boundaries = copy.deepcopy(FIELDS), copy.deepcopy([SYSTEM]), copy.deepcopy(BOUNDARIES)
+    extra = mutate(fields, systems, boundaries) or {}
+    return fields, systems, boundaries, extra
+
+
+CONFIG_ERRORS = {
+    "bdf2": (lambda f, s, b: s[0].update(time_integration="bdf2"), "backward_euler only"),
+    "physiology": (lambda f, s, b: {"physiology": {"enabled": True}}, "physiology"),
+    "unknown_operator": (lambda f, s, b: s[0]["terms"].append({"operator": "reaction", "equation": "oxygen"}),
+                         "unknown term operator"),
+    "foreign_trial": (lambda f, s, b: s[0]["terms"].append({"operator": "linear_coupling", "equation": "oxygen",
+                                                            "trial": "pressure"}), "unknowns of their system"),
+    "advection_without_velocity": (lambda f, s, b: s[0]["terms"].append({"operator": "advection",
+                                                                         "equation": "tracer"}),
+                                   "advection requires a velocity source"),
+    "bad_weighting": (lambda f, s, b: s[0]["stabilization"][0].update(diffusion_weighting="upwind"),
+                      "diffusion_weighting"),
+    "scalar_waveform": (lambda f, s, b: b[1]["conditions"][0].update(waveform="w") or
+                        {"temporal_functions": [{"name": "w", "kind": "constant", "units": "1", "value": 1.0}]},
+                        "time-dependent scalar boundary"),
+    "traction_on_scalar": (lambda f, s, b: b[2]["conditions"].__setitem__(
+        0, {"field": "oxygen", "type": "pressure_traction", "value": 1.0}), "unsupported scalar boundary"),
+    "unconfigured_label": (lambda f, s, b: (b.pop(2), None)[1], "has no simulation_config.json boundary definition"),
+    "vector_unknown": (lambda f, s, b: s[0]["unknowns"].append("velocity"), "requires scalar fields"),
+    "snapshot_velocity": (lambda f, s, b: [t.update(velocity="series") for t in s[0]["terms"]
+                                           if t["operator"] == "advection"] and
+                          [x.update(velocity="series") for x in s[0]["stabilization"]] and
+                          {"velocity_sources": [{"name": "series", "kind": "snapshot_series",
+                                                 "manifest": "v.json"}]}, "is not supported by iga_hip solve"),
+}
+
+
+@pytest.mark.parametrize("name", list(CONFIG_ERRORS))
+def test_configuration_errors(tmp_path, name):
+    mutate, message = CONFIG_ERRORS[name]
+    fields, systems, boundaries, extra = bad(mutate)
+    d = write(tmp_path, fields, systems, boundaries, extra=extra)
+    out = run("solve-check", d / "case.ntiga", d)
+    assert out.returncode != 0
+    assert message in out.stderr, out.stderr
+
+
+@pytest.mark.parametrize("args,message", [
+    (["--output-every", "2"], "--output-every requires --output"),
+    (["--checkpoint-every", "2"], "--checkpoint-every requires --checkpoint"),
+    (["--stop-after-step", "9"], "--stop-after-step exceeds configured transport steps"),
+    (["--visualization-format", "vtkhdf"], "vtkhdf is not available"),
+    (["--linear-rtol", "-1"], "--linear-rtol requires a finite positive value"),
+    (["--bogus", "1"], "unknown option: --bogus"),
+    (["species", "out.txt", "vel.txt", "extra"], "too many positional arguments"),
+    (["--system", "nope"], "unknown equation system 'nope'"),
+])
+def test_usage_errors(tmp_path, args, message):
+    d = write(tmp_path, FIELDS, [SYSTEM], BOUNDARIES)
+    out = run("solve", d / "case.ntiga", d, *args)
+    assert out.returncode != 0
+    assert message in out.stderr, out.stderr

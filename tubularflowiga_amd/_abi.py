@@ -19,6 +19,8 @@ TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2
 TFI_WARM_EXTRAPOLATE4, TFI_WARM_MPE = 5, 6
 TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL = 0, 1
 TFI_TANGENT_LEGACY, TFI_TANGENT_EXACT = 0, 1
+TERM_KINDS = {"time_derivative": 0, "diffusion": 1, "advection": 2, "linear_coupling": 3, "volume_source": 4}
+TFI_SUPG_NONE, TFI_SUPG_GALERKIN, TFI_SUPG_TEST, TFI_SUPG_CONSISTENT = 0, 1, 2, 3
 MAX_NEWTON_RECORDS = 64
 
 c_int, c_double, c_ll, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_void_p
@@ -65,6 +67,18 @@ class TransportParams(ctypes.Structure):
     _fields_ = [("diffusion", c_double), ("vplus", c_double), ("vminus", c_double), ("kplus", c_double),
                 ("kminus", c_double), ("detach_plus", c_double), ("detach_minus", c_double), ("dt", c_double),
                 ("artificial_diffusion", c_double)]
+
+
+class SolveTerm(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("equation", ctypes.c_int32), ("trial", ctypes.c_int32),
+                ("coefficient", c_double)]
+
+
+class SolveSystem(ctypes.Structure):
+    _fields_ = [("fields", ctypes.c_int32), ("term_count", ctypes.c_int32), ("terms", c_void_p),
+                ("supg", c_void_p), ("tau_diffusivity", c_void_p), ("dt", c_double),
+                ("surface_count", ctypes.c_int32), ("surface_element", c_void_p), ("surface_face", c_void_p),
+                ("surface_flux", c_void_p), ("surface_robin", c_void_p), ("surface_robin_source", c_void_p)]
 
 
 class KernelTimers(ctypes.Structure):
@@ -133,6 +147,13 @@ SIGNATURES = {
     "tfi_transport_step": (c_int, [c_void_p, c_int, c_int, c_double, P(c_int), P(c_double)]),
     "tfi_transport_matrices": (c_int, [c_void_p, c_void_p, c_void_p, P(c_int)]),
     "tfi_transport_bench_spmv": (c_int, [c_void_p, c_int, P(c_double)]),
+    "tfi_solve_create": (c_int, [c_void_p, P(SolveSystem), _d, _i32, P(c_void_p)]),
+    "tfi_solve_destroy": (None, [c_void_p]),
+    "tfi_solve_set_boundary": (c_int, [c_void_p, _d]),
+    "tfi_solve_set_state": (c_int, [c_void_p, _d]),
+    "tfi_solve_get_state": (c_int, [c_void_p, _d]),
+    "tfi_solve_step": (c_int, [c_void_p, c_int, c_int, c_double, P(c_int), P(c_double)]),
+    "tfi_solve_matrices": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, P(c_int)]),
     "tfi_timers_get": (c_int, [c_void_p, P(KernelTimers)]),
     "tfi_timers_enable": (c_int, [c_void_p, c_int]),
     "tfi_timers_reset": (c_int, [c_void_p]),

@@ -26,7 +26,8 @@ LIB = PKG / "libtfi_hip.so"
 CLI = PKG / "iga_hip"
 
 SOURCES = ["tfi_core.hip", "tfi_mesh.hip", "tfi_assembly.hip", "tfi_sparse.hip", "tfi_krylov.hip",
-           "tfi_amg.hip", "tfi_mlp.hip", "tfi_warmstart.hip", "tfi_flow.hip", "tfi_transport.hip"]
+           "tfi_amg.hip", "tfi_mlp.hip", "tfi_warmstart.hip", "tfi_flow.hip", "tfi_transport.hip",
+           "tfi_solve.hip"]
 
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Wall", "-Wno-unused-function",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -565,6 +565,24 @@ def write_transport_case(case: Case, directory, params: dict, steps: int, veloci
     return d
 
 
+def write_solve_case(case: Case, directory, fields, systems, boundaries, dt: float, steps: int, velocity=None,
+                     extra=None):
+    """A case directory for `solve`: the flow case files with a
+    simulation_config.json holding the given scalar `fields` (dicts with name,
+    kind, initial_value), linear_transport `systems` and `boundaries`; the
+    prescribed velocity in initial_velocityfield.txt (the tube's Poiseuille
+    field unless `velocity` is given).  `extra` adds top-level keys."""
+    d = write_case(case, directory)
+    if velocity is None:
+        velocity = tube_velocity(case)
+    np.savetxt(d / "initial_velocityfield.txt", np.asarray(velocity).reshape(-1, 3), fmt="%.17g")
+    config = {"schema_version": 3, "dimension": "3d", "fields": list(fields),
+              "time": {"dt": dt, "steps": steps}, "equation_systems": list(systems), "boundaries": list(boundaries)}
+    config.update(extra or {})
+    (d / "simulation_config.json").write_text(json.dumps(config, indent=2) + "\n")
+    return d
+
+
 # Named workloads (sizes chosen so the DOF counts match the workload text).
 WORKLOADS = {
     # straight tube R=1 mm, L=10 mm, 5 hexes per section x 16 axial, Poiseuille 0.1 m/s

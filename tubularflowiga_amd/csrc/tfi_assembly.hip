@@ -759,6 +759,84 @@ __global__ void FaceTractionKernel(int count, const int* __restrict__ felem, con
 	}
 }
 
+// Face mass of one labelled face for the scalar flux / Robin terms
+// (AssembleGenericSurfaceKernel, solvers/cuda/include/GenericTransportKernels.cuh:146-215):
+// with m = w_i w_j / 4 |t_0 x t_1| at the 4x4 Gauss points of the face,
+// mass[f][a][b] = sum_pt N_a N_b m and load[f][a] = sum_pt N_a m.  |t_0 x t_1|
+// is the length of the cofactor vector FaceTractionKernel integrates.  One
+// 64-lane block per face; stride kMaxBasis in both outputs.
+__global__ void FaceMassKernel(int count, const int* __restrict__ felem, const int* __restrict__ fface,
+	const int* __restrict__ eoff, const int* __restrict__ xoff, const uint8_t* __restrict__ xcol,
+	const double* __restrict__ xval, const double* __restrict__ bezier, double* __restrict__ mass,
+	double* __restrict__ load)
+{
+	const int f = blockIdx.x;
+	if (f >= count) return;
+	__shared__ double bern[16][64];
+	__shared__ double measure[16];
+	__shared__ double basis[16][kMaxBasis];
+	const int e = felem[f], face = fface[f];
+	const int fixed_axis[6] = {2, 1, 0, 1, 0, 2};
+	const int var0[6] = {0, 0, 1, 0, 1, 0}, var1[6] = {1, 2, 2, 2, 2, 1};
+	const double fixed_value[6] = {0.0, 0.0, 1.0, 1.0, 0.0, 1.0};
+	const double gw[4] = {0.3478548451374539, 0.6521451548625461, 0.6521451548625461, 0.3478548451374539};
+	const int lane = threadIdx.x;  // 64 threads
+	for (int idx = lane; idx < 16*64; idx += 64) {
+		const int pt = idx >> 6, p = idx & 63;
+		double c[3];
+		c[fixed_axis[face]] = fixed_value[face];
+		c[var0[face]] = kGp[pt >> 2];
+		c[var1[face]] = kGp[pt & 3];
+		double b[3][4], db[3][4], d2b[3][4];
+		for (int d = 0; d < 3; ++d) Bern3(c[d], b[d], db[d], d2b[d]);
+		bern[pt][p] = b[0][p & 3]*b[1][(p >> 2) & 3]*b[2][p >> 4];
+	}
+	if (lane < 16) {
+		const int pt = lane, qi = pt >> 2, qj = pt & 3;
+		double c[3];
+		c[fixed_axis[face]] = fixed_value[face];
+		c[var0[face]] = kGp[qi];
+		c[var1[face]] = kGp[qj];
+		double b[3][4], db[3][4], d2b[3][4];
+		for (int d = 0; d < 3; ++d) Bern3(c[d], b[d], db[d], d2b[d]);
+		const double* X = bezier + (size_t)e*192;
+		double t0[3] = {}, t1[3] = {};
+		for (int k = 0; k < 4; ++k)
+			for (int j = 0; j < 4; ++j)
+				for (int i = 0; i < 4; ++i) {
+					const int p = i + 4*j + 16*k;
+					const double g[3] = {db[0][i]*b[1][j]*b[2][k], b[0][i]*db[1][j]*b[2][k], b[0][i]*b[1][j]*db[2][k]};
+					for (int x = 0; x < 3; ++x) {
+						t0[x] += X[3*p + x]*g[var0[face]];
+						t1[x] += X[3*p + x]*g[var1[face]];
+					}
+				}
+		const double n[3] = {t0[1]*t1[2] - t0[2]*t1[1], t0[2]*t1[0] - t0[0]*t1[2], t0[0]*t1[1] - t0[1]*t1[0]};
+		measure[pt] = gw[qi]*gw[qj]*0.25*sqrt(n[0]*n[0] + n[1]*n[1] + n[2]*n[2]);
+	}
+	__syncthreads();
+	const int b0 = eoff[e], nen = eoff[e + 1] - b0;
+	for (int idx = lane; idx < 16*nen; idx += 64) {
+		const int pt = idx/nen, a = idx - pt*nen;
+		double N = 0.0;
+		for (int k = xoff[b0 + a]; k < xoff[b0 + a + 1]; ++k) N += xval[k]*bern[pt][xcol[k]];
+		basis[pt][a] = N;
+	}
+	__syncthreads();
+	double* mf = mass + (size_t)f*kMaxBasis*kMaxBasis;
+	for (int pair = lane; pair < nen*nen; pair += 64) {
+		const int a = pair/nen, b = pair - a*nen;
+		double s = 0.0;
+		for (int pt = 0; pt < 16; ++pt) s += basis[pt][a]*basis[pt][b]*measure[pt];
+		mf[a*kMaxBasis + b] = s;
+	}
+	for (int a = lane; a < nen; a += 64) {
+		double s = 0.0;
+		for (int pt = 0; pt < 16; ++pt) s += basis[pt][a]*measure[pt];
+		load[(size_t)f*kMaxBasis + a] = s;
+	}
+}
+
 // Residual-only assembly from the basis table, for elements of at most 32
 // basis functions: one workgroup of 256 threads per element and no slabs.
 // Phase A: four lanes per point sum the fields over the basis functions and
@@ -1032,6 +1110,14 @@ void FaceFlowsFromVectors(tfi_mesh* m, int count, const int* element, const doub
 	hipLaunchKernelGGL(FaceFlowVecKernel, dim3(count), dim3(64), 0, m->ctx->stream, count, element, m->eoff.data(),
 		m->conn.data(), vec, state, flows);
 	TFI_LAUNCH_CHECK("FaceFlowVecKernel");
+}
+
+void FaceMass(tfi_mesh* m, int count, const int* element, const int* face, double* mass, double* load)
+{
+	if (count <= 0) return;
+	hipLaunchKernelGGL(FaceMassKernel, dim3(count), dim3(64), 0, m->ctx->stream, count, element, face,
+		m->eoff.data(), m->xoff.data(), m->xcol.data(), m->xval.data(), m->bezier.data(), mass, load);
+	TFI_LAUNCH_CHECK("FaceMassKernel");
 }
 
 void FaceTraction(tfi_mesh* m, int count, const int* element, const int* face, const double* pressure,

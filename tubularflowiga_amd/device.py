@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, TransportParams, WarmOptions, TFI_FP64, TFI_MIXED,
+from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, SolveSystem, SolveTerm, TransportParams, WarmOptions, TFI_FP64, TFI_MIXED,
                    TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3,
                    TFI_WARM_EXTRAPOLATE4,
                    TFI_WARM_LEARNED, TFI_WARM_MPE,
@@ -391,4 +391,82 @@ class Transport:
         """Release the device buffers (before the context is closed)."""
         if self.handle:
             self.lib.tfi_transport_destroy(self.handle)
+            self.handle = None
+
+
+class ConfiguredTransport:
+    """A linear_transport system of F coupled scalar fields (F x F node
+    blocks); see tfi_solve_* in include/tfi.h.
+
+    terms: [(kind, equation, trial, coefficient)] with kind a TERM_KINDS
+    value; supg, tau_diffusivity: per field; surfaces: optional dict with
+    element, face and per-face [count][F] flux, robin, robin_source."""
+
+    def __init__(self, dmesh: DeviceMesh, fields, terms, supg, tau_diffusivity, dt, velocity, constrained,
+                 surfaces=None):
+        self.dmesh = dmesh
+        self.ctx = dmesh.ctx
+        self.lib = dmesh.lib
+        self.nodes = dmesh.nodes
+        self.fields = int(fields)
+        term_array = (SolveTerm * max(1, len(terms)))(*[SolveTerm(int(k), int(e), int(t), float(c))
+                                                        for k, e, t, c in terms])
+        keep = [term_array]
+
+        def arr(values, dtype):
+            a = np.ascontiguousarray(values, dtype=dtype).ravel()
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.c_void_p)
+
+        sysdef = SolveSystem()
+        sysdef.fields = self.fields
+        sysdef.term_count = len(terms)
+        sysdef.terms = ctypes.cast(term_array, ctypes.c_void_p)
+        sysdef.supg = arr(supg, np.int32)
+        sysdef.tau_diffusivity = arr(tau_diffusivity, np.float64)
+        sysdef.dt = float(dt)
+        if surfaces is not None and len(surfaces["element"]):
+            sysdef.surface_count = len(surfaces["element"])
+            sysdef.surface_element = arr(surfaces["element"], np.int32)
+            sysdef.surface_face = arr(surfaces["face"], np.int32)
+            sysdef.surface_flux = arr(surfaces["flux"], np.float64)
+            sysdef.surface_robin = arr(surfaces["robin"], np.float64)
+            sysdef.surface_robin_source = arr(surfaces["robin_source"], np.float64)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.lib.tfi_solve_create(
+            dmesh.handle, ctypes.byref(sysdef), np.ascontiguousarray(velocity, np.float64).ravel(),
+            np.ascontiguousarray(constrained, np.int32).ravel(), ctypes.byref(h)))
+        self.handle = h
+
+    def set_boundary(self, values):
+        self.ctx.check(self.lib.tfi_solve_set_boundary(self.handle,
+                                                       np.ascontiguousarray(values, np.float64).ravel()))
+
+    def set_state(self, state):
+        self.ctx.check(self.lib.tfi_solve_set_state(self.handle, np.ascontiguousarray(state, np.float64).ravel()))
+
+    def state(self):
+        out = np.zeros(self.fields * self.nodes)
+        self.ctx.check(self.lib.tfi_solve_get_state(self.handle, out))
+        return out.reshape(-1, self.fields)
+
+    def step(self, zero_guess=False, maximum_iterations=10000, rtol=1e-8):
+        it, res = ctypes.c_int(), ctypes.c_double()
+        self.ctx.check(self.lib.tfi_solve_step(self.handle, int(zero_guess), maximum_iterations, rtol,
+                                               ctypes.byref(it), ctypes.byref(res)))
+        return it.value, res.value
+
+    def matrices(self):
+        F, nnzb = self.fields, self.dmesh.nnzb
+        left, prev, src = np.zeros(F * F * nnzb), np.zeros(F * F * nnzb), np.zeros(F * self.nodes)
+        s = ctypes.c_int()
+        self.ctx.check(self.lib.tfi_solve_matrices(self.handle, left.ctypes.data_as(ctypes.c_void_p),
+                                                   prev.ctypes.data_as(ctypes.c_void_p),
+                                                   src.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s)))
+        return left.reshape(-1, F, F), prev.reshape(-1, F, F), src.reshape(-1, F), s.value
+
+    def close(self):
+        """Release the device buffers (before the context is closed)."""
+        if self.handle:
+            self.lib.tfi_solve_destroy(self.handle)
             self.handle = None

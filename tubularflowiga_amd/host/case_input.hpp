@@ -899,7 +899,15 @@ inline void WriteStateVtk(const fs::path& mesh_path, const fs::path& path, const
 }
 
 // WriteVtu (include/VtkOutput.hpp:92-148) with point arrays velocity / pressure
-inline void WriteStateVtu(const fs::path& mesh_path, const fs::path& path, const std::vector<double>& s, double time)
+// Named point arrays of a VTU file (VtkPointArray, VtkOutput.hpp)
+struct PointArray {
+	std::string name;
+	int components = 1;
+	std::vector<double> values;  // [points * components]
+};
+
+inline void WritePointArraysVtu(const fs::path& mesh_path, const fs::path& path, const std::vector<PointArray>& arrays,
+	double time)
 {
 	std::ifstream in(mesh_path);
 	if (!in) throw std::runtime_error("cannot open VTK mesh: " + mesh_path.string());
@@ -940,7 +948,9 @@ inline void WriteStateVtu(const fs::path& mesh_path, const fs::path& path, const
 	}
 	if (npts == 0 || ncells == 0 || offsets.size() != ncells || types.size() != ncells)
 		throw std::runtime_error("VTK mesh is not a complete ASCII unstructured grid: " + mesh_path.string());
-	if (s.size() != 4*npts) throw std::runtime_error("VTU point-array size mismatch");
+	for (const auto& a : arrays)
+		if (a.values.size() != static_cast<std::size_t>(a.components)*npts)
+			throw std::runtime_error("VTU point-array size mismatch");
 	if (!path.parent_path().empty()) fs::create_directories(path.parent_path());
 	std::ofstream out(path);
 	if (!out) throw std::runtime_error("cannot create VTU output: " + path.string());
@@ -949,13 +959,14 @@ inline void WriteStateVtu(const fs::path& mesh_path, const fs::path& path, const
 		<< "  <UnstructuredGrid>\n"
 		<< "    <FieldData><DataArray type=\"Float64\" Name=\"TimeValue\" NumberOfTuples=\"1\" format=\"ascii\">" << time
 		<< "</DataArray></FieldData>\n"
-		<< "    <Piece NumberOfPoints=\"" << npts << "\" NumberOfCells=\"" << ncells << "\">\n      <PointData>\n"
-		<< "        <DataArray type=\"Float64\" Name=\"velocity\" NumberOfComponents=\"3\" format=\"ascii\">\n          ";
-	for (std::size_t n = 0; n < npts; ++n) out << s[4*n] << ' ' << s[4*n + 1] << ' ' << s[4*n + 2] << ' ';
-	out << "\n        </DataArray>\n"
-		<< "        <DataArray type=\"Float64\" Name=\"pressure\" NumberOfComponents=\"1\" format=\"ascii\">\n          ";
-	for (std::size_t n = 0; n < npts; ++n) out << s[4*n + 3] << ' ';
-	out << "\n        </DataArray>\n      </PointData>\n"
+		<< "    <Piece NumberOfPoints=\"" << npts << "\" NumberOfCells=\"" << ncells << "\">\n      <PointData>\n";
+	for (const auto& a : arrays) {
+		out << "        <DataArray type=\"Float64\" Name=\"" << a.name << "\" NumberOfComponents=\"" << a.components
+			<< "\" format=\"ascii\">\n          ";
+		for (double v : a.values) out << v << ' ';
+		out << "\n        </DataArray>\n";
+	}
+	out << "      </PointData>\n"
 		<< "      <Points><DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n        ";
 	for (double v : points) out << v << ' ';
 	out << "\n      </DataArray></Points>\n      <Cells>\n"
@@ -968,6 +979,17 @@ inline void WriteStateVtu(const fs::path& mesh_path, const fs::path& path, const
 	out << "\n        </DataArray>\n      </Cells>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n";
 	out.close();
 	if (!out) throw std::runtime_error("cannot write VTU output: " + path.string());
+}
+
+inline void WriteStateVtu(const fs::path& mesh_path, const fs::path& path, const std::vector<double>& s, double time)
+{
+	const std::size_t n = s.size()/4;
+	PointArray velocity{"velocity", 3, std::vector<double>(3*n)}, pressure{"pressure", 1, std::vector<double>(n)};
+	for (std::size_t i = 0; i < n; ++i) {
+		for (int d = 0; d < 3; ++d) velocity.values[3*i + d] = s[4*i + d];
+		pressure.values[i] = s[4*i + 3];
+	}
+	WritePointArraysVtu(mesh_path, path, {velocity, pressure}, time);
 }
 
 // WritePvd (VtkOutput.hpp:175-190)

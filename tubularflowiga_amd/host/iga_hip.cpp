@@ -6,6 +6,7 @@
 // precision, warm start, tangent) and default to the reproducing settings.
 #include "case_input.hpp"
 #include "outlets.hpp"
+#include "solve_input.hpp"
 #include "transport_input.hpp"
 #include "tfi.h"
 
@@ -724,19 +725,254 @@ int Transport(Ctx& c, const TransportCase& t)
 	return 0;
 }
 
+struct SolveOptions {
+	fs::path database, case_dir, output, velocity, checkpoint, restart;
+	std::string system;
+	int output_every = 0, checkpoint_every = 0, stop_after_step = 0;
+	double linear_rtol = 1e-8;
+};
+
+// ParseCudaConfiguredTransportOptions (ConfiguredTransportHost.inc:13-48), plus
+// --linear-rtol R (the GMRES relative tolerance, 1e-8 as in :391-392)
+SolveOptions ParseSolveOptions(int argc, char** argv)
+{
+	if (argc < 4)
+		throw std::runtime_error("usage: iga_hip solve DATABASE.ntiga CASE_DIR [SYSTEM] [OUTPUT] [VELOCITY] "
+			"[--system NAME] [--output PATH] [--velocity PATH] [--output-every N] [--checkpoint PREFIX] "
+			"[--checkpoint-every N] [--restart PREFIX] [--stop-after-step N] [--visualization-format auto|vtu] "
+			"[--linear-rtol R]");
+	SolveOptions o;
+	o.database = argv[2];
+	o.case_dir = argv[3];
+	int positional = 0;
+	for (int i = 4; i < argc; ++i) {
+		const std::string arg(argv[i]);
+		if (arg.rfind("--", 0) != 0) {
+			if (positional == 0) o.system = arg;
+			else if (positional == 1) o.output = arg;
+			else if (positional == 2) o.velocity = arg;
+			else throw std::runtime_error("too many positional arguments");
+			++positional;
+			continue;
+		}
+		if (i + 1 >= argc) throw std::runtime_error(arg + " requires a value");
+		const std::string v(argv[++i]);
+		if (arg == "--system") o.system = v;
+		else if (arg == "--output") o.output = v;
+		else if (arg == "--velocity") o.velocity = v;
+		else if (arg == "--output-every") o.output_every = PositiveInt(v, arg);
+		else if (arg == "--checkpoint") o.checkpoint = v;
+		else if (arg == "--checkpoint-every") o.checkpoint_every = PositiveInt(v, arg);
+		else if (arg == "--restart") o.restart = v;
+		else if (arg == "--stop-after-step") o.stop_after_step = PositiveInt(v, arg);
+		else if (arg == "--visualization-format") {
+			if (v == "vtkhdf") throw std::runtime_error("--visualization-format vtkhdf is not available in iga_hip; use vtu");
+			if (v != "auto" && v != "vtu") throw std::runtime_error("--visualization-format must be auto, vtu, or vtkhdf");
+		} else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
+		else throw std::runtime_error("unknown option: " + arg);
+	}
+	if (o.output_every > 0 && o.output.empty()) throw std::runtime_error("--output-every requires --output");
+	if (o.checkpoint_every > 0 && o.checkpoint.empty()) throw std::runtime_error("--checkpoint-every requires --checkpoint");
+	return o;
+}
+
+// Inputs of the solve command, read (and validated) before the device is touched.
+struct SolveCase {
+	SolveOptions o;
+	Clock::time_point start;
+	FlatDatabase db;
+	LinearConfig config;
+	CompiledSystem system;
+	ScalarBoundaries bc;
+	std::vector<double> velocity;  // [3N]
+	std::vector<double> state;     // [N*F] initial or restart state
+	int start_step = 0, end_step = 0;
+};
+
+SolveCase LoadSolveCase(int argc, char** argv)
+{
+	SolveCase t;
+	t.o = ParseSolveOptions(argc, argv);
+	t.start = Clock::now();
+	t.db = ReadDatabase(t.o.database.string());
+	t.config = ReadLinearConfig((t.o.case_dir/"simulation_config.json").string());
+	t.system = CompileSystem(t.config, t.o.system);
+	const CompiledSystem& s = t.system;
+	if (s.velocity_source != "prescribed") {
+		if (!t.o.velocity.empty())
+			throw std::runtime_error("a positional VELOCITY file cannot override a configured snapshot_series");
+		throw std::runtime_error("velocity source '" + s.velocity_source + "' (" + t.config.velocity_source_kind[s.velocity_source]
+			+ ") is not supported by iga_hip solve; use a prescribed velocity");
+	}
+	if (t.o.stop_after_step > s.steps) throw std::runtime_error("--stop-after-step exceeds configured transport steps");
+	t.end_step = t.o.stop_after_step > 0 ? t.o.stop_after_step : s.steps;
+	const auto labeled = ReadLabeledHexMesh((t.o.case_dir/"controlmesh.vtk").string(), t.db.nodes,
+		static_cast<std::uint64_t>(t.db.elements()));
+	const auto vel = ReadVelocity((t.o.velocity.empty() ? t.o.case_dir/"initial_velocityfield.txt" : t.o.velocity).string(),
+		t.db.nodes);
+	t.velocity.resize(3*t.db.nodes);
+	for (std::size_t n = 0; n < t.db.nodes; ++n)
+		for (int d = 0; d < 3; ++d) t.velocity[3*n + d] = vel[n][d];
+	t.bc = ResolveScalarBoundaries(t.config, s, labeled.labels, t.db);
+	const std::size_t F = s.fields.size();
+	t.state.resize(t.db.nodes*F);
+	for (std::size_t n = 0; n < t.db.nodes; ++n)
+		for (std::size_t f = 0; f < F; ++f)
+			t.state[n*F + f] = t.bc.constrained[n*F + f] ? t.bc.value[n*F + f] : s.initial[f];
+	if (!t.o.restart.empty()) {
+		const TransportCheckpointMeta m = ReadTransportCheckpoint(t.o.restart, &t.state);
+		ValidateTransportCheckpoint(m, t.db.nodes, s);
+		t.start_step = m.completed_step;
+	}
+	return t;
+}
+
+// solve-check DATABASE CASE_DIR [options]: the compiled system and its
+// boundaries without a GPU (no reference counterpart)
+int SolveCheck(int argc, char** argv)
+{
+	const SolveCase t = LoadSolveCase(argc, argv);
+	const CompiledSystem& s = t.system;
+	std::cout << std::setprecision(17) << "solve_check system=" << s.name << " fields=";
+	for (std::size_t i = 0; i < s.fields.size(); ++i) std::cout << (i ? "," : "") << s.fields[i];
+	std::cout << " velocity_source=" << s.velocity_source << " dt=" << s.dt << " steps=" << s.steps
+		<< " terms=" << s.terms.size() << " constrained_dofs=" << t.bc.constrained_dofs
+		<< " surfaces=" << t.bc.surface_element.size() << '\n';
+	for (const auto& term : s.terms)
+		std::cout << "term kind=" << term.kind << " equation=" << term.equation << " trial=" << term.trial
+			<< " coefficient=" << term.coefficient << '\n';
+	for (std::size_t f = 0; f < s.fields.size(); ++f)
+		std::cout << "field index=" << f << " supg=" << s.supg[f] << " tau_diffusivity=" << s.tau_diffusivity[f]
+			<< " initial=" << s.initial[f] << '\n';
+	long long index_sum = 0;
+	double value_sum = 0.0;
+	for (std::size_t i = 0; i < t.bc.constrained.size(); ++i)
+		if (t.bc.constrained[i]) {
+			index_sum += static_cast<long long>(i);
+			value_sum += t.bc.value[i];
+		}
+	std::cout << "boundary constrained_index_sum=" << index_sum << " value_sum=" << value_sum << '\n';
+	return 0;
+}
+
+int Solve(Ctx& c, const SolveCase& t)
+{
+	const SolveOptions& o = t.o;
+	const FlatDatabase& db = t.db;
+	const CompiledSystem& s = t.system;
+	const int F = static_cast<int>(s.fields.size());
+	Mesh m;
+	CreateMesh(c, db, m);
+	tfi_mesh_stats st{};
+	c.Check(tfi_mesh_stats_get(m.mesh, &st), "mesh statistics");
+	PrintMesh(db, &st);
+	if (st.bad_samples)
+		throw std::runtime_error("mesh has " + std::to_string(st.bad_samples)
+			+ " non-positive quadrature Jacobians; minimum detJ " + std::to_string(st.minimum_determinant));
+	const auto preprocess_end = Clock::now();
+	if (!o.restart.empty())
+		std::cout << "restart=" << o.restart.string() << " completed_step=" << t.start_step
+			<< " physical_time=" << t.start_step*s.dt << '\n';
+	tfi_solve_system sys{};
+	sys.fields = F;
+	sys.term_count = static_cast<int>(s.terms.size());
+	sys.terms = s.terms.data();
+	sys.supg = s.supg.data();
+	sys.tau_diffusivity = s.tau_diffusivity.data();
+	sys.dt = s.dt;
+	sys.surface_count = static_cast<int>(t.bc.surface_element.size());
+	sys.surface_element = t.bc.surface_element.data();
+	sys.surface_face = t.bc.surface_face.data();
+	sys.surface_flux = t.bc.surface_flux.data();
+	sys.surface_robin = t.bc.surface_robin.data();
+	sys.surface_robin_source = t.bc.surface_robin_source.data();
+	struct Handle {
+		tfi_solve* s = nullptr;
+		~Handle() { tfi_solve_destroy(s); }
+	} h;
+	const auto assembly_start = Clock::now();
+	c.Check(tfi_solve_create(m.mesh, &sys, t.velocity.data(), t.bc.constrained.data(), &h.s), "configured transport assembly");
+	c.Check(tfi_solve_set_boundary(h.s, t.bc.value.data()), "configured transport boundary values");
+	c.Check(tfi_solve_set_state(h.s, t.state.data()), "configured transport state");
+	const double assembly_s = Seconds(assembly_start, Clock::now());
+	int singular = 0;
+	c.Check(tfi_solve_matrices(h.s, nullptr, nullptr, nullptr, &singular), "configured transport block inverses");
+	std::vector<double> state(t.state);
+	const fs::path mesh_vtk = o.case_dir/"controlmesh.vtk";
+	std::vector<std::pair<double, fs::path>> snapshots;
+	auto write_output = [&](int step) {
+		c.Check(tfi_solve_get_state(h.s, state.data()), "configured transport state download");
+		const fs::path vtu = VtuStepPath(o.output, step);
+		WriteConfiguredOutput(TimeIndexedPath(o.output, step), s.fields, state, mesh_vtk, vtu, step*s.dt);
+		snapshots.push_back({step*s.dt, vtu});
+	};
+	if (o.output_every > 0) write_output(t.start_step);
+	long long total_iterations = 0;
+	double linear_s = 0.0;
+	const auto solve_start = Clock::now();
+	for (int step = t.start_step; step < t.end_step; ++step) {
+		int it = 0;
+		double res = 0.0;
+		const auto ls = Clock::now();
+		if (tfi_solve_step(h.s, step == 0 ? 1 : 0, 10000, o.linear_rtol, &it, &res) != TFI_OK)
+			throw std::runtime_error("configured transport GMRES failed at step " + std::to_string(step) + ": "
+				+ tfi_last_error(c.ctx));
+		linear_s += Seconds(ls, Clock::now());
+		total_iterations += it;
+		const int completed = step + 1;
+		if (o.output_every > 0 && completed % o.output_every == 0) write_output(completed);
+		if (!o.checkpoint.empty()
+			&& (completed == t.end_step || (o.checkpoint_every > 0 && completed % o.checkpoint_every == 0))) {
+			c.Check(tfi_solve_get_state(h.s, state.data()), "configured transport state download");
+			TransportCheckpointMeta meta;
+			meta.nodes = db.nodes;
+			meta.fields = s.fields;
+			meta.system = s.name;
+			meta.velocity_source = s.velocity_source;
+			meta.completed_step = completed;
+			meta.physical_time = completed*s.dt;
+			meta.dt = s.dt;
+			WriteTransportCheckpoint(o.checkpoint, meta, state);
+			std::cout << "checkpoint=" << o.checkpoint.string() << " completed_step=" << completed << '\n';
+		}
+	}
+	const auto solve_end = Clock::now();
+	c.Check(tfi_solve_get_state(h.s, state.data()), "configured transport state download");
+	double norm = 0.0;
+	for (double v : state) norm += v*v;
+	norm = std::sqrt(norm);
+	if (!o.output.empty()) {
+		WriteConfiguredOutput(o.output, s.fields, state, mesh_vtk, VtuFinalPath(o.output), t.end_step*s.dt);
+		if (snapshots.empty()) snapshots.push_back({t.end_step*s.dt, VtuFinalPath(o.output)});
+		WritePvd(PvdPath(o.output), snapshots);
+	}
+	long long peak = 0, live = 0;
+	c.Check(tfi_allocation_stats(c.ctx, &peak, &live), "allocation statistics");
+	std::cout << "iga_cuda_solve system=" << s.name << " fields=" << F << " steps=" << s.steps
+		<< " run_end_step=" << t.end_step << " preprocess_s=" << Seconds(t.start, preprocess_end)
+		<< " assembly_s=" << assembly_s << " solve_s=" << linear_s
+		<< " time_loop_s=" << Seconds(solve_start, solve_end) << " total_iterations=" << total_iterations
+		<< " final_l2=" << norm << " singular_diagonal_blocks=" << singular << " gpu_used_gib=" << Gib(peak)
+		<< " backend=hip\n";
+	return 0;
+}
+
 int main(int argc, char** argv)
 {
 	std::unique_ptr<Ctx> ctx;
 	try {
 		if (argc < 2)
-			throw std::runtime_error("usage: iga_hip device-info|mesh-check|transport|navier-stokes|case-check ...");
+			throw std::runtime_error("usage: iga_hip device-info|mesh-check|transport|solve|navier-stokes|case-check ...");
 		const std::string command(argv[1]);
 		if (command == "case-check") return CaseCheck(argc, argv);
 		if (command == "transport-check") return TransportCheck(argc, argv);
+		if (command == "solve-check") return SolveCheck(argc, argv);
 		// usage errors before the device is touched
 		if (command == "navier-stokes") ParseOptions(argc, argv);
 		std::unique_ptr<TransportCase> transport_case;
 		if (command == "transport") transport_case = std::make_unique<TransportCase>(LoadTransportCase(argc, argv));
+		std::unique_ptr<SolveCase> solve_case;
+		if (command == "solve") solve_case = std::make_unique<SolveCase>(LoadSolveCase(argc, argv));
 		ctx = std::make_unique<Ctx>(0);
 		PrintDevice(*ctx);
 		if (command == "device-info") return 0;
@@ -747,6 +983,7 @@ int main(int argc, char** argv)
 		int status = 0;
 		if (command == "navier-stokes") status = NavierStokes(*ctx, argc, argv);
 		else if (command == "transport") status = Transport(*ctx, *transport_case);
+		else if (command == "solve") status = Solve(*ctx, *solve_case);
 		else throw std::runtime_error("unknown command: " + command);
 		PrintAllocations(std::cout, ctx->ctx);
 		return status;

@@ -1,0 +1,590 @@
+// Inputs of the configured linear transport (`iga_cuda solve`,
+// solvers/cuda/include/ConfiguredTransportHost.inc:474-503): the
+// linear_transport systems of simulation_config.json compiled to weak-form
+// terms (CompileLinearSystem, include/SimulationConfig.hpp:743-785), the SUPG
+// settings, the scalar boundaries (ResolveScalarBoundaries,
+// solvers/cpu/include/GenericCaseInput.hpp:63-141), the flux / Robin faces
+// (ConfiguredTransportHost.inc:204-255), text / VTU output and the
+// raw_float64 transport checkpoint (include/TransportCheckpoint.hpp).
+#pragma once
+
+#include "case_input.hpp"
+#include "tfi.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace tfi_host {
+
+struct ScalarFieldDef {
+	std::string name, kind = "scalar";
+	double initial_value = 0.0;
+};
+
+struct WeakTermDef {
+	std::string op, equation, trial, velocity;
+	double coefficient = 1.0;
+};
+
+struct StabilizationDef {
+	std::string equation, method, velocity, diffusion_weighting = "consistent", tau = "advective_diffusive";
+};
+
+struct LinearSystemDef {
+	std::string name;
+	std::vector<std::string> unknowns;
+	std::vector<WeakTermDef> terms;
+	std::vector<StabilizationDef> stabilization;
+	std::string time_integration = "backward_euler";
+};
+
+struct ScalarConditionDef {
+	std::string field, type, dirichlet_dofs = "face_support", waveform, profile;
+	std::vector<double> value;
+	double coefficient = 0.0, exterior_value = 0.0;
+};
+
+struct ScalarBoundaryDef {
+	int label = -1;
+	std::string name;
+	std::vector<ScalarConditionDef> conditions;
+};
+
+struct LinearConfig {
+	std::vector<ScalarFieldDef> fields;
+	std::vector<LinearSystemDef> systems;   // linear_transport systems in file order
+	std::vector<ScalarBoundaryDef> boundaries;
+	std::set<std::string> velocity_sources;
+	std::map<std::string, std::string> velocity_source_kind;
+	double dt = 0.0;
+	int steps = 0;
+	bool physiology = false;
+};
+
+// The parts of ParseSimulationConfiguration (include/SimulationConfig.hpp:229-717)
+// the linear_transport path reads, with its checks on them.
+inline LinearConfig ReadLinearConfig(const std::string& path)
+{
+	std::ifstream in(path);
+	if (!in) throw std::runtime_error("cannot open simulation configuration: " + path);
+	std::stringstream ss;
+	ss << in.rdbuf();
+	const Json root_value = ParseJson(ss.str());
+	const Json& root = RequireObject(root_value, "simulation_config.json");
+	RequireKnownKeys(root, {"schema_version", "dimension", "geometry", "mesh", "fields", "equation_systems",
+		"boundaries", "time", "temporal_functions", "velocity_sources", "physiology", "simulation_scope", "perfusate",
+		"external_circuit", "coupling"}, "simulation_config.json");
+	LinearConfig c;
+	const int schema = RequireInteger(Required(root, "schema_version", "root"), "schema_version");
+	if (schema < 2 || schema > 4)
+		throw std::runtime_error("simulation_config.json: supported schema versions are 2, 3, and 4");
+	std::set<std::string> names;
+	const auto& farr = RequireArray(Required(root, "fields", "root"), "fields").array;
+	for (std::size_t i = 0; i < farr.size(); ++i) {
+		const std::string ctx = "fields[" + std::to_string(i) + "]";
+		const Json& f = RequireObject(farr[i], ctx);
+		RequireKnownKeys(f, {"name", "kind", "initial_value"}, ctx);
+		ScalarFieldDef d;
+		d.name = RequireString(Required(f, "name", ctx), ctx + ".name");
+		d.kind = RequireString(Required(f, "kind", ctx), ctx + ".kind");
+		if (d.kind != "scalar" && d.kind != "vector3" && d.kind != "pressure")
+			throw std::runtime_error("simulation_config.json: unsupported field kind '" + d.kind + "'");
+		if (const Json* v = f.Find("initial_value")) d.initial_value = RequireNumber(*v, ctx + ".initial_value");
+		if (d.name.empty() || !names.insert(d.name).second)
+			throw std::runtime_error("simulation_config.json: field names must be non-empty and unique");
+		c.fields.push_back(d);
+	}
+	if (c.fields.empty()) throw std::runtime_error("simulation_config.json: fields cannot be empty");
+	const Json& time = RequireObject(Required(root, "time", "root"), "time");
+	RequireKnownKeys(time, {"dt", "steps"}, "time");
+	c.dt = RequireNumber(Required(time, "dt", "time"), "time.dt");
+	c.steps = RequireInteger(Required(time, "steps", "time"), "time.steps");
+	if (!(c.dt > 0.0)) throw std::runtime_error("simulation_config.json: time.dt must be positive");
+	if (const Json* vs = root.Find("velocity_sources")) {
+		const auto& arr = RequireArray(*vs, "velocity_sources").array;
+		for (std::size_t i = 0; i < arr.size(); ++i) {
+			const std::string ctx = "velocity_sources[" + std::to_string(i) + "]";
+			const Json& o = RequireObject(arr[i], ctx);
+			const std::string name = RequireString(Required(o, "name", ctx), ctx + ".name");
+			if (name.empty() || name == "prescribed" || !c.velocity_sources.insert(name).second)
+				throw std::runtime_error("simulation_config.json: velocity source names must be unique and not 'prescribed'");
+			c.velocity_source_kind[name] = RequireString(Required(o, "kind", ctx), ctx + ".kind");
+		}
+	}
+	if (const Json* ph = root.Find("physiology")) {
+		const Json& o = RequireObject(*ph, "physiology");
+		if (const Json* en = o.Find("enabled")) {
+			if (en->kind != Json::Kind::Bool) throw std::runtime_error("physiology.enabled must be a boolean");
+			c.physiology = en->boolean;
+		}
+	}
+	const auto& sarr = RequireArray(Required(root, "equation_systems", "root"), "equation_systems").array;
+	std::set<std::string> snames;
+	for (std::size_t i = 0; i < sarr.size(); ++i) {
+		const std::string ctx = "equation_systems[" + std::to_string(i) + "]";
+		const Json& o = RequireObject(sarr[i], ctx);
+		RequireKnownKeys(o, {"name", "kind", "unknowns", "terms", "stabilization", "viscosity", "density",
+			"time_integration"}, ctx);
+		const std::string name = RequireString(Required(o, "name", ctx), ctx + ".name");
+		if (name.empty() || !snames.insert(name).second)
+			throw std::runtime_error("simulation_config.json: equation-system names must be non-empty and unique");
+		const std::string kind = RequireString(Required(o, "kind", ctx), ctx + ".kind");
+		if (kind == "navier_stokes") continue;
+		if (kind != "linear_transport")
+			throw std::runtime_error("simulation_config.json: unsupported equation-system kind '" + kind + "'");
+		LinearSystemDef s;
+		s.name = name;
+		std::set<std::string> local;
+		const auto& unk = RequireArray(Required(o, "unknowns", ctx), ctx + ".unknowns").array;
+		for (std::size_t j = 0; j < unk.size(); ++j) {
+			const std::string f = RequireString(unk[j], ctx + ".unknowns[" + std::to_string(j) + "]");
+			if (!names.count(f)) throw std::runtime_error("simulation_config.json: unknown field '" + f + "'");
+			if (!local.insert(f).second) throw std::runtime_error("simulation_config.json: duplicate system unknown");
+			s.unknowns.push_back(f);
+		}
+		if (s.unknowns.empty()) throw std::runtime_error("simulation_config.json: system unknowns cannot be empty");
+		if (o.Find("density")) throw std::runtime_error("simulation_config.json: density applies only to navier_stokes");
+		if (const Json* v = o.Find("time_integration")) s.time_integration = RequireString(*v, ctx + ".time_integration");
+		if (s.time_integration != "backward_euler" && s.time_integration != "bdf2")
+			throw std::runtime_error("simulation_config.json: linear_transport time_integration must be 'backward_euler' or 'bdf2'");
+		if (const Json* terms = o.Find("terms")) {
+			const auto& arr = RequireArray(*terms, ctx + ".terms").array;
+			for (std::size_t j = 0; j < arr.size(); ++j) {
+				const std::string tctx = ctx + ".terms[" + std::to_string(j) + "]";
+				const Json& t = RequireObject(arr[j], tctx);
+				RequireKnownKeys(t, {"operator", "equation", "trial", "coefficient", "velocity"}, tctx);
+				WeakTermDef w;
+				w.op = RequireString(Required(t, "operator", tctx), tctx + ".operator");
+				static const std::set<std::string> ops = {"time_derivative", "diffusion", "advection", "linear_coupling",
+					"volume_source"};
+				if (!ops.count(w.op)) throw std::runtime_error("simulation_config.json: unknown term operator '" + w.op + "'");
+				w.equation = RequireString(Required(t, "equation", tctx), tctx + ".equation");
+				w.trial = w.equation;
+				if (const Json* v = t.Find("trial")) w.trial = RequireString(*v, tctx + ".trial");
+				if (const Json* v = t.Find("coefficient")) w.coefficient = RequireNumber(*v, tctx + ".coefficient");
+				if (const Json* v = t.Find("velocity")) w.velocity = RequireString(*v, tctx + ".velocity");
+				if (!local.count(w.equation) || !local.count(w.trial))
+					throw std::runtime_error("simulation_config.json: term equation and trial must be unknowns of their system");
+				if (w.op == "advection" && w.velocity.empty())
+					throw std::runtime_error("simulation_config.json: advection requires a velocity source");
+				if (w.op == "advection" && w.velocity != "prescribed" && !c.velocity_sources.count(w.velocity))
+					throw std::runtime_error("simulation_config.json: advection references an unknown velocity source");
+				s.terms.push_back(w);
+			}
+		}
+		if (s.terms.empty()) throw std::runtime_error("simulation_config.json: linear_transport requires terms");
+		if (const Json* st = o.Find("stabilization")) {
+			const auto& arr = RequireArray(*st, ctx + ".stabilization").array;
+			for (std::size_t j = 0; j < arr.size(); ++j) {
+				const std::string ictx = ctx + ".stabilization[" + std::to_string(j) + "]";
+				const Json& it = RequireObject(arr[j], ictx);
+				RequireKnownKeys(it, {"equation", "method", "velocity", "diffusion_weighting", "tau"}, ictx);
+				StabilizationDef d;
+				d.equation = RequireString(Required(it, "equation", ictx), ictx + ".equation");
+				d.method = RequireString(Required(it, "method", ictx), ictx + ".method");
+				d.velocity = RequireString(Required(it, "velocity", ictx), ictx + ".velocity");
+				if (const Json* v = it.Find("diffusion_weighting"))
+					d.diffusion_weighting = RequireString(*v, ictx + ".diffusion_weighting");
+				if (d.diffusion_weighting != "consistent" && d.diffusion_weighting != "galerkin"
+					&& d.diffusion_weighting != "supg_test")
+					throw std::runtime_error("simulation_config.json: stabilization diffusion_weighting must be 'consistent', 'galerkin' or 'supg_test'");
+				if (const Json* v = it.Find("tau")) d.tau = RequireString(*v, ictx + ".tau");
+				if (d.tau != "advective_diffusive" && d.tau != "advective")
+					throw std::runtime_error("simulation_config.json: stabilization tau must be 'advective_diffusive' or 'advective'");
+				if (!local.count(d.equation) || d.method != "supg")
+					throw std::runtime_error("simulation_config.json: stabilization requires a system equation and supported method 'supg'");
+				if (d.velocity != "prescribed" && !c.velocity_sources.count(d.velocity))
+					throw std::runtime_error("simulation_config.json: stabilization references an unknown velocity source");
+				s.stabilization.push_back(d);
+			}
+		}
+		c.systems.push_back(s);
+	}
+	if (sarr.empty()) throw std::runtime_error("simulation_config.json: equation_systems cannot be empty");
+	const auto& barr = RequireArray(Required(root, "boundaries", "root"), "boundaries").array;
+	std::set<int> labels;
+	for (std::size_t i = 0; i < barr.size(); ++i) {
+		const std::string ctx = "boundaries[" + std::to_string(i) + "]";
+		const Json& o = RequireObject(barr[i], ctx);
+		RequireKnownKeys(o, {"label", "name", "conditions"}, ctx);
+		ScalarBoundaryDef b;
+		b.label = RequireInteger(Required(o, "label", ctx), ctx + ".label");
+		b.name = RequireString(Required(o, "name", ctx), ctx + ".name");
+		if (!labels.insert(b.label).second) throw std::runtime_error("simulation_config.json: duplicate boundary label");
+		const auto& carr = RequireArray(Required(o, "conditions", ctx), ctx + ".conditions").array;
+		std::set<std::string> conditioned;
+		for (std::size_t j = 0; j < carr.size(); ++j) {
+			const std::string cctx = ctx + ".conditions[" + std::to_string(j) + "]";
+			const Json& co = RequireObject(carr[j], cctx);
+			RequireKnownKeys(co, {"field", "type", "value", "coefficient", "exterior_value", "profile", "scale", "waveform",
+				"resistance", "proximal_resistance", "distal_resistance", "capacitance", "reference_pressure",
+				"initial_pressure", "pressure_gauge", "dirichlet_dofs"}, cctx);
+			ScalarConditionDef d;
+			d.field = RequireString(Required(co, "field", cctx), cctx + ".field");
+			d.type = RequireString(Required(co, "type", cctx), cctx + ".type");
+			static const std::set<std::string> kinds = {"dirichlet", "no_flux", "flux", "robin", "advective_outflow",
+				"pressure_traction", "resistance", "windkessel_rc", "windkessel_rcr"};
+			if (!kinds.count(d.type)) throw std::runtime_error("simulation_config.json: unknown boundary type '" + d.type + "'");
+			if (!names.count(d.field) || !conditioned.insert(d.field).second)
+				throw std::runtime_error("simulation_config.json: boundary fields must exist and be unique per boundary");
+			if (const Json* v = co.Find("value")) d.value = ValueVector(*v, cctx + ".value");
+			if (const Json* v = co.Find("coefficient")) d.coefficient = RequireNumber(*v, cctx + ".coefficient");
+			if (const Json* v = co.Find("exterior_value")) d.exterior_value = RequireNumber(*v, cctx + ".exterior_value");
+			if (const Json* v = co.Find("dirichlet_dofs")) d.dirichlet_dofs = RequireString(*v, cctx + ".dirichlet_dofs");
+			if (d.dirichlet_dofs != "face_support" && d.dirichlet_dofs != "node_label")
+				throw std::runtime_error("simulation_config.json: dirichlet_dofs must be 'face_support' or 'node_label'");
+			if (const Json* v = co.Find("profile")) d.profile = RequireString(*v, cctx + ".profile");
+			if (const Json* v = co.Find("waveform")) d.waveform = RequireString(*v, cctx + ".waveform");
+			if (d.type == "dirichlet" && d.value.empty() && d.profile.empty())
+				throw std::runtime_error("simulation_config.json: dirichlet requires value or profile");
+			b.conditions.push_back(d);
+		}
+		c.boundaries.push_back(b);
+	}
+	return c;
+}
+
+// CompiledLinearSystem (include/SimulationConfig.hpp:167-185, 743-785) in the
+// form the C-ABI takes
+struct CompiledSystem {
+	std::string name, velocity_source = "prescribed", time_integration;
+	std::vector<std::string> fields;
+	std::map<std::string, int> index;
+	std::vector<tfi_solve_term> terms;
+	std::vector<std::int32_t> supg;
+	std::vector<double> tau_diffusivity, initial;
+	double dt = 0.0;
+	int steps = 0;
+};
+
+inline CompiledSystem CompileSystem(const LinearConfig& c, const std::string& requested)
+{
+	if (c.physiology)
+		throw std::runtime_error("physiology-enabled configurations (metabolism sources and derived outputs) are not "
+			"supported by this build");
+	if (c.systems.empty()) throw std::runtime_error("simulation configuration has no linear_transport equation system");
+	const LinearSystemDef* s = requested.empty() ? &c.systems.front() : nullptr;
+	for (const auto& it : c.systems)
+		if (!s && it.name == requested) s = &it;
+	if (!s) throw std::runtime_error("unknown equation system '" + requested + "'");
+	CompiledSystem r;
+	r.name = s->name;
+	r.fields = s->unknowns;
+	r.dt = c.dt;
+	r.steps = c.steps;
+	r.time_integration = s->time_integration;
+	if (r.time_integration != "backward_euler")
+		throw std::runtime_error("CUDA transport supports backward_euler only; use iga_solve for bdf2");
+	std::set<std::string> velocities;
+	for (const auto& t : s->terms)
+		if (t.op == "advection") velocities.insert(t.velocity);
+	for (const auto& d : s->stabilization) velocities.insert(d.velocity);
+	if (velocities.size() > 1)
+		throw std::runtime_error("linear_transport currently supports one velocity source per equation system");
+	if (!velocities.empty()) r.velocity_source = *velocities.begin();
+	for (std::size_t i = 0; i < r.fields.size(); ++i) r.index[r.fields[i]] = static_cast<int>(i);
+	if (r.fields.size() > 8) throw std::runtime_error("CUDA supports 1 through 8 coupled scalar fields per system");
+	static const std::map<std::string, int> kind = {{"time_derivative", TFI_TERM_TIME_DERIVATIVE},
+		{"diffusion", TFI_TERM_DIFFUSION}, {"advection", TFI_TERM_ADVECTION},
+		{"linear_coupling", TFI_TERM_LINEAR_COUPLING}, {"volume_source", TFI_TERM_VOLUME_SOURCE}};
+	for (const auto& t : s->terms) {
+		if (t.op == "advection" && t.velocity != r.velocity_source)
+			throw std::runtime_error("CUDA transport received inconsistent velocity sources");
+		r.terms.push_back({kind.at(t.op), r.index.at(t.equation), r.index.at(t.trial), t.coefficient});
+	}
+	if (r.terms.size() > 64) throw std::runtime_error("configured transport supports at most 64 weak-form terms");
+	const std::size_t F = r.fields.size();
+	r.supg.assign(F, TFI_SUPG_NONE);
+	r.tau_diffusivity.assign(F, 0.0);
+	for (const auto& d : s->stabilization) {
+		if (d.method != "supg" || d.velocity != r.velocity_source)
+			throw std::runtime_error("CUDA transport requires SUPG and advection to use the same velocity source");
+		r.supg[r.index.at(d.equation)] = d.diffusion_weighting == "supg_test" ? TFI_SUPG_TEST
+			: d.diffusion_weighting == "consistent" ? TFI_SUPG_CONSISTENT : TFI_SUPG_GALERKIN;
+		// SupgTauDiffusivity (SimulationConfig.hpp:728-741)
+		if (d.tau != "advective_diffusive") continue;
+		const int f = r.index.at(d.equation);
+		for (const auto& t : r.terms)
+			if (t.kind == TFI_TERM_DIFFUSION && t.equation == f && t.trial == f) r.tau_diffusivity[f] += t.coefficient;
+	}
+	r.initial.assign(F, 0.0);
+	for (std::size_t i = 0; i < F; ++i)
+		for (const auto& f : c.fields)
+			if (f.name == r.fields[i]) r.initial[i] = f.initial_value;
+	return r;
+}
+
+// NodeFaceLabels (GenericCaseInput.hpp:38-61): labels of the element faces on
+// which each node's basis has a nonzero trace
+inline std::vector<std::vector<int>> NodeFaceLabels(const FlatDatabase& db)
+{
+	static const int fixed_axis[6] = {2, 1, 0, 1, 0, 2};
+	static const int fixed_index[6] = {0, 0, 3, 3, 0, 3};
+	std::vector<std::vector<int>> out(db.nodes);
+	for (int e = 0; e < db.elements(); ++e)
+		for (int f = 0; f < 6; ++f) {
+			const int label = db.labels[6*static_cast<std::size_t>(e) + f];
+			if (label < 0) continue;
+			for (int a = db.element_offsets[e]; a < db.element_offsets[e + 1]; ++a)
+				for (int k = db.extraction_offsets[a]; k < db.extraction_offsets[a + 1]; ++k) {
+					const int p = db.extraction_columns[k];
+					const int ijk[3] = {p % 4, (p/4) % 4, p/16};
+					if (ijk[fixed_axis[f]] != fixed_index[f]) continue;
+					auto& l = out.at(static_cast<std::size_t>(db.connectivity[a]));
+					if (std::find(l.begin(), l.end(), label) == l.end()) l.push_back(label);
+					break;
+				}
+		}
+	return out;
+}
+
+struct ScalarBoundaries {
+	std::vector<std::int32_t> constrained;
+	std::vector<double> value;
+	long long constrained_dofs = 0;
+	// flux / Robin faces
+	std::vector<std::int32_t> surface_element, surface_face;
+	std::vector<double> surface_flux, surface_robin, surface_robin_source;
+};
+
+// ResolveScalarBoundaries (GenericCaseInput.hpp:63-141) with the face-support
+// Dirichlet entries, then the flux / Robin faces (ConfiguredTransportHost.inc:204-255)
+inline ScalarBoundaries ResolveScalarBoundaries(const LinearConfig& c, const CompiledSystem& s,
+	const std::vector<int>& labels, const FlatDatabase& db)
+{
+	for (const auto& name : s.fields) {
+		const auto it = std::find_if(c.fields.begin(), c.fields.end(), [&](const ScalarFieldDef& f) { return f.name == name; });
+		if (it == c.fields.end() || it->kind != "scalar")
+			throw std::runtime_error("linear_transport currently requires scalar fields: " + name);
+	}
+	const std::size_t F = s.fields.size();
+	ScalarBoundaries r;
+	r.constrained.assign(labels.size()*F, 0);
+	r.value.assign(labels.size()*F, 0.0);
+	std::set<int> mesh_labels, configured;
+	for (int l : labels)
+		if (l >= 0) mesh_labels.insert(l);
+	for (const auto& b : c.boundaries) {
+		if (!mesh_labels.count(b.label))
+			throw std::runtime_error("simulation_config.json boundary label " + std::to_string(b.label)
+				+ " is not present in controlmesh.vtk");
+		configured.insert(b.label);
+		for (const auto& cond : b.conditions) {
+			const auto f = s.index.find(cond.field);
+			if (f == s.index.end()) continue;
+			if (!cond.waveform.empty()) throw std::runtime_error("time-dependent scalar boundary execution is not implemented");
+			if (cond.type == "no_flux" || cond.type == "advective_outflow" || cond.type == "flux" || cond.type == "robin")
+				continue;
+			if (cond.type != "dirichlet")
+				throw std::runtime_error("unsupported scalar boundary condition for field '" + cond.field + "'");
+			if (cond.value.size() != 1)
+				throw std::runtime_error("scalar Dirichlet field '" + cond.field + "' requires one value");
+			for (std::size_t n = 0; n < labels.size(); ++n)
+				if (labels[n] == b.label) {
+					r.constrained[n*F + f->second] = 1;
+					r.value[n*F + f->second] = cond.value[0];
+				}
+		}
+	}
+	for (int l : mesh_labels)
+		if (!configured.count(l))
+			throw std::runtime_error("mesh boundary label " + std::to_string(l) + " has no simulation_config.json boundary definition");
+	const auto touched = NodeFaceLabels(db);
+	std::set<std::pair<int, int>> labelled;
+	for (const auto& b : c.boundaries)
+		for (const auto& cond : b.conditions) {
+			const auto f = s.index.find(cond.field);
+			if (f != s.index.end() && cond.type == "dirichlet") labelled.insert({b.label, f->second});
+		}
+	for (const auto& b : c.boundaries)
+		for (const auto& cond : b.conditions) {
+			const auto f = s.index.find(cond.field);
+			if (f == s.index.end() || cond.type != "dirichlet" || cond.dirichlet_dofs != "face_support") continue;
+			for (std::size_t n = 0; n < labels.size(); ++n) {
+				const std::size_t dof = n*F + f->second;
+				const auto& t = touched[n];
+				if (r.constrained[dof] || labelled.count({labels[n], f->second})
+					|| std::find(t.begin(), t.end(), b.label) == t.end())
+					continue;
+				r.constrained[dof] = 1;
+				r.value[dof] = cond.value[0];
+			}
+		}
+	r.constrained_dofs = std::count(r.constrained.begin(), r.constrained.end(), 1);
+	// flux / Robin faces
+	std::vector<int> required, found;
+	for (const auto& b : c.boundaries)
+		for (const auto& cond : b.conditions)
+			if (s.index.count(cond.field) && (cond.type == "flux" || cond.type == "robin")
+				&& std::find(required.begin(), required.end(), b.label) == required.end())
+				required.push_back(b.label);
+	for (int e = 0; e < db.elements(); ++e)
+		for (int face = 0; face < 6; ++face) {
+			const int label = db.labels[6*static_cast<std::size_t>(e) + face];
+			if (label < 0) continue;
+			const auto b = std::find_if(c.boundaries.begin(), c.boundaries.end(),
+				[&](const ScalarBoundaryDef& x) { return x.label == label; });
+			if (b == c.boundaries.end())
+				throw std::runtime_error("element boundary face has no simulation_config.json definition");
+			std::vector<double> flux(F, 0.0), robin(F, 0.0), source(F, 0.0);
+			bool active = false;
+			for (const auto& cond : b->conditions) {
+				const auto f = s.index.find(cond.field);
+				if (f == s.index.end()) continue;
+				if (cond.type == "flux") {
+					if (cond.value.empty()) throw std::runtime_error("scalar flux field '" + cond.field + "' requires a value");
+					flux[f->second] = cond.value[0];
+					active = true;
+				} else if (cond.type == "robin") {
+					robin[f->second] = cond.coefficient;
+					source[f->second] = cond.coefficient*cond.exterior_value;
+					active = true;
+				}
+			}
+			if (!active) continue;
+			if (std::find(found.begin(), found.end(), label) == found.end()) found.push_back(label);
+			r.surface_element.push_back(e);
+			r.surface_face.push_back(face);
+			r.surface_flux.insert(r.surface_flux.end(), flux.begin(), flux.end());
+			r.surface_robin.insert(r.surface_robin.end(), robin.begin(), robin.end());
+			r.surface_robin_source.insert(r.surface_robin_source.end(), source.begin(), source.end());
+		}
+	for (int l : required)
+		if (std::find(found.begin(), found.end(), l) == found.end())
+			throw std::runtime_error("configured scalar surface boundary label " + std::to_string(l)
+				+ " has no boundary faces in the .ntiga database; repack with iga_pack");
+	return r;
+}
+
+// WriteCudaConfiguredOutput (ConfiguredTransportHost.inc:96-140): "node v0 v1 ..."
+// lines at 17 digits, the field names in PATH.fields, and a VTU with one
+// point array per field
+inline void WriteConfiguredOutput(const fs::path& path, const std::vector<std::string>& fields,
+	const std::vector<double>& state, const fs::path& mesh_vtk, const fs::path& vtu, double time)
+{
+	const std::size_t F = fields.size(), nodes = state.size()/F;
+	if (!path.parent_path().empty()) fs::create_directories(path.parent_path());
+	std::ofstream values(path), meta(path.string() + ".fields");
+	if (!values || !meta) throw std::runtime_error("cannot create configured CUDA output");
+	values << std::setprecision(17);
+	for (const auto& name : fields) meta << name << '\n';
+	for (std::size_t n = 0; n < nodes; ++n) {
+		values << n;
+		for (std::size_t f = 0; f < F; ++f) values << ' ' << state[n*F + f];
+		values << '\n';
+	}
+	if (!values || !meta) throw std::runtime_error("cannot write configured CUDA output");
+	std::vector<PointArray> arrays;
+	for (std::size_t f = 0; f < F; ++f) {
+		PointArray a{fields[f], 1, std::vector<double>(nodes)};
+		for (std::size_t n = 0; n < nodes; ++n) a.values[n] = state[n*F + f];
+		arrays.push_back(std::move(a));
+	}
+	WritePointArraysVtu(mesh_vtk, vtu, arrays, time);
+}
+
+// raw_float64 transport checkpoint, schema 1 (include/TransportCheckpoint.hpp:18-219)
+struct TransportCheckpointMeta {
+	std::uint64_t nodes = 0;
+	std::vector<std::string> fields;
+	std::string system, velocity_source, state_file, state_format = "raw_float64";
+	int schema_version = 1, completed_step = 0;
+	double physical_time = 0.0, dt = 0.0;
+};
+
+inline void WriteTransportCheckpoint(const fs::path& prefix, TransportCheckpointMeta m, const std::vector<double>& state)
+{
+	const fs::path state_path = prefix.string() + ".state";
+	{
+		std::ofstream out(state_path, std::ios::binary);
+		if (!out) throw std::runtime_error("cannot create CUDA transport checkpoint state: " + state_path.string());
+		out.write(reinterpret_cast<const char*>(state.data()), static_cast<std::streamsize>(state.size()*sizeof(double)));
+		out.close();
+		if (!out) throw std::runtime_error("cannot write CUDA transport checkpoint state: " + state_path.string());
+	}
+	m.state_file = state_path.filename().string();
+	const fs::path meta_path = prefix.string() + ".json";
+	std::ofstream out(meta_path);
+	if (!out) throw std::runtime_error("cannot create transport checkpoint metadata: " + meta_path.string());
+	out << std::setprecision(17) << "{\n  \"schema_version\": 1,\n  \"nodes\": " << m.nodes << ",\n  \"fields\": [";
+	for (std::size_t i = 0; i < m.fields.size(); ++i) out << (i ? ", " : "") << '"' << JsonEscape(m.fields[i]) << '"';
+	out << "],\n  \"system\": \"" << JsonEscape(m.system) << "\",\n  \"velocity_source\": \"" << JsonEscape(m.velocity_source)
+		<< "\",\n  \"completed_step\": " << m.completed_step << ",\n  \"physical_time\": " << m.physical_time
+		<< ",\n  \"dt\": " << m.dt << ",\n  \"state_file\": \"" << JsonEscape(m.state_file)
+		<< "\",\n  \"state_format\": \"raw_float64\"\n}\n";
+	if (!out) throw std::runtime_error("cannot write transport checkpoint metadata: " + meta_path.string());
+}
+
+inline TransportCheckpointMeta ReadTransportCheckpoint(const fs::path& prefix, std::vector<double>* state)
+{
+	const fs::path meta_path = prefix.string() + ".json";
+	std::ifstream in(meta_path);
+	if (!in) throw std::runtime_error("cannot open transport checkpoint metadata: " + meta_path.string());
+	std::stringstream ss;
+	ss << in.rdbuf();
+	const Json v = ParseJson(ss.str());
+	const Json& root = RequireObject(v, "transport checkpoint");
+	RequireKnownKeys(root, {"schema_version", "nodes", "fields", "system", "velocity_source", "completed_step",
+		"physical_time", "dt", "state_file", "state_format", "time_integration", "history_file"}, "transport checkpoint");
+	auto req = [&](const char* key) -> const Json& {
+		const Json* x = root.Find(key);
+		if (!x) throw std::runtime_error(std::string("transport checkpoint requires '") + key + "'");
+		return *x;
+	};
+	TransportCheckpointMeta m;
+	m.schema_version = RequireInteger(req("schema_version"), "transport checkpoint.schema_version");
+	const double nodes = RequireNumber(req("nodes"), "transport checkpoint.nodes");
+	if (!(nodes > 0.0) || std::floor(nodes) != nodes)
+		throw std::runtime_error("transport checkpoint.nodes must be a positive integer");
+	m.nodes = static_cast<std::uint64_t>(nodes);
+	for (const auto& f : RequireArray(req("fields"), "transport checkpoint.fields").array)
+		m.fields.push_back(RequireString(f, "transport checkpoint.fields[]"));
+	m.system = RequireString(req("system"), "transport checkpoint.system");
+	m.velocity_source = RequireString(req("velocity_source"), "transport checkpoint.velocity_source");
+	m.completed_step = RequireInteger(req("completed_step"), "transport checkpoint.completed_step");
+	m.physical_time = RequireNumber(req("physical_time"), "transport checkpoint.physical_time");
+	m.dt = RequireNumber(req("dt"), "transport checkpoint.dt");
+	m.state_file = RequireString(req("state_file"), "transport checkpoint.state_file");
+	m.state_format = RequireString(req("state_format"), "transport checkpoint.state_format");
+	if (m.schema_version == 2) throw std::runtime_error("CUDA transport restart supports backward_euler checkpoints only");
+	if (m.schema_version != 1 || m.fields.empty() || m.system.empty() || m.velocity_source.empty()
+		|| m.completed_step < 0 || m.physical_time < 0.0 || !(m.dt > 0.0) || m.state_file.empty()
+		|| (m.state_format != "petsc_binary" && m.state_format != "raw_float64"))
+		throw std::runtime_error("transport checkpoint metadata contains invalid values");
+	if (m.state_format != "raw_float64") throw std::runtime_error("CUDA transport restart requires raw_float64 state");
+	fs::path path(m.state_file);
+	if (path.is_relative()) path = meta_path.parent_path()/path;
+	std::ifstream bin(path, std::ios::binary);
+	if (!bin) throw std::runtime_error("cannot open CUDA transport checkpoint state: " + path.string());
+	state->assign(m.nodes*m.fields.size(), 0.0);
+	bin.read(reinterpret_cast<char*>(state->data()), static_cast<std::streamsize>(state->size()*sizeof(double)));
+	char extra = 0;
+	if (!bin || bin.read(&extra, 1)) throw std::runtime_error("CUDA transport checkpoint state is truncated or has trailing data");
+	return m;
+}
+
+// ValidateTransportCheckpoint (TransportCheckpoint.hpp:190-216)
+inline void ValidateTransportCheckpoint(const TransportCheckpointMeta& m, std::uint64_t nodes, const CompiledSystem& s)
+{
+	auto close = [](double a, double b) { return std::abs(a - b) <= 1e-12*std::max({1.0, std::abs(a), std::abs(b)}); };
+	if (m.nodes != nodes) throw std::runtime_error("transport checkpoint node count does not match database");
+	if (m.fields != s.fields) throw std::runtime_error("transport checkpoint fields do not match equation system");
+	if (m.system != s.name) throw std::runtime_error("transport checkpoint system does not match configuration");
+	if (m.velocity_source != s.velocity_source)
+		throw std::runtime_error("transport checkpoint velocity source does not match configuration");
+	if (m.completed_step > s.steps) throw std::runtime_error("transport checkpoint is beyond configured final step");
+	if (!close(m.physical_time, m.completed_step*s.dt))
+		throw std::runtime_error("transport checkpoint physical time is inconsistent with completed step");
+	if (!close(m.dt, s.dt)) throw std::runtime_error("transport checkpoint dt does not match configuration");
+}
+
+}  // namespace tfi_host
