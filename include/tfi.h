@@ -350,6 +350,10 @@ int tfi_solve_step(tfi_solve* solve, int zero_guess, int maximum_iterations, dou
  * blocks), the source vector [N*fields] and the count of diagonal blocks
  * inverted by their diagonal (any pointer may be NULL). */
 int tfi_solve_matrices(tfi_solve* solve, double* left, double* previous, double* source, int* singular_blocks);
+/* Device-resident timing of the F x F SpMV (y = L x on the current state)
+ * with HIP events around `repetitions` back-to-back launches: average
+ * milliseconds per launch.  No reference counterpart. */
+int tfi_solve_bench_spmv(tfi_solve* solve, int repetitions, double* milliseconds);
 
 /* Kernel timers accumulated on the context stream (HIP events). */
 typedef struct {

@@ -154,6 +154,7 @@ SIGNATURES = {
     "tfi_solve_get_state": (c_int, [c_void_p, _d]),
     "tfi_solve_step": (c_int, [c_void_p, c_int, c_int, c_double, P(c_int), P(c_double)]),
     "tfi_solve_matrices": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, P(c_int)]),
+    "tfi_solve_bench_spmv": (c_int, [c_void_p, c_int, P(c_double)]),
     "tfi_timers_get": (c_int, [c_void_p, P(KernelTimers)]),
     "tfi_timers_enable": (c_int, [c_void_p, c_int]),
     "tfi_timers_reset": (c_int, [c_void_p]),

@@ -764,8 +764,8 @@ __global__ void FaceTractionKernel(int count, const int* __restrict__ felem, con
 // with m = w_i w_j / 4 |t_0 x t_1| at the 4x4 Gauss points of the face,
 // mass[f][a][b] = sum_pt N_a N_b m and load[f][a] = sum_pt N_a m.  |t_0 x t_1|
 // is the length of the cofactor vector FaceTractionKernel integrates.  One
-// 64-lane block per face; stride kMaxBasis in both outputs.
-__global__ void FaceMassKernel(int count, const int* __restrict__ felem, const int* __restrict__ fface,
+// 64-lane block per face; `stride` (>= the element's basis count) in both outputs.
+__global__ void FaceMassKernel(int count, int stride, const int* __restrict__ felem, const int* __restrict__ fface,
 	const int* __restrict__ eoff, const int* __restrict__ xoff, const uint8_t* __restrict__ xcol,
 	const double* __restrict__ xval, const double* __restrict__ bezier, double* __restrict__ mass,
 	double* __restrict__ load)
@@ -823,17 +823,17 @@ __global__ void FaceMassKernel(int count, const int* __restrict__ felem, const i
 		basis[pt][a] = N;
 	}
 	__syncthreads();
-	double* mf = mass + (size_t)f*kMaxBasis*kMaxBasis;
+	double* mf = mass + (size_t)f*stride*stride;
 	for (int pair = lane; pair < nen*nen; pair += 64) {
 		const int a = pair/nen, b = pair - a*nen;
 		double s = 0.0;
 		for (int pt = 0; pt < 16; ++pt) s += basis[pt][a]*basis[pt][b]*measure[pt];
-		mf[a*kMaxBasis + b] = s;
+		mf[a*stride + b] = s;
 	}
 	for (int a = lane; a < nen; a += 64) {
 		double s = 0.0;
 		for (int pt = 0; pt < 16; ++pt) s += basis[pt][a]*measure[pt];
-		load[(size_t)f*kMaxBasis + a] = s;
+		load[(size_t)f*stride + a] = s;
 	}
 }
 
@@ -1115,7 +1115,8 @@ void FaceFlowsFromVectors(tfi_mesh* m, int count, const int* element, const doub
 void FaceMass(tfi_mesh* m, int count, const int* element, const int* face, double* mass, double* load)
 {
 	if (count <= 0) return;
-	hipLaunchKernelGGL(FaceMassKernel, dim3(count), dim3(64), 0, m->ctx->stream, count, element, face,
+	if (m->max_nen > kMaxBasis) throw Error("face integrals support at most 80 basis functions");
+	hipLaunchKernelGGL(FaceMassKernel, dim3(count), dim3(64), 0, m->ctx->stream, count, m->max_nen, element, face,
 		m->eoff.data(), m->xoff.data(), m->xcol.data(), m->xval.data(), m->bezier.data(), mass, load);
 	TFI_LAUNCH_CHECK("FaceMassKernel");
 }

@@ -229,8 +229,8 @@ void FaceFlowsFromVectors(tfi_mesh* mesh, int count, const int* element, const d
 	double* flows);
 void FaceTraction(tfi_mesh* mesh, int count, const int* element, const int* face, const double* pressure,
 	double* local);
-// Per-face surface mass[count][kMaxBasis][kMaxBasis] = int_face N_a N_b dA and
-// load[count][kMaxBasis] = int_face N_a dA (device), 4x4 Gauss rule.
+// Per-face surface mass[count][S][S] = int_face N_a N_b dA and
+// load[count][S] = int_face N_a dA (device), S = mesh->max_nen, 4x4 Gauss rule.
 void FaceMass(tfi_mesh* mesh, int count, const int* element, const int* face, double* mass, double* load);
 
 // ---- sparse (tfi_sparse.hip)

@@ -213,8 +213,8 @@ template <int F>
 __global__ void __launch_bounds__(kThreads) SolveSurfaceKernel(int count, const int* __restrict__ faces,
 	const int* __restrict__ felem, const int* __restrict__ eoff, const int* __restrict__ conn,
 	const long long* __restrict__ pair_offsets, const int* __restrict__ pair_block, const double* __restrict__ mass,
-	const double* __restrict__ load, const double* __restrict__ robin, const double* __restrict__ flux_total,
-	double dt, double* __restrict__ left, double* __restrict__ source)
+	const double* __restrict__ load, int stride, const double* __restrict__ robin,
+	const double* __restrict__ flux_total, double dt, double* __restrict__ left, double* __restrict__ source)
 {
 	if ((int)blockIdx.x >= count) return;
 	const int f = faces[blockIdx.x];
@@ -224,11 +224,11 @@ __global__ void __launch_bounds__(kThreads) SolveSurfaceKernel(int count, const 
 	const long long pbase = pair_offsets[e];
 	for (int pair = threadIdx.x; pair < nen*nen; pair += kThreads) {
 		const int a = pair/nen, b = pair - a*nen;
-		const double m = mass[((size_t)f*kMaxBasis + a)*kMaxBasis + b];
+		const double m = mass[((size_t)f*stride + a)*stride + b];
 		double* l = left + (size_t)F*F*pair_block[pbase + pair];
 		for (int field = 0; field < F; ++field) l[field*F + field] += dt*robin[(size_t)f*F + field]*m;
 		if (b == 0) {
-			const double s = load[(size_t)f*kMaxBasis + a];
+			const double s = load[(size_t)f*stride + a];
 			const size_t row = conn[b0 + a];
 			for (int field = 0; field < F; ++field) source[F*row + field] += dt*flux_total[(size_t)f*F + field]*s;
 		}
@@ -487,8 +487,9 @@ void AssembleSurfaces(tfi_solve* s, const tfi_solve_system* sys)
 	total_dev.Allocate(ctx, (size_t)count*F);
 	robin_dev.Upload(robin.data(), robin.size());
 	total_dev.Upload(total.data(), total.size());
-	mass.Allocate(ctx, (size_t)count*kMaxBasis*kMaxBasis);
-	load.Allocate(ctx, (size_t)count*kMaxBasis);
+	const int S = mesh->max_nen;
+	mass.Allocate(ctx, (size_t)count*S*S);
+	load.Allocate(ctx, (size_t)count*S);
 	FaceMass(mesh, count, element.data(), face.data(), mass.data(), load.data());
 	std::vector<int> sorted;
 	std::vector<std::pair<int, int>> spans;
@@ -502,7 +503,7 @@ void AssembleSurfaces(tfi_solve* s, const tfi_solve_system* sys)
 			constexpr int FF = decltype(fc)::value;
 			hipLaunchKernelGGL(SolveSurfaceKernel<FF>, dim3(span.second), dim3(kThreads), 0, ctx->stream, span.second,
 				order.data() + span.first, element.data(), mesh->eoff.data(), mesh->conn.data(),
-				mesh->pair_offsets.data(), mesh->pair_block.data(), mass.data(), load.data(), robin_dev.data(),
+				mesh->pair_offsets.data(), mesh->pair_block.data(), mass.data(), load.data(), S, robin_dev.data(),
 				total_dev.data(), sys->dt, s->left.data(), s->source.data());
 			TFI_LAUNCH_CHECK("SolveSurfaceKernel");
 		});
@@ -691,5 +692,21 @@ int tfi_solve_matrices(tfi_solve* s, double* left, double* previous, double* sou
 		if (previous) s->previous.Download(previous, s->previous.size());
 		if (source) s->source.Download(source, s->source.size());
 		if (singular_blocks) *singular_blocks = s->singular;
+	});
+}
+
+int tfi_solve_bench_spmv(tfi_solve* s, int repetitions, double* milliseconds)
+{
+	return Guard(s->ctx, [&] {
+		tfi_ctx* ctx = s->ctx;
+		for (int i = 0; i < 3; ++i) Spmv(s, s->left.data(), s->state.data(), s->work.data());
+		Sync(ctx);
+		Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event");
+		for (int i = 0; i < repetitions; ++i) Spmv(s, s->left.data(), s->state.data(), s->work.data());
+		Check(hipEventRecord(ctx->ev_end, ctx->stream), "event");
+		Check(hipEventSynchronize(ctx->ev_end), "event sync");
+		float ms = 0.0f;
+		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
+		*milliseconds = ms/std::max(1, repetitions);
 	});
 }

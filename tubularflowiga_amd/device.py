@@ -456,6 +456,12 @@ class ConfiguredTransport:
                                                ctypes.byref(it), ctypes.byref(res)))
         return it.value, res.value
 
+    def bench_spmv(self, repetitions=50):
+        """Average ms per launch of the F x F SpMV, timed back to back."""
+        ms = ctypes.c_double()
+        self.ctx.check(self.lib.tfi_solve_bench_spmv(self.handle, repetitions, ctypes.byref(ms)))
+        return ms.value
+
     def matrices(self):
         F, nnzb = self.fields, self.dmesh.nnzb
         left, prev, src = np.zeros(F * F * nnzb), np.zeros(F * F * nnzb), np.zeros(F * self.nodes)
