@@ -333,6 +333,10 @@ typedef struct {
 int tfi_solve_create(tfi_mesh* mesh, const tfi_solve_system* system, const double* velocity,
 	const int32_t* constrained, tfi_solve** out);
 void tfi_solve_destroy(tfi_solve* solve);
+/* Re-assembles L, P and the source for a new nodal velocity [3N] and rebuilds
+ * the block-Jacobi scaling, as the reference does before every step of a
+ * snapshot-series velocity source (ConfiguredTransportHost.inc:337-358, 376). */
+int tfi_solve_set_velocity(tfi_solve* solve, const double* velocity);
 /* Prescribed values [N*fields] of the constrained entries
  * (SetGenericBoundaryVectorKernel). */
 int tfi_solve_set_boundary(tfi_solve* solve, const double* values);

@@ -217,3 +217,60 @@ def test_cli_solve_matches_the_oracle_and_restarts(tmp_path):
     assert "restart=" in r2.stdout and "completed_step=3" in r2.stdout
     again = _read_output(out2, 3)
     assert np.abs(again - got).max() <= 1e-12 * np.abs(got).max()
+
+
+def test_cli_solve_with_a_velocity_series(tmp_path):
+    """A snapshot-series velocity source: the operators are re-assembled before
+    every step for the velocity interpolated linearly at the step's end time
+    (ConfiguredTransportHost.inc:337-376); the oracle does the same with its
+    own assembly and a direct solve per step."""
+    import json
+    import subprocess
+    import sys
+    from pathlib import Path
+    import scipy.sparse.linalg as spla
+    sys.path.insert(0, str(Path(__file__).resolve().parent))
+    case, d, sysdef, constrained, values, start, surfaces = _cli_case(tmp_path)
+    config = json.loads((d / "simulation_config.json").read_text())
+    for t in config["equation_systems"][0]["terms"]:
+        if t["operator"] == "advection":
+            t["velocity"] = "series"
+    for x in config["equation_systems"][0]["stabilization"]:
+        x["velocity"] = "series"
+    config["velocity_sources"] = [{"name": "series", "kind": "snapshot_series", "manifest": "velocity.csv",
+                                   "interpolation": "linear", "out_of_range": "hold"}]
+    config["time"]["steps"] = 4
+    (d / "simulation_config.json").write_text(json.dumps(config))
+    base = casegen.tube_velocity(case)
+    snaps = [(0.0, base), (0.005, 2.0 * base), (0.0075, 0.5 * base)]
+    for k, (_, v) in enumerate(snaps):
+        np.savetxt(d / f"v{k}.txt", v, fmt="%.17g")
+    (d / "velocity.csv").write_text("time,file\n" + "".join(f"{t!r},v{k}.txt\n" for k, (t, _) in enumerate(snaps)))
+    cli = Path(__file__).resolve().parent.parent / "tubularflowiga_amd" / "iga_hip"
+    out = tmp_path / "series" / "species.txt"
+    print("case: iga_hip solve with a velocity series, 4 steps", flush=True)
+    r = subprocess.run([str(cli), "solve", str(d / "case.ntiga"), str(d), "--output", str(out), "--linear-rtol",
+                        "1e-11"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    got = _read_output(out, 3)
+    pattern = O.block_pattern(case.mesh)
+    mask = constrained.ravel().astype(bool)
+    c = start.ravel().copy()
+    for step in range(4):
+        t = (step + 1) * 2e-3
+        times = [s[0] for s in snaps]
+        if t >= times[-1]:
+            vel = snaps[-1][1]
+        else:
+            k = int(np.searchsorted(times, t))
+            w = (t - times[k - 1]) / (times[k] - times[k - 1])
+            vel = (1.0 - w) * snaps[k - 1][1] + w * snaps[k][1]
+        L, P, Sv = S.assemble(case.mesh, pattern, vel, sysdef, constrained, surfaces)
+        b = S.to_csr(pattern, P) @ c + Sv.ravel()
+        b[mask] = values.ravel()[mask]
+        c = spla.spsolve(S.to_csr(pattern, L).tocsc(), b)
+    want = c.reshape(-1, 3)
+    whole = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print(f"series state: rel L2 = {whole:.2e}", flush=True)
+    assert whole <= 1e-8

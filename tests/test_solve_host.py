@@ -135,7 +135,15 @@ CONFIG_ERRORS = {
                                            if t["operator"] == "advection"] and
                           [x.update(velocity="series") for x in s[0]["stabilization"]] and
                           {"velocity_sources": [{"name": "series", "kind": "snapshot_series",
-                                                 "manifest": "v.json"}]}, "is not supported by iga_hip solve"),
+                                                 "manifest": "v.csv"}]}, "requires 'interpolation'"),
+    "missing_manifest": (lambda f, s, b: [t.update(velocity="series") for t in s[0]["terms"]
+                                          if t["operator"] == "advection"] and
+                         [x.update(velocity="series") for x in s[0]["stabilization"]] and
+                         {"velocity_sources": [{"name": "series", "kind": "snapshot_series", "manifest": "v.csv",
+                                                "interpolation": "linear"}]}, "cannot open velocity manifest"),
+    "nearest_interpolation": (lambda f, s, b: {"velocity_sources": [{"name": "series", "kind": "snapshot_series",
+                                                                     "manifest": "v.csv", "interpolation": "nearest"}]},
+                              "requires manifest, linear interpolation"),
 }
 
 
@@ -164,3 +172,40 @@ def test_usage_errors(tmp_path, args, message):
     out = run("solve", d / "case.ntiga", d, *args)
     assert out.returncode != 0
     assert message in out.stderr, out.stderr
+
+
+def series_system():
+    system = copy.deepcopy(SYSTEM)
+    for t in system["terms"]:
+        if t["operator"] == "advection":
+            t["velocity"] = "series"
+    for x in system["stabilization"]:
+        x["velocity"] = "series"
+    return system
+
+
+@pytest.mark.parametrize("manifest,message", [
+    ("time,file\n0.0,v0.txt\n0.01,v1.txt\n", None),
+    ("# comment\n0.0, v0.txt\n\n0.01 ,v1.txt\n", None),
+    ("0.01,v0.txt\n0.0,v1.txt\n", "strictly increasing"),
+    ("0.0,v0.txt,extra\n", "must contain exactly time,file"),
+    ("-1,v0.txt\n", "finite and nonnegative"),
+    ("abc,v0.txt\n", "invalid time"),
+    ("0.0,/abs/v0.txt\n", "relative to the case directory"),
+    ("time,file\n", "contains no snapshots"),
+])
+def test_velocity_series_manifest(tmp_path, manifest, message):
+    """ParseVelocityManifest (include/VelocitySeries.hpp:39-75)."""
+    extra = {"velocity_sources": [{"name": "series", "kind": "snapshot_series", "manifest": "v.csv",
+                                   "interpolation": "linear", "out_of_range": "hold"}]}
+    d = write(tmp_path, FIELDS, [series_system()], BOUNDARIES, extra=extra)
+    (d / "v.csv").write_text(manifest)
+    out = run("solve-check", d / "case.ntiga", d)
+    if message is None:
+        assert out.returncode == 0, out.stderr
+        assert "velocity_source=series" in out.stdout
+    else:
+        assert out.returncode != 0 and message in out.stderr, out.stderr
+    positional = run("solve", d / "case.ntiga", d, "species", "out.txt", "velocity.txt")
+    assert positional.returncode != 0
+    assert "cannot override a configured snapshot_series" in positional.stderr or message in positional.stderr

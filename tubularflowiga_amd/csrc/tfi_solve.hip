@@ -28,6 +28,7 @@
 #include "tfi_internal.hpp"
 #include "tfi_solver.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <type_traits>
@@ -41,6 +42,17 @@ struct tfi_solve {
 	std::unique_ptr<tfi::Fgmres> krylov;
 	int restart = 50;
 	int singular = 0;
+	// the compiled system, kept for re-assembly with a new velocity
+	std::vector<tfi_solve_term> terms;
+	std::vector<int> supg;
+	std::vector<double> tau_diffusivity;
+	double dt = 0.0;
+	tfi::DevBuf<double> velocity;
+	// flux / Robin faces: masses, loads, coefficients and launch groups
+	int surface_count = 0;
+	tfi::DevBuf<double> face_mass, face_load, face_robin, face_total;
+	tfi::DevBuf<int> face_element, face_order;
+	std::vector<std::pair<int, int>> face_spans;
 };
 
 namespace tfi {
@@ -459,54 +471,133 @@ void Validate(const tfi_mesh* mesh, const tfi_solve_system* sys)
 	if (mesh->max_nen > kMaxBasis) throw Error("configured transport supports at most 80 basis functions");
 }
 
-void AssembleSurfaces(tfi_solve* s, const tfi_solve_system* sys)
+// Face masses of the flux / Robin faces (geometry only, computed once) and
+// their grouping by (colour of the element, occurrence of the element in the
+// list): the elements of one group are distinct and share no node
+void SetupSurfaces(tfi_solve* s, const tfi_solve_system* sys)
 {
 	tfi_ctx* ctx = s->ctx;
 	tfi_mesh* mesh = s->mesh;
 	const int count = sys->surface_count;
 	const int F = s->fields;
-	// faces grouped by (colour of the element, occurrence of the element in
-	// the list): the elements of one group are distinct and share no node
+	s->surface_count = count;
+	if (count == 0) return;
 	std::map<std::pair<int, int>, std::vector<int>> groups;
 	std::vector<int> seen(mesh->elements, 0);
 	for (int i = 0; i < count; ++i) {
 		const int e = sys->surface_element[i];
 		groups[{mesh->h_color[e], seen[e]++}].push_back(i);
 	}
-	std::vector<double> robin(sys->surface_robin, sys->surface_robin + (size_t)count*F);
 	std::vector<double> total((size_t)count*F);
 	for (size_t i = 0; i < total.size(); ++i) total[i] = sys->surface_flux[i] + sys->surface_robin_source[i];
-	DevBuf<int> element, face, order;
-	DevBuf<double> mass, load, robin_dev, total_dev;
-	element.Allocate(ctx, count);
+	DevBuf<int> face;
 	face.Allocate(ctx, count);
-	order.Allocate(ctx, count);
-	element.Upload(sys->surface_element, count);
 	face.Upload(sys->surface_face, count);
-	robin_dev.Allocate(ctx, (size_t)count*F);
-	total_dev.Allocate(ctx, (size_t)count*F);
-	robin_dev.Upload(robin.data(), robin.size());
-	total_dev.Upload(total.data(), total.size());
+	s->face_element.Allocate(ctx, count);
+	s->face_element.Upload(sys->surface_element, count);
+	s->face_robin.Allocate(ctx, (size_t)count*F);
+	s->face_total.Allocate(ctx, (size_t)count*F);
+	s->face_robin.Upload(sys->surface_robin, (size_t)count*F);
+	s->face_total.Upload(total.data(), total.size());
 	const int S = mesh->max_nen;
-	mass.Allocate(ctx, (size_t)count*S*S);
-	load.Allocate(ctx, (size_t)count*S);
-	FaceMass(mesh, count, element.data(), face.data(), mass.data(), load.data());
+	s->face_mass.Allocate(ctx, (size_t)count*S*S);
+	s->face_load.Allocate(ctx, (size_t)count*S);
+	FaceMass(mesh, count, s->face_element.data(), face.data(), s->face_mass.data(), s->face_load.data());
 	std::vector<int> sorted;
-	std::vector<std::pair<int, int>> spans;
 	for (const auto& g : groups) {
-		spans.push_back({static_cast<int>(sorted.size()), static_cast<int>(g.second.size())});
+		s->face_spans.push_back({static_cast<int>(sorted.size()), static_cast<int>(g.second.size())});
 		sorted.insert(sorted.end(), g.second.begin(), g.second.end());
 	}
-	order.Upload(sorted.data(), sorted.size());
-	for (const auto& span : spans)
+	s->face_order.Allocate(ctx, count);
+	s->face_order.Upload(sorted.data(), sorted.size());
+	Sync(ctx);
+}
+
+// L, P and the source for the nodal velocity [3N] on the device: element
+// colours, then the faces, then the constrained rows, the diagonal-block
+// inverses and D^-1 L
+void Assemble(tfi_solve* s)
+{
+	tfi_ctx* ctx = s->ctx;
+	tfi_mesh* mesh = s->mesh;
+	const int F = s->fields;
+	const int N = mesh->nodes;
+	Fill(ctx, s->left.data(), 0.0, s->left.size());
+	Fill(ctx, s->previous.data(), 0.0, s->previous.size());
+	Fill(ctx, s->source.data(), 0.0, s->source.size());
+	SolveArgs a{};
+	a.eoff = mesh->eoff.data();
+	a.conn = mesh->conn.data();
+	a.geometry = mesh->geometry.data();
+	a.table = mesh->basis_table.data();
+	a.pair_offsets = mesh->pair_offsets.data();
+	a.pair_block = mesh->pair_block.data();
+	a.velocity = s->velocity.data();
+	a.dt = s->dt;
+	a.term_count = static_cast<int>(s->terms.size());
+	for (size_t t = 0; t < s->terms.size(); ++t)
+		a.terms[t] = {s->terms[t].kind, s->terms[t].equation, s->terms[t].trial, s->terms[t].coefficient};
+	for (int f = 0; f < F; ++f) {
+		a.supg[f] = s->supg[f];
+		a.tau_diffusivity[f] = s->tau_diffusivity[f];
+	}
+	a.left = s->left.data();
+	a.previous = s->previous.data();
+	a.source = s->source.data();
+	const size_t point_bytes = sizeof(double)*kQ*(4 + F);
+	const size_t staged_bytes = sizeof(double)*(size_t)kQ*mesh->max_nen*kRec;
+	const bool staged = point_bytes + staged_bytes <= 160*1024;
+	const size_t lds = point_bytes + (staged ? staged_bytes : 0);
+	{
+		StreamTimer timer(ctx, &ctx->timers.transport_assembly_ms);
 		Dispatch(F, [&](auto fc) {
 			constexpr int FF = decltype(fc)::value;
-			hipLaunchKernelGGL(SolveSurfaceKernel<FF>, dim3(span.second), dim3(kThreads), 0, ctx->stream, span.second,
-				order.data() + span.first, element.data(), mesh->eoff.data(), mesh->conn.data(),
-				mesh->pair_offsets.data(), mesh->pair_block.data(), mass.data(), load.data(), S, robin_dev.data(),
-				total_dev.data(), sys->dt, s->left.data(), s->source.data());
-			TFI_LAUNCH_CHECK("SolveSurfaceKernel");
+			const void* kernel = staged ? reinterpret_cast<const void*>(&SolveAssembleKernel<FF, true>)
+				: reinterpret_cast<const void*>(&SolveAssembleKernel<FF, false>);
+			if (lds > 65536)
+				Check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+					"hipFuncSetAttribute");
+			for (int c = 0; c < mesh->colors; ++c) {
+				a.elements = mesh->color_elements.data() + mesh->color_offsets[c];
+				a.count = mesh->color_offsets[c + 1] - mesh->color_offsets[c];
+				if (a.count == 0) continue;
+				if (staged)
+					hipLaunchKernelGGL((SolveAssembleKernel<FF, true>), dim3(a.count), dim3(kThreads), lds, ctx->stream,
+						a);
+				else
+					hipLaunchKernelGGL((SolveAssembleKernel<FF, false>), dim3(a.count), dim3(kThreads), lds,
+						ctx->stream, a);
+				TFI_LAUNCH_CHECK("SolveAssembleKernel");
+			}
+			for (const auto& span : s->face_spans) {
+				hipLaunchKernelGGL(SolveSurfaceKernel<FF>, dim3(span.second), dim3(kThreads), 0, ctx->stream,
+					span.second, s->face_order.data() + span.first, s->face_element.data(), mesh->eoff.data(),
+					mesh->conn.data(), mesh->pair_offsets.data(), mesh->pair_block.data(), s->face_mass.data(),
+					s->face_load.data(), mesh->max_nen, s->face_robin.data(), s->face_total.data(), s->dt,
+					s->left.data(), s->source.data());
+				TFI_LAUNCH_CHECK("SolveSurfaceKernel");
+			}
 		});
+	}
+	const BsrView v = mesh->view();
+	DevBuf<int> singular;
+	singular.Allocate(ctx, 1);
+	FillInt(ctx, singular.data(), 0, 1);
+	Dispatch(F, [&](auto fc) {
+		constexpr int FF = decltype(fc)::value;
+		hipLaunchKernelGGL(SolveBoundaryKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
+			v.row_offsets, v.diagonal, s->constrained.data(), s->left.data(), s->previous.data());
+		TFI_LAUNCH_CHECK("SolveBoundaryKernel");
+		hipLaunchKernelGGL(SolveBlockInverseKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
+			v.diagonal, s->left.data(), s->inverse.data(), singular.data());
+		TFI_LAUNCH_CHECK("SolveBlockInverseKernel");
+		hipLaunchKernelGGL(SolveScaleRowsKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
+			v.row_offsets, s->inverse.data(), s->left.data(), s->scaled_left.data());
+		TFI_LAUNCH_CHECK("SolveScaleRowsKernel");
+	});
+	int count = 0;
+	singular.Download(&count, 1);
+	s->singular = std::max(s->singular, count);
 	Sync(ctx);
 }
 
@@ -529,91 +620,38 @@ int tfi_solve_create(tfi_mesh* mesh, const tfi_solve_system* system, const doubl
 		s->mesh = mesh;
 		const int F = system->fields;
 		s->fields = F;
+		s->terms.assign(system->terms, system->terms + system->term_count);
+		s->supg.assign(system->supg, system->supg + F);
+		s->tau_diffusivity.assign(system->tau_diffusivity, system->tau_diffusivity + F);
+		s->dt = system->dt;
 		const int N = mesh->nodes;
 		const long long n = (long long)F*N;
 		if (!mesh->basis_table.size()) SetBasisTable(mesh, 1);
 		s->left.Allocate(ctx, (size_t)mesh->nnzb*F*F);
 		s->previous.Allocate(ctx, (size_t)mesh->nnzb*F*F);
+		s->scaled_left.Allocate(ctx, (size_t)mesh->nnzb*F*F);
 		s->inverse.Allocate(ctx, (size_t)N*F*F);
 		for (auto* b : {&s->source, &s->state, &s->rhs, &s->next, &s->scaled, &s->values, &s->work})
 			b->Allocate(ctx, n);
 		s->constrained.Allocate(ctx, n);
 		s->constrained.Upload(constrained, n);
-		Fill(ctx, s->left.data(), 0.0, s->left.size());
-		Fill(ctx, s->previous.data(), 0.0, s->previous.size());
-		for (auto* b : {&s->source, &s->state, &s->values}) Fill(ctx, b->data(), 0.0, n);
-		DevBuf<double> vel;
-		vel.Allocate(ctx, 3LL*N);
-		vel.Upload(velocity, 3LL*N);
-		SolveArgs a{};
-		a.eoff = mesh->eoff.data();
-		a.conn = mesh->conn.data();
-		a.geometry = mesh->geometry.data();
-		a.table = mesh->basis_table.data();
-		a.pair_offsets = mesh->pair_offsets.data();
-		a.pair_block = mesh->pair_block.data();
-		a.velocity = vel.data();
-		a.dt = system->dt;
-		a.term_count = system->term_count;
-		for (int t = 0; t < system->term_count; ++t)
-			a.terms[t] = {system->terms[t].kind, system->terms[t].equation, system->terms[t].trial,
-				system->terms[t].coefficient};
-		for (int f = 0; f < F; ++f) {
-			a.supg[f] = system->supg[f];
-			a.tau_diffusivity[f] = system->tau_diffusivity[f];
-		}
-		a.left = s->left.data();
-		a.previous = s->previous.data();
-		a.source = s->source.data();
-		const size_t point_bytes = sizeof(double)*kQ*(4 + F);
-		const size_t staged_bytes = sizeof(double)*(size_t)kQ*mesh->max_nen*kRec;
-		const bool staged = point_bytes + staged_bytes <= 160*1024;
-		const size_t lds = point_bytes + (staged ? staged_bytes : 0);
-		{
-			StreamTimer timer(ctx, &ctx->timers.transport_assembly_ms);
-			Dispatch(F, [&](auto fc) {
-				constexpr int FF = decltype(fc)::value;
-				const void* kernel = staged ? reinterpret_cast<const void*>(&SolveAssembleKernel<FF, true>)
-					: reinterpret_cast<const void*>(&SolveAssembleKernel<FF, false>);
-				if (lds > 65536)
-					Check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-						static_cast<int>(lds)), "hipFuncSetAttribute");
-				for (int c = 0; c < mesh->colors; ++c) {
-					a.elements = mesh->color_elements.data() + mesh->color_offsets[c];
-					a.count = mesh->color_offsets[c + 1] - mesh->color_offsets[c];
-					if (a.count == 0) continue;
-					if (staged)
-						hipLaunchKernelGGL((SolveAssembleKernel<FF, true>), dim3(a.count), dim3(kThreads), lds,
-							ctx->stream, a);
-					else
-						hipLaunchKernelGGL((SolveAssembleKernel<FF, false>), dim3(a.count), dim3(kThreads), lds,
-							ctx->stream, a);
-					TFI_LAUNCH_CHECK("SolveAssembleKernel");
-				}
-			});
-		}
-		if (system->surface_count > 0) AssembleSurfaces(s.get(), system);
-		const BsrView v = mesh->view();
-		DevBuf<int> singular;
-		singular.Allocate(ctx, 1);
-		FillInt(ctx, singular.data(), 0, 1);
-		s->scaled_left.Allocate(ctx, (size_t)mesh->nnzb*F*F);
-		Dispatch(F, [&](auto fc) {
-			constexpr int FF = decltype(fc)::value;
-			hipLaunchKernelGGL(SolveBoundaryKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
-				v.row_offsets, v.diagonal, s->constrained.data(), s->left.data(), s->previous.data());
-			TFI_LAUNCH_CHECK("SolveBoundaryKernel");
-			hipLaunchKernelGGL(SolveBlockInverseKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
-				v.diagonal, s->left.data(), s->inverse.data(), singular.data());
-			TFI_LAUNCH_CHECK("SolveBlockInverseKernel");
-			hipLaunchKernelGGL(SolveScaleRowsKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
-				v.row_offsets, s->inverse.data(), s->left.data(), s->scaled_left.data());
-			TFI_LAUNCH_CHECK("SolveScaleRowsKernel");
-		});
-		singular.Download(&s->singular, 1);
+		for (auto* b : {&s->state, &s->values}) Fill(ctx, b->data(), 0.0, n);
+		s->velocity.Allocate(ctx, 3LL*N);
+		s->velocity.Upload(velocity, 3LL*N);
+		SetupSurfaces(s.get(), system);
+		Assemble(s.get());
 		s->krylov = std::make_unique<Fgmres>(ctx, n, s->restart);
 		Sync(ctx);
 		*out = s.release();
+	});
+}
+
+int tfi_solve_set_velocity(tfi_solve* s, const double* velocity)
+{
+	return Guard(s->ctx, [&] {
+		if (!velocity) throw Error("tfi_solve_set_velocity needs a velocity");
+		s->velocity.Upload(velocity, 3LL*s->mesh->nodes);
+		Assemble(s);
 	});
 }
 

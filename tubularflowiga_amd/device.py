@@ -438,6 +438,11 @@ class ConfiguredTransport:
             np.ascontiguousarray(constrained, np.int32).ravel(), ctypes.byref(h)))
         self.handle = h
 
+    def set_velocity(self, velocity):
+        """Re-assemble for a new nodal velocity [N][3]."""
+        self.ctx.check(self.lib.tfi_solve_set_velocity(self.handle,
+                                                       np.ascontiguousarray(velocity, np.float64).ravel()))
+
     def set_boundary(self, values):
         self.ctx.check(self.lib.tfi_solve_set_boundary(self.handle,
                                                        np.ascontiguousarray(values, np.float64).ravel()))

@@ -784,7 +784,9 @@ struct SolveCase {
 	LinearConfig config;
 	CompiledSystem system;
 	ScalarBoundaries bc;
-	std::vector<double> velocity;  // [3N]
+	std::vector<double> velocity;  // [3N] prescribed velocity (empty for a snapshot series)
+	std::vector<VelocitySnapshot> snapshots;
+	VelocitySourceDef source;
 	std::vector<double> state;     // [N*F] initial or restart state
 	int start_step = 0, end_step = 0;
 };
@@ -801,18 +803,23 @@ SolveCase LoadSolveCase(int argc, char** argv)
 	if (s.velocity_source != "prescribed") {
 		if (!t.o.velocity.empty())
 			throw std::runtime_error("a positional VELOCITY file cannot override a configured snapshot_series");
-		throw std::runtime_error("velocity source '" + s.velocity_source + "' (" + t.config.velocity_source_kind[s.velocity_source]
-			+ ") is not supported by iga_hip solve; use a prescribed velocity");
+		const auto it = std::find_if(t.config.sources.begin(), t.config.sources.end(),
+			[&](const VelocitySourceDef& d) { return d.name == s.velocity_source; });
+		if (it == t.config.sources.end()) throw std::runtime_error("unknown velocity source '" + s.velocity_source + "'");
+		t.source = *it;
+		t.snapshots = ReadVelocityManifest(t.o.case_dir/t.source.manifest);
 	}
 	if (t.o.stop_after_step > s.steps) throw std::runtime_error("--stop-after-step exceeds configured transport steps");
 	t.end_step = t.o.stop_after_step > 0 ? t.o.stop_after_step : s.steps;
 	const auto labeled = ReadLabeledHexMesh((t.o.case_dir/"controlmesh.vtk").string(), t.db.nodes,
 		static_cast<std::uint64_t>(t.db.elements()));
-	const auto vel = ReadVelocity((t.o.velocity.empty() ? t.o.case_dir/"initial_velocityfield.txt" : t.o.velocity).string(),
-		t.db.nodes);
-	t.velocity.resize(3*t.db.nodes);
-	for (std::size_t n = 0; n < t.db.nodes; ++n)
-		for (int d = 0; d < 3; ++d) t.velocity[3*n + d] = vel[n][d];
+	if (t.snapshots.empty()) {
+		const auto vel = ReadVelocity(
+			(t.o.velocity.empty() ? t.o.case_dir/"initial_velocityfield.txt" : t.o.velocity).string(), t.db.nodes);
+		t.velocity.resize(3*t.db.nodes);
+		for (std::size_t n = 0; n < t.db.nodes; ++n)
+			for (int d = 0; d < 3; ++d) t.velocity[3*n + d] = vel[n][d];
+	}
 	t.bc = ResolveScalarBoundaries(t.config, s, labeled.labels, t.db);
 	const std::size_t F = s.fields.size();
 	t.state.resize(t.db.nodes*F);
@@ -855,6 +862,22 @@ int SolveCheck(int argc, char** argv)
 	return 0;
 }
 
+// velocity_at (ConfiguredTransportHost.inc:343-357): the snapshot series
+// interpolated linearly at `time`, read from the case directory
+std::vector<double> SeriesVelocity(const SolveCase& t, double time)
+{
+	const VelocityInterpolation w = ResolveVelocityInterpolation(t.snapshots, time, t.source.out_of_range);
+	const auto lower = ReadVelocity((t.o.case_dir/t.snapshots[w.lower].file).string(), t.db.nodes);
+	std::vector<double> out(3*t.db.nodes);
+	for (std::size_t n = 0; n < t.db.nodes; ++n)
+		for (int d = 0; d < 3; ++d) out[3*n + d] = lower[n][d];
+	if (w.lower == w.upper) return out;
+	const auto upper = ReadVelocity((t.o.case_dir/t.snapshots[w.upper].file).string(), t.db.nodes);
+	for (std::size_t n = 0; n < t.db.nodes; ++n)
+		for (int d = 0; d < 3; ++d) out[3*n + d] = (1.0 - w.upper_weight)*out[3*n + d] + w.upper_weight*upper[n][d];
+	return out;
+}
+
 int Solve(Ctx& c, const SolveCase& t)
 {
 	const SolveOptions& o = t.o;
@@ -890,11 +913,16 @@ int Solve(Ctx& c, const SolveCase& t)
 		tfi_solve* s = nullptr;
 		~Handle() { tfi_solve_destroy(s); }
 	} h;
+	const bool series = !t.snapshots.empty();
 	const auto assembly_start = Clock::now();
-	c.Check(tfi_solve_create(m.mesh, &sys, t.velocity.data(), t.bc.constrained.data(), &h.s), "configured transport assembly");
+	// with a snapshot series the operators are assembled before every step
+	// for the velocity at its end time; the first assembly uses the first one
+	const std::vector<double> first_velocity = series ? SeriesVelocity(t, (t.start_step + 1)*s.dt) : t.velocity;
+	c.Check(tfi_solve_create(m.mesh, &sys, first_velocity.data(), t.bc.constrained.data(), &h.s),
+		"configured transport assembly");
 	c.Check(tfi_solve_set_boundary(h.s, t.bc.value.data()), "configured transport boundary values");
 	c.Check(tfi_solve_set_state(h.s, t.state.data()), "configured transport state");
-	const double assembly_s = Seconds(assembly_start, Clock::now());
+	double assembly_s = Seconds(assembly_start, Clock::now());
 	int singular = 0;
 	c.Check(tfi_solve_matrices(h.s, nullptr, nullptr, nullptr, &singular), "configured transport block inverses");
 	std::vector<double> state(t.state);
@@ -913,6 +941,12 @@ int Solve(Ctx& c, const SolveCase& t)
 	for (int step = t.start_step; step < t.end_step; ++step) {
 		int it = 0;
 		double res = 0.0;
+		if (series && step > t.start_step) {
+			const auto as = Clock::now();
+			const auto vel = SeriesVelocity(t, (step + 1)*s.dt);
+			c.Check(tfi_solve_set_velocity(h.s, vel.data()), "configured transport assembly");
+			assembly_s += Seconds(as, Clock::now());
+		}
 		const auto ls = Clock::now();
 		if (tfi_solve_step(h.s, step == 0 ? 1 : 0, 10000, o.linear_rtol, &it, &res) != TFI_OK)
 			throw std::runtime_error("configured transport GMRES failed at step " + std::to_string(step) + ": "

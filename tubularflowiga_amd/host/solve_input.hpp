@@ -59,7 +59,12 @@ struct ScalarBoundaryDef {
 	std::vector<ScalarConditionDef> conditions;
 };
 
+struct VelocitySourceDef {
+	std::string name, kind, manifest, interpolation, out_of_range = "error";
+};
+
 struct LinearConfig {
+	std::vector<VelocitySourceDef> sources;
 	std::vector<ScalarFieldDef> fields;
 	std::vector<LinearSystemDef> systems;   // linear_transport systems in file order
 	std::vector<ScalarBoundaryDef> boundaries;
@@ -114,10 +119,23 @@ inline LinearConfig ReadLinearConfig(const std::string& path)
 		for (std::size_t i = 0; i < arr.size(); ++i) {
 			const std::string ctx = "velocity_sources[" + std::to_string(i) + "]";
 			const Json& o = RequireObject(arr[i], ctx);
-			const std::string name = RequireString(Required(o, "name", ctx), ctx + ".name");
-			if (name.empty() || name == "prescribed" || !c.velocity_sources.insert(name).second)
-				throw std::runtime_error("simulation_config.json: velocity source names must be unique and not 'prescribed'");
-			c.velocity_source_kind[name] = RequireString(Required(o, "kind", ctx), ctx + ".kind");
+			RequireKnownKeys(o, {"name", "kind", "manifest", "interpolation", "out_of_range"}, ctx);
+			VelocitySourceDef d;
+			d.name = RequireString(Required(o, "name", ctx), ctx + ".name");
+			d.kind = RequireString(Required(o, "kind", ctx), ctx + ".kind");
+			d.manifest = RequireString(Required(o, "manifest", ctx), ctx + ".manifest");
+			d.interpolation = RequireString(Required(o, "interpolation", ctx), ctx + ".interpolation");
+			if (const Json* v = o.Find("out_of_range")) d.out_of_range = RequireString(*v, ctx + ".out_of_range");
+			if (d.name.empty() || d.name == "prescribed" || !c.velocity_sources.insert(d.name).second)
+				throw std::runtime_error("simulation_config.json: velocity source names must be non-empty, unique, and not 'prescribed'");
+			if (d.kind != "snapshot_series" || d.manifest.empty() || d.interpolation != "linear"
+				|| (d.out_of_range != "error" && d.out_of_range != "hold"))
+				throw std::runtime_error("simulation_config.json: snapshot velocity source requires manifest, linear "
+					"interpolation, and error or hold out_of_range");
+			if (fs::path(d.manifest).is_absolute())
+				throw std::runtime_error("simulation_config.json: velocity source manifest must be relative to the case directory");
+			c.velocity_source_kind[d.name] = d.kind;
+			c.sources.push_back(d);
 		}
 	}
 	if (const Json* ph = root.Find("physiology")) {
@@ -321,6 +339,84 @@ inline CompiledSystem CompileSystem(const LinearConfig& c, const std::string& re
 		for (const auto& f : c.fields)
 			if (f.name == r.fields[i]) r.initial[i] = f.initial_value;
 	return r;
+}
+
+// Velocity snapshot series (include/VelocitySeries.hpp:19-117): "time,file"
+// lines (an optional "time,file" header, '#' comments), times finite,
+// nonnegative and strictly increasing, files relative to the case directory
+struct VelocitySnapshot {
+	double time = 0.0;
+	fs::path file;
+};
+
+inline std::string TrimToken(const std::string& t)
+{
+	const auto a = t.find_first_not_of(" \t\r");
+	if (a == std::string::npos) return {};
+	return t.substr(a, t.find_last_not_of(" \t\r") - a + 1);
+}
+
+inline std::vector<VelocitySnapshot> ReadVelocityManifest(const fs::path& path)
+{
+	std::ifstream in(path);
+	if (!in) throw std::runtime_error("cannot open velocity manifest: " + path.string());
+	const std::string ctx = path.string();
+	std::vector<VelocitySnapshot> out;
+	std::string line;
+	std::size_t number = 0;
+	while (std::getline(in, line)) {
+		++number;
+		line = TrimToken(line);
+		if (line.empty() || line.front() == '#') continue;
+		const auto comma = line.find(',');
+		if (comma == std::string::npos || line.find(',', comma + 1) != std::string::npos)
+			throw std::runtime_error(ctx + " line " + std::to_string(number) + " must contain exactly time,file");
+		const std::string tt = TrimToken(line.substr(0, comma)), ft = TrimToken(line.substr(comma + 1));
+		if (out.empty() && tt == "time" && ft == "file") continue;
+		std::size_t used = 0;
+		double t = 0.0;
+		try {
+			t = std::stod(tt, &used);
+		} catch (const std::exception&) {
+			throw std::runtime_error(ctx + " line " + std::to_string(number) + " has an invalid time");
+		}
+		if (used != tt.size() || !std::isfinite(t) || t < 0.0)
+			throw std::runtime_error(ctx + " times must be finite and nonnegative");
+		if (!out.empty() && !(t > out.back().time)) throw std::runtime_error(ctx + " times must be strictly increasing");
+		const fs::path file(ft);
+		if (file.empty() || file.is_absolute())
+			throw std::runtime_error(ctx + " files must be non-empty paths relative to the case directory");
+		out.push_back({t, file});
+	}
+	if (out.empty()) throw std::runtime_error(ctx + " contains no snapshots");
+	return out;
+}
+
+// ResolveVelocityInterpolation (VelocitySeries.hpp:105-125): the bracketing
+// snapshots and the upper one's weight; "hold" clamps outside the series
+struct VelocityInterpolation {
+	std::size_t lower = 0, upper = 0;
+	double upper_weight = 0.0;
+};
+
+inline VelocityInterpolation ResolveVelocityInterpolation(const std::vector<VelocitySnapshot>& s, double time,
+	const std::string& out_of_range)
+{
+	if (s.empty() || !std::isfinite(time))
+		throw std::runtime_error("cannot interpolate an empty velocity series or non-finite time");
+	if (time < s.front().time) {
+		if (out_of_range == "hold") return {0, 0, 0.0};
+		throw std::runtime_error("transport time precedes the first velocity snapshot");
+	}
+	if (time > s.back().time) {
+		if (out_of_range == "hold") return {s.size() - 1, s.size() - 1, 0.0};
+		throw std::runtime_error("transport time exceeds the last velocity snapshot");
+	}
+	std::size_t upper = 0;
+	while (upper < s.size() && s[upper].time < time) ++upper;
+	if (upper == s.size()) return {s.size() - 1, s.size() - 1, 0.0};
+	if (s[upper].time == time || upper == 0) return {upper, upper, 0.0};
+	return {upper - 1, upper, (time - s[upper - 1].time)/(s[upper].time - s[upper - 1].time)};
 }
 
 // NodeFaceLabels (GenericCaseInput.hpp:38-61): labels of the element faces on
