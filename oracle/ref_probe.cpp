@@ -43,7 +43,10 @@
 //                                          SupgTauDiffusivity, InitialScalarValues,
 //                                          and ResolveScalarBoundaries with
 //                                          NodeFaceLabels: "term", "field" and
-//                                          "boundary" lines as `iga_hip solve-check`
+//                                          "boundary" lines as `iga_hip solve-check`;
+//                                          with CASE_DIR/physiology_state.txt (an
+//                                          interleaved state), ComputePhysiologyPointArrays
+//                                          as "derived" lines
 // ELEMENT file: nen, then nen connectivity ints, nen x 64 extraction values,
 // 64 x 3 Bezier points, nen x 4 nodal state values (whitespace separated).
 #include "BoundaryFlow.hpp"
@@ -53,6 +56,7 @@
 #include "OutletModel.hpp"
 #include "CaseInput.hpp"
 #include "GenericCaseInput.hpp"
+#include "PhysiologyOutput.hpp"
 
 #include <cstdio>
 #include <fstream>
@@ -228,6 +232,18 @@ int main(int argc, char** argv)
 					value_sum += bc.value[i];
 				}
 			std::printf("boundary constrained_index_sum=%lld value_sum=%.17g\n", index_sum, value_sum);
+			// derived physiology arrays of the state in CASE_DIR/physiology_state.txt (if present)
+			std::ifstream state_file(dir + "/physiology_state.txt");
+			if (state_file) {
+				std::vector<double> values;
+				double v = 0.0;
+				while (state_file >> v) values.push_back(v);
+				for (const auto& a : iga::ComputePhysiologyPointArrays(config.physiology, system.fields, values)) {
+					std::printf("derived name=%s", a.name.c_str());
+					for (double x : a.values) std::printf(" %.17g", x);
+					std::printf("\n");
+				}
+			}
 		} else if (cmd == "outlet" && argc == 12) {
 			iga::OutletModelState m;
 			m.label = 2;

@@ -192,6 +192,8 @@ def test_cli_solve_matches_the_oracle_and_restarts(tmp_path):
     assert (out.parent / "species.txt.fields").read_text().split() == ["oxygen", "carrier", "tracer"]
     assert (out.parent / "species.pvd").exists() and (out.parent / "species.step000003.vtu").exists()
     assert (out.parent / "species.step000003.txt").exists() and (out.parent / "species.vtu").exists()
+    manifest = (out.parent / "physiology_fields.json").read_text()
+    assert '"enabled": false' in manifest and '"oxygen": {"status": "solved"}' in manifest
     got = _read_output(out, 3)
     pattern = O.block_pattern(case.mesh)
     vel = casegen.tube_velocity(case)

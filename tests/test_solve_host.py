@@ -60,6 +60,9 @@ def variants():
     b = copy.deepcopy(BOUNDARIES)
     b[1]["conditions"][1]["dirichlet_dofs"] = "face_support"
     out["face_support"] = (FIELDS, [SYSTEM], b, [])
+    out["metabolism"] = (FIELDS, [SYSTEM], BOUNDARIES, [], {"physiology": {
+        "metabolism": {"tracer": -0.05, "oxygen": -0.2},
+        "oxygen_capacity": {"hematocrit_percent": 40.0}, "derived_fields": ["pO2", "SaO2", "hematocrit"]}})
     return out
 
 
@@ -73,15 +76,58 @@ def run(*args):
 
 
 def check_lines(out):
-    return [line for line in out.splitlines() if line.split(" ")[0] in ("term", "field", "boundary")]
+    return [line for line in out.splitlines() if line.split(" ")[0] in ("term", "field", "boundary", "derived")]
+
+
+PHYSIOLOGY_FIELDS = [{"name": "velocity", "kind": "vector3"}, {"name": "pressure", "kind": "pressure"},
+                     {"name": "total_oxygen", "kind": "scalar"}, {"name": "carbon_dioxide", "kind": "scalar"},
+                     {"name": "bicarbonate", "kind": "scalar"}]
+
+
+@pytest.mark.parametrize("state", ["dissolved_oxygen", "total_oxygen"])
+def test_derived_physiology_arrays_match_the_reference(tmp_path, state):
+    """ComputePhysiologyPointArrays (include/PhysiologyOutput.hpp:29-91) with the
+    oxygen equilibrium of OxygenCapacity.hpp, on a state spanning pO2 from 0 to
+    far above P50, against the reference's own function."""
+    import numpy as np
+    if not PROBE.exists():
+        pytest.skip("oracle/_ref/ref_probe is not built (needs the reference sources)")
+    o2 = "oxygen" if state == "dissolved_oxygen" else "total_oxygen"
+    fields = copy.deepcopy(PHYSIOLOGY_FIELDS)
+    fields[2]["name"] = o2
+    system = {"name": "gas", "kind": "linear_transport", "unknowns": [o2, "carbon_dioxide", "bicarbonate"],
+              "terms": [{"operator": "time_derivative", "equation": u} for u in (o2, "carbon_dioxide", "bicarbonate")]}
+    boundaries = [{"label": lab, "name": f"b{lab}", "conditions": []} for lab in (0, 1, 2)]
+    physiology = {"physiology": {"oxygen_capacity": {"oxygen_state": state, "hematocrit_percent": 42.0,
+                                                     "hemoglobin_g_dl": 14.0},
+                                 "derived_fields": ["pO2", "SaO2", "SvO2", "dissolved_oxygen", "bound_oxygen",
+                                                    "total_oxygen", "hematocrit", "pCO2", "pH"]}}
+    d = write(tmp_path, fields, [system], boundaries, extra=physiology)
+    n = casegen.workload("tube_c0").mesh.nodes
+    rng = np.random.default_rng(3)
+    top = 0.5 if state == "dissolved_oxygen" else 12.0
+    values = np.stack([np.linspace(0.0, top, n), rng.uniform(0.5, 2.0, n), rng.uniform(10.0, 30.0, n)], axis=1)
+    np.savetxt(d / "physiology_state.txt", values.ravel(), fmt="%.17g")
+    ours = run("solve-check", d / "case.ntiga", d)
+    assert ours.returncode == 0, ours.stderr
+    ref = subprocess.run([str(PROBE), "solve-case", str(d), str(d / "case.ntiga")], capture_output=True, text=True,
+                         timeout=120)
+    assert ref.returncode == 0, ref.stderr
+    mine = {line.split()[1]: np.array(line.split()[2:], float) for line in ours.stdout.splitlines()
+            if line.startswith("derived")}
+    theirs = {line.split()[1]: np.array(line.split()[2:], float) for line in ref.stdout.splitlines()
+              if line.startswith("derived")}
+    assert list(mine) == list(theirs) and len(mine) == 9
+    for k in mine:
+        assert np.allclose(mine[k], theirs[k], rtol=1e-13, atol=1e-300), k
 
 
 @pytest.mark.parametrize("name", list(variants()))
 def test_solve_check_matches_the_reference_compile(tmp_path, name):
     if not PROBE.exists():
         pytest.skip("oracle/_ref/ref_probe is not built (needs the reference sources)")
-    fields, systems, boundaries, opts = variants()[name]
-    d = write(tmp_path, fields, systems, boundaries)
+    fields, systems, boundaries, opts, *extra = variants()[name]
+    d = write(tmp_path, fields, systems, boundaries, extra=extra[0] if extra else None)
     ours = run("solve-check", d / "case.ntiga", d, *opts)
     assert ours.returncode == 0, ours.stderr
     system = opts[1] if opts else []
@@ -114,7 +160,6 @@ def bad(mutate):
 
 CONFIG_ERRORS = {
     "bdf2": (lambda f, s, b: s[0].update(time_integration="bdf2"), "backward_euler only"),
-    "physiology": (lambda f, s, b: {"physiology": {"enabled": True}}, "physiology"),
     "unknown_operator": (lambda f, s, b: s[0]["terms"].append({"operator": "reaction", "equation": "oxygen"}),
                          "unknown term operator"),
     "foreign_trial": (lambda f, s, b: s[0]["terms"].append({"operator": "linear_coupling", "equation": "oxygen",
@@ -130,6 +175,12 @@ CONFIG_ERRORS = {
     "traction_on_scalar": (lambda f, s, b: b[2]["conditions"].__setitem__(
         0, {"field": "oxygen", "type": "pressure_traction", "value": 1.0}), "unsupported scalar boundary"),
     "unconfigured_label": (lambda f, s, b: (b.pop(2), None)[1], "has no simulation_config.json boundary definition"),
+    "vasodilation": (lambda f, s, b: {"physiology": {"vasodilation": {"field": "tracer"}}}, "3d vasodilation"),
+    "metabolism_untransported": (lambda f, s, b: {"physiology": {"metabolism": {"pressure": 1.0}}},
+                                 "is not a transported species"),
+    "derived_without_capacity": (lambda f, s, b: {"physiology": {"derived_fields": ["hematocrit"]}},
+                                 "requires oxygen_capacity"),
+    "perfusate": (lambda f, s, b: {"perfusate": {}}, "multidomain coupling"),
     "vector_unknown": (lambda f, s, b: s[0]["unknowns"].append("velocity"), "requires scalar fields"),
     "snapshot_velocity": (lambda f, s, b: [t.update(velocity="series") for t in s[0]["terms"]
                                            if t["operator"] == "advection"] and

@@ -859,6 +859,18 @@ int SolveCheck(int argc, char** argv)
 			value_sum += t.bc.value[i];
 		}
 	std::cout << "boundary constrained_index_sum=" << index_sum << " value_sum=" << value_sum << '\n';
+	// derived physiology arrays of an interleaved state in CASE_DIR/physiology_state.txt
+	std::ifstream state_file(t.o.case_dir/"physiology_state.txt");
+	if (state_file) {
+		std::vector<double> values;
+		double v = 0.0;
+		while (state_file >> v) values.push_back(v);
+		for (const auto& a : PhysiologyArrays(t.config.physiology, s.fields, values)) {
+			std::cout << "derived name=" << a.name;
+			for (double x : a.values) std::cout << ' ' << x;
+			std::cout << '\n';
+		}
+	}
 	return 0;
 }
 
@@ -931,7 +943,8 @@ int Solve(Ctx& c, const SolveCase& t)
 	auto write_output = [&](int step) {
 		c.Check(tfi_solve_get_state(h.s, state.data()), "configured transport state download");
 		const fs::path vtu = VtuStepPath(o.output, step);
-		WriteConfiguredOutput(TimeIndexedPath(o.output, step), s.fields, state, mesh_vtk, vtu, step*s.dt);
+		WriteConfiguredOutput(TimeIndexedPath(o.output, step), s.fields, state, mesh_vtk, vtu, step*s.dt,
+			t.config.physiology);
 		snapshots.push_back({step*s.dt, vtu});
 	};
 	if (o.output_every > 0) write_output(t.start_step);
@@ -976,9 +989,11 @@ int Solve(Ctx& c, const SolveCase& t)
 	for (double v : state) norm += v*v;
 	norm = std::sqrt(norm);
 	if (!o.output.empty()) {
-		WriteConfiguredOutput(o.output, s.fields, state, mesh_vtk, VtuFinalPath(o.output), t.end_step*s.dt);
+		WriteConfiguredOutput(o.output, s.fields, state, mesh_vtk, VtuFinalPath(o.output), t.end_step*s.dt,
+			t.config.physiology);
 		if (snapshots.empty()) snapshots.push_back({t.end_step*s.dt, VtuFinalPath(o.output)});
 		WritePvd(PvdPath(o.output), snapshots);
+		WritePhysiologyManifest(o.output.parent_path()/"physiology_fields.json", t.config.physiology, s.fields);
 	}
 	long long peak = 0, live = 0;
 	c.Check(tfi_allocation_stats(c.ctx, &peak, &live), "allocation statistics");

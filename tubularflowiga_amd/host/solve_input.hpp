@@ -63,7 +63,18 @@ struct VelocitySourceDef {
 	std::string name, kind, manifest, interpolation, out_of_range = "error";
 };
 
+// PhysiologyDefinition (include/SimulationConfig.hpp:124-141), the parts a
+// linear_transport run uses: metabolism sources and derived output fields
+struct PhysiologyDef {
+	bool enabled = false, oxygen_capacity = false, total_oxygen = false;
+	std::map<std::string, double> metabolism;  // ordered by field name
+	double hematocrit_percent = 0.0, oxygen_solubility = 0.0031, hemoglobin_g_dl = 15.0, p50_mmhg = 26.8,
+		hill_exponent = 2.7, gas_molar_volume_ml_mmol = 22.4;
+	std::vector<std::string> derived_fields;
+};
+
 struct LinearConfig {
+	PhysiologyDef physiology;
 	std::vector<VelocitySourceDef> sources;
 	std::vector<ScalarFieldDef> fields;
 	std::vector<LinearSystemDef> systems;   // linear_transport systems in file order
@@ -72,8 +83,203 @@ struct LinearConfig {
 	std::map<std::string, std::string> velocity_source_kind;
 	double dt = 0.0;
 	int steps = 0;
-	bool physiology = false;
 };
+
+// The physiology section (include/SimulationConfig.hpp:594-697) with its checks
+// against the transported species of the linear_transport systems
+inline void ParsePhysiology(const Json& value, const std::set<std::string>& fields, LinearConfig& c)
+{
+	const Json& o = RequireObject(value, "physiology");
+	RequireKnownKeys(o, {"enabled", "metabolism", "oxygen_capacity", "vasodilation", "derived_fields"}, "physiology");
+	auto boolean = [](const Json& v, const std::string& ctx) {
+		if (v.kind != Json::Kind::Bool) throw std::runtime_error(ctx + " must be a boolean");
+		return v.boolean;
+	};
+	PhysiologyDef& p = c.physiology;
+	p.enabled = o.Find("enabled") ? boolean(*o.Find("enabled"), "physiology.enabled") : true;
+	if (const Json* m = o.Find("metabolism")) {
+		const Json& rates = RequireObject(*m, "physiology.metabolism");
+		for (const auto& kv : rates.object) {
+			if (!fields.count(kv.first))
+				throw std::runtime_error("simulation_config.json: metabolism references missing field '" + kv.first + "'");
+			const double v = RequireNumber(kv.second, "physiology.metabolism." + kv.first);
+			if (!std::isfinite(v)) throw std::runtime_error("simulation_config.json: metabolism rates must be finite");
+			p.metabolism.emplace(kv.first, v);
+		}
+	}
+	if (const Json* ox = o.Find("oxygen_capacity")) {
+		const Json& it = RequireObject(*ox, "physiology.oxygen_capacity");
+		RequireKnownKeys(it, {"enabled", "oxygen_state", "hematocrit_percent", "oxygen_solubility", "hemoglobin_g_dl",
+			"p50_mmhg", "hill_exponent", "gas_molar_volume_ml_mmol"}, "physiology.oxygen_capacity");
+		auto number = [&](const char* name, double fallback) {
+			const Json* v = it.Find(name);
+			return v ? RequireNumber(*v, std::string("physiology.oxygen_capacity.") + name) : fallback;
+		};
+		p.oxygen_capacity = it.Find("enabled") ? boolean(*it.Find("enabled"), "physiology.oxygen_capacity.enabled") : true;
+		if (const Json* st = it.Find("oxygen_state")) {
+			const std::string state = RequireString(*st, "physiology.oxygen_capacity.oxygen_state");
+			if (state != "dissolved_oxygen" && state != "total_oxygen")
+				throw std::runtime_error("unsupported oxygen transport state '" + state + "'");
+			p.total_oxygen = state == "total_oxygen";
+		}
+		p.hematocrit_percent = number("hematocrit_percent", 0.0);
+		p.oxygen_solubility = number("oxygen_solubility", 0.0031);
+		p.hemoglobin_g_dl = number("hemoglobin_g_dl", 15.0);
+		p.p50_mmhg = number("p50_mmhg", 26.8);
+		p.hill_exponent = number("hill_exponent", 2.7);
+		p.gas_molar_volume_ml_mmol = number("gas_molar_volume_ml_mmol", 22.4);
+		if (p.hematocrit_percent < 0.0 || p.hematocrit_percent > 100.0 || !(p.oxygen_solubility > 0.0)
+			|| p.hemoglobin_g_dl < 0.0 || !(p.p50_mmhg > 0.0) || !(p.hill_exponent > 0.0)
+			|| !(p.gas_molar_volume_ml_mmol > 0.0))
+			throw std::runtime_error("simulation_config.json: oxygen-capacity parameters are invalid");
+	}
+	if (const Json* vd = o.Find("vasodilation")) {
+		const Json& it = RequireObject(*vd, "physiology.vasodilation");
+		RequireKnownKeys(it, {"enabled", "field", "emax_radius_fraction", "ec50", "relaxation_tau"}, "physiology.vasodilation");
+		if (it.Find("field")) RequireString(*it.Find("field"), "physiology.vasodilation.field");
+		if (it.Find("enabled") ? boolean(*it.Find("enabled"), "physiology.vasodilation.enabled") : true)
+			throw std::runtime_error("simulation_config.json: 3d vasodilation requires FSI and is not supported; disable it "
+				"or use the 1d solver");
+	}
+	if (const Json* df = o.Find("derived_fields")) {
+		static const std::set<std::string> allowed{"pO2", "pCO2", "pH", "SaO2", "SvO2", "dissolved_oxygen",
+			"bound_oxygen", "total_oxygen", "hematocrit"};
+		std::set<std::string> unique;
+		const auto& arr = RequireArray(*df, "physiology.derived_fields").array;
+		for (std::size_t i = 0; i < arr.size(); ++i) {
+			const std::string name = RequireString(arr[i], "physiology.derived_fields[" + std::to_string(i) + "]");
+			if (!allowed.count(name) || !unique.insert(name).second)
+				throw std::runtime_error("simulation_config.json: unsupported or duplicate physiology derived field '" + name + "'");
+			p.derived_fields.push_back(name);
+		}
+	}
+	std::set<std::string> transported;
+	for (const auto& s : c.systems) transported.insert(s.unknowns.begin(), s.unknowns.end());
+	for (const auto& r : p.metabolism)
+		if (!transported.count(r.first))
+			throw std::runtime_error("simulation_config.json: metabolism field '" + r.first + "' is not a transported species");
+	const std::string oxygen = p.total_oxygen ? "total_oxygen" : "oxygen";
+	if (p.total_oxygen && !p.oxygen_capacity)
+		throw std::runtime_error("simulation_config.json: total_oxygen transport requires oxygen_capacity");
+	if (transported.count("oxygen") && transported.count("total_oxygen"))
+		throw std::runtime_error("simulation_config.json: oxygen and total_oxygen cannot both be transported states");
+	for (const auto& name : p.derived_fields) {
+		if ((name == "pO2" || name == "SaO2" || name == "SvO2" || name == "dissolved_oxygen" || name == "bound_oxygen"
+			|| name == "total_oxygen") && !transported.count(oxygen))
+			throw std::runtime_error("simulation_config.json: derived field '" + name + "' requires transported " + oxygen);
+		if (name == "pCO2" && !transported.count("carbon_dioxide"))
+			throw std::runtime_error("simulation_config.json: pCO2 requires transported carbon_dioxide");
+		if (name == "pH" && (!transported.count("carbon_dioxide") || !transported.count("bicarbonate")))
+			throw std::runtime_error("simulation_config.json: pH requires transported carbon_dioxide and bicarbonate");
+		if ((name == "bound_oxygen" || name == "total_oxygen" || name == "hematocrit") && !p.oxygen_capacity)
+			throw std::runtime_error("simulation_config.json: derived field '" + name + "' requires oxygen_capacity");
+	}
+}
+
+// Oxygen equilibrium (include/OxygenCapacity.hpp:57-138): Hill saturation
+// S = p^n / (p^n + p50^n), dissolved = alpha p 10 / Vm, bound = 1.34 Hb S 10 / Vm
+// (mol/m^3), pO2 from the transported dissolved or total oxygen (the total
+// inverted by bisection on [0, 5000 mmHg], the upper end doubled until it
+// brackets, to 1e-10 relative, at most 100 halvings)
+struct OxygenState {
+	double po2 = 0.0, saturation = 0.0, dissolved = 0.0, bound = 0.0, total = 0.0;
+};
+
+inline OxygenState OxygenFromPo2(double po2, const PhysiologyDef& p)
+{
+	OxygenState r;
+	r.po2 = std::max(po2, 0.0);
+	const double num = std::pow(r.po2, p.hill_exponent);
+	r.saturation = num > 0.0 ? num/(num + std::pow(p.p50_mmhg, p.hill_exponent)) : 0.0;
+	r.dissolved = p.oxygen_solubility*r.po2*10.0/p.gas_molar_volume_ml_mmol;
+	r.bound = 1.34*p.hemoglobin_g_dl*r.saturation*10.0/p.gas_molar_volume_ml_mmol;
+	r.total = r.dissolved + r.bound;
+	return r;
+}
+
+inline OxygenState OxygenFromTransported(double concentration, const PhysiologyDef& p)
+{
+	if (!std::isfinite(concentration) || concentration < 0.0)
+		throw std::runtime_error(std::string(p.total_oxygen ? "total" : "dissolved")
+			+ " oxygen must be finite and nonnegative");
+	if (!p.total_oxygen) return OxygenFromPo2(concentration*p.gas_molar_volume_ml_mmol/(10.0*p.oxygen_solubility), p);
+	if (concentration == 0.0) return OxygenFromPo2(0.0, p);
+	double lo = 0.0, hi = 5000.0;
+	while (OxygenFromPo2(hi, p).total < concentration && hi < 1.0e7) hi *= 2.0;
+	if (OxygenFromPo2(hi, p).total < concentration)
+		throw std::runtime_error("total oxygen exceeds the equilibrium inversion bound");
+	for (int it = 0; it < 100; ++it) {
+		const double mid = 0.5*(lo + hi);
+		if (OxygenFromPo2(mid, p).total < concentration) lo = mid;
+		else hi = mid;
+		if (hi - lo <= 1.0e-10*std::max(1.0, hi)) break;
+	}
+	return OxygenFromPo2(0.5*(lo + hi), p);
+}
+
+// ComputePhysiologyPointArrays (include/PhysiologyOutput.hpp:29-91): one
+// point array per derived field from the interleaved state of `fields`
+inline std::vector<PointArray> PhysiologyArrays(const PhysiologyDef& p, const std::vector<std::string>& fields,
+	const std::vector<double>& state)
+{
+	std::vector<PointArray> out;
+	if (!p.enabled || fields.empty()) return out;
+	const std::size_t F = fields.size(), nodes = state.size()/F;
+	auto column = [&](const std::string& name) {
+		const auto it = std::find(fields.begin(), fields.end(), name);
+		if (it == fields.end()) throw std::runtime_error("physiology output needs field '" + name + "'");
+		const std::size_t f = static_cast<std::size_t>(it - fields.begin());
+		std::vector<double> v(nodes);
+		for (std::size_t n = 0; n < nodes; ++n) v[n] = state[n*F + f];
+		return v;
+	};
+	for (const auto& name : p.derived_fields) {
+		PointArray a{name, 1, std::vector<double>(nodes, 0.0)};
+		if (name == "hematocrit") {
+			std::fill(a.values.begin(), a.values.end(), p.hematocrit_percent);
+		} else if (name == "pO2" || name == "SaO2" || name == "SvO2" || name == "dissolved_oxygen"
+			|| name == "bound_oxygen" || name == "total_oxygen") {
+			const auto oxygen = column(p.total_oxygen ? "total_oxygen" : "oxygen");
+			for (std::size_t n = 0; n < nodes; ++n) {
+				const OxygenState e = OxygenFromTransported(oxygen[n], p);
+				a.values[n] = name == "pO2" ? e.po2 : (name == "SaO2" || name == "SvO2") ? e.saturation
+					: name == "dissolved_oxygen" ? e.dissolved : name == "bound_oxygen" ? e.bound : e.total;
+			}
+		} else if (name == "pCO2") {
+			const auto co2 = column("carbon_dioxide");
+			for (std::size_t n = 0; n < nodes; ++n) a.values[n] = co2[n]/0.0301;
+		} else if (name == "pH") {
+			const auto co2 = column("carbon_dioxide"), hco3 = column("bicarbonate");
+			for (std::size_t n = 0; n < nodes; ++n) {
+				const double pco2 = std::max(co2[n]/0.0301, 1.0e-30);
+				a.values[n] = 6.1 + std::log10(std::max(hco3[n], 1.0e-30)/(0.0301*pco2));
+			}
+		}
+		out.push_back(std::move(a));
+	}
+	return out;
+}
+
+// WritePhysiologyManifest (include/PhysiologyOutput.hpp:93-117)
+inline void WritePhysiologyManifest(const fs::path& path, const PhysiologyDef& p, const std::vector<std::string>& fields)
+{
+	std::ofstream out(path);
+	if (!out) throw std::runtime_error("cannot create physiology manifest: " + path.string());
+	out << "{\n  \"enabled\": " << (p.enabled ? "true" : "false") << ",\n  \"fields\": {";
+	bool first = true;
+	for (const auto& f : fields) {
+		out << (first ? "\n" : ",\n") << "    \"" << JsonEscape(f) << "\": {\"status\": \"solved\"}";
+		first = false;
+	}
+	for (const auto& f : p.derived_fields) {
+		out << (first ? "\n" : ",\n") << "    \"" << JsonEscape(f) << "\": {\"status\": \"derived\"}";
+		first = false;
+	}
+	if (!first) out << '\n';
+	out << "  }\n}\n";
+	out.close();
+	if (!out) throw std::runtime_error("cannot write physiology manifest: " + path.string());
+}
 
 // The parts of ParseSimulationConfiguration (include/SimulationConfig.hpp:229-717)
 // the linear_transport path reads, with its checks on them.
@@ -138,13 +344,11 @@ inline LinearConfig ReadLinearConfig(const std::string& path)
 			c.sources.push_back(d);
 		}
 	}
-	if (const Json* ph = root.Find("physiology")) {
-		const Json& o = RequireObject(*ph, "physiology");
-		if (const Json* en = o.Find("enabled")) {
-			if (en->kind != Json::Kind::Bool) throw std::runtime_error("physiology.enabled must be a boolean");
-			c.physiology = en->boolean;
-		}
-	}
+	for (const char* key : {"perfusate", "coupling", "external_circuit"})
+		if (root.Find(key))
+			throw std::runtime_error(std::string("simulation_config.json: '") + key
+				+ "' (multidomain coupling) is not supported by iga_hip solve");
+	const Json* physiology = root.Find("physiology");
 	const auto& sarr = RequireArray(Required(root, "equation_systems", "root"), "equation_systems").array;
 	std::set<std::string> snames;
 	for (std::size_t i = 0; i < sarr.size(); ++i) {
@@ -228,6 +432,7 @@ inline LinearConfig ReadLinearConfig(const std::string& path)
 		c.systems.push_back(s);
 	}
 	if (sarr.empty()) throw std::runtime_error("simulation_config.json: equation_systems cannot be empty");
+	if (physiology) ParsePhysiology(*physiology, names, c);
 	const auto& barr = RequireArray(Required(root, "boundaries", "root"), "boundaries").array;
 	std::set<int> labels;
 	for (std::size_t i = 0; i < barr.size(); ++i) {
@@ -286,9 +491,6 @@ struct CompiledSystem {
 
 inline CompiledSystem CompileSystem(const LinearConfig& c, const std::string& requested)
 {
-	if (c.physiology)
-		throw std::runtime_error("physiology-enabled configurations (metabolism sources and derived outputs) are not "
-			"supported by this build");
 	if (c.systems.empty()) throw std::runtime_error("simulation configuration has no linear_transport equation system");
 	const LinearSystemDef* s = requested.empty() ? &c.systems.front() : nullptr;
 	for (const auto& it : c.systems)
@@ -319,6 +521,12 @@ inline CompiledSystem CompileSystem(const LinearConfig& c, const std::string& re
 			throw std::runtime_error("CUDA transport received inconsistent velocity sources");
 		r.terms.push_back({kind.at(t.op), r.index.at(t.equation), r.index.at(t.trial), t.coefficient});
 	}
+	// metabolism rates as volume sources of their fields (CompileLinearSystem, :778-784)
+	if (c.physiology.enabled)
+		for (const auto& rate : c.physiology.metabolism) {
+			const auto f = r.index.find(rate.first);
+			if (f != r.index.end()) r.terms.push_back({TFI_TERM_VOLUME_SOURCE, f->second, f->second, rate.second});
+		}
 	if (r.terms.size() > 64) throw std::runtime_error("configured transport supports at most 64 weak-form terms");
 	const std::size_t F = r.fields.size();
 	r.supg.assign(F, TFI_SUPG_NONE);
@@ -565,7 +773,8 @@ inline ScalarBoundaries ResolveScalarBoundaries(const LinearConfig& c, const Com
 // lines at 17 digits, the field names in PATH.fields, and a VTU with one
 // point array per field
 inline void WriteConfiguredOutput(const fs::path& path, const std::vector<std::string>& fields,
-	const std::vector<double>& state, const fs::path& mesh_vtk, const fs::path& vtu, double time)
+	const std::vector<double>& state, const fs::path& mesh_vtk, const fs::path& vtu, double time,
+	const PhysiologyDef& physiology)
 {
 	const std::size_t F = fields.size(), nodes = state.size()/F;
 	if (!path.parent_path().empty()) fs::create_directories(path.parent_path());
@@ -585,6 +794,7 @@ inline void WriteConfiguredOutput(const fs::path& path, const std::vector<std::s
 		for (std::size_t n = 0; n < nodes; ++n) a.values[n] = state[n*F + f];
 		arrays.push_back(std::move(a));
 	}
+	for (auto& a : PhysiologyArrays(physiology, fields, state)) arrays.push_back(std::move(a));
 	WritePointArraysVtu(mesh_vtk, vtu, arrays, time);
 }
 
