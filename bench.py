@@ -250,6 +250,11 @@ def solve_measurement(dm, case, state, steps=10, warmup=2):
         its = [tr.step()[0] for _ in range(steps)]
         wall = time.perf_counter() - t0
         spmv_ms = tr.bench_spmv(50)
+        tr.set_preconditioner(device.TFI_PC_AMG)
+        tr.step()  # AMG set-up
+        t0 = time.perf_counter()
+        its_amg = [tr.step()[0] for _ in range(3)]
+        wall_amg = time.perf_counter() - t0
     finally:
         tr.close()
     nnzb = dm.nnzb
@@ -263,7 +268,8 @@ def solve_measurement(dm, case, state, steps=10, warmup=2):
             "spmv_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                               "kernel": "SolveSpmvKernel<3> (3x3 FP64 values)", "avg_ms": round(spmv_ms, 4),
-                              "algorithmic_bytes": int(spmv_bytes)}}
+                              "algorithmic_bytes": int(spmv_bytes)},
+            "amg": {"ms_per_step": round(1e3 * wall_amg / 3, 3), "krylov_per_step": float(np.mean(its_amg))}}
 
 
 def transport_cpu_baseline(dm, case, velocity, params, iterations, n_el=2000, products=20):

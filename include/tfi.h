@@ -337,6 +337,13 @@ void tfi_solve_destroy(tfi_solve* solve);
  * the block-Jacobi scaling, as the reference does before every step of a
  * snapshot-series velocity source (ConfiguredTransportHost.inc:337-358, 376). */
 int tfi_solve_set_velocity(tfi_solve* solve, const double* velocity);
+/* Preconditioner of the step's FGMRES on D^-1 L: TFI_PC_BJACOBI (default;
+ * none beyond the block-Jacobi scaling itself, as the reference's GMRES) or
+ * TFI_PC_AMG (fields <= 4: the coupled aggregation AMG with FP32 V-cycles, the
+ * F x F blocks embedded in its 4x4 hierarchy with an identity on the padding
+ * fields; set up again after each re-assembly).  No reference counterpart;
+ * the solution is the same either way. */
+int tfi_solve_set_preconditioner(tfi_solve* solve, int preconditioner);
 /* Prescribed values [N*fields] of the constrained entries
  * (SetGenericBoundaryVectorKernel). */
 int tfi_solve_set_boundary(tfi_solve* solve, const double* values);

@@ -151,6 +151,7 @@ SIGNATURES = {
     "tfi_solve_destroy": (None, [c_void_p]),
     "tfi_solve_set_boundary": (c_int, [c_void_p, _d]),
     "tfi_solve_set_velocity": (c_int, [c_void_p, _d]),
+    "tfi_solve_set_preconditioner": (c_int, [c_void_p, c_int]),
     "tfi_solve_set_state": (c_int, [c_void_p, _d]),
     "tfi_solve_get_state": (c_int, [c_void_p, _d]),
     "tfi_solve_step": (c_int, [c_void_p, c_int, c_int, c_double, P(c_int), P(c_double)]),

@@ -53,6 +53,13 @@ struct tfi_solve {
 	tfi::DevBuf<double> face_mass, face_load, face_robin, face_total;
 	tfi::DevBuf<int> face_element, face_order;
 	std::vector<std::pair<int, int>> face_spans;
+	// optional coupled AMG on D^-1 L, its F x F blocks embedded in the 4x4
+	// hierarchy with an identity on the padding fields (F <= 4)
+	int preconditioner = TFI_PC_BJACOBI;
+	std::unique_ptr<tfi::Amg> amg;
+	bool amg_stale = true;
+	tfi::DevBuf<double> padded64, pad_in, pad_out;
+	tfi::DevBuf<float> padded32;
 };
 
 namespace tfi {
@@ -393,6 +400,39 @@ __global__ void __launch_bounds__(kThreads) SolveSpmvKernel(int rows, const int*
 	}
 }
 
+// 4x4 embedding of the F x F blocks (F <= 4): [[S, 0], [0, I if diagonal block else 0]]
+template <int F>
+__global__ void SolvePadBlocksKernel(int rows, const int* __restrict__ ro, const int* __restrict__ diag,
+	const double* __restrict__ s, double* __restrict__ out)
+{
+	const int row = blockIdx.x*blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	for (int k = ro[row]; k < ro[row + 1]; ++k) {
+		const double* a = s + (size_t)F*F*k;
+		double* o = out + 16*(size_t)k;
+		for (int i = 0; i < 4; ++i)
+			for (int j = 0; j < 4; ++j)
+				o[4*i + j] = (i < F && j < F) ? a[F*i + j] : (i == j && k == diag[row] ? 1.0 : 0.0);
+	}
+}
+
+// [F per node] <-> [4 per node, padding zero]
+template <int F>
+__global__ void SolvePadVectorKernel(int rows, const double* __restrict__ v, double* __restrict__ p)
+{
+	const int row = blockIdx.x*blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	for (int i = 0; i < 4; ++i) p[4*(size_t)row + i] = i < F ? v[(size_t)F*row + i] : 0.0;
+}
+
+template <int F>
+__global__ void SolveUnpadVectorKernel(int rows, const double* __restrict__ p, double* __restrict__ v)
+{
+	const int row = blockIdx.x*blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	for (int i = 0; i < F; ++i) v[(size_t)F*row + i] = p[4*(size_t)row + i];
+}
+
 __global__ void SolveValuesKernel(long long n, const int* __restrict__ constrained, const double* __restrict__ values,
 	double* __restrict__ v)
 {
@@ -598,7 +638,37 @@ void Assemble(tfi_solve* s)
 	int count = 0;
 	singular.Download(&count, 1);
 	s->singular = std::max(s->singular, count);
+	s->amg_stale = true;
 	Sync(ctx);
+}
+
+// The AMG hierarchy for the current D^-1 L (numeric set-up again after a
+// re-assembly); FP32 V-cycles
+void SetupAmg(tfi_solve* s)
+{
+	tfi_ctx* ctx = s->ctx;
+	const BsrView a = s->mesh->view();
+	const int N = s->mesh->nodes;
+	if (!s->amg) {
+		s->padded64.Allocate(ctx, (size_t)a.nnzb*16);
+		s->padded32.Allocate(ctx, (size_t)a.nnzb*16);
+		s->pad_in.Allocate(ctx, 4LL*N);
+		s->pad_out.Allocate(ctx, 4LL*N);
+		s->amg = std::make_unique<Amg>(s->mesh, TFI_MIXED, AmgOptions());
+	}
+	Dispatch(s->fields, [&](auto fc) {
+		constexpr int FF = decltype(fc)::value;
+		if constexpr (FF <= 4) {
+			hipLaunchKernelGGL(SolvePadBlocksKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
+				a.row_offsets, a.diagonal, s->scaled_left.data(), s->padded64.data());
+			TFI_LAUNCH_CHECK("SolvePadBlocksKernel");
+		} else {
+			throw Error("the AMG preconditioner of the configured transport supports at most 4 fields");
+		}
+	});
+	ToFloat(ctx, s->padded64.data(), s->padded32.data(), (long long)a.nnzb*16);
+	s->amg->Setup(s->padded64.data(), s->padded32.data());
+	s->amg_stale = false;
 }
 
 }  // namespace
@@ -698,6 +768,24 @@ int tfi_solve_step(tfi_solve* s, int zero_guess, int maximum_iterations, double 
 		BlockApply(s, s->rhs.data(), s->scaled.data());
 		Operator A = [&](const double* x, double* y) { Spmv(s, s->scaled_left.data(), x, y); };
 		Precond M = [&](const double* v, double* z) { Copy(ctx, v, z, n); };
+		if (s->preconditioner == TFI_PC_AMG) {
+			if (s->amg_stale) SetupAmg(s);
+			const int N = s->mesh->nodes;
+			M = [&, N](const double* v, double* z) {
+				Dispatch(s->fields, [&](auto fc) {
+					constexpr int FF = decltype(fc)::value;
+					if constexpr (FF <= 4) {
+						hipLaunchKernelGGL(SolvePadVectorKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream, N,
+							v, s->pad_in.data());
+						TFI_LAUNCH_CHECK("SolvePadVectorKernel");
+						s->amg->Apply(s->pad_in.data(), s->pad_out.data());
+						hipLaunchKernelGGL(SolveUnpadVectorKernel<FF>, dim3(Grid(N)), dim3(kThreads), 0, ctx->stream,
+							N, s->pad_out.data(), z);
+						TFI_LAUNCH_CHECK("SolveUnpadVectorKernel");
+					}
+				});
+			};
+		}
 		// the reference's GMRES stops at rtol times the initial preconditioned
 		// residual (Gmres.hpp:105-124)
 		A(s->next.data(), s->work.data());
@@ -746,5 +834,16 @@ int tfi_solve_bench_spmv(tfi_solve* s, int repetitions, double* milliseconds)
 		float ms = 0.0f;
 		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
 		*milliseconds = ms/std::max(1, repetitions);
+	});
+}
+
+int tfi_solve_set_preconditioner(tfi_solve* s, int preconditioner)
+{
+	return Guard(s->ctx, [&] {
+		if (preconditioner != TFI_PC_AMG && preconditioner != TFI_PC_BJACOBI)
+			throw Error("configured transport preconditioner must be TFI_PC_AMG or TFI_PC_BJACOBI");
+		if (preconditioner == TFI_PC_AMG && s->fields > 4)
+			throw Error("the AMG preconditioner of the configured transport supports at most 4 fields");
+		s->preconditioner = preconditioner;
 	});
 }

@@ -438,6 +438,10 @@ class ConfiguredTransport:
             np.ascontiguousarray(constrained, np.int32).ravel(), ctypes.byref(h)))
         self.handle = h
 
+    def set_preconditioner(self, preconditioner):
+        """TFI_PC_BJACOBI (default) or TFI_PC_AMG (at most 4 fields)."""
+        self.ctx.check(self.lib.tfi_solve_set_preconditioner(self.handle, preconditioner))
+
     def set_velocity(self, velocity):
         """Re-assemble for a new nodal velocity [N][3]."""
         self.ctx.check(self.lib.tfi_solve_set_velocity(self.handle,

@@ -730,6 +730,7 @@ struct SolveOptions {
 	std::string system;
 	int output_every = 0, checkpoint_every = 0, stop_after_step = 0;
 	double linear_rtol = 1e-8;
+	int preconditioner = TFI_PC_BJACOBI;
 };
 
 // ParseCudaConfiguredTransportOptions (ConfiguredTransportHost.inc:13-48), plus
@@ -740,7 +741,7 @@ SolveOptions ParseSolveOptions(int argc, char** argv)
 		throw std::runtime_error("usage: iga_hip solve DATABASE.ntiga CASE_DIR [SYSTEM] [OUTPUT] [VELOCITY] "
 			"[--system NAME] [--output PATH] [--velocity PATH] [--output-every N] [--checkpoint PREFIX] "
 			"[--checkpoint-every N] [--restart PREFIX] [--stop-after-step N] [--visualization-format auto|vtu] "
-			"[--linear-rtol R]");
+			"[--linear-rtol R] [--preconditioner bjacobi|amg]");
 	SolveOptions o;
 	o.database = argv[2];
 	o.case_dir = argv[3];
@@ -769,6 +770,7 @@ SolveOptions ParseSolveOptions(int argc, char** argv)
 			if (v == "vtkhdf") throw std::runtime_error("--visualization-format vtkhdf is not available in iga_hip; use vtu");
 			if (v != "auto" && v != "vtu") throw std::runtime_error("--visualization-format must be auto, vtu, or vtkhdf");
 		} else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
+		else if (arg == "--preconditioner") o.preconditioner = Choice(v, arg, {{"bjacobi", TFI_PC_BJACOBI}, {"amg", TFI_PC_AMG}});
 		else throw std::runtime_error("unknown option: " + arg);
 	}
 	if (o.output_every > 0 && o.output.empty()) throw std::runtime_error("--output-every requires --output");
@@ -932,6 +934,7 @@ int Solve(Ctx& c, const SolveCase& t)
 	const std::vector<double> first_velocity = series ? SeriesVelocity(t, (t.start_step + 1)*s.dt) : t.velocity;
 	c.Check(tfi_solve_create(m.mesh, &sys, first_velocity.data(), t.bc.constrained.data(), &h.s),
 		"configured transport assembly");
+	c.Check(tfi_solve_set_preconditioner(h.s, o.preconditioner), "configured transport preconditioner");
 	c.Check(tfi_solve_set_boundary(h.s, t.bc.value.data()), "configured transport boundary values");
 	c.Check(tfi_solve_set_state(h.s, t.state.data()), "configured transport state");
 	double assembly_s = Seconds(assembly_start, Clock::now());
