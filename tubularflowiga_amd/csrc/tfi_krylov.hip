@@ -20,7 +20,7 @@ namespace tfi {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPerThread = 8;
+constexpr int kPerThread = 2;
 constexpr int kGroup = 4;  // basis vectors per multi-dot pass
 
 __device__ inline double WaveSum(double v)
