@@ -380,6 +380,43 @@ def test_legacy_tangent_equals_exact_on_uniform_flow():
         assert np.max(np.abs(jl2 - je2)) > 1e-3 * np.abs(je2).max()
 
 
+def test_legacy_tangent_equals_exact_at_rest_with_viscosity():
+    """At the zero state (u = u_prev = 0, p = 0) with mu != 0 the fine scales and
+    their derivatives with respect to tau vanish, so the exact tangent's momentum
+    rows reduce to mass + mu grad N_a . grad N_b + mu d_jN_a d_iN_b +
+    rho tau_C d_iN_a d_jN_b and -d_iN_a N_b: the legacy tangent's momentum rows
+    (velocity-velocity and velocity-pressure) and its [3,3] entry must equal the
+    finite-difference-pinned exact tangent's.  This pins the legacy viscous and
+    tau_C terms (a transposed index in mu d_jN_a d_iN_b or a sign slip fails).
+    The continuity-velocity entries differ by the strong viscous term the legacy
+    tangent drops, -nu tau_M d_iN_a lap N_b, which vanishes with mu."""
+    curved = unit_bezier().copy()
+    curved[:, 0] += 0.08 * curved[:, 0] * curved[:, 1]
+    curved[:, 1] += 0.05 * curved[:, 1] * curved[:, 2]
+    curved[:, 2] += 0.04 * curved[:, 0] * curved[:, 2]
+    zero = np.zeros((64, 4))
+    for bez in (unit_bezier(), curved):
+        for rho, mu, dt in ((2.5, 0.17, 0.2), (1.0, 3.0, 0.05)):
+            jl, rl = O.ns_element(bez, EYE, zero, zero, rho, mu, dt, "legacy")
+            je, re = O.ns_element(bez, EYE, zero, zero, rho, mu, dt, "exact")
+            assert np.array_equal(rl, re)
+            L = jl.reshape(64, 4, 64, 4)
+            X = je.reshape(64, 4, 64, 4)
+            scale = np.abs(X).max()
+            assert np.max(np.abs(L[:, :3] - X[:, :3])) <= 1e-12 * scale
+            assert np.max(np.abs(L[:, 3, :, 3] - X[:, 3, :, 3])) <= 1e-12 * scale
+            # the viscous entries themselves are present (not vanishing)
+            visc = np.abs(L[:, 0, :, 1]).max()
+            assert visc > 1e-3 * scale
+            # continuity-velocity: the dropped strong viscous term is O(mu)
+            d = np.abs(L[:, 3, :, :3] - X[:, 3, :, :3]).max()
+            assert d > 1e-6 * scale
+            jl0, _ = O.ns_element(bez, EYE, zero, zero, rho, mu * 1e-6, dt, "legacy")
+            je0, _ = O.ns_element(bez, EYE, zero, zero, rho, mu * 1e-6, dt, "exact")
+            d0 = np.abs(jl0.reshape(64, 4, 64, 4)[:, 3, :, :3] - je0.reshape(64, 4, 64, 4)[:, 3, :, :3]).max()
+            assert d0 <= 2e-6 * d
+
+
 # ------------------------------------------------------------ global operators
 @pytest.fixture(scope="module", params=["tube_c0", "ybif_small"])
 def case(request):
