@@ -29,8 +29,8 @@ namespace tfi {
 
 namespace {
 
-constexpr int kSlab = 32;
 constexpr int kThreads = 256;
+constexpr size_t kLdsLimit = 160*1024;  // gfx950 LDS per workgroup
 constexpr int kQd = 56;       // per-point record in LDS
 constexpr int kBasisRec = 7;  // N, dN/dx, dN/dy, dN/dz, lap N, u.grad N, w.grad N (w = u + u')
 
@@ -182,6 +182,7 @@ struct AsmArgs {
 	double* negres;
 	const double* table;   // per-element basis records [q][a][kTableRec] at eoff[e]*kQ*kTableRec, or null
 	int qstride;           // doubles between the basis records of consecutive points (odd)
+	int max_nen;           // largest element of the mesh (LDS carve-up)
 };
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
@@ -211,9 +212,13 @@ __device__ inline void Term2(doublex4 (&c)[2], double a, double b0, double b1)
 // the LDS records, and two 16 x 16 output tiles per wave cover the trial
 // functions of the element's (padded) 32 x 32 basis pairs.
 // kJac: residual and tangent; otherwise the residual alone.
-template <bool kExact, bool kMfma, bool kJac>
+// kSlab: quadrature points per slab (32, or 16 / 8 for elements whose basis
+// records would not fit the LDS at 32 points: nen up to 80).
+template <bool kExact, bool kMfma, bool kJac, int kSlab>
 __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs args)
 {
+	static_assert(kSlab == 32 || kSlab == 16 || kSlab == 8, "slab of 8, 16 or 32 points");
+	static_assert(!kMfma || kSlab == 32, "the MFMA tangent works on slabs of 32 points");
 	constexpr int kT = kMfma ? 512 : 256;  // threads of the workgroup
 	extern __shared__ double lds[];
 	const int tid = threadIdx.x;
@@ -231,9 +236,10 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 
 	// LDS carve-up
 	const int qstride = args.qstride;
+	const double* table = args.table;
 	double* nodal = lds;                                   // [nen][8]
-	double* C = nodal + kMaxBasis*8;                        // [nen][64]
-	double* basis = C + (size_t)nen*64;                     // [kSlab][qstride] records
+	double* C = nodal + kMaxBasis*8;                        // [nen][64], only without the table
+	double* basis = C + (table ? 0 : (size_t)args.max_nen*64);  // [kSlab][qstride] records
 	double* qd = basis + (size_t)kSlab*qstride;             // [kSlab][kQd]
 	double* resid = qd + kSlab*kQd;                         // [a_tile][4]
 	int* lconn = reinterpret_cast<int*>(resid + 4*kMaxBasis);
@@ -244,7 +250,6 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 		for (int f = 0; f < 4; ++f) nodal[8*a + f] = args.state[4*(size_t)node + f];
 		for (int f = 0; f < 3; ++f) nodal[8*a + 4 + f] = args.previous[4*(size_t)node + f];
 	}
-	const double* table = args.table;
 	if (!kMfma && !table)
 		for (int i = tid; i < nen*64; i += kT) C[i] = 0.0;
 	for (int i = tid; i < 4*kMaxBasis; i += kT) resid[i] = 0.0;
@@ -337,10 +342,10 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 					R[3] = -(N*r[18] - gx*r[19] - gy*r[20] - gz*r[21])*m;
 				}
 			}
-			// 32 consecutive lanes hold the 32 points of one a: fixed-order tree sum
+			// kSlab consecutive lanes hold the slab's points of one a: fixed-order tree sum
 #pragma unroll
 			for (int c = 0; c < 4; ++c)
-				for (int off = 1; off < 32; off <<= 1) R[c] += __shfl_xor(R[c], off, 64);
+				for (int off = 1; off < kSlab; off <<= 1) R[c] += __shfl_xor(R[c], off, 64);
 			if (valid && ql == 0 && a >= a0 && a < a1)
 				for (int c = 0; c < 4; ++c) resid[4*(a - a0) + c] += R[c];
 		}
@@ -969,22 +974,62 @@ void SetBasisTable(tfi_mesh* m, int enable)
 	Sync(ctx);
 }
 
+namespace {
+
+// LDS bytes of one AssembleNsKernel workgroup (nodal records, extraction rows
+// when there is no basis table, slab basis records, point records, residual
+// rows, connectivity).
+size_t AssemblyLds(int nen_max, bool table, int slab, int qstride)
+{
+	return sizeof(double)*(kMaxBasis*8 + (table ? 0 : (size_t)nen_max*64) + (size_t)slab*qstride + (size_t)slab*kQd
+		+ 4*kMaxBasis) + sizeof(int)*kMaxBasis;
+}
+
+template <bool kExact, bool kMfma, bool kJac, int kSlab>
+void LaunchAssembly(tfi_mesh* m, AsmArgs a, int tiles, size_t lds)
+{
+	const void* kernel = reinterpret_cast<const void*>(AssembleNsKernel<kExact, kMfma, kJac, kSlab>);
+	static size_t raised = 0;  // per instantiation
+	if (lds > 65536 && lds > raised) {
+		Check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+			"hipFuncSetAttribute");
+		raised = lds;
+	}
+	tfi_ctx* ctx = m->ctx;
+	for (int c = 0; c < m->colors; ++c) {
+		a.elements = m->color_elements.data() + m->color_offsets[c];
+		a.count = m->color_offsets[c + 1] - m->color_offsets[c];
+		if (a.count == 0) continue;
+		hipLaunchKernelGGL((AssembleNsKernel<kExact, kMfma, kJac, kSlab>), dim3(a.count*tiles),
+			dim3(kMfma ? 512 : kThreads), lds, ctx->stream, a);
+		TFI_LAUNCH_CHECK("AssembleNsKernel");
+	}
+}
+
+template <bool kExact, bool kJac>
+void LaunchVectorAssembly(tfi_mesh* m, const AsmArgs& a, int tiles, size_t lds, int slab)
+{
+	switch (slab) {
+	case 32: LaunchAssembly<kExact, false, kJac, 32>(m, a, tiles, lds); break;
+	case 16: LaunchAssembly<kExact, false, kJac, 16>(m, a, tiles, lds); break;
+	default: LaunchAssembly<kExact, false, kJac, 8>(m, a, tiles, lds); break;
+	}
+}
+
+}  // namespace
+
 void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const NsParams& p, double* jacobian,
 	double* negative_residual)
 {
 	const bool exact = p.tangent == TFI_TANGENT_EXACT;
 	const bool jac = jacobian != nullptr;
 	const int nen_max = m->max_nen;
-	// elements of at most 32 basis functions take the FP64-MFMA tangent (which
-	// reads the basis table)
-	const bool mfma = jac && nen_max <= 32 && m->basis_table.size() > 0;
-	const int variant = jac ? (exact ? 1 : 0) + (mfma ? 2 : 0) : 4;
-	static const void* const kKernels[5] = {
-		reinterpret_cast<const void*>(AssembleNsKernel<false, false, true>),
-		reinterpret_cast<const void*>(AssembleNsKernel<true, false, true>),
-		reinterpret_cast<const void*>(AssembleNsKernel<false, true, true>),
-		reinterpret_cast<const void*>(AssembleNsKernel<true, true, true>),
-		reinterpret_cast<const void*>(AssembleNsKernel<false, false, false>)};
+	if (nen_max > kMaxBasis) throw Error("element assembly supports at most 80 basis functions per element");
+	const bool table = m->basis_table.size() > 0;
+	// elements of at most 32 basis functions take the FP64-MFMA tangent, which
+	// reads the basis table (evaluating the basis in the same kernel would
+	// push it past its 256 registers)
+	const bool mfma = jac && nen_max <= 32 && table;
 	tfi_ctx* ctx = m->ctx;
 	if (jacobian) Fill(ctx, jacobian, 0.0, m->nnzb*16);
 	Fill(ctx, negative_residual, 0.0, 4LL*m->nodes);
@@ -996,8 +1041,14 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	// same field of different points hit different LDS banks
 	const int records = mfma ? 32 : nen_max;
 	const int qstride = records*kBasisRec | 1;
-	const size_t lds = sizeof(double)*(kMaxBasis*8 + (size_t)nen_max*64 + (size_t)kSlab*qstride
-		+ kSlab*kQd + 4*kMaxBasis) + sizeof(int)*kMaxBasis;
+	// slab of quadrature points: the largest of 32 / 16 / 8 whose LDS fits
+	int slab = 32;
+	if (!mfma)
+		while (slab > 8 && AssemblyLds(nen_max, table, slab, qstride) > kLdsLimit) slab >>= 1;
+	const size_t lds = AssemblyLds(nen_max, table, slab, qstride);
+	if (lds > kLdsLimit)
+		throw Error("element assembly needs " + std::to_string(lds) + " bytes of LDS for " + std::to_string(nen_max)
+			+ " basis functions per element, above the " + std::to_string(kLdsLimit) + " bytes of a workgroup");
 	AsmArgs a{};
 	a.eoff = m->eoff.data();
 	a.conn = m->conn.data();
@@ -1016,18 +1067,11 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	a.dt = p.dt;
 	a.jac = jacobian;
 	a.negres = negative_residual;
-	a.table = m->basis_table.size() ? m->basis_table.data() : nullptr;
+	a.table = table ? m->basis_table.data() : nullptr;
 	a.qstride = qstride;
-	if (lds > 65536) {
-		static size_t raised[5] = {0, 0, 0, 0, 0};
-		if (lds > raised[variant]) {
-			Check(hipFuncSetAttribute(kKernels[variant], hipFuncAttributeMaxDynamicSharedMemorySize,
-				static_cast<int>(lds)), "hipFuncSetAttribute");
-			raised[variant] = lds;
-		}
-	}
+	a.max_nen = nen_max;
 	StreamTimer timer(ctx, jac ? &ctx->timers.assembly_ms : &ctx->timers.residual_ms);
-	if (!jac && a.table && nen_max <= 32) {
+	if (!jac && table && nen_max <= 32) {
 		for (int c = 0; c < m->colors; ++c) {
 			a.elements = m->color_elements.data() + m->color_offsets[c];
 			a.count = m->color_offsets[c + 1] - m->color_offsets[c];
@@ -1035,19 +1079,15 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 			hipLaunchKernelGGL(NsResidualKernel, dim3(a.count), dim3(kThreads), 0, ctx->stream, a);
 			TFI_LAUNCH_CHECK("NsResidualKernel");
 		}
-	} else for (int c = 0; c < m->colors; ++c) {
-		a.elements = m->color_elements.data() + m->color_offsets[c];
-		a.count = m->color_offsets[c + 1] - m->color_offsets[c];
-		if (a.count == 0) continue;
-		const dim3 grid(a.count*tiles), block(mfma ? 512 : kThreads);
-		switch (variant) {
-		case 0: hipLaunchKernelGGL((AssembleNsKernel<false, false, true>), grid, block, lds, ctx->stream, a); break;
-		case 1: hipLaunchKernelGGL((AssembleNsKernel<true, false, true>), grid, block, lds, ctx->stream, a); break;
-		case 2: hipLaunchKernelGGL((AssembleNsKernel<false, true, true>), grid, block, lds, ctx->stream, a); break;
-		case 3: hipLaunchKernelGGL((AssembleNsKernel<true, true, true>), grid, block, lds, ctx->stream, a); break;
-		default: hipLaunchKernelGGL((AssembleNsKernel<false, false, false>), grid, block, lds, ctx->stream, a); break;
-		}
-		TFI_LAUNCH_CHECK("AssembleNsKernel");
+	} else if (mfma) {
+		if (exact) LaunchAssembly<true, true, true, 32>(m, a, tiles, lds);
+		else LaunchAssembly<false, true, true, 32>(m, a, tiles, lds);
+	} else if (!jac) {
+		LaunchVectorAssembly<false, false>(m, a, tiles, lds, slab);
+	} else if (exact) {
+		LaunchVectorAssembly<true, true>(m, a, tiles, lds, slab);
+	} else {
+		LaunchVectorAssembly<false, true>(m, a, tiles, lds, slab);
 	}
 	if (jac) {
 		ctx->timers.assembly_calls += 1;
