@@ -14,25 +14,24 @@ int main(int argc, char** argv)
 {
 	try {
 		if (argc != 11) throw std::runtime_error("usage: outlet_host_check KIND R RP RD C PREF PCAP PRESSURE DT FLOW");
-		tfi_host::OutletModel m;
-		m.label = 2;
-		m.kind = argv[1];
-		if (!tfi_host::IsOutletKind(m.kind)) throw std::runtime_error("unknown outlet kind");
-		m.resistance = std::stod(argv[2]);
-		m.proximal_resistance = std::stod(argv[3]);
-		m.distal_resistance = std::stod(argv[4]);
-		m.capacitance = std::stod(argv[5]);
-		m.reference_pressure = std::stod(argv[6]);
-		m.capacitor_pressure = std::stod(argv[7]);
-		m.pressure = std::stod(argv[8]);
+		tfi_host::OutletKind kind;
+		if (!tfi_host::ParseOutletKind(argv[1], &kind)) throw std::runtime_error("unknown outlet kind");
 		const double dt = std::stod(argv[9]), flow = std::stod(argv[10]);
-		const auto e = tfi_host::EvaluateOutlet(m, m.capacitor_pressure, flow, dt);
-		const auto c = tfi_host::EvaluateOutlets({m}, {m.capacitor_pressure}, {flow}, dt);
-		std::vector<tfi_host::OutletModel> relaxed{m}, committed{m};
-		tfi_host::RelaxOutlets(relaxed, c);
-		tfi_host::CommitOutlets(committed, c);
-		for (double v : {e.pressure, e.capacitor_pressure, c.maximum_pressure_change, c.Tolerance(), relaxed[0].pressure,
-			committed[0].pressure, committed[0].capacitor_pressure})
+		tfi_host::OutletBank bank;
+		bank.Add(2, kind, std::stod(argv[2]), std::stod(argv[3]), std::stod(argv[4]), std::stod(argv[5]),
+			std::stod(argv[6]), std::stod(argv[7]), std::stod(argv[8]));
+		tfi_host::OutletBank relaxed = bank, committed = bank;
+		bank.StartStep();
+		const double change = bank.Respond({flow}, dt);
+		relaxed.StartStep();
+		relaxed.Respond({flow}, dt);
+		relaxed.Relax();
+		committed.StartStep();
+		committed.Respond({flow}, dt);
+		committed.Commit();
+		// the model's own response: the committed pressure and capacitor pressure
+		for (double v : {committed.applied(0), committed.capacitor(0), change, bank.Tolerance(), relaxed.applied(0),
+			committed.applied(0), committed.capacitor(0)})
 			std::printf("%.17g\n", v);
 		return 0;
 	} catch (const std::exception& ex) {

@@ -126,6 +126,9 @@ def host_check(tmp_path_factory):
 
 
 def test_driver_models_match_the_reference_header_outputs(host_check):
+    """The driver's outlet bank reproduces the reference header's outputs on
+    every golden case (to rounding: it evaluates each model as an affine
+    response in the flow), and its error messages."""
     for r in GOLDEN["evaluations"]:
         args = r["args"]
         out = subprocess.run([str(host_check), args[0], *map(repr, args[1:])], capture_output=True, text=True)
@@ -133,7 +136,12 @@ def test_driver_models_match_the_reference_header_outputs(host_check):
             assert out.returncode == 1 and r["error"] in out.stderr, (r, out.stderr)
         else:
             assert out.returncode == 0, out.stderr
-            assert [float(v) for v in out.stdout.split()] == r["out"], r
+            got = [float(v) for v in out.stdout.split()]
+            # the bank evaluates each model as an affine response in the flow:
+            # the reference's values up to rounding
+            assert len(got) == len(r["out"]), r
+            for g, w in zip(got, r["out"]):
+                assert abs(g - w) <= 4e-15 * max(abs(w), 1.0), (r, got)
 
 
 @pytest.mark.skipif(not CLI.exists(), reason="iga_hip is not built (run __graft_entry__.build())")

@@ -10,7 +10,11 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
+#include <cerrno>
+#include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <filesystem>
@@ -143,119 +147,181 @@ inline FlatDatabase ReadDatabase(const std::string& path)
 }
 
 // ---------------------------------------------------------------- control mesh
-// ReadLabeledHexMesh (solvers/cpu/include/BoundarySupport.hpp:25-73)
+// The labelled control mesh controlmesh.vtk: eight-node hexahedra and the
+// integer point label array.  What is accepted and each error message follow
+// ReadLabeledHexMesh (solvers/cpu/include/BoundarySupport.hpp:25-73); the
+// reader is a cursor over the whole file held in memory.
 struct LabeledHexMesh {
 	std::vector<std::array<std::int32_t, 8>> cells;
 	std::vector<int> labels;
 };
 
+class VtkTokenCursor {
+public:
+	explicit VtkTokenCursor(const std::string& path)
+	{
+		std::ifstream in(path, std::ios::binary);
+		if (!in) throw std::runtime_error("cannot open mesh: " + path);
+		text_.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+	}
+	// next whitespace-separated token; false at the end of the file
+	bool Token(std::string* out)
+	{
+		const std::size_t n = text_.size();
+		while (pos_ < n && std::isspace(static_cast<unsigned char>(text_[pos_]))) ++pos_;
+		if (pos_ >= n) return false;
+		const std::size_t begin = pos_;
+		while (pos_ < n && !std::isspace(static_cast<unsigned char>(text_[pos_]))) ++pos_;
+		out->assign(text_, begin, pos_ - begin);
+		return true;
+	}
+	// moves past the next token equal to `keyword`; false if there is none
+	bool After(const char* keyword)
+	{
+		std::string t;
+		while (Token(&t))
+			if (t == keyword) return true;
+		return false;
+	}
+	bool Real(double* v)
+	{
+		std::string t;
+		if (!Token(&t)) return false;
+		char* end = nullptr;
+		*v = std::strtod(t.c_str(), &end);
+		return end && *end == '\0';
+	}
+	bool Integer(long long* v)
+	{
+		std::string t;
+		if (!Token(&t)) return false;
+		char* end = nullptr;
+		errno = 0;
+		*v = std::strtoll(t.c_str(), &end, 10);
+		return end && *end == '\0' && errno == 0;
+	}
+	bool Count(std::uint64_t* v)
+	{
+		long long x = 0;
+		if (!Integer(&x) || x < 0) return false;
+		*v = static_cast<std::uint64_t>(x);
+		return true;
+	}
+
+private:
+	std::string text_;
+	std::size_t pos_ = 0;
+};
+
 inline LabeledHexMesh ReadLabeledHexMesh(const std::string& path, std::uint64_t nodes_expected,
 	std::uint64_t elements_expected)
 {
-	std::ifstream in(path);
-	if (!in) throw std::runtime_error("cannot open mesh: " + path);
-	std::string token;
-	std::uint64_t nodes = 0;
-	while (in >> token)
-		if (token == "POINTS") {
-			std::string type;
-			if (!(in >> nodes >> type)) throw std::runtime_error("invalid POINTS record");
-			break;
-		}
-	if (nodes != nodes_expected) throw std::runtime_error("VTK point count does not match database nodes");
-	for (std::uint64_t i = 0; i < 3*nodes; ++i) {
-		double ignored;
-		if (!(in >> ignored)) throw std::runtime_error("VTK POINTS record is truncated");
-	}
-	std::uint64_t cells = 0, values = 0;
-	while (in >> token)
-		if (token == "CELLS") {
-			if (!(in >> cells >> values)) throw std::runtime_error("invalid CELLS record");
-			break;
-		}
-	if (cells != elements_expected) throw std::runtime_error("VTK cell count does not match database elements");
+	VtkTokenCursor vtk(path);
+	std::string word;
+	std::uint64_t points = 0;
+	if (vtk.After("POINTS") && !(vtk.Count(&points) && vtk.Token(&word)))
+		throw std::runtime_error("invalid POINTS record");
+	if (points != nodes_expected) throw std::runtime_error("VTK point count does not match database nodes");
+	double coordinate = 0.0;
+	for (std::uint64_t k = 3*points; k > 0; --k)
+		if (!vtk.Real(&coordinate)) throw std::runtime_error("VTK POINTS record is truncated");
+	std::uint64_t cell_count = 0, cell_values = 0;
+	if (vtk.After("CELLS") && !(vtk.Count(&cell_count) && vtk.Count(&cell_values)))
+		throw std::runtime_error("invalid CELLS record");
+	if (cell_count != elements_expected) throw std::runtime_error("VTK cell count does not match database elements");
 	LabeledHexMesh mesh;
-	mesh.cells.resize(cells);
-	for (auto& cell : mesh.cells) {
-		int count = 0;
-		if (!(in >> count) || count != 8) throw std::runtime_error("only eight-node VTK hexahedra are supported");
-		for (auto& node : cell)
-			if (!(in >> node) || node < 0 || static_cast<std::uint64_t>(node) >= nodes)
+	mesh.cells.assign(cell_count, {});
+	const long long limit = static_cast<long long>(points);
+	for (std::uint64_t e = 0; e < cell_count; ++e) {
+		long long corners = 0;
+		if (!vtk.Integer(&corners) || corners != 8) throw std::runtime_error("only eight-node VTK hexahedra are supported");
+		for (int c = 0; c < 8; ++c) {
+			long long id = -1;
+			if (!vtk.Integer(&id) || id < 0 || id >= limit || id > INT32_MAX)
 				throw std::runtime_error("invalid hexahedron connectivity");
+			mesh.cells[e][c] = static_cast<std::int32_t>(id);
+		}
 	}
 	std::uint64_t point_data = 0;
-	while (in >> token)
-		if (token == "POINT_DATA") {
-			if (!(in >> point_data)) throw std::runtime_error("invalid POINT_DATA record");
-			break;
-		}
-	if (point_data != nodes) throw std::runtime_error("VTK POINT_DATA count does not match database nodes");
-	while (in >> token)
-		if (token == "LOOKUP_TABLE") {
-			in >> token;
-			break;
-		}
-	if (!in) throw std::runtime_error("VTK point labels were not found");
-	mesh.labels.resize(nodes);
+	if (vtk.After("POINT_DATA") && !vtk.Count(&point_data)) throw std::runtime_error("invalid POINT_DATA record");
+	if (point_data != points) throw std::runtime_error("VTK POINT_DATA count does not match database nodes");
+	if (!(vtk.After("LOOKUP_TABLE") && vtk.Token(&word))) throw std::runtime_error("VTK point labels were not found");
+	mesh.labels.assign(points, 0);
 	for (auto& label : mesh.labels) {
-		double v = 0.0;
-		if (!(in >> v)) throw std::runtime_error("VTK point-label array is truncated");
-		label = static_cast<int>(v);
-		if (v != static_cast<double>(label)) throw std::runtime_error("VTK point label is not integral");
+		double value = 0.0;
+		if (!vtk.Real(&value)) throw std::runtime_error("VTK point-label array is truncated");
+		if (!(std::floor(value) == value) || std::abs(value) > 2147483647.0)
+			throw std::runtime_error("VTK point label is not integral");
+		label = static_cast<int>(value);
 	}
 	return mesh;
 }
 
-// WallTraceBasis (BoundarySupport.hpp:75-150): every basis with a nonzero
-// extraction coefficient on a Bezier column of an external control-mesh face
-// whose four corners are labelled wall (0).
+// Wall trace: the basis functions that do not vanish on the wall.  A face of
+// the control mesh is external when exactly one hexahedron uses it (a face
+// shared by more than two is an error), and a wall face when its four corner
+// points are labelled 0; every basis function of the face's element with a
+// non-zero extraction coefficient on one of the 16 Bezier points of that face
+// is in the trace.  Same rule and messages as WallTraceBasis
+// (solvers/cpu/include/BoundarySupport.hpp:75-150).  Here the faces are found
+// by sorting one record per (element, local face) on its sorted corner ids,
+// and each local face's Bezier points are a 64-bit mask.
 inline std::vector<std::int32_t> WallTraceBasis(const FlatDatabase& db, const LabeledHexMesh& mesh)
 {
-	static const int corners[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
-	struct Face {
-		std::size_t element = 0;
-		int local = -1;
-		std::array<std::int32_t, 4> nodes{};
-		int uses = 0;
-	};
-	std::map<std::array<std::int32_t, 4>, Face> faces;
-	for (std::size_t e = 0; e < mesh.cells.size(); ++e)
-		for (int f = 0; f < 6; ++f) {
-			std::array<std::int32_t, 4> nodes{};
-			for (int c = 0; c < 4; ++c) nodes[c] = mesh.cells[e][corners[f][c]];
-			auto key = nodes;
-			std::sort(key.begin(), key.end());
-			Face& face = faces[key];
-			if (face.uses == 0) {
-				face.element = e;
-				face.local = f;
-				face.nodes = nodes;
-			}
-			if (++face.uses > 2) throw std::runtime_error("non-manifold control-mesh face");
-		}
-	std::set<std::int32_t> result;
-	for (const auto& item : faces) {
-		const Face& face = item.second;
-		if (face.uses != 1) continue;
-		if (!std::all_of(face.nodes.begin(), face.nodes.end(), [&](std::int32_t n) { return mesh.labels.at(n) == 0; }))
-			continue;
-		if (db.ids.at(face.element) != face.element)
-			throw std::runtime_error("database element IDs do not match VTK cell order");
-		const int b0 = db.element_offsets[face.element], nen = db.element_offsets[face.element + 1] - b0;
-		for (int a = 0; a < nen; ++a)
-			for (int k = db.extraction_offsets[b0 + a]; k < db.extraction_offsets[b0 + a + 1]; ++k) {
-				const int p = db.extraction_columns[k];
-				const int i = p % 4, j = (p/4) % 4, kk = p/16;
-				const int f = face.local;
-				const bool on = (f == 0 && kk == 0) || (f == 1 && j == 0) || (f == 2 && i == 3) || (f == 3 && j == 3)
-					|| (f == 4 && i == 0) || (f == 5 && kk == 3);
-				if (on) {
-					result.insert(db.connectivity[b0 + a]);
-					break;
-				}
-			}
+	// corner tuples of the six local faces, and the Bezier points (i + 4j + 16k) on each
+	static const int kFaceCorners[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3},
+		{4, 5, 6, 7}};
+	std::uint64_t face_mask[6] = {0, 0, 0, 0, 0, 0};
+	for (int pt = 0; pt < 64; ++pt) {
+		const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
+		const bool on[6] = {k == 0, j == 0, i == 3, j == 3, i == 0, k == 3};
+		for (int f = 0; f < 6; ++f)
+			if (on[f]) face_mask[f] |= std::uint64_t{1} << pt;
 	}
-	return std::vector<std::int32_t>(result.begin(), result.end());
+	struct Record {
+		std::array<std::int32_t, 4> key;
+		std::uint32_t element;
+		std::uint8_t face;
+	};
+	const std::size_t E = mesh.cells.size();
+	std::vector<Record> records(6*E);
+	for (std::size_t e = 0; e < E; ++e)
+		for (int f = 0; f < 6; ++f) {
+			Record& r = records[6*e + f];
+			for (int c = 0; c < 4; ++c) r.key[c] = mesh.cells[e][kFaceCorners[f][c]];
+			std::sort(r.key.begin(), r.key.end());
+			r.element = static_cast<std::uint32_t>(e);
+			r.face = static_cast<std::uint8_t>(f);
+		}
+	std::sort(records.begin(), records.end(), [](const Record& a, const Record& b) {
+		return a.key != b.key ? a.key < b.key : 6ULL*a.element + a.face < 6ULL*b.element + b.face;
+	});
+	std::vector<const Record*> external;
+	for (std::size_t lo = 0, hi = 0; lo < records.size(); lo = hi) {
+		hi = lo + 1;
+		while (hi < records.size() && records[hi].key == records[lo].key) ++hi;
+		if (hi - lo > 2) throw std::runtime_error("non-manifold control-mesh face");
+		if (hi - lo == 1) external.push_back(&records[lo]);
+	}
+	std::vector<char> in_trace(db.nodes, 0);
+	for (const Record* r : external) {
+		const std::size_t e = r->element;
+		bool wall = true;
+		for (int c = 0; c < 4; ++c) wall = wall && mesh.labels.at(mesh.cells[e][kFaceCorners[r->face][c]]) == 0;
+		if (!wall) continue;
+		if (db.ids.at(e) != e) throw std::runtime_error("database element IDs do not match VTK cell order");
+		const std::uint64_t mask = face_mask[r->face];
+		for (int a = db.element_offsets[e]; a < db.element_offsets[e + 1]; ++a) {
+			std::uint64_t row = 0;
+			for (int k = db.extraction_offsets[a]; k < db.extraction_offsets[a + 1]; ++k)
+				if (db.extraction_values[k] != 0.0) row |= std::uint64_t{1} << db.extraction_columns[k];
+			if (row & mask) in_trace.at(static_cast<std::size_t>(db.connectivity[a])) = 1;
+		}
+	}
+	std::vector<std::int32_t> trace;
+	for (std::size_t n = 0; n < in_trace.size(); ++n)
+		if (in_trace[n]) trace.push_back(static_cast<std::int32_t>(n));
+	return trace;
 }
 
 // ReadVelocity (solvers/cpu/include/CaseInput.hpp:72-83)

@@ -241,8 +241,8 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	const double dt = transient ? config.dt : 0.0;
 	const int physical_steps = transient ? config.steps : 1;
 	auto step_config = [&](double time) { return transient ? MaterializeWaveforms(config, o.case_dir, time) : config; };
-	std::vector<OutletModel> outlets = InitOutletModels(config, sys);
-	auto outlet_p = OutletPressures(outlets);
+	OutletBank outlets = OutletBank::FromConfig(config, sys);
+	auto outlet_p = outlets.Tractions();
 	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(step_config(0.0), sys, labeled.labels, profile, &outlet_p);
 	ApplyWallTrace(bc, wall_trace);
 	if (o.stop_after_step > physical_steps) throw std::runtime_error("--stop-after-step exceeds configured physical steps");
@@ -321,7 +321,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		ValidateCheckpoint(meta, db.nodes, physical_steps, dt, sys.density, sys.viscosity);
 		state = ReadCheckpointState(o.restart, meta);
 		start_step = meta.completed_step;
-		RestoreOutlets(meta.outlets, outlets);
+		outlets.Restore(meta.outlets);
 		std::cout << "restart=" << o.restart.string() << " completed_step=" << start_step
 			<< " physical_time=" << meta.physical_time << '\n';
 	} else if (transient) {
@@ -346,8 +346,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	long long total_linear_iterations = 0;
 	double total_assembly = 0.0, total_linear = 0.0;
 	std::vector<double> traction(4*N, 0.0);
-	std::vector<std::int32_t> outlet_labels;
-	for (const auto& om : outlets) outlet_labels.push_back(om.label);
+	const std::vector<std::int32_t> outlet_labels(outlets.labels().begin(), outlets.labels().end());
 	for (int step = start_step; step < run_end; ++step) {
 		const double time = transient ? (step + 1)*dt : 0.0;
 		const SimulationConfig cfg = step_config(time);
@@ -361,13 +360,12 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		// Outlet fixed point (iga_cuda.cu:927-1084): each pass solves the step
 		// with the outlet models' pressures as tractions, from where the last
 		// pass ended; the models then turn the outlet flows into new pressures.
-		std::vector<double> capacitor_start;
-		for (const auto& om : outlets) capacitor_start.push_back(om.capacitor_pressure);
+		outlets.StartStep();
 		const int passes = outlets.empty() ? 1 : 12;
 		bool outlets_converged = false;
 		tfi_newton_report rep{}, first_rep{};  // last and first outlet pass of the step
 		for (int coupling = 0; coupling < passes; ++coupling) {
-			outlet_p = OutletPressures(outlets);
+			outlet_p = outlets.Tractions();
 			bc = ResolveFlowBoundaries(cfg, sys, labeled.labels, profile, &outlet_p);
 			ApplyWallTrace(bc, wall_trace);
 			std::vector<std::int32_t> te, tf;
@@ -422,18 +420,18 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 			std::vector<double> flows(outlets.size());
 			c.Check(tfi_flow_boundary_flows(flow.flow, static_cast<int>(outlet_labels.size()), outlet_labels.data(),
 				flows.data()), "outlet flows");
-			const OutletCoupling e = EvaluateOutlets(outlets, capacitor_start, flows, dt);
+			const double change = outlets.Respond(flows, dt);
 			std::cout << "step=" << step + 1 << " time=" << time << " outlet_iteration=" << coupling
-				<< " pressure_change=" << e.maximum_pressure_change << " tolerance=" << e.Tolerance() << '\n';
-			if (e.maximum_pressure_change <= e.Tolerance()) {
-				CommitOutlets(outlets, e);
-				for (const auto& om : outlets)
-					std::cout << "outlet label=" << om.label << " flow=" << om.flow << " pressure=" << om.pressure
-						<< " capacitor_pressure=" << om.capacitor_pressure << '\n';
+				<< " pressure_change=" << change << " tolerance=" << outlets.Tolerance() << '\n';
+			if (change <= outlets.Tolerance()) {
+				outlets.Commit();
+				for (std::size_t i = 0; i < outlets.size(); ++i)
+					std::cout << "outlet label=" << outlets.labels()[i] << " flow=" << outlets.flow(i)
+						<< " pressure=" << outlets.applied(i) << " capacitor_pressure=" << outlets.capacitor(i) << '\n';
 				outlets_converged = true;
 				break;
 			}
-			RelaxOutlets(outlets, e);
+			outlets.Relax();
 		}
 		if (!outlets_converged)
 			throw std::runtime_error("outlet fixed-point iteration did not converge at physical step "
@@ -462,7 +460,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 			meta.dt = dt;
 			meta.density = sys.density;
 			meta.viscosity = sys.viscosity;
-			meta.outlets = OutletCheckpoint(outlets);
+			meta.outlets = outlets.Checkpoint();
 			WriteCheckpoint(o.checkpoint, meta, state);
 			std::cout << "checkpoint=" << o.checkpoint.string() << " completed_step=" << completed << '\n';
 		}
@@ -525,8 +523,8 @@ int CaseCheck(int argc, char** argv)
 	const FlowSystem& sys = FindFlowSystem(config, system);
 	const bool transient = sys.time_integration == "backward_euler";
 	const double time = transient ? config.dt : 0.0;
-	const auto outlets = InitOutletModels(config, sys);
-	const auto outlet_p = OutletPressures(outlets);
+	const OutletBank outlets = OutletBank::FromConfig(config, sys);
+	const auto outlet_p = outlets.Tractions();
 	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(transient ? MaterializeWaveforms(config, case_dir, time) : config,
 		sys, labeled.labels, profile, &outlet_p);
 	ApplyWallTrace(bc, trace);
@@ -550,9 +548,9 @@ int CaseCheck(int argc, char** argv)
 		first = false;
 	}
 	std::cout << '\n';
-	for (const auto& om : outlets)
-		std::cout << "outlet label=" << om.label << " kind=" << om.kind << " pressure=" << om.pressure
-			<< " capacitor_pressure=" << om.capacitor_pressure << '\n';
+	for (std::size_t i = 0; i < outlets.size(); ++i)
+		std::cout << "outlet label=" << outlets.labels()[i] << " kind=" << outlets.kind(i) << " pressure="
+			<< outlets.applied(i) << " capacitor_pressure=" << outlets.capacitor(i) << '\n';
 	return 0;
 }
 

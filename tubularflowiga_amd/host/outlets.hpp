@@ -1,9 +1,19 @@
 // Lumped outlet models coupled to the Navier-Stokes step: resistance, RC and
-// RCR Windkessel outlets (include/OutletModel.hpp, include/OutletCheckpoint.hpp), the
-// driver-level fixed point the reference runs around each time step
-// (iga_cuda.cu:927-1084).  Each model turns the volumetric flow through its
-// outlet label into a pressure that the step applies as a pressure traction;
-// the step is repeated with relaxed pressures until they settle.
+// RCR Windkessel outlets.  The models, their tolerance and the under-relaxed
+// fixed point are the reference's contract (include/OutletModel.hpp:70-149,
+// the driver loop of iga_cuda.cu:927-1084, checkpoint fields of
+// include/OutletCheckpoint.hpp:22-48); the organisation here is different.
+//
+// All outlets of a case live in one OutletBank, as parallel arrays.  Over one
+// time step every model is affine in its outlet flow Q, because the
+// capacitor pressure it starts from is fixed for the step:
+//   resistance       p = p_ref + R Q
+//   RC  (R_d = R)    pc = (C pc0/dt + p_ref/R_d + Q) / (C/dt + 1/R_d),  p = pc
+//   RCR              pc as RC with the distal R_d,                      p = pc + R_p Q
+// (backward Euler for C dpc/dt = Q - (pc - p_ref)/R_d).  StartStep() fixes
+// pc0; each outlet pass then evaluates the responses of all outlets to the
+// measured flows, compares them with the pressures the pass applied, and
+// either relaxes the applied pressures by 1/2 or commits the responses.
 #pragma once
 
 #include "case_input.hpp"
@@ -17,140 +27,171 @@
 
 namespace tfi_host {
 
-struct OutletModel {
-	int label = -1;
-	std::string kind;  // "resistance", "windkessel_rc" or "windkessel_rcr"
-	double resistance = 0.0, proximal_resistance = 0.0, distal_resistance = 0.0, capacitance = 0.0;
-	double reference_pressure = 0.0;
-	double capacitor_pressure = 0.0;  // committed state of the capacitor
-	double flow = 0.0;
-	double pressure = 0.0;            // the pressure the next solve applies
-};
+enum class OutletKind { kResistance, kRc, kRcr };
 
-inline bool IsOutletKind(const std::string& type)
+inline bool ParseOutletKind(const std::string& name, OutletKind* kind)
 {
-	return type == "resistance" || type == "windkessel_rc" || type == "windkessel_rcr";
-}
-
-// InitializeOutletModels (include/OutletModel.hpp:41-68): one model per outlet
-// condition on the system's pressure field; the first applied pressure is the
-// reference pressure (resistance) or the initial capacitor pressure (RC, RCR).
-inline std::vector<OutletModel> InitOutletModels(const SimulationConfig& c, const FlowSystem& s)
-{
-	std::vector<OutletModel> models;
-	for (const auto& b : c.boundaries)
-		for (const auto& cond : b.conditions) {
-			if (cond.field != s.pressure || !IsOutletKind(cond.type)) continue;
-			OutletModel m;
-			m.label = b.label;
-			m.kind = cond.type;
-			m.resistance = cond.resistance;
-			m.proximal_resistance = cond.proximal_resistance;
-			m.distal_resistance = cond.distal_resistance;
-			m.capacitance = cond.capacitance;
-			m.reference_pressure = cond.reference_pressure;
-			m.capacitor_pressure = cond.initial_pressure;
-			m.pressure = cond.type == "resistance" ? cond.reference_pressure : cond.initial_pressure;
-			models.push_back(m);
+	static const std::pair<const char*, OutletKind> table[] = {
+		{"resistance", OutletKind::kResistance}, {"windkessel_rc", OutletKind::kRc}, {"windkessel_rcr", OutletKind::kRcr}};
+	for (const auto& entry : table)
+		if (name == entry.first) {
+			if (kind) *kind = entry.second;
+			return true;
 		}
-	return models;
+	return false;
 }
 
-struct OutletPressure {
-	double pressure = 0.0, capacitor_pressure = 0.0;
+inline const char* OutletKindName(OutletKind kind)
+{
+	switch (kind) {
+	case OutletKind::kResistance: return "resistance";
+	case OutletKind::kRc: return "windkessel_rc";
+	default: return "windkessel_rcr";
+	}
+}
+
+class OutletBank {
+public:
+	// One outlet per condition of an outlet kind on the flow system's
+	// pressure field, in boundary order.  The first applied pressure is the
+	// reference pressure (resistance) or the initial capacitor pressure.
+	static OutletBank FromConfig(const SimulationConfig& config, const FlowSystem& system)
+	{
+		OutletBank bank;
+		for (const auto& boundary : config.boundaries)
+			for (const auto& cond : boundary.conditions) {
+				OutletKind kind;
+				if (cond.field != system.pressure || !ParseOutletKind(cond.type, &kind)) continue;
+				const double first = kind == OutletKind::kResistance ? cond.reference_pressure : cond.initial_pressure;
+				bank.Add(boundary.label, kind, cond.resistance, cond.proximal_resistance, cond.distal_resistance,
+					cond.capacitance, cond.reference_pressure, cond.initial_pressure, first);
+			}
+		return bank;
+	}
+
+	void Add(int label, OutletKind kind, double resistance, double proximal, double distal, double capacitance,
+		double reference_pressure, double capacitor_pressure, double applied_pressure)
+	{
+		label_.push_back(label);
+		kind_.push_back(kind);
+		r_.push_back(resistance);
+		rp_.push_back(proximal);
+		rd_.push_back(distal);
+		c_.push_back(capacitance);
+		pref_.push_back(reference_pressure);
+		capacitor_.push_back(capacitor_pressure);
+		applied_.push_back(applied_pressure);
+		flow_.push_back(0.0);
+		capacitor_start_.push_back(capacitor_pressure);
+		response_.push_back(0.0);
+		response_capacitor_.push_back(capacitor_pressure);
+	}
+
+	bool empty() const { return label_.empty(); }
+	std::size_t size() const { return label_.size(); }
+	const std::vector<int>& labels() const { return label_; }
+	double applied(std::size_t i) const { return applied_[i]; }
+	double flow(std::size_t i) const { return flow_[i]; }
+	double capacitor(std::size_t i) const { return capacitor_[i]; }
+	const char* kind(std::size_t i) const { return OutletKindName(kind_[i]); }
+
+	// Outlet pressures the next solve applies as tractions, by label.
+	std::map<int, double> Tractions() const
+	{
+		std::map<int, double> out;
+		for (std::size_t i = 0; i < size(); ++i) out.emplace(label_[i], applied_[i]);
+		return out;
+	}
+
+	// Fixes the capacitor pressures the step's passes start from.
+	void StartStep() { capacitor_start_ = capacitor_; }
+
+	// Responses of every outlet to the measured flows; returns the largest
+	// change against the applied pressures.  Tolerance() then holds
+	// 1e-8 + 1e-6 max(1, |responses|).
+	double Respond(const std::vector<double>& flows, double dt)
+	{
+		if (flows.size() != size()) throw std::runtime_error("outlet coupling vectors have inconsistent sizes");
+		double change = 0.0;
+		scale_ = 1.0;
+		for (std::size_t i = 0; i < size(); ++i) {
+			const double q = flows[i];
+			if (!std::isfinite(q)) throw std::runtime_error("outlet flow must be finite");
+			double p, pc = capacitor_start_[i];
+			if (kind_[i] == OutletKind::kResistance) {
+				p = pref_[i] + r_[i]*q;
+			} else {
+				if (!(dt > 0.0)) throw std::runtime_error("RC/RCR outlets require positive transient dt");
+				const double rd = kind_[i] == OutletKind::kRc ? r_[i] : rd_[i];
+				const double g = c_[i]/dt + 1.0/rd;
+				pc = (c_[i]*capacitor_start_[i]/dt + pref_[i]/rd)/g + q/g;
+				p = kind_[i] == OutletKind::kRcr ? pc + rp_[i]*q : pc;
+			}
+			response_[i] = p;
+			response_capacitor_[i] = pc;
+			change = std::max(change, std::abs(p - applied_[i]));
+			scale_ = std::max(scale_, std::abs(p));
+		}
+		measured_ = flows;
+		return change;
+	}
+
+	double Tolerance() const { return 1e-8 + 1e-6*scale_; }
+
+	// Applied pressures move halfway (by `weight`) to the last responses.
+	void Relax(double weight = 0.5)
+	{
+		if (measured_.size() != size() || !(weight > 0.0 && weight <= 1.0))
+			throw std::runtime_error("invalid outlet coupling relaxation");
+		for (std::size_t i = 0; i < size(); ++i) {
+			flow_[i] = measured_[i];
+			applied_[i] += weight*(response_[i] - applied_[i]);
+		}
+	}
+
+	// The last responses become the committed outlet state.
+	void Commit()
+	{
+		if (measured_.size() != size()) throw std::runtime_error("invalid outlet coupling commit");
+		for (std::size_t i = 0; i < size(); ++i) {
+			flow_[i] = measured_[i];
+			applied_[i] = response_[i];
+			capacitor_[i] = response_capacitor_[i];
+		}
+	}
+
+	std::vector<CheckpointOutlet> Checkpoint() const
+	{
+		std::vector<CheckpointOutlet> out(size());
+		for (std::size_t i = 0; i < size(); ++i)
+			out[i] = CheckpointOutlet{label_[i], OutletKindName(kind_[i]), flow_[i], applied_[i], capacitor_[i]};
+		return out;
+	}
+
+	void Restore(const std::vector<CheckpointOutlet>& saved)
+	{
+		if (saved.size() != size()) throw std::runtime_error("flow checkpoint outlet count does not match configuration");
+		std::map<int, const CheckpointOutlet*> by_label;
+		for (const auto& s : saved) by_label.emplace(s.label, &s);
+		for (std::size_t i = 0; i < size(); ++i) {
+			const auto it = by_label.find(label_[i]);
+			if (it == by_label.end() || it->second->kind != OutletKindName(kind_[i]))
+				throw std::runtime_error("flow checkpoint outlet labels or kinds do not match configuration");
+			flow_[i] = it->second->flow;
+			applied_[i] = it->second->pressure;
+			capacitor_[i] = it->second->capacitor_pressure;
+		}
+	}
+
+private:
+	std::vector<int> label_;
+	std::vector<OutletKind> kind_;
+	std::vector<double> r_, rp_, rd_, c_, pref_;
+	std::vector<double> applied_, capacitor_, flow_;       // committed / applied state
+	std::vector<double> capacitor_start_;                  // capacitor pressures of the step start
+	std::vector<double> response_, response_capacitor_;    // last responses
+	std::vector<double> measured_;                         // flows of the last responses
+	double scale_ = 1.0;
 };
-
-// EvaluateOutletModel (include/OutletModel.hpp:70-90).  Resistance: p = p_ref + R Q.
-// RC / RCR: backward Euler for C dPc/dt = Q - (Pc - p_ref)/R_d from the
-// capacitor pressure at the start of the step, and p = Pc (+ R_p Q for RCR).
-inline OutletPressure EvaluateOutlet(const OutletModel& m, double capacitor_start, double flow, double dt)
-{
-	if (!std::isfinite(flow)) throw std::runtime_error("outlet flow must be finite");
-	if (m.kind == "resistance") return {m.reference_pressure + m.resistance*flow, capacitor_start};
-	if (!(dt > 0.0)) throw std::runtime_error("RC/RCR outlets require positive transient dt");
-	const double r = m.kind == "windkessel_rc" ? m.resistance : m.distal_resistance;
-	const double pc = (m.capacitance*capacitor_start/dt + flow + m.reference_pressure/r)/(m.capacitance/dt + 1.0/r);
-	return {pc + (m.kind == "windkessel_rcr" ? m.proximal_resistance*flow : 0.0), pc};
-}
-
-struct OutletCoupling {
-	std::vector<double> flow, pressure, capacitor_pressure;
-	double maximum_pressure_change = 0.0;
-	double pressure_scale = 1.0;
-	// OutletCouplingTolerance (include/OutletModel.hpp:118-121)
-	double Tolerance() const { return 1e-8 + 1e-6*pressure_scale; }
-};
-
-// EvaluateOutletCoupling (include/OutletModel.hpp:92-116)
-inline OutletCoupling EvaluateOutlets(const std::vector<OutletModel>& models,
-	const std::vector<double>& capacitor_start, const std::vector<double>& flow, double dt)
-{
-	if (models.size() != capacitor_start.size() || models.size() != flow.size())
-		throw std::runtime_error("outlet coupling vectors have inconsistent sizes");
-	OutletCoupling r;
-	r.flow = flow;
-	for (std::size_t i = 0; i < models.size(); ++i) {
-		const OutletPressure e = EvaluateOutlet(models[i], capacitor_start[i], flow[i], dt);
-		r.pressure.push_back(e.pressure);
-		r.capacitor_pressure.push_back(e.capacitor_pressure);
-		r.maximum_pressure_change = std::max(r.maximum_pressure_change, std::abs(e.pressure - models[i].pressure));
-		r.pressure_scale = std::max(r.pressure_scale, std::abs(e.pressure));
-	}
-	return r;
-}
-
-// RelaxOutletCoupling (include/OutletModel.hpp:123-135): under-relaxed pressure update.
-inline void RelaxOutlets(std::vector<OutletModel>& models, const OutletCoupling& e, double relaxation = 0.5)
-{
-	if (models.size() != e.flow.size() || models.size() != e.pressure.size() || !(relaxation > 0.0 && relaxation <= 1.0))
-		throw std::runtime_error("invalid outlet coupling relaxation");
-	for (std::size_t i = 0; i < models.size(); ++i) {
-		models[i].flow = e.flow[i];
-		models[i].pressure = (1.0 - relaxation)*models[i].pressure + relaxation*e.pressure[i];
-	}
-}
-
-// CommitOutletCoupling (include/OutletModel.hpp:137-149)
-inline void CommitOutlets(std::vector<OutletModel>& models, const OutletCoupling& e)
-{
-	if (models.size() != e.flow.size() || models.size() != e.pressure.size() || models.size() != e.capacitor_pressure.size())
-		throw std::runtime_error("invalid outlet coupling commit");
-	for (std::size_t i = 0; i < models.size(); ++i) {
-		models[i].flow = e.flow[i];
-		models[i].pressure = e.pressure[i];
-		models[i].capacitor_pressure = e.capacitor_pressure[i];
-	}
-}
-
-// MaterializeOutletPressures (include/OutletModel.hpp:151-170): the pressures the
-// boundary resolution applies as tractions on the outlet labels.
-inline std::map<int, double> OutletPressures(const std::vector<OutletModel>& models)
-{
-	std::map<int, double> p;
-	for (const auto& m : models) p.emplace(m.label, m.pressure);
-	return p;
-}
-
-// AppendOutletCheckpoint / RestoreOutletCheckpoint (include/OutletCheckpoint.hpp:22-48)
-inline std::vector<CheckpointOutlet> OutletCheckpoint(const std::vector<OutletModel>& models)
-{
-	std::vector<CheckpointOutlet> out;
-	for (const auto& m : models) out.push_back({m.label, m.kind, m.flow, m.pressure, m.capacitor_pressure});
-	return out;
-}
-
-inline void RestoreOutlets(const std::vector<CheckpointOutlet>& saved, std::vector<OutletModel>& models)
-{
-	if (saved.size() != models.size())
-		throw std::runtime_error("flow checkpoint outlet count does not match configuration");
-	for (auto& m : models) {
-		const auto it = std::find_if(saved.begin(), saved.end(), [&](const CheckpointOutlet& o) { return o.label == m.label; });
-		if (it == saved.end() || it->kind != m.kind)
-			throw std::runtime_error("flow checkpoint outlet labels or kinds do not match configuration");
-		m.flow = it->flow;
-		m.pressure = it->pressure;
-		m.capacitor_pressure = it->capacitor_pressure;
-	}
-}
 
 }  // namespace tfi_host
