@@ -103,6 +103,17 @@ int tfi_ns_assemble(tfi_mesh* mesh, const double* state, const double* previous,
 int tfi_bsr_spmv(tfi_mesh* mesh, const double* values, const double* x, double* y, int precision);
 /* 4x4 diagonal-block inverses (BuildBlockInverseKernel<4>, SparseKernels.cuh:42-90). */
 int tfi_block_inverse(tfi_mesh* mesh, const double* values, double* inverse, int* singular);
+/* Allocation-free variant of tfi_block_inverse.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * values [nnzb*16] and inverse [16N] are DEVICE pointers; *singular (host)
+ * receives the number of blocks that took the diagonal fallback.  The kernels
+ * work in a fixed library-wide device array (one count per 256-row
+ * workgroup, 256 KiB per GPU, held until the process ends), so the mesh may
+ * have at most 16,777,216 nodes; larger meshes are rejected with an error.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  Synchronous on the context stream; allocates nothing. */
+int tfi_block_inverse_noalloc(tfi_mesh* mesh, const double* values, double* inverse, int* singular);
 /* Dirichlet rows and rhs (ApplyNavierStokesBoundaryKernel /
  * SetNavierStokesBoundaryRhsKernel, IgaCudaKernels.cuh:436-463), in place. */
 int tfi_apply_dirichlet(tfi_mesh* mesh, const int32_t* velocity_mask, const int32_t* pressure_mask,

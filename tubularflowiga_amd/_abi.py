@@ -113,6 +113,7 @@ SIGNATURES = {
     "tfi_ns_assemble": (c_int, [c_void_p, _d, _d, c_double, c_double, c_double, c_void_p, _d, c_int]),
     "tfi_bsr_spmv": (c_int, [c_void_p, _d, _d, _d, c_int]),
     "tfi_block_inverse": (c_int, [c_void_p, _d, _d, P(c_int)]),
+    "tfi_block_inverse_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, P(c_int)]),
     "tfi_apply_dirichlet": (c_int, [c_void_p, _i32, _i32, _d, _d, _d, c_void_p, c_void_p]),
     "tfi_pressure_traction": (c_int, [c_void_p, c_int, _i32, _i32, _d, _d]),
     "tfi_linear_solve": (c_int, [c_void_p, _d, _d, _d, c_int, c_int, c_int, c_int, c_double, P(c_int),

@@ -223,6 +223,7 @@ int tfi_ctx_create(int device, tfi_ctx** out)
 		if (device < 0 || device >= count) throw Error("no such HIP device");
 		Check(hipSetDevice(device), "hipSetDevice");
 		ctx->device = device;
+		InitNoAllocScratch(device);
 		Check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
 		Check(hipEventCreate(&ctx->ev_begin), "hipEventCreate");
 		Check(hipEventCreate(&ctx->ev_end), "hipEventCreate");

@@ -248,6 +248,9 @@ void Residual64(tfi_ctx* ctx, const BsrView& a, const double* values, const doub
 int BlockInverse64(tfi_ctx* ctx, const BsrView& a, const double* values, double* inverse);
 void BlockApply64(tfi_ctx* ctx, int rows, const double* inverse, const double* x, double* y);
 void BlockInverseTo32(tfi_ctx* ctx, int rows, const double* inverse, float* inverse32);
+// Looks up, once per device (the device current), the device address of the
+// fixed working memory of tfi_block_inverse_noalloc.
+void InitNoAllocScratch(int device);
 
 // ---- Krylov (tfi_krylov.hip)
 // Preconditioner: z = M^-1 v (both length 4N, device).

@@ -12,6 +12,16 @@
 #include "tfi_internal.hpp"
 
 #include <cmath>
+#include <mutex>
+
+// Working memory of tfi_block_inverse_noalloc: one singular-block count per
+// 256-row workgroup (at most kNoAllocGroups of them, so meshes of up to
+// 16,777,216 nodes), then the total at kNoAllocGroups.  Every GPU the library
+// runs on holds its own copy (256 KiB) until the process ends.  Host code
+// must not pass this name to a kernel: it is a host-side stand-in, and the
+// device address of each GPU's copy comes from hipGetSymbolAddress.
+constexpr int kNoAllocGroups = 65536;
+static __device__ __align__(256) unsigned int scratch[kNoAllocGroups + 64];
 
 namespace tfi {
 
@@ -46,11 +56,11 @@ __global__ void __launch_bounds__(256) BsrSpmvKernel(int rows, const int* __rest
 	}
 }
 
-__global__ void BlockInverseKernel(int rows, const int* __restrict__ diag, const double* __restrict__ vals,
-	double* __restrict__ inv, unsigned int* __restrict__ singular)
+// Inverse of the diagonal block of `node` into inv[16 node]; returns true when
+// a pivot fell below 1e-14 and the diagonal fallback was taken.
+__device__ inline bool InvertDiagonalBlock(int node, const int* __restrict__ diag, const double* __restrict__ vals,
+	double* __restrict__ inv)
 {
-	const int node = blockIdx.x*blockDim.x + threadIdx.x;
-	if (node >= rows) return;
 	const double* s = vals + (size_t)diag[node]*16;
 	double m[4][8];
 	for (int i = 0; i < 4; ++i)
@@ -83,11 +93,47 @@ __global__ void BlockInverseKernel(int rows, const int* __restrict__ diag, const
 		for (int i = 0; i < 4; ++i)
 			for (int j = 0; j < 4; ++j) d[4*i + j] = m[i][4 + j];
 	} else {
-		atomicAdd(singular, 1u);
 		for (int i = 0; i < 4; ++i)
 			for (int j = 0; j < 4; ++j)
 				d[4*i + j] = i == j && fabs(s[5*i]) > 1e-14 ? 1.0/s[5*i] : (i == j ? 1.0 : 0.0);
 	}
+	return failed;
+}
+
+__global__ void BlockInverseKernel(int rows, const int* __restrict__ diag, const double* __restrict__ vals,
+	double* __restrict__ inv, unsigned int* __restrict__ singular)
+{
+	const int node = blockIdx.x*blockDim.x + threadIdx.x;
+	if (node >= rows) return;
+	if (InvertDiagonalBlock(node, diag, vals, inv)) atomicAdd(singular, 1u);
+}
+
+// Same inverse; the singular-block count of each 256-row workgroup goes to
+// partial[blockIdx.x] (every slot below gridDim.x is written, so nothing
+// needs clearing beforehand).
+__global__ void __launch_bounds__(256) BlockInverseCountKernel(int rows, const int* __restrict__ diag,
+	const double* __restrict__ vals, double* __restrict__ inv, unsigned int* __restrict__ partial)
+{
+	const int node = blockIdx.x*blockDim.x + threadIdx.x;
+	const bool failed = node < rows && InvertDiagonalBlock(node, diag, vals, inv);
+	const int count = __syncthreads_count(failed);
+	if (threadIdx.x == 0) partial[blockIdx.x] = static_cast<unsigned int>(count);
+}
+
+// total[0] = sum of partial[0 .. count), one workgroup of 256 threads.
+__global__ void __launch_bounds__(256) CountSumKernel(const unsigned int* __restrict__ partial, int count,
+	unsigned int* __restrict__ total)
+{
+	__shared__ unsigned int lds[256];
+	unsigned int s = 0;
+	for (int i = threadIdx.x; i < count; i += 256) s += partial[i];
+	lds[threadIdx.x] = s;
+	__syncthreads();
+	for (int half = 128; half > 0; half >>= 1) {
+		if (threadIdx.x < half) lds[threadIdx.x] += lds[threadIdx.x + half];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) total[0] = lds[0];
 }
 
 __global__ void BlockApplyKernel(int rows, const double* __restrict__ inv, const double* __restrict__ x,
@@ -167,4 +213,59 @@ void BlockInverseTo32(tfi_ctx* ctx, int rows, const double* inverse, float* inve
 	TFI_LAUNCH_CHECK("ToFloatBlocksKernel");
 }
 
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned int* g_scratch[kMaxDevices] = {};  // device address of `scratch` per GPU
+
+}  // namespace
+
+void InitNoAllocScratch(int device)
+{
+	if (device < 0 || device >= kMaxDevices) throw Error("device index beyond the allocation-free scratch table");
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if (g_scratch[device]) return;
+	void* ptr = nullptr;
+	Check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+	if (!ptr) throw Error("hipGetSymbolAddress(scratch) returned a null address");
+	g_scratch[device] = static_cast<unsigned int*>(ptr);
+}
+
 }  // namespace tfi
+
+extern "C" int tfi_block_inverse_noalloc(tfi_mesh* m, const double* values, double* inverse, int* singular)
+{
+	using namespace tfi;
+	if (!m) return TFI_ERROR;
+	tfi_ctx* ctx = m->ctx;
+	try {
+		if (!values || !inverse) throw Error("tfi_block_inverse_noalloc needs device arrays of values and inverses");
+		const int rows = m->nodes;
+		const int groups = (rows + 255)/256;
+		if (groups > kNoAllocGroups)
+			throw Error("tfi_block_inverse_noalloc supports at most " + std::to_string(256LL*kNoAllocGroups)
+				+ " nodes (its fixed working memory), the mesh has " + std::to_string(rows));
+		unsigned int* ptr = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch[ctx->device] : nullptr;
+		if (!ptr) throw Error("allocation-free working memory was not set up for this device");
+		if (rows > 0) {
+			const BsrView a = m->view();
+			hipLaunchKernelGGL(BlockInverseCountKernel, dim3(groups), dim3(256), 0, ctx->stream, rows, a.diagonal,
+				values, inverse, ptr);
+			TFI_LAUNCH_CHECK("BlockInverseCountKernel");
+		}
+		hipLaunchKernelGGL(CountSumKernel, dim3(1), dim3(256), 0, ctx->stream, ptr, groups, ptr + kNoAllocGroups);
+		TFI_LAUNCH_CHECK("CountSumKernel");
+		// the total lands in the context's pinned host buffer (no allocation)
+		Check(hipMemcpyAsync(ctx->host_pinned, ptr + kNoAllocGroups, sizeof(unsigned int), hipMemcpyDeviceToHost,
+			ctx->stream), "download singular count");
+		Check(hipStreamSynchronize(ctx->stream), "stream sync");
+		unsigned int count = 0;
+		std::memcpy(&count, ctx->host_pinned, sizeof(count));
+		if (singular) *singular = static_cast<int>(count);
+		return TFI_OK;
+	} catch (const std::exception& e) {
+		ctx->error = e.what();
+		return TFI_ERROR;
+	}
+}
