@@ -1,22 +1,29 @@
 """Benchmark of the body-fitted IGA Navier-Stokes time step on one MI355X.
 
-Workload: the symmetric Y-bifurcation (parent R = 2 mm, daughters of 1.6 mm
-equivalent radius at +-35 deg, 20 mm branches, quadratic C1 splines refined to
-~0.6M velocity+pressure DOFs), steady inflow Q = 5 ml/s, backward Euler with
-dt = 1e-3 s, rho = 1060 kg/m^3, mu = 3.5e-3 Pa s.  Every Newton linear solve
-goes to a true relative residual of 1e-8 and Newton to a relative nonlinear
-residual of 1e-10.  Two methods run the same physical steps from the same
-initial state:
+Default workload (`tree_c4`): the seeded vessel tree -- binary, depth 4
+(15 segments), root radius 3 mm, Murray radii (r^3 conserved), segment
+lengths U[8, 12] x radius, branching angles U[25, 45] deg, quadratic C1
+splines at 8.1M velocity+pressure DOFs (1.68M elements) -- with pulsatile
+inflow Q(t) = 5 (1 + 0.5 sin 2 pi t) ml/s at 1 Hz, zero-traction outlets,
+backward Euler with dt = 1e-3 s, rho = 1060 kg/m^3, mu = 3.5e-3 Pa s.  Every
+Newton linear solve goes to a true relative residual of 1e-8 and Newton to a
+relative nonlinear residual of 1e-12 (--newton-rtol).  Two methods run the same physical
+steps from the same initial state (rest):
   * plain: FP64 FGMRES + FP64 block AMG, Newton from the previous step;
   * mixed + learned: FP32 SpMV/AMG inside FP64 FGMRES with iterative
-    refinement, Newton from the online-trained MLP prediction (residual-checked).
+    refinement, Newton from the online-trained MLP prediction over the
+    increment extrapolation (residual-checked, previous step as fallback).
 `value` is the mixed+learned wall time per time step (lower is better); the
 plain FP64 time, the speedup, the assembly kernel's FP64-MFMA roofline and the
-SpMV HBM roofline sit beside it.  A secondary comparison (`pulsatile`) runs the
-learned-warm-start workload on the same mesh: inflow Q(t) = 5 (1 + 0.6 sin 2 pi t
-+ 0.2 sin 4 pi t) ml/s, 100 untimed steps, then 10 timed steps of each method.
+SpMV HBM roofline sit beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+`--workload ybif_c1` runs the symmetric Y-bifurcation (0.6M DOFs; steady
+Q = 5 ml/s by default) with its secondary measurements: a pulsatile
+comparison (Q(t) = 5 (1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t) ml/s, 100 untimed
++ 10 timed steps), the two-species transport and the configured three-field
+transport on the same mesh.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 from __future__ import annotations
 
@@ -45,14 +52,28 @@ def pulsatile(t):
     return 1.0 + 0.6 * math.sin(2 * math.pi * t) + 0.2 * math.sin(4 * math.pi * t)
 
 
+def tree_pulse(t):
+    """Inflow scale of the vessel-tree workload: Q(t)/Q0 = 1 + 0.5 sin 2 pi t (1 Hz)."""
+    return 1.0 + 0.5 * math.sin(2 * math.pi * t)
+
+
+WAVEFORMS = {"pulsatile": (pulsatile, "pulsatile Q(t)=5(1+0.6 sin 2 pi t+0.2 sin 4 pi t) ml/s"),
+             "pulse": (tree_pulse, "pulsatile Q(t)=5(1+0.5 sin 2 pi t) ml/s at 1 Hz"),
+             "steady": (None, "steady Q=5 ml/s")}
+DESCRIPTIONS = {"tree_c4": "seeded vessel tree (binary, depth 4, 15 segments, root radius 3 mm, Murray radii, "
+                           "L/r ~ U[8,12], angles ~ U[25,45] deg, seed 20261202)",
+                "ybif_c1": "symmetric Y-bifurcation (parent 2 mm, daughters 1.6 mm at +-35 deg, 20 mm branches)"}
+
+
 def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol, train_steps=20,
-               waveform=None):
+               waveform=None, smoother=None):
     from tubularflowiga_amd import boundary, device
     vmask, pmask, vt, pt = bc
     flow = device.FlowSolver(dm, density=rho, viscosity=mu, dt=dt, nonlinear_rtol=tol["rtol"],
                              nonlinear_atol=tol["atol"], mass_rtol=tol["mass"], linear_rtol=1e-8,
                              preconditioner=device.TFI_PC_AMG, precision=precision, warm_start=warm,
-                             restart=100, linear_maximum_iterations=4000)
+                             restart=100, linear_maximum_iterations=4000,
+                             amg_smoother=device.TFI_SMOOTH_GAUSS_SEIDEL if smoother is None else smoother)
     if warm == device.TFI_WARM_LEARNED:
         flow.configure_warm_start(hidden=128, train_steps=train_steps, learning_rate=1e-3, min_snapshots=2)
     scale0 = waveform(0.0) if waveform else 1.0
@@ -96,11 +117,11 @@ def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm,
     return timed, records, timers, state, warm_stats
 
 
-def measured_traffic():
+def measured_traffic(workload):
     """HBM bytes per launch of the roofline kernels from the committed rocprofv3
-    counter summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the factors a
-    plain copy kernel of known size gives); {} when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_ybif_c1.txt")
+    counter summary of the workload (FETCH_SIZE and WRITE_SIZE passes, scaled by
+    the factors a plain copy kernel of known size gives); {} when there is none."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
     out = {}
     if not os.path.exists(path):
         return out
@@ -109,6 +130,14 @@ def measured_traffic():
             fields = dict(f.split("=", 1) for f in line.split()[1:])
             out[fields["kernel"].replace("_", " ")] = int(float(fields["bytes_per_launch"]))
     return out
+
+
+def kernel_breakdown(timers, steps):
+    """GPU milliseconds per time step of the timed kernels (HIP events)."""
+    steps = max(steps, 1)
+    return {"assembly_tangent": round(timers["assembly_ms"] / steps, 3),
+            "assembly_residual_only": round(timers["residual_ms"] / steps, 3),
+            "spmv_fp64": round(timers["spmv_ms"] / steps, 3), "spmv_fp32": round(timers["spmv32_ms"] / steps, 3)}
 
 
 def method_summary(timed):
@@ -365,15 +394,22 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=45)
-    ap.add_argument("--workload", default="ybif_c1")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="tree_c4", help="tree_c4 (default), ybif_c1, ...")
     ap.add_argument("--dt", type=float, default=1e-3)
+    ap.add_argument("--newton-rtol", type=float, default=1e-12,
+                    help="relative nonlinear residual of every step (both methods)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20, help="Adam steps of the predictor per time step")
-    ap.add_argument("--inflow", choices=["steady", "pulsatile"], default="steady",
-                    help="steady Q = 5 ml/s, or Q(t) = 5 (1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t) ml/s")
-    ap.add_argument("--no-pulsatile", action="store_true",
-                    help="skip the secondary pulsatile-inflow comparison (100 + 10 steps per method)")
+    ap.add_argument("--inflow", choices=sorted(WAVEFORMS), default=None,
+                    help="pulse: Q0 (1 + 0.5 sin 2 pi t) (tree default); steady: Q0 (Y default); "
+                         "pulsatile: Q0 (1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t)")
+    ap.add_argument("--extras", action="store_true",
+                    help="secondary measurements on the mesh (pulsatile comparison, transport, configured "
+                         "transport); on by default for ybif_c1")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--smoother", choices=["gauss-seidel", "chebyshev", "jacobi"], default="gauss-seidel",
+                    help="AMG smoother of both methods")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernel on the workload's pattern plus a 2 GiB copy (for counter passes)")
     ap.add_argument("--assembly-only", action="store_true",
@@ -387,6 +423,8 @@ def main(argv=None):
         return 2
     from tubularflowiga_amd import boundary, casegen, device
 
+    inflow = a.inflow or ("pulse" if a.workload.startswith("tree") else "steady")
+    extras = (a.extras or a.workload == "ybif_c1") and not a.no_extras
     t0 = time.perf_counter()
     case = casegen.workload(a.workload)
     rho, mu = casegen.flow_parameters(case)
@@ -409,27 +447,26 @@ def main(argv=None):
         return 0
     bc = boundary.resolve_flow_boundaries(case)
     faces = boundary.boundary_faces(case.mesh)
-    # Newton to 1e-10 relative nonlinear residual (both methods), so that the
-    # state itself is within ~1e-10 of the converged solution; every Newton
-    # linear solve to a true relative residual of 1e-8
-    tol = {"rtol": 1e-10, "atol": 1e-14, "mass": 1e-6}
-    wf = pulsatile if a.inflow == "pulsatile" else None
-    inflow_text = ("steady Q=5 ml/s" if wf is None else "pulsatile Q(t)=5(1+0.6 sin 2 pi t+0.2 sin 4 pi t) ml/s")
+    # Newton to a relative nonlinear residual of 1e-12 by default (both
+    # methods): on the tree two states that each meet 1e-10 differ by up to
+    # ~4e-8 in pressure (relative L2), at 1e-12 by less than 1e-10; every
+    # Newton linear solve to a true relative residual of 1e-8
+    tol = {"rtol": a.newton_rtol, "atol": 1e-14, "mass": 1e-6}
+    wf, inflow_text = WAVEFORMS[inflow]
 
-    log("plain FP64 (AMG, previous-step start)")
+    smoother = {"gauss-seidel": device.TFI_SMOOTH_GAUSS_SEIDEL, "chebyshev": device.TFI_SMOOTH_CHEBYSHEV,
+                "jacobi": device.TFI_SMOOTH_JACOBI}[a.smoother]
+    log(f"plain FP64 (AMG with {a.smoother} smoothing, previous-step start)")
     plain, plain_all, plain_timers, plain_state, _ = run_method(
         dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol,
-        waveform=wf)
+        waveform=wf, smoother=smoother)
     log("mixed precision + learned warm start")
     fast, fast_all, fast_timers, fast_state, warm_stats = run_method(
         dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_MIXED, device.TFI_WARM_LEARNED, tol,
-        a.train_steps, waveform=wf)
+        a.train_steps, waveform=wf, smoother=smoother)
 
     ms_plain = 1e3 * float(np.mean([r["wall_s"] for r in plain]))
     ms_fast = 1e3 * float(np.mean([r["wall_s"] for r in fast]))
-    ps, fs = plain_state.reshape(-1, 4), fast_state.reshape(-1, 4)
-    rel_v = float(np.linalg.norm(fs[:, :3] - ps[:, :3]) / np.linalg.norm(ps[:, :3]))
-    rel_p = float(np.linalg.norm(fs[:, 3] - ps[:, 3]) / max(np.linalg.norm(ps[:, 3]), 1e-300))
 
     # SpMV roofline: FP64 SpMVs of the plain run's timed region (HIP events on the library stream)
     spmv_ms = plain_timers["spmv_ms"] / max(plain_timers["spmv_calls"], 1)
@@ -443,7 +480,9 @@ def main(argv=None):
     asm_tflops = asm_launch_flops / (asm_launch_ms * 1e-3) / 1e12
     asm_call_ms = plain_timers["assembly_ms"] / max(plain_timers["assembly_calls"], 1)
     res_call_ms = plain_timers["residual_ms"] / max(plain_timers["residual_calls"], 1)
-    traffic = measured_traffic()
+    traffic = measured_traffic(a.workload)
+    mesh = case.mesh
+    desc = DESCRIPTIONS.get(a.workload, a.workload)
 
     out = {
         "metric": "NS time-step wall time @ rel. residual 1e-8: FP64 vs mixed/learned; SpMV GB/s",
@@ -457,32 +496,24 @@ def main(argv=None):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 (plain) / f32+f64 refinement (mixed); mlp bf16 inference",
-        "data": "synthetic (generated Y-bifurcation case; no external data)",
-        "config": {"workload": f"{a.workload}: symmetric Y-bifurcation, quadratic IGA, {4*case.mesh.nodes} DOFs, "
-                               f"{case.mesh.elements} elements, {inflow_text}, backward Euler dt={a.dt}",
-                   "nodes": case.mesh.nodes, "elements": case.mesh.elements, "dofs": 4 * case.mesh.nodes,
-                   "block_nonzeros": int(dm.nnzb), "newton_rtol": tol["rtol"], "linear_rtol": 1e-8},
+        "data": f"synthetic (generated {a.workload} case; no external data)",
+        "config": {"workload": f"{a.workload}: {desc}, quadratic IGA, {4*mesh.nodes} DOFs, {mesh.elements} elements, "
+                               f"{inflow_text}, zero-traction outlets, backward Euler dt={a.dt} from rest; "
+                               f"{a.warmup} untimed + {a.steps} timed steps per method",
+                   "nodes": mesh.nodes, "elements": mesh.elements, "dofs": 4 * mesh.nodes,
+                   "block_nonzeros": int(dm.nnzb), "newton_rtol": tol["rtol"], "linear_rtol": 1e-8,
+                   "amg_smoother": a.smoother},
         "fp64_ms_per_step": round(ms_plain, 3),
         "speedup_vs_fp64": round(ms_plain / ms_fast, 3),
-        "plain_fp64": {"ms_per_step": round(ms_plain, 3),
-                       "assembly_ms_per_step": round(1e3 * float(np.mean([r["assembly_s"] for r in plain])), 3),
-                       "linear_ms_per_step": round(1e3 * float(np.mean([r["linear_s"] for r in plain])), 3),
-                       "newton_per_step": float(np.mean([r["newton"] for r in plain])),
-                       "krylov_per_step": float(np.mean([r["linear"] for r in plain])),
-                       "chord_steps": int(sum(r["chord"] for r in plain))},
-        "mixed_learned": {"ms_per_step": round(ms_fast, 3),
-                          "assembly_ms_per_step": round(1e3 * float(np.mean([r["assembly_s"] for r in fast])), 3),
-                          "linear_ms_per_step": round(1e3 * float(np.mean([r["linear_s"] for r in fast])), 3),
-                          "newton_per_step": float(np.mean([r["newton"] for r in fast])),
-                          "krylov_per_step": float(np.mean([r["linear"] for r in fast])),
-                          "chord_steps": int(sum(r["chord"] for r in fast)),
-                          "correction_blend": float(np.mean([r["blend"] for r in fast])),
-                          "learned_guess_accepted": int(sum(r["warm_used"] for r in fast)),
-                          "warm_start": warm_stats},
+        "plain_fp64": method_summary(plain),
+        "mixed_learned": dict(method_summary(fast),
+                              correction_blend=float(np.mean([r["blend"] for r in fast])),
+                              learned_guess_accepted=int(sum(r["warm_used"] for r in fast)),
+                              warm_start=warm_stats),
         "whole_run_ms_per_step": {"plain_fp64": round(1e3 * float(np.mean([r["wall_s"] for r in plain_all])), 3),
                                   "mixed_learned": round(1e3 * float(np.mean([r["wall_s"] for r in fast_all])), 3),
                                   "steps": a.warmup + a.steps},
-        "rel_l2_mixed_vs_fp64": {"velocity": rel_v, "pressure": rel_p},
+        "rel_l2_mixed_vs_fp64": rel_l2(fast_state, plain_state),
         "roofline": {"bound": "mfma", "achieved": round(asm_tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(asm_tflops / FP64_PEAK_TFLOPS, 4),
                      "traffic": traffic.get("AssembleNsKernel<true, true, true>"),
@@ -496,13 +527,16 @@ def main(argv=None):
                           "traffic": traffic.get("BsrSpmvKernel<double, double, double, false>"),
                           "kernel": "BsrSpmvKernel<double> (FGMRES operator, FP64 values)",
                           "avg_ms": round(spmv_ms, 4), "algorithmic_bytes": int(spmv_bytes)},
+        "gpu_time_ms_per_step": {"plain_fp64": kernel_breakdown(plain_timers, len(plain)),
+                                 "mixed_learned": kernel_breakdown(fast_timers, len(fast))},
     }
-    out["transport"] = transport_measurement(dm, case, plain_state, cpu=not a.no_cpu)
-    out["transport"]["spmv_roofline"]["traffic"] = traffic.get("Bsr2SpmvKernel")
-    out["solve"] = solve_measurement(dm, case, plain_state)
-    if a.inflow == "steady" and not a.no_pulsatile:
-        out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, a.dt, dict(tol, rtol=3e-11),
-                                                a.train_steps)
+    if extras:
+        out["transport"] = transport_measurement(dm, case, plain_state, cpu=not a.no_cpu)
+        out["transport"]["spmv_roofline"]["traffic"] = traffic.get("Bsr2SpmvKernel")
+        out["solve"] = solve_measurement(dm, case, plain_state)
+        if inflow == "steady":
+            out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, a.dt, dict(tol, rtol=3e-11),
+                                                    a.train_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, dm, rho, mu, a.dt, plain)
     print(json.dumps(out), flush=True)

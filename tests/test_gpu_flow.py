@@ -42,15 +42,18 @@ def _oracle_steps(case, steps, dt):
     return state
 
 
-@pytest.mark.parametrize("pc,precision,tangent", [(0, 0, 0), (1, 0, 0), (1, 0, 1), (1, 1, 1)])
-def test_backward_euler_matches_oracle(ctx, tube, pc, precision, tangent):
+@pytest.mark.parametrize("pc,precision,tangent,smoother",
+                         [(0, 0, 0, 1), (1, 0, 0, 1), (1, 0, 1, 1), (1, 1, 1, 1), (1, 0, 1, 2), (1, 1, 1, 2)])
+def test_backward_euler_matches_oracle(ctx, tube, pc, precision, tangent, smoother):
+    """smoother 1: multicolour Gauss-Seidel on the large levels (block Jacobi
+    on this small mesh's levels); 2: Chebyshev on every level."""
     from tubularflowiga_amd import device
     dm = device.DeviceMesh(ctx, tube.mesh)
     vmask, pmask, vt, pt = boundary.resolve_flow_boundaries(tube)
     s0 = boundary.initial_state(vmask, pmask, vt, pt)
     flow = device.FlowSolver(dm, density=RHO, viscosity=MU, dt=1e-3, maximum_newton=40, nonlinear_rtol=1e-13,
                              nonlinear_atol=1e-16, mass_rtol=1e30, linear_rtol=1e-12, preconditioner=pc,
-                             precision=precision, tangent=tangent)
+                             precision=precision, tangent=tangent, amg_smoother=smoother)
     flow.set_state(s0)
     flow.set_boundary(vmask, pmask, vt, pt)
     flow.set_boundary_faces(*boundary.boundary_faces(tube.mesh))
@@ -62,7 +65,7 @@ def test_backward_euler_matches_oracle(ctx, tube, pc, precision, tangent):
     gpu = flow.state()
     ref = _oracle_steps(tube, 2, 1e-3)
     v, p = _rel(gpu, ref)
-    print(f"pc={pc} precision={precision} tangent={tangent} newton={[r['newton_iterations'] for r in reports]} "
+    print(f"pc={pc} precision={precision} tangent={tangent} smoother={smoother} newton={[r['newton_iterations'] for r in reports]} "
           f"linear={[r['linear_iterations'] for r in reports]} rel_l2 v={v:.3e} p={p:.3e}", flush=True)
     assert v <= 1e-10 and p <= 1e-10
     for r in reports:

@@ -15,9 +15,11 @@
 //     dense Gauss-Jordan inverse of the coarsest operator;
 //   * V-cycle: symmetric multicolour block Gauss-Seidel (greedy node colouring
 //     per level, one launch per colour, in place -- rows of one colour are
-//     uncoupled) or damped block Jacobi with omega = 1.33 / lambda_max(D^-1 A)
-//     (power-iteration estimate); restriction = aggregate sums, prolongation =
-//     injection; optionally entirely in FP32.
+//     uncoupled), damped block Jacobi with omega = 1.33 / lambda_max(D^-1 A),
+//     or a Chebyshev polynomial in D^-1 A on [0.1, 1.1] lambda_max
+//     (power-iteration estimate; one fused launch per degree, no colours);
+//     restriction = aggregate sums, prolongation = injection; optionally
+//     entirely in FP32.
 // The prolongator is the 0/1 aggregate map, stored as integers, so it is exact
 // in any precision.
 #include "tfi_solver.hpp"
@@ -33,7 +35,8 @@ namespace {
 constexpr double kStrength = 0.25;     // row-relative strength threshold
 constexpr int kMaxLevels = 16;
 constexpr double kOmegaScale = 1.33;   // block-Jacobi damping = kOmegaScale / lambda_max(D^-1 A)
-constexpr int kPre = 2, kPost = 2;      // Jacobi sweeps
+constexpr int kPre = 2, kPost = 2;      // Jacobi sweeps / Chebyshev degree
+constexpr double kChebLow = 0.1, kChebHigh = 1.1;  // Chebyshev interval, in units of lambda_max(D^-1 A)
 constexpr int kPowerIterations = 12;
 
 template <class T>
@@ -109,6 +112,38 @@ __global__ void __launch_bounds__(256) SmoothKernel(int rows, const int* __restr
 		const T* d = inv + (size_t)row*16 + 4*r;
 		const T corr = d[0]*r0 + d[1]*r1 + d[2]*r2 + d[3]*r3;
 		xout[4*(size_t)row + r] = (xin ? xin[4*(size_t)row + r] : T(0)) + omega*corr;
+	}
+}
+
+// One Chebyshev step (three-term form): z = D^-1 (b - A x_in), d = cd d + cz z,
+// x_out = x_in + d; x_in == nullptr means x_in = 0 (and A x_in = 0), cd == 0
+// ignores the old d.  d is per row, so it is updated in place.
+template <class T>
+__global__ void __launch_bounds__(256) ChebyshevKernel(int rows, const int* __restrict__ ro,
+	const int* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ inv, const T* __restrict__ b,
+	const T* __restrict__ xin, T* __restrict__ xout, T* __restrict__ d, T cd, T cz)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 2, r = lane & 3;
+	T acc = 0;
+	if (xin) {
+		acc = RowPartial(ro, cols, vals, xin, row, sub, r);
+		acc += __shfl_xor(acc, 4, 64);
+		acc += __shfl_xor(acc, 8, 64);
+		acc += __shfl_xor(acc, 16, 64);
+		acc += __shfl_xor(acc, 32, 64);
+	}
+	const T res = b[4*(size_t)row + r] - acc;
+	const T r0 = __shfl(res, 0, 64), r1 = __shfl(res, 1, 64), r2 = __shfl(res, 2, 64), r3 = __shfl(res, 3, 64);
+	if (sub == 0) {
+		const size_t i = 4*(size_t)row + r;
+		const T* dv = inv + (size_t)row*16 + 4*r;
+		const T z = dv[0]*r0 + dv[1]*r1 + dv[2]*r2 + dv[3]*r3;
+		const T dn = (cd != T(0) ? cd*d[i] : T(0)) + cz*z;
+		d[i] = dn;
+		xout[i] = (xin ? xin[i] : T(0)) + dn;
 	}
 }
 
@@ -745,6 +780,28 @@ void Amg::Impl::Cycle(int l)
 	}
 	const T omega = static_cast<T>(kOmegaScale/v.lambda_max);
 	const bool gs = v.gs;
+	const bool cheb = !gs && opt.smoother == TFI_SMOOTH_CHEBYSHEV;
+	// Chebyshev steps on [kChebLow, kChebHigh] lambda_max from x (x_in == nullptr: from 0);
+	// d lives in the level's r buffer, which the smoothing phases do not otherwise use
+	auto chebyshev = [&](int degree, bool from_zero) {
+		const double hi = kChebHigh*v.lambda_max, lo = kChebLow*v.lambda_max;
+		const double theta = 0.5*(hi + lo), delta = 0.5*(hi - lo), sigma = theta/delta;
+		double rho = 1.0/sigma;
+		for (int k = 0; k < degree; ++k) {
+			double cd = 0.0, cz = 1.0/theta;
+			if (k > 0) {
+				const double rn = 1.0/(2.0*sigma - rho);
+				cd = rn*rho;
+				cz = 2.0*rn/delta;
+				rho = rn;
+			}
+			hipLaunchKernelGGL(ChebyshevKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals,
+				inv, b, (k == 0 && from_zero) ? static_cast<const T*>(nullptr) : static_cast<const T*>(x), y, r,
+				static_cast<T>(cd), static_cast<T>(cz));
+			std::swap(x, y);
+		}
+		TFI_LAUNCH_CHECK("ChebyshevKernel");
+	};
 	const int ncol = static_cast<int>(v.color_offsets.size()) - 1;
 	// after_forward: this backward sweep directly follows a forward one, whose
 	// last colour it would repeat with nothing changed in between (a colour's
@@ -771,6 +828,8 @@ void Amg::Impl::Cycle(int l)
 		else Fill(ctx, reinterpret_cast<double*>(x), 0.0, n);
 		gs_sweep(true);
 		if (opt.gs_sweeps > 1) gs_sweep(false, true);
+	} else if (cheb) {
+		chebyshev(kPre, true);
 	} else {
 		// damped block Jacobi from x = 0
 		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
@@ -798,6 +857,8 @@ void Amg::Impl::Cycle(int l)
 	if (gs) {
 		if (opt.gs_sweeps > 1) gs_sweep(true);
 		gs_sweep(false, opt.gs_sweeps > 1);
+	} else if (cheb) {
+		chebyshev(kPost, false);
 	} else {
 		for (int s = 0; s < kPost; ++s) {
 			hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv,

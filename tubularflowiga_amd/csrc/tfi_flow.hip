@@ -427,14 +427,18 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		double pre_norm = 0.0, pre_asm = 0.0;
 		const bool fresh = f->fresh_step;
 		f->fresh_step = false;
-		if (fresh && f->opt.warm_start != TFI_WARM_PREVIOUS && f->warm->Ready(f->opt.warm_start)) {
+		// until the network has trained on its first snapshots, the learned mode
+		// starts from its base prediction alone
+		int mode = f->opt.warm_start;
+		if (mode == TFI_WARM_LEARNED && !f->warm->Ready(mode)) mode = f->warm->BaseMode();
+		if (fresh && mode != TFI_WARM_PREVIOUS && f->warm->Ready(mode)) {
 			const auto a0 = Clock::now();
 			const double base = assemble(false, nullptr);
 			Copy(ctx, f->state.data(), f->update.data(), n);  // keep the plain guess
 			double blend = 1.0;
 			double predicted = std::numeric_limits<double>::infinity();  // linearised ||r|| of the guess
 			bool guess_assembled = false;  // rhs and pre_mt already hold the guess's residual
-			if (f->opt.warm_start == TFI_WARM_LEARNED) {
+			if (mode == TFI_WARM_LEARNED) {
 				// Residual line search along the learned correction: with u_b the
 				// base prediction and u_L = u_b + g c the network's guess, take
 				// u = u_b + t (u_L - u_b) with t minimising the linearised
@@ -466,7 +470,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				}
 			} else {
 				// a plain prediction: its residual decides between a chord start and a new tangent
-				f->warm->Predict(f->opt.warm_start, f->state.data(), f->phase, f->phase_count);
+				f->warm->Predict(mode, f->state.data(), f->phase, f->phase_count);
 				SetDirichletState(m, f->vmask.data(), f->pmask.data(), f->vtarget.data(), f->ptarget.data(),
 					f->state.data());
 				predicted = assemble(false, &pre_mt);

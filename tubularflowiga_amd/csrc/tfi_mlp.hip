@@ -438,6 +438,123 @@ uint64_t SplitMix(uint64_t& s)
 
 }  // namespace
 
+// ------------------------------------------------------------------ set-up on the device
+// Control-point position of each basis entry (e, a): sum_k C[a][k] X_k over its
+// extraction row, and the row sum (the weight of the entry).
+__global__ void EntryPositionKernel(int elements, const int* __restrict__ eoff, const int* __restrict__ xoff,
+	const uint8_t* __restrict__ xcol, const double* __restrict__ xval, const double* __restrict__ bezier,
+	double* __restrict__ entry)
+{
+	const int e = blockIdx.x;
+	if (e >= elements) return;
+	const double* X = bezier + (size_t)e*192;
+	for (int a = eoff[e] + threadIdx.x; a < eoff[e + 1]; a += blockDim.x) {
+		double s[4] = {0.0, 0.0, 0.0, 0.0};
+		for (int k = xoff[a]; k < xoff[a + 1]; ++k) {
+			const double c = xval[k];
+			const double* x = X + 3*xcol[k];
+			s[0] += c*x[0];
+			s[1] += c*x[1];
+			s[2] += c*x[2];
+			s[3] += c;
+		}
+		for (int d = 0; d < 4; ++d) entry[4*(size_t)a + d] = s[d];
+	}
+}
+
+// Node position = (sum of its entries' positions) / (sum of their weights),
+// entries summed in ascending order (node_entries lists them per node).
+__global__ void NodePositionKernel(int nodes, const long long* __restrict__ offsets,
+	const long long* __restrict__ node_entries, const double* __restrict__ entry, float* __restrict__ pos)
+{
+	const int n = blockIdx.x*blockDim.x + threadIdx.x;
+	if (n >= nodes) return;
+	double s[4] = {0.0, 0.0, 0.0, 0.0};
+	for (long long k = offsets[n]; k < offsets[n + 1]; ++k)
+		for (int d = 0; d < 4; ++d) s[d] += entry[4*node_entries[k] + d];
+	const double w = s[3] != 0.0 ? s[3] : 1.0;
+	for (int d = 0; d < 3; ++d) pos[3*(size_t)n + d] = static_cast<float>(s[d]/w);
+}
+
+// 27-point stencil of each node: the node, then its 26 nearest pattern
+// neighbours by control-point distance (squared distance of the float
+// positions summed in double; ties by column index), and the RMS distance of
+// the chosen neighbours (1 when there are none).  One 64-lane wave per node:
+// every round takes the wave-wide smallest (d2, column) not yet chosen.
+constexpr int kStencilLanes = 64;
+constexpr int kStencilPerLane = 8;   // rows of up to 512 blocks
+__global__ void __launch_bounds__(256) StencilKernel(int nodes, const int* __restrict__ ro,
+	const int* __restrict__ cols, const float* __restrict__ pos, int* __restrict__ stencil, float* __restrict__ hscale,
+	int* __restrict__ status)
+{
+	const int lane = threadIdx.x & 63;
+	const int n = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (n >= nodes) return;
+	const int begin = ro[n], end = ro[n + 1];
+	if (end - begin > kStencilLanes*kStencilPerLane) {
+		if (lane == 0) atomicOr(status, 1);
+		return;
+	}
+	double d2[kStencilPerLane];
+	int col[kStencilPerLane];
+	const float px = pos[3*(size_t)n], py = pos[3*(size_t)n + 1], pz = pos[3*(size_t)n + 2];
+#pragma unroll
+	for (int i = 0; i < kStencilPerLane; ++i) {
+		const int k = begin + lane + i*kStencilLanes;
+		col[i] = k < end ? cols[k] : -1;
+		d2[i] = __builtin_huge_val();
+		if (col[i] >= 0 && col[i] != n) {
+			const int j = col[i];
+			const double tx = static_cast<double>(pos[3*(size_t)j] - px);
+			const double ty = static_cast<double>(pos[3*(size_t)j + 1] - py);
+			const double tz = static_cast<double>(pos[3*(size_t)j + 2] - pz);
+			d2[i] = tx*tx + ty*ty + tz*tz;
+		} else {
+			col[i] = -1;
+		}
+	}
+	int* out = stencil + (size_t)n*kStencil;
+	if (lane == 0) out[0] = n;
+	double h2 = 0.0;
+	int used = 0;
+	for (int s = 1; s < kStencil; ++s) {
+		// lane-local smallest (d2, column)
+		double bd = __builtin_huge_val();
+		int bc = 0x7fffffff, bi = -1;
+#pragma unroll
+		for (int i = 0; i < kStencilPerLane; ++i)
+			if (col[i] >= 0 && (d2[i] < bd || (d2[i] == bd && col[i] < bc))) {
+				bd = d2[i];
+				bc = col[i];
+				bi = i;
+			}
+		// wave-wide smallest
+		double wd = bd;
+		int wc = bc;
+		for (int off = 32; off > 0; off >>= 1) {
+			const double od = __shfl_xor(wd, off, 64);
+			const int oc = __shfl_xor(wc, off, 64);
+			if (od < wd || (od == wd && oc < wc)) {
+				wd = od;
+				wc = oc;
+			}
+		}
+		if (wc == 0x7fffffff) {
+			if (lane == 0) out[s] = n;
+			continue;
+		}
+		if (bi >= 0 && bc == wc) col[bi] = -1;  // columns are unique in a row: one lane owns it
+		if (lane == 0) out[s] = wc;
+		h2 += wd;
+		++used;
+	}
+	if (lane == 0) {
+		float h = static_cast<float>(used ? sqrt(h2/used) : 1.0);
+		if (!(h > 0.0f)) h = 1.0f;
+		hscale[n] = h;
+	}
+}
+
 struct Mlp::Impl {
 	tfi_mesh* mesh;
 	tfi_ctx* ctx;
@@ -546,61 +663,45 @@ Mlp::Mlp(tfi_mesh* mesh, const tfi_warm_options& options) : impl_(std::make_uniq
 	if (I.H%16 != 0 || I.H > 256) throw Error("MLP width must be a multiple of 16 and at most 256");
 	if (options.stencil != kStencil) throw Error("the MLP stencil has 27 control points");
 	const int N = mesh->nodes;
-	// control-point positions: extraction-weighted average of the Bezier points
-	std::vector<double> px(3LL*N, 0.0), pw(N, 0.0);
-	for (int e = 0; e < mesh->elements; ++e)
-		for (int a = mesh->h_eoff[e]; a < mesh->h_eoff[e + 1]; ++a) {
-			const int node = mesh->h_conn[a];
-			for (int k = mesh->h_xoff[a]; k < mesh->h_xoff[a + 1]; ++k) {
-				const double c = mesh->h_xval[k];
-				const double* X = mesh->h_bezier.data() + (size_t)e*192 + 3*mesh->h_xcol[k];
-				for (int d = 0; d < 3; ++d) px[3LL*node + d] += c*X[d];
-				pw[node] += c;
-			}
-		}
-	std::vector<float> pos(3LL*N), hs(N);
-	for (int n = 0; n < N; ++n)
-		for (int d = 0; d < 3; ++d) pos[3LL*n + d] = static_cast<float>(px[3LL*n + d]/(pw[n] != 0.0 ? pw[n] : 1.0));
-	// 27-point stencils: the node and its 26 nearest pattern neighbours
-	std::vector<int> ro(N + 1), cols(mesh->nnzb);
-	mesh->row_offsets.Download(ro.data(), N + 1);
-	mesh->columns.Download(cols.data(), mesh->nnzb);
-	std::vector<int> st((size_t)N*kStencil);
-	std::vector<std::pair<double, int>> cand;
-	for (int n = 0; n < N; ++n) {
-		cand.clear();
-		for (int k = ro[n]; k < ro[n + 1]; ++k) {
-			const int j = cols[k];
-			if (j == n) continue;
-			double d2 = 0.0;
-			for (int d = 0; d < 3; ++d) {
-				const double t = pos[3LL*j + d] - pos[3LL*n + d];
-				d2 += t*t;
-			}
-			cand.push_back({d2, j});
-		}
-		std::sort(cand.begin(), cand.end());
-		st[(size_t)n*kStencil] = n;
-		double h2 = 0.0;
-		int used = 0;
-		for (int s = 1; s < kStencil; ++s) {
-			if (s - 1 < (int)cand.size()) {
-				st[(size_t)n*kStencil + s] = cand[s - 1].second;
-				h2 += cand[s - 1].first;
-				++used;
-			} else {
-				st[(size_t)n*kStencil + s] = n;
-			}
-		}
-		hs[n] = static_cast<float>(used ? std::sqrt(h2/used) : 1.0);
-		if (!(hs[n] > 0.0f)) hs[n] = 1.0f;
+	// control-point positions: extraction-weighted average of the Bezier points,
+	// per basis entry on the device, then per node over its entries in order
+	const long long entries = mesh->basis_entries;
+	std::vector<long long> noff(N + 1, 0);
+	for (long long a = 0; a < entries; ++a) ++noff[mesh->h_conn[a] + 1];
+	for (int n = 0; n < N; ++n) noff[n + 1] += noff[n];
+	std::vector<long long> nent(entries);
+	{
+		std::vector<long long> fill(noff.begin(), noff.end() - 1);
+		for (long long a = 0; a < entries; ++a) nent[fill[mesh->h_conn[a]]++] = a;
 	}
-	I.stencil.Allocate(I.ctx, st.size());
-	I.stencil.Upload(st.data(), st.size());
-	I.pos.Allocate(I.ctx, pos.size());
-	I.pos.Upload(pos.data(), pos.size());
+	{
+		DevBuf<double> entry(I.ctx, 4*(size_t)entries);
+		DevBuf<long long> d_noff(I.ctx, N + 1), d_nent(I.ctx, entries);
+		d_noff.Upload(noff.data(), N + 1);
+		d_nent.Upload(nent.data(), entries);
+		hipLaunchKernelGGL(EntryPositionKernel, dim3(mesh->elements), dim3(64), 0, I.ctx->stream, mesh->elements,
+			mesh->eoff.data(), mesh->xoff.data(), mesh->xcol.data(), mesh->xval.data(), mesh->bezier.data(),
+			entry.data());
+		TFI_LAUNCH_CHECK("EntryPositionKernel");
+		I.pos.Allocate(I.ctx, 3*(size_t)N);
+		hipLaunchKernelGGL(NodePositionKernel, dim3((N + kT - 1)/kT), dim3(kT), 0, I.ctx->stream, N, d_noff.data(),
+			d_nent.data(), entry.data(), I.pos.data());
+		TFI_LAUNCH_CHECK("NodePositionKernel");
+		Sync(I.ctx);
+	}
+	// 27-point stencils: the node and its 26 nearest pattern neighbours
+	I.stencil.Allocate(I.ctx, (size_t)N*kStencil);
 	I.hscale.Allocate(I.ctx, N);
-	I.hscale.Upload(hs.data(), N);
+	{
+		DevBuf<int> status(I.ctx, 1);
+		FillInt(I.ctx, status.data(), 0, 1);
+		hipLaunchKernelGGL(StencilKernel, dim3((N + 3)/4), dim3(256), 0, I.ctx->stream, N, mesh->row_offsets.data(),
+			mesh->columns.data(), I.pos.data(), I.stencil.data(), I.hscale.data(), status.data());
+		TFI_LAUNCH_CHECK("StencilKernel");
+		int st = 0;
+		status.Download(&st, 1);
+		if (st) throw Error("a pattern row has more blocks than the stencil search takes (512)");
+	}
 	I.batch = std::min(4096, N);
 	I.rows.Allocate(I.ctx, I.batch);
 	I.Layout();

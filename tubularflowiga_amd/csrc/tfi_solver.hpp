@@ -7,7 +7,7 @@ namespace tfi {
 
 // Node-block plain-aggregation AMG (tfi_amg.hip).
 struct AmgOptions {
-	int smoother = TFI_SMOOTH_GAUSS_SEIDEL;  // on levels with at least gs_min_rows rows
+	int smoother = TFI_SMOOTH_GAUSS_SEIDEL;  // GS on levels with at least gs_min_rows rows; CHEBYSHEV on all
 	int coarse_nodes = 256;                  // block rows on the coarsest (dense direct) level
 	int gs_min_rows = 8192;                  // smaller levels use damped block Jacobi
 	int gs_sweeps = 2;                       // 2: symmetric (forward+backward) pre and post; 1: forward pre, backward post
@@ -72,6 +72,8 @@ public:
 	void PredictBase(double* state);
 	// increments combined by the last TFI_WARM_MPE prediction
 	int MpeOrder() const;
+	// the mode of the learned prediction's base (TFI_WARM_MPE or TFI_WARM_EXTRAPOLATE4)
+	int BaseMode() const;
 	void Accepted(bool used);
 	void Stats(int* snapshots, double* last_loss, int* accepted, int* rejected) const;
 	double Gate() const;

@@ -350,6 +350,11 @@ int WarmStart::MpeOrder() const
 	return impl_->mpe_order;
 }
 
+int WarmStart::BaseMode() const
+{
+	return impl_->BaseMode();
+}
+
 void WarmStart::Accepted(bool used)
 {
 	if (used) ++impl_->accepted;

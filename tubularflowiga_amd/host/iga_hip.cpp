@@ -147,7 +147,7 @@ const char* kUsage =
 	"[--stop-after-step N] [--nonlinear-rtol R] [--nonlinear-atol A] "
 	"[--mass-rtol R] [--visualization-format auto|vtu] "
 	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|mpe|learned] "
-	"[--tangent exact|legacy] [--amg-smoother gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
+	"[--tangent exact|legacy] [--amg-smoother gauss-seidel|jacobi|chebyshev] [--linear-rtol R] [--restart-length N] "
 	"[--linear-max-iterations N] [--chord-polish on|off] [--train-steps N]";
 
 // ParseCudaFlowOptions (iga_cuda.cu:447-498) plus the solver-path options
@@ -194,7 +194,8 @@ Options ParseOptions(int argc, char** argv)
 				{"learned", TFI_WARM_LEARNED}});
 		else if (arg == "--tangent") o.tangent = Choice(v, arg, {{"exact", TFI_TANGENT_EXACT}, {"legacy", TFI_TANGENT_LEGACY}});
 		else if (arg == "--amg-smoother")
-			o.smoother = Choice(v, arg, {{"gauss-seidel", TFI_SMOOTH_GAUSS_SEIDEL}, {"jacobi", TFI_SMOOTH_JACOBI}});
+			o.smoother = Choice(v, arg, {{"gauss-seidel", TFI_SMOOTH_GAUSS_SEIDEL}, {"jacobi", TFI_SMOOTH_JACOBI},
+				{"chebyshev", TFI_SMOOTH_CHEBYSHEV}});
 		else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
 		else if (arg == "--restart-length") o.restart_length = PositiveInt(v, arg);
 		else if (arg == "--linear-max-iterations") o.linear_maximum_iterations = PositiveInt(v, arg);
