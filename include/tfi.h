@@ -172,6 +172,7 @@ typedef struct {
 	double residual_l2, continuity_l2, continuity_sum, net_boundary_flow, relative_mass_imbalance;
 	double update_l2, linear_residual, assembly_s, linear_s;
 	int linear_iterations, singular_diagonal_blocks;
+	double step;             /* fraction of the Newton update applied (1, or less under the velocity step limit) */
 } tfi_newton_record;
 
 typedef struct {

@@ -46,7 +46,8 @@ class NewtonRecord(ctypes.Structure):
     _fields_ = [("residual_l2", c_double), ("continuity_l2", c_double), ("continuity_sum", c_double),
                 ("net_boundary_flow", c_double), ("relative_mass_imbalance", c_double),
                 ("update_l2", c_double), ("linear_residual", c_double), ("assembly_s", c_double),
-                ("linear_s", c_double), ("linear_iterations", c_int), ("singular_diagonal_blocks", c_int)]
+                ("linear_s", c_double), ("linear_iterations", c_int), ("singular_diagonal_blocks", c_int),
+                ("step", c_double)]
 
 
 class NewtonReport(ctypes.Structure):

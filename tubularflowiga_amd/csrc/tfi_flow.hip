@@ -49,6 +49,21 @@ __global__ void ContinityPartialKernel(const double* __restrict__ rhs, int nodes
 	}
 }
 
+// Largest |x| over the entries i with i % stride < width (stride 4, width 3:
+// the velocity components of a state vector), per block of 256 threads.
+__global__ void MaxAbsPartialKernel(const double* __restrict__ x, long long n, int stride, int width,
+	double* __restrict__ partial)
+{
+	__shared__ double w[4];
+	double v = 0.0;
+	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
+		if (i % stride < width) v = fmax(v, fabs(x[i]));
+	for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+	if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(w[0], w[1]), fmax(w[2], w[3]));
+}
+
 }  // namespace
 
 LinearSolver::LinearSolver(tfi_mesh* mesh, int preconditioner, int precision, int restart, const AmgOptions& amg)
@@ -209,6 +224,20 @@ int Guard(tfi_ctx* ctx, F&& fn)
 	}
 }
 
+// max |x_i| over the components i % stride < width (device vector, host result)
+double MaxAbs(tfi_flow* f, const double* x, long long n, int stride, int width)
+{
+	const int blocks = 256;
+	hipLaunchKernelGGL(MaxAbsPartialKernel, dim3(blocks), dim3(256), 0, f->ctx->stream, x, n, stride, width,
+		f->cont_partial.data());
+	TFI_LAUNCH_CHECK("MaxAbsPartialKernel");
+	std::vector<double> part(blocks);
+	f->cont_partial.Download(part.data(), blocks);
+	double v = 0.0;
+	for (double p : part) v = std::max(v, p);
+	return v;
+}
+
 }  // namespace
 
 namespace {
@@ -241,7 +270,7 @@ void tfi_flow_default_options(tfi_flow_options* o)
 	o->precision = TFI_FP64;
 	o->warm_start = TFI_WARM_PREVIOUS;
 	o->verbose = 0;
-	o->amg_smoother = TFI_SMOOTH_GAUSS_SEIDEL;
+	o->amg_smoother = TFI_SMOOTH_CHEBYSHEV;
 	o->tangent = TFI_TANGENT_EXACT;
 	o->amg_coarse_nodes = 0;
 	o->amg_gs_min_rows = 0;
@@ -391,6 +420,12 @@ constexpr double kStaleStart = 1e-3;
 // A Newton update that raises the residual, or after which the next solve
 // fails, is backtracked by halves (at most this many times in a step).
 constexpr int kMaxBacktracks = 6;
+// Velocity step limit: a Newton update whose largest velocity change exceeds
+// this fraction of the largest prescribed velocity is shortened to it (an
+// impulsive start otherwise overshoots into states whose tangents the
+// Krylov solver cannot handle).  Ordinary steps change the velocity far less
+// and take the full update.
+constexpr double kVelocityStep = 0.5;
 constexpr double kChordContraction = 4.0;
 // A learned correction whose gate (its least-squares scale on the latest unseen
 // snapshot) is below this is not evaluated: the guess is the base prediction.
@@ -500,6 +535,8 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			}
 		}
 		bool converged = false;
+		// largest prescribed velocity of the step: the scale of the velocity step limit
+		const double velocity_scale = MaxAbs(f, f->vtarget.data(), 3LL*m->nodes, 3, 3);
 		double prev_norm = -1.0, contraction = 0.0;  // observed ||r_k|| / ||r_{k-1}||^2 this step
 		double applied = 0.0;                         // fraction of last_update in the state (0: none)
 		double norm_before = 0.0;                     // residual of the state last_update was computed at
@@ -639,10 +676,16 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 					+ ", continuity_l2=" + Precise(mt.continuity_l2) + ", net_boundary_flow=" + Precise(mt.net)
 					+ ", relative_mass_imbalance=" + Precise(mt.imbalance) + ")");
 			}
-			rec.update_l2 = Norm2(ctx, f->update.data(), n);
-			Axpy(ctx, 1.0, f->update.data(), f->state.data(), n);
+			double step = 1.0;
+			if (velocity_scale > 0.0) {
+				const double dv = MaxAbs(f, f->update.data(), n, 4, 3);
+				if (dv > kVelocityStep*velocity_scale) step = kVelocityStep*velocity_scale/dv;
+			}
+			rec.update_l2 = step*Norm2(ctx, f->update.data(), n);
+			rec.step = step;
+			Axpy(ctx, step, f->update.data(), f->state.data(), n);
 			Copy(ctx, f->update.data(), f->last_update.data(), n);
-			applied = 1.0;
+			applied = step;
 			norm_before = norm;
 			if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
 			rep.newton_iterations = it + 1;

@@ -7,7 +7,7 @@ namespace tfi {
 
 // Node-block plain-aggregation AMG (tfi_amg.hip).
 struct AmgOptions {
-	int smoother = TFI_SMOOTH_GAUSS_SEIDEL;  // GS on levels with at least gs_min_rows rows; CHEBYSHEV on all
+	int smoother = TFI_SMOOTH_CHEBYSHEV;  // CHEBYSHEV on every level; GS on levels with at least gs_min_rows rows
 	int coarse_nodes = 256;                  // block rows on the coarsest (dense direct) level
 	int gs_min_rows = 8192;                  // smaller levels use damped block Jacobi
 	int gs_sweeps = 2;                       // 2: symmetric (forward+backward) pre and post; 1: forward pre, backward post

@@ -98,7 +98,7 @@ struct Options {
 	std::string visualization = "auto";
 	// MI355X solver path
 	int preconditioner = TFI_PC_AMG, precision = TFI_FP64, warm_start = TFI_WARM_PREVIOUS;
-	int tangent = TFI_TANGENT_EXACT, smoother = TFI_SMOOTH_GAUSS_SEIDEL, restart_length = 200;
+	int tangent = TFI_TANGENT_EXACT, smoother = TFI_SMOOTH_CHEBYSHEV, restart_length = 200;
 	int linear_maximum_iterations = 5000;
 	double linear_rtol = 1e-8;
 	int chord_polish = 1, train_steps = 0;
@@ -147,7 +147,7 @@ const char* kUsage =
 	"[--stop-after-step N] [--nonlinear-rtol R] [--nonlinear-atol A] "
 	"[--mass-rtol R] [--visualization-format auto|vtu] "
 	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|mpe|learned] "
-	"[--tangent exact|legacy] [--amg-smoother gauss-seidel|jacobi|chebyshev] [--linear-rtol R] [--restart-length N] "
+	"[--tangent exact|legacy] [--amg-smoother chebyshev|gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
 	"[--linear-max-iterations N] [--chord-polish on|off] [--train-steps N]";
 
 // ParseCudaFlowOptions (iga_cuda.cu:447-498) plus the solver-path options
@@ -406,7 +406,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 						<< " relative_mass_imbalance=" << rec.relative_mass_imbalance << " update_l2=" << rec.update_l2
 						<< " linear_iterations=" << rec.linear_iterations << " linear_residual=" << rec.linear_residual
 						<< " singular_diagonal_blocks=" << rec.singular_diagonal_blocks << " assembly_s=" << rec.assembly_s
-						<< " linear_s=" << rec.linear_s << '\n';
+						<< " linear_s=" << rec.linear_s << " step=" << rec.step << '\n';
 				}
 			}
 			if (status != TFI_OK)
