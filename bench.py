@@ -62,7 +62,9 @@ WAVEFORMS = {"pulsatile": (pulsatile, "pulsatile Q(t)=5(1+0.6 sin 2 pi t+0.2 sin
              "steady": (None, "steady Q=5 ml/s")}
 DESCRIPTIONS = {"tree_c4": "seeded vessel tree (binary, depth 4, 15 segments, root radius 3 mm, Murray radii, "
                            "L/r ~ U[8,12], angles ~ U[25,45] deg, seed 20261202)",
-                "ybif_c1": "symmetric Y-bifurcation (parent 2 mm, daughters 1.6 mm at +-35 deg, 20 mm branches)"}
+                "ybif_c1": "symmetric Y-bifurcation (parent 2 mm, daughters 1.6 mm at +-35 deg, 20 mm branches)",
+                "tube_c2": "straight tube R = 1 mm, L = 64 mm (O-grid 20 control hexes per section x 256 axial, "
+                           "refined 5.5-fold)"}
 
 
 def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol, train_steps=20,
@@ -73,7 +75,7 @@ def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm,
                              nonlinear_atol=tol["atol"], mass_rtol=tol["mass"], linear_rtol=1e-8,
                              preconditioner=device.TFI_PC_AMG, precision=precision, warm_start=warm,
                              restart=100, linear_maximum_iterations=4000,
-                             amg_smoother=device.TFI_SMOOTH_GAUSS_SEIDEL if smoother is None else smoother)
+                             amg_smoother=device.TFI_SMOOTH_AUTO if smoother is None else smoother)
     if warm == device.TFI_WARM_LEARNED:
         flow.configure_warm_start(hidden=128, train_steps=train_steps, learning_rate=1e-3, min_snapshots=2)
     scale0 = waveform(0.0) if waveform else 1.0
@@ -409,8 +411,8 @@ def main(argv=None):
     ap.add_argument("--pulsatile-steps", type=int, default=10,
                     help="timed steps of the pulsatile comparison after its 100 untimed ones (300: the full "
                          "400-step workload)")
-    ap.add_argument("--smoother", choices=["gauss-seidel", "chebyshev", "jacobi"], default="chebyshev",
-                    help="AMG smoother of both methods")
+    ap.add_argument("--smoother", choices=["auto", "gauss-seidel", "chebyshev", "jacobi"], default="auto",
+                    help="AMG smoother of both methods (auto: Chebyshev for transient steps, Gauss-Seidel for steady)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernel on the workload's pattern plus a 2 GiB copy (for counter passes)")
     ap.add_argument("--assembly-only", action="store_true",
@@ -424,7 +426,7 @@ def main(argv=None):
         return 2
     from tubularflowiga_amd import boundary, casegen, device
 
-    inflow = a.inflow or ("pulse" if a.workload.startswith("tree") else "steady")
+    inflow = a.inflow or ("pulse" if a.workload.startswith(("tree", "tube")) else "steady")
     extras = (a.extras or a.workload == "ybif_c1") and not a.no_extras
     t0 = time.perf_counter()
     case = casegen.workload(a.workload)
@@ -457,7 +459,7 @@ def main(argv=None):
     wf, inflow_text = WAVEFORMS[inflow]
 
     smoother = {"gauss-seidel": device.TFI_SMOOTH_GAUSS_SEIDEL, "chebyshev": device.TFI_SMOOTH_CHEBYSHEV,
-                "jacobi": device.TFI_SMOOTH_JACOBI}[a.smoother]
+                "jacobi": device.TFI_SMOOTH_JACOBI, "auto": device.TFI_SMOOTH_AUTO}[a.smoother]
     log(f"plain FP64 (AMG with {a.smoother} smoothing, previous-step start)")
     plain, plain_all, plain_timers, plain_state, _ = run_method(
         dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol,

@@ -34,7 +34,9 @@ enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
 enum { TFI_FP64 = 0, TFI_MIXED = 1 };
 enum { TFI_WARM_PREVIOUS = 0, TFI_WARM_LEARNED = 1, TFI_WARM_EXTRAPOLATE = 2, TFI_WARM_EXTRAPOLATE2 = 3,
 	TFI_WARM_EXTRAPOLATE3 = 4, TFI_WARM_EXTRAPOLATE4 = 5, TFI_WARM_MPE = 6 };
-enum { TFI_SMOOTH_JACOBI = 0, TFI_SMOOTH_GAUSS_SEIDEL = 1, TFI_SMOOTH_CHEBYSHEV = 2 };
+/* AMG smoothers; TFI_SMOOTH_AUTO: Chebyshev for transient flow (dt > 0),
+ * multicolour Gauss-Seidel for steady flow (the default). */
+enum { TFI_SMOOTH_JACOBI = 0, TFI_SMOOTH_GAUSS_SEIDEL = 1, TFI_SMOOTH_CHEBYSHEV = 2, TFI_SMOOTH_AUTO = 3 };
 enum { TFI_TANGENT_LEGACY = 0, TFI_TANGENT_EXACT = 1 };
 
 /* ------------------------------------------------------------------ context */

@@ -17,7 +17,7 @@ TFI_PC_BJACOBI, TFI_PC_AMG = 0, 1
 TFI_FP64, TFI_MIXED = 0, 1
 TFI_WARM_PREVIOUS, TFI_WARM_LEARNED, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3 = 0, 1, 2, 3, 4
 TFI_WARM_EXTRAPOLATE4, TFI_WARM_MPE = 5, 6
-TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL, TFI_SMOOTH_CHEBYSHEV = 0, 1, 2
+TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL, TFI_SMOOTH_CHEBYSHEV, TFI_SMOOTH_AUTO = 0, 1, 2, 3
 TFI_TANGENT_LEGACY, TFI_TANGENT_EXACT = 0, 1
 TERM_KINDS = {"time_derivative": 0, "diffusion": 1, "advection": 2, "linear_coupling": 3, "volume_source": 4}
 TFI_SUPG_NONE, TFI_SUPG_GALERKIN, TFI_SUPG_TEST, TFI_SUPG_CONSISTENT = 0, 1, 2, 3

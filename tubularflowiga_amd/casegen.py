@@ -935,6 +935,10 @@ WORKLOADS = {
     # small Y for CPU tests
     "ybif_small": dict(kind="ybif", r_parent_mm=2.0, r_daughter_mm=1.6, angle_deg=35.0, l_parent_mm=20.0,
                        l_daughter_mm=20.0, n_core=2, m_ring=1, n_parent=8, n_daughter=8, flow_ml_s=5.0),
+    # refined straight tube, ~4M DOFs: the O-grid section of 2 x 2 core + 4 rings of 2 x 2 control hexes
+    # (20 per section; the O-grid cannot make the 24 the workload names) and 256 axial, refined 5.5-fold
+    # uniformly (11 x 11 core, rings 11 x 11, 1408 axial); R = 1 mm, L = 64 mm, Poiseuille inlet 0.1 m/s
+    "tube_c2": dict(kind="tube", radius_mm=1.0, length_mm=64.0, n_core=11, m_ring=11, n_axial=1408, u_max=0.1),
     # seeded vessel tree: binary, depth 4 (15 segments), root radius 3 mm, Murray radii (r^3 conserved),
     # segment lengths U[8, 12] x radius, branching angles U[25, 45] deg, Q0 = 5 ml/s; ~8M DOFs
     "tree_c4": dict(kind="tree", seed=20261202, depth=4, r_root_mm=3.0, n_section=48, axial_aspect=1.05,

@@ -429,6 +429,7 @@ struct Amg::Impl {
 
 	void Symbolic(const double* fine_values);
 	void Numeric(const double* fine_values64, const float* fine_values32);
+	void Colour(Level& v);
 	template <class T>
 	void Cycle(int l);
 };
@@ -525,6 +526,32 @@ std::vector<int> Aggregate(int n, const std::vector<int>& ro, const std::vector<
 }
 
 }  // namespace
+
+// Greedy node colouring of a Gauss-Seidel level (rows of a colour are uncoupled).
+void Amg::Impl::Colour(Level& v)
+{
+	std::vector<int> col(v.rows, -1), mark;
+	int ncol = 0;
+	for (int i = 0; i < v.rows; ++i) {
+		if ((int)mark.size() < ncol + 1) mark.resize(ncol + 1, -1);
+		for (int k = v.h_ro[i]; k < v.h_ro[i + 1]; ++k) {
+			const int c = col[v.h_cols[k]];
+			if (c >= 0) mark[c] = i;
+		}
+		int c = 0;
+		while (c < ncol && mark[c] == i) ++c;
+		if (c == ncol) ++ncol;
+		col[i] = c;
+	}
+	v.color_offsets.assign(ncol + 1, 0);
+	for (int i = 0; i < v.rows; ++i) ++v.color_offsets[col[i] + 1];
+	for (int c = 0; c < ncol; ++c) v.color_offsets[c + 1] += v.color_offsets[c];
+	std::vector<int> rows(v.rows);
+	std::vector<int> fill(v.color_offsets.begin(), v.color_offsets.end() - 1);
+	for (int i = 0; i < v.rows; ++i) rows[fill[col[i]]++] = i;
+	v.color_rows.Allocate(ctx, v.rows);
+	v.color_rows.Upload(rows.data(), v.rows);
+}
 
 void Amg::Impl::Symbolic(const double* fine_values)
 {
@@ -635,29 +662,7 @@ void Amg::Impl::Symbolic(const double* fine_values)
 	for (size_t l = 0; l < L.size(); ++l) {
 		Level& v = *L[l];
 		v.gs = l + 1 < L.size() && opt.smoother == TFI_SMOOTH_GAUSS_SEIDEL && v.rows >= opt.gs_min_rows;
-		if (v.gs) {
-			std::vector<int> col(v.rows, -1), mark;
-			int ncol = 0;
-			for (int i = 0; i < v.rows; ++i) {
-				if ((int)mark.size() < ncol + 1) mark.resize(ncol + 1, -1);
-				for (int k = v.h_ro[i]; k < v.h_ro[i + 1]; ++k) {
-					const int c = col[v.h_cols[k]];
-					if (c >= 0) mark[c] = i;
-				}
-				int c = 0;
-				while (c < ncol && mark[c] == i) ++c;
-				if (c == ncol) ++ncol;
-				col[i] = c;
-			}
-			v.color_offsets.assign(ncol + 1, 0);
-			for (int i = 0; i < v.rows; ++i) ++v.color_offsets[col[i] + 1];
-			for (int c = 0; c < ncol; ++c) v.color_offsets[c + 1] += v.color_offsets[c];
-			std::vector<int> rows(v.rows);
-			std::vector<int> fill(v.color_offsets.begin(), v.color_offsets.end() - 1);
-			for (int i = 0; i < v.rows; ++i) rows[fill[col[i]]++] = i;
-			v.color_rows.Allocate(ctx, v.rows);
-			v.color_rows.Upload(rows.data(), v.rows);
-		}
+		if (v.gs) Colour(v);
 		const long long n = 4LL*v.rows;
 		v.inv64.Allocate(ctx, 16LL*v.rows);
 		if (precision == TFI_MIXED) {
@@ -902,6 +907,20 @@ void Amg::Describe(std::vector<int>& rows, std::vector<long long>& nnzb, std::ve
 }
 
 Amg::~Amg() = default;
+
+int Amg::smoother() const { return impl_->opt.smoother; }
+
+void Amg::UseGaussSeidel()
+{
+	Impl& I = *impl_;
+	I.opt.smoother = TFI_SMOOTH_GAUSS_SEIDEL;
+	if (!I.built) return;
+	for (size_t l = 0; l + 1 < I.L.size(); ++l) {
+		Level& v = *I.L[l];
+		v.gs = v.rows >= I.opt.gs_min_rows;
+		if (v.gs && v.color_offsets.empty()) I.Colour(v);
+	}
+}
 
 int Amg::levels() const { return static_cast<int>(impl_->L.size()); }
 

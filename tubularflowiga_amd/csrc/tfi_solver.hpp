@@ -7,7 +7,7 @@ namespace tfi {
 
 // Node-block plain-aggregation AMG (tfi_amg.hip).
 struct AmgOptions {
-	int smoother = TFI_SMOOTH_CHEBYSHEV;  // CHEBYSHEV on every level; GS on levels with at least gs_min_rows rows
+	int smoother = TFI_SMOOTH_GAUSS_SEIDEL;  // GS on levels with at least gs_min_rows rows; CHEBYSHEV on every level
 	int coarse_nodes = 256;                  // block rows on the coarsest (dense direct) level
 	int gs_min_rows = 8192;                  // smaller levels use damped block Jacobi
 	int gs_sweeps = 2;                       // 2: symmetric (forward+backward) pre and post; 1: forward pre, backward post
@@ -25,6 +25,10 @@ public:
 	int levels() const;
 	// rows, block nonzeros and Gauss-Seidel colours (0: Jacobi / direct) per level
 	void Describe(std::vector<int>& rows, std::vector<long long>& nnzb, std::vector<int>& colors) const;
+	int smoother() const;
+	// switch a Chebyshev-smoothed hierarchy to multicolour Gauss-Seidel (for good);
+	// Setup must run again before the next Apply
+	void UseGaussSeidel();
 	struct Impl;
 
 private:
@@ -38,6 +42,9 @@ public:
 		const AmgOptions& amg = AmgOptions());
 	~LinearSolver();
 	void Setup(const double* values);
+	// A Chebyshev-smoothed AMG whose solve has not converged within
+	// kChebyshevCap iterations is switched to Gauss-Seidel and the solve rerun
+	// from the same initial guess.
 	KrylovResult Solve(const double* b, double* x, int maxit, double rtol);
 	int singular_blocks() const { return singular_; }
 	int refinements() const { return refinements_; }
@@ -54,6 +61,8 @@ private:
 	std::unique_ptr<Amg> amg_;
 	int singular_ = 0;
 	int refinements_ = 0;
+	DevBuf<double> guess_;
+	KrylovResult SolveOnce(const double* b, double* x, int maxit, double rtol);
 };
 
 // Initial-state predictor for a time step (tfi_warmstart.hip).

@@ -16,7 +16,8 @@ from ._abi import (FlowOptions, KernelTimers, MeshStats, NewtonReport, SolveSyst
                    TFI_PC_AMG, TFI_PC_BJACOBI, TFI_WARM_EXTRAPOLATE, TFI_WARM_EXTRAPOLATE2, TFI_WARM_EXTRAPOLATE3,
                    TFI_WARM_EXTRAPOLATE4,
                    TFI_WARM_LEARNED, TFI_WARM_MPE,
-                   TFI_WARM_PREVIOUS, TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL, TFI_SMOOTH_CHEBYSHEV)
+                   TFI_WARM_PREVIOUS, TFI_SMOOTH_JACOBI, TFI_SMOOTH_GAUSS_SEIDEL, TFI_SMOOTH_CHEBYSHEV,
+                   TFI_SMOOTH_AUTO)
 
 __all__ = ["Context", "DeviceMesh", "FlowSolver", "TFI_FP64", "TFI_MIXED", "TFI_PC_AMG", "TFI_PC_BJACOBI",
            "TFI_WARM_PREVIOUS", "TFI_WARM_LEARNED", "TFI_WARM_EXTRAPOLATE",

@@ -98,7 +98,7 @@ struct Options {
 	std::string visualization = "auto";
 	// MI355X solver path
 	int preconditioner = TFI_PC_AMG, precision = TFI_FP64, warm_start = TFI_WARM_PREVIOUS;
-	int tangent = TFI_TANGENT_EXACT, smoother = TFI_SMOOTH_CHEBYSHEV, restart_length = 200;
+	int tangent = TFI_TANGENT_EXACT, smoother = TFI_SMOOTH_AUTO, restart_length = 200;
 	int linear_maximum_iterations = 5000;
 	double linear_rtol = 1e-8;
 	int chord_polish = 1, train_steps = 0;
@@ -147,7 +147,7 @@ const char* kUsage =
 	"[--stop-after-step N] [--nonlinear-rtol R] [--nonlinear-atol A] "
 	"[--mass-rtol R] [--visualization-format auto|vtu] "
 	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|mpe|learned] "
-	"[--tangent exact|legacy] [--amg-smoother chebyshev|gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
+	"[--tangent exact|legacy] [--amg-smoother auto|chebyshev|gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
 	"[--linear-max-iterations N] [--chord-polish on|off] [--train-steps N]";
 
 // ParseCudaFlowOptions (iga_cuda.cu:447-498) plus the solver-path options
@@ -194,8 +194,8 @@ Options ParseOptions(int argc, char** argv)
 				{"learned", TFI_WARM_LEARNED}});
 		else if (arg == "--tangent") o.tangent = Choice(v, arg, {{"exact", TFI_TANGENT_EXACT}, {"legacy", TFI_TANGENT_LEGACY}});
 		else if (arg == "--amg-smoother")
-			o.smoother = Choice(v, arg, {{"gauss-seidel", TFI_SMOOTH_GAUSS_SEIDEL}, {"jacobi", TFI_SMOOTH_JACOBI},
-				{"chebyshev", TFI_SMOOTH_CHEBYSHEV}});
+			o.smoother = Choice(v, arg, {{"auto", TFI_SMOOTH_AUTO}, {"gauss-seidel", TFI_SMOOTH_GAUSS_SEIDEL},
+				{"jacobi", TFI_SMOOTH_JACOBI}, {"chebyshev", TFI_SMOOTH_CHEBYSHEV}});
 		else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
 		else if (arg == "--restart-length") o.restart_length = PositiveInt(v, arg);
 		else if (arg == "--linear-max-iterations") o.linear_maximum_iterations = PositiveInt(v, arg);
