@@ -103,8 +103,9 @@ def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm,
         rec = {"step": step + 1, "wall_s": t1 - t0, "newton": rep["newton_iterations"],
                "linear": rep["linear_iterations"], "assembly_s": rep["assembly_s"], "linear_s": rep["linear_s"],
                "warm_used": rep["warm_start_used"], "warm_ratio": rep["warm_start_initial_ratio"],
-               "chord": rep["chord_steps"], "blend": rep["warm_start_blend"], "final_residual_rel": rep["final_residual"] / max(rep["tolerance"], 1e-300)
-               * tol["rtol"]}
+               "chord": rep["chord_steps"], "blend": rep["warm_start_blend"],
+               "base_ratio": rep["warm_start_base_ratio"],
+               "final_residual_rel": rep["final_residual"] / max(rep["tolerance"], 1e-300) * tol["rtol"]}
         records.append(rec)
         log(f"  step {step + 1} {'timed' if is_timed else 'warmup'} wall={rec['wall_s']*1e3:.1f} ms "
             f"newton={rec['newton']} linear={rec['linear']} asm={rec['assembly_s']*1e3:.1f} ms "
@@ -132,6 +133,19 @@ def measured_traffic(workload):
             fields = dict(f.split("=", 1) for f in line.split()[1:])
             out[fields["kernel"].replace("_", " ")] = int(float(fields["bytes_per_launch"]))
     return out
+
+
+def network_gain(timed):
+    """What the trained network adds to its base prediction, over the timed
+    steps where both were evaluated: the geometric mean of the step's initial
+    residual (relative to the previous-step start) of the learned guess and of
+    the base prediction alone, and their ratio (< 1: the network helps)."""
+    pairs = [(r["warm_ratio"], r["base_ratio"]) for r in timed if r["warm_ratio"] > 0 and r["base_ratio"] > 0]
+    if not pairs:
+        return None
+    g = float(np.exp(np.mean([math.log(a) for a, _ in pairs])))
+    b = float(np.exp(np.mean([math.log(c) for _, c in pairs])))
+    return {"steps": len(pairs), "learned_initial_ratio": g, "base_initial_ratio": b, "learned_over_base": g / b}
 
 
 def kernel_breakdown(timers, steps):
@@ -362,6 +376,72 @@ def cpu_baseline(case, dm, rho, mu, dt, plain_timed, budget_s=20.0):
                        f"SpMV-equivalents (counts from the plain GPU run)")}
 
 
+def cpu_model():
+    """CPU model name and physical core count of this host (from /proc/cpuinfo)."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "model name":
+                model = val
+            elif key == "physical id":
+                phys = val
+            elif key == "core id":
+                core = val
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return model, len(cores) or (os.cpu_count() or 1)
+
+
+def cpu_full_step(ctx, rho_unused=None):
+    """A complete plain FP64 backward-Euler Newton step of the CPU oracle (OpenMP)
+    on the coarse version of the tree (tree_small, the same seeded geometry),
+    from rest to a relative residual of 1e-8 with FGMRES + block Jacobi, beside
+    the GPU's plain step on the same case."""
+    from oracle import oracle as O
+    from tubularflowiga_amd import boundary, casegen, device
+    case = casegen.workload("tree_small")
+    rho, mu = casegen.flow_parameters(case)
+    vm, pm, vt, pt = boundary.resolve_flow_boundaries(case)
+    s0 = boundary.initial_state(vm, pm, vt, pt)
+    pattern = O.block_pattern(case.mesh)
+    t0 = time.perf_counter()
+    _, newton, linear, _ = O.ns_newton_step(case.mesh, pattern, s0, s0.copy(), rho, mu, 1e-3, vm, pm, vt, pt,
+                                            max_newton=30, rtol=1e-8, atol=1e-16, linear_rtol=1e-8, tangent="exact")
+    cpu_s = time.perf_counter() - t0
+    dm = device.DeviceMesh(ctx, case.mesh)
+    flow = device.FlowSolver(dm, density=rho, viscosity=mu, dt=1e-3, nonlinear_rtol=1e-8, nonlinear_atol=1e-16,
+                             mass_rtol=1e30, linear_rtol=1e-8, maximum_newton=30)
+    try:
+        flow.set_state(s0)
+        flow.set_boundary(vm, pm, vt, pt)
+        flow.set_boundary_faces(*boundary.boundary_faces(case.mesh))
+        flow.begin_step()
+        t0 = time.perf_counter()
+        rep = flow.newton()
+        gpu_s = time.perf_counter() - t0
+    finally:
+        flow.close()
+        dm.close()
+    return {"case": f"tree_small ({4 * case.mesh.nodes} DOFs, {case.mesh.elements} elements), first step from rest, "
+                    "relative residual 1e-8",
+            "cpu_ms": round(1e3 * cpu_s, 1), "cpu_newton": newton, "cpu_krylov": linear,
+            "cpu_solver": "FGMRES(200) + block Jacobi, exact tangent (oracle)",
+            "gpu_plain_fp64_ms": round(1e3 * gpu_s, 1), "gpu_newton": rep["newton_iterations"],
+            "gpu_krylov": rep["linear_iterations"]}
+
+
+def measured_peaks(ctx):
+    """Stream (2 GiB device copy, read + write bytes) and FP64 matrix-core
+    (probe kernel) peaks measured on this GPU."""
+    n = 1 << 28
+    ms = ctx.bench_copy(n, repetitions=5)
+    return {"hbm_copy_gbs": round(16 * n / (ms * 1e-3) / 1e9, 1), "fp64_mfma_tflops": round(ctx.bench_mfma_fp64(3), 2)}
+
+
 def spmv_only(dm, workload, repetitions=20):
     """SpMV launches on the workload's pattern (FP64 and FP32 values) and a
     plain 2 GiB device copy (CopyKernel), whose HBM counters calibrate the SpMV ones."""
@@ -513,6 +593,7 @@ def main(argv=None):
         "mixed_learned": dict(method_summary(fast),
                               correction_blend=float(np.mean([r["blend"] for r in fast])),
                               learned_guess_accepted=int(sum(r["warm_used"] for r in fast)),
+                              network_gain=network_gain(fast),
                               warm_start=warm_stats),
         "whole_run_ms_per_step": {"plain_fp64": round(1e3 * float(np.mean([r["wall_s"] for r in plain_all])), 3),
                                   "mixed_learned": round(1e3 * float(np.mean([r["wall_s"] for r in fast_all])), 3),
@@ -541,8 +622,16 @@ def main(argv=None):
         if inflow == "steady":
             out["pulsatile"] = pulsatile_comparison(dm, case, bc, faces, rho, mu, 5e-3, tol, a.train_steps,
                                                     warmup=100, steps=a.pulsatile_steps)
+    peaks = measured_peaks(ctx)
+    out["measured_peaks"] = peaks
+    out["roofline"]["frac_of_measured_peak"] = round(asm_tflops / peaks["fp64_mfma_tflops"], 4)
+    out["spmv_roofline"]["frac_of_measured_peak"] = round(spmv_gbs / peaks["hbm_copy_gbs"], 4)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, dm, rho, mu, a.dt, plain)
+        model, cores = cpu_model()
+        out["cpu_baseline"]["cpu_model"] = model
+        out["cpu_baseline"]["physical_cores"] = cores
+        out["cpu_baseline"]["full_step"] = cpu_full_step(ctx)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -138,6 +138,11 @@ int tfi_bench_assembly(tfi_mesh* mesh, int tangent, int with_jacobian, int repet
  * grid-stride copy kernel), average milliseconds per launch.  No reference
  * counterpart. */
 int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milliseconds);
+/* Measured FP64 matrix-core throughput: a probe kernel of independent
+ * v_mfma_f64_16x16x4f64 chains on every SIMD, TFLOP/s over `repetitions`
+ * launches (HIP events).  The measured peak beside the published one.  No
+ * reference counterpart. */
+int tfi_bench_mfma_fp64(tfi_ctx* ctx, int repetitions, double* tflops);
 
 /* ------------------------------------------------------------------ flow solver */
 typedef struct {

@@ -73,6 +73,25 @@ __global__ void ScaleKernel(double a, double* __restrict__ x, long long n)
 		x[i] *= a;
 }
 
+typedef double doublex4_t __attribute__((ext_vector_type(4)));
+
+// FP64 matrix-core throughput probe: every wave runs `iterations` rounds of
+// four independent v_mfma_f64_16x16x4f64 chains (2048 flops each) and writes
+// one value per lane so that nothing is optimised away.
+__global__ void __launch_bounds__(256) MfmaPeakKernel(int iterations, double seed, double* __restrict__ out)
+{
+	const int lane = threadIdx.x & 63;
+	const double a = seed + lane*1e-3, b = seed - lane*1e-3;
+	doublex4_t c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0, c2 = c0, c3 = c0;
+	for (int i = 0; i < iterations; ++i) {
+		c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c0, 0, 0, 0);
+		c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, c1, 0, 0, 0);
+		c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, a, c2, 0, 0, 0);
+		c3 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, b, c3, 0, 0, 0);
+	}
+	out[blockIdx.x*(size_t)blockDim.x + threadIdx.x] = c0[0] + c1[1] + c2[2] + c3[3];
+}
+
 __global__ void CopyKernel(const double* __restrict__ x, double* __restrict__ y, long long n)
 {
 	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
@@ -150,6 +169,12 @@ void Scale(tfi_ctx* ctx, double a, double* x, long long n)
 {
 	hipLaunchKernelGGL(ScaleKernel, dim3(Blocks(n)), dim3(kThreads), 0, ctx->stream, a, x, n);
 	TFI_LAUNCH_CHECK("ScaleKernel");
+}
+
+void MfmaPeak(tfi_ctx* ctx, int blocks, int threads, int iterations, double* out)
+{
+	hipLaunchKernelGGL(MfmaPeakKernel, dim3(blocks), dim3(threads), 0, ctx->stream, iterations, 0.5, out);
+	TFI_LAUNCH_CHECK("MfmaPeakKernel");
 }
 
 void Copy(tfi_ctx* ctx, const double* x, double* y, long long n)

@@ -986,6 +986,25 @@ int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milli
 	});
 }
 
+int tfi_bench_mfma_fp64(tfi_ctx* ctx, int repetitions, double* tflops)
+{
+	return Guard(ctx, [&] {
+		if (repetitions <= 0 || !tflops) throw Error("FP64 MFMA probe needs a positive repetition count and an output");
+		const int blocks = 8*ctx->cus, threads = 256, iterations = 20000;
+		DevBuf<double> out(ctx, (size_t)blocks*threads);
+		MfmaPeak(ctx, blocks, threads, iterations, out.data());
+		Sync(ctx);
+		Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event");
+		for (int i = 0; i < repetitions; ++i) MfmaPeak(ctx, blocks, threads, iterations, out.data());
+		Check(hipEventRecord(ctx->ev_end, ctx->stream), "event");
+		Check(hipEventSynchronize(ctx->ev_end), "event sync");
+		float ms = 0.0f;
+		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
+		const double flops = 2048.0*4.0*iterations*(blocks*(threads/64.0))*repetitions;
+		*tflops = flops/(ms*1e-3)/1e12;
+	});
+}
+
 int tfi_bench_spmv(tfi_mesh* m, int precision, int repetitions, double* milliseconds)
 {
 	return Guard(m->ctx, [&] {

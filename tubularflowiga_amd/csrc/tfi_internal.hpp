@@ -173,6 +173,8 @@ void Axpy(tfi_ctx* ctx, double a, const double* x, double* y, long long n);
 void Axpby(tfi_ctx* ctx, double a, const double* x, double b, double* y, long long n);
 void Scale(tfi_ctx* ctx, double a, double* x, long long n);
 void Copy(tfi_ctx* ctx, const double* x, double* y, long long n);
+// FP64 matrix-core throughput probe (blocks x threads lanes, iterations of 4 MFMA chains)
+void MfmaPeak(tfi_ctx* ctx, int blocks, int threads, int iterations, double* out);
 void Fill(tfi_ctx* ctx, double* x, double value, long long n);
 void FillInt(tfi_ctx* ctx, int* x, int value, long long n);
 void ToFloat(tfi_ctx* ctx, const double* x, float* y, long long n);

@@ -47,6 +47,12 @@ class Context:
         self.check(self.lib.tfi_bench_copy(self.handle, int(count), repetitions, ctypes.byref(ms)))
         return ms.value
 
+    def bench_mfma_fp64(self, repetitions=5):
+        """Measured FP64 matrix-core TFLOP/s (probe kernel)."""
+        t = ctypes.c_double()
+        self.check(self.lib.tfi_bench_mfma_fp64(self.handle, repetitions, ctypes.byref(t)))
+        return t.value
+
     def device_info(self) -> str:
         buf = ctypes.create_string_buffer(512)
         self.check(self.lib.tfi_device_info(self.handle, buf, 512))
