@@ -296,3 +296,71 @@ def test_cli_outlets_with_predicted_starts_reach_the_plain_solution(tmp_path):
           f"outlet {outlet(fast)} vs {outlet(plain)}", flush=True)
     assert v <= 1e-10 and p <= 1e-10
     assert np.allclose(outlet(fast), outlet(plain), rtol=1e-10, atol=0.0)
+
+
+def steady_direct(m, state, rho, mu, vmask, pmask, vt, pt, rtol=1e-13, atol=1e-16, max_newton=40):
+    """The oracle's steady Newton iteration (the loop of O.ns_newton_step)
+    with each linear system solved by sparse LU: block-Jacobi FGMRES does not
+    converge on the steady saddle-point systems."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    pattern = O.block_pattern(m)
+    ro, cols, _ = pattern
+    n4 = 4 * m.nodes
+    state = np.array(state, dtype=np.float64)
+    r0 = None
+    for _ in range(max_newton):
+        jac, rhs = O.ns_assemble(m, pattern, state, state, rho, mu, 0.0, "exact")
+        O.apply_dirichlet(pattern, vmask, pmask, vt, pt, state, jac, rhs)
+        norm = float(np.linalg.norm(rhs))
+        r0 = norm if r0 is None else r0
+        if norm <= max(atol * np.sqrt(n4), rtol * r0):
+            return state
+        state += spl.spsolve(sp.bsr_matrix((jac, cols, ro), shape=(n4, n4)).tocsc(), rhs)
+    raise RuntimeError("oracle steady Newton reached max_newton")
+
+
+@pytest.mark.parametrize("with_case_config", [False, True])
+def test_cli_legacy_flow_input_matches_the_oracle(tube, tmp_path, with_case_config):
+    """Without simulation_config.json, `navier-stokes` reads the legacy input
+    (iga_cuda.cu:791-795): simulation_parameter.txt (vplus scales the inlet
+    profile) and an optional case_config.json; the flow is steady with
+    viscosity 0.1 and density 1, wall and inlet nodes carry velocity data and
+    outlet nodes a pressure (CaseInput.hpp:162-259).  The steady state matches
+    the oracle's Newton solution with the same Dirichlet data."""
+    case, base = tube
+    d = tmp_path / "legacy"
+    shutil.copytree(base, d)
+    (d / "simulation_config.json").unlink()
+    vplus = 1e-4
+    (d / "simulation_parameter.txt").write_text(
+        f"D 1e-3\nvplus {vplus}\nvminus 0\nkplus 0.5\nkminus 0.1\nk'plus 0.3\nk'minus 0\ndt 1e-3\nnstep 1\n"
+        "N0bc 1\nNplusbc 0.5\nNminusbc 0\n")
+    outlet_pressure = 0.0
+    if with_case_config:
+        outlet_pressure = 1e-4
+        (d / "case_config.json").write_text(json.dumps({"schema_version": 1, "boundaries": {
+            "inherit_legacy": True, "conditions": [{"label": 2, "type": "outlet", "pressure": outlet_pressure}]}}))
+    out = tmp_path / "flow.txt"
+    r = run("navier-stokes", d / "case.ntiga", d, "--output", out, *TIGHT)
+    assert r.returncode == 0, r.stderr
+    assert f"boundary_config={'case_config.json' if with_case_config else 'legacy-defaults'}" in r.stdout
+    assert "time_integration=steady" in r.stdout and "viscosity=0.1 " in r.stdout
+    gpu = read_state(out)
+    # the same data for the oracle
+    lab = case.point_labels
+    m = case.mesh
+    vmask = ((lab == 0) | (lab == 1)).astype(np.int32)
+    pmask = (lab >= 2).astype(np.int32)
+    vt = np.zeros((m.nodes, 3))
+    vt[lab == 1] = vplus * case.inlet_velocity[lab == 1]
+    pt = np.where(lab >= 2, outlet_pressure, 0.0)
+    trace = boundary.wall_trace_basis(m, case.cells, lab)
+    vmask[trace] = 1
+    vt[trace] = 0.0
+    state = boundary.initial_state(vmask, pmask, vt, pt)
+    ref = steady_direct(m, state, 1.0, 0.1, vmask, pmask, vt, pt)
+    v, p = rel(gpu, ref.reshape(-1, 4))
+    print(f"legacy input (case_config={with_case_config}) vs oracle rel_l2 velocity={v:.3e} pressure={p:.3e}",
+          flush=True)
+    assert v <= 1e-10 and p <= 1e-10

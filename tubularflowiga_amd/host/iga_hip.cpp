@@ -233,22 +233,49 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		static_cast<std::uint64_t>(db.elements()));
 	const auto wall_trace = WallTraceBasis(db, labeled);
 	const auto profile = ReadVelocity((o.case_dir/"initial_velocityfield.txt").string(), db.nodes);
-	if (!fs::exists(o.case_dir/"simulation_config.json"))
-		throw std::runtime_error("legacy simulation_parameter.txt/case_config.json input is not supported by iga_hip; "
-			"write simulation_config.json");
-	const SimulationConfig config = ReadSimulationConfig((o.case_dir/"simulation_config.json").string());
-	const FlowSystem& sys = FindFlowSystem(config, o.system);
+	// Without simulation_config.json the case is the legacy input
+	// (iga_cuda.cu:791-795): simulation_parameter.txt and an optional
+	// case_config.json, a steady flow with viscosity 0.1 and density 1, node
+	// labels as Dirichlet data (wall and inlet velocity, outlet pressure).
+	const bool legacy = !fs::exists(o.case_dir/"simulation_config.json");
+	SimulationConfig config;
+	FlowSystem legacy_system;
+	ResolvedFlowBoundaries legacy_bc;
+	std::string boundary_config = "simulation_config.json";
+	if (legacy) {
+		const TransportParameters params = ReadTransportParameters((o.case_dir/"simulation_parameter.txt").string());
+		const CaseRules rules = ReadCaseRules((o.case_dir/"case_config.json").string());
+		const TransportBoundaries tb = ResolveTransportBoundaries(rules, labeled.labels, params, &profile);
+		legacy_bc.velocity_mask = tb.velocity_mask;
+		legacy_bc.pressure_mask = tb.pressure_mask;
+		legacy_bc.velocity = tb.velocity;
+		legacy_bc.pressure = tb.pressure;
+		legacy_bc.velocity_nodes = tb.velocity_nodes;
+		legacy_bc.pressure_nodes = tb.pressure_nodes;
+		legacy_system.viscosity = 0.1;
+		legacy_system.density = 1.0;
+		legacy_system.time_integration = "steady";
+		boundary_config = rules.present ? "case_config.json" : "legacy-defaults";
+	} else {
+		config = ReadSimulationConfig((o.case_dir/"simulation_config.json").string());
+	}
+	const FlowSystem& sys = legacy ? legacy_system : FindFlowSystem(config, o.system);
 	const bool transient = sys.time_integration == "backward_euler";
 	const double dt = transient ? config.dt : 0.0;
 	const int physical_steps = transient ? config.steps : 1;
 	auto step_config = [&](double time) { return transient ? MaterializeWaveforms(config, o.case_dir, time) : config; };
-	OutletBank outlets = OutletBank::FromConfig(config, sys);
+	OutletBank outlets = legacy ? OutletBank() : OutletBank::FromConfig(config, sys);
 	auto outlet_p = outlets.Tractions();
-	ResolvedFlowBoundaries bc = ResolveFlowBoundaries(step_config(0.0), sys, labeled.labels, profile, &outlet_p);
-	ApplyWallTrace(bc, wall_trace);
+	auto resolve = [&](const SimulationConfig& cfg) {
+		ResolvedFlowBoundaries r = legacy ? legacy_bc
+			: ResolveFlowBoundaries(cfg, sys, labeled.labels, profile, &outlet_p);
+		ApplyWallTrace(r, wall_trace);
+		return r;
+	};
+	ResolvedFlowBoundaries bc = resolve(step_config(0.0));
 	if (o.stop_after_step > physical_steps) throw std::runtime_error("--stop-after-step exceeds configured physical steps");
 	const int run_end = o.stop_after_step > 0 ? o.stop_after_step : physical_steps;
-	std::cout << "boundary_config=simulation_config.json viscosity=" << sys.viscosity << " density=" << sys.density
+	std::cout << "boundary_config=" << boundary_config << " viscosity=" << sys.viscosity << " density=" << sys.density
 		<< " time_integration=" << (transient ? "backward_euler" : "steady") << " dt=" << dt
 		<< " steps=" << physical_steps << " run_end_step=" << run_end << " velocity_nodes=" << bc.velocity_nodes
 		<< " pressure_nodes=" << bc.pressure_nodes << '\n';
@@ -367,8 +394,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		tfi_newton_report rep{}, first_rep{};  // last and first outlet pass of the step
 		for (int coupling = 0; coupling < passes; ++coupling) {
 			outlet_p = outlets.Tractions();
-			bc = ResolveFlowBoundaries(cfg, sys, labeled.labels, profile, &outlet_p);
-			ApplyWallTrace(bc, wall_trace);
+			bc = resolve(cfg);
 			std::vector<std::int32_t> te, tf;
 			std::vector<double> tp;
 			for (std::size_t k = 0; k < face_el.size(); ++k) {

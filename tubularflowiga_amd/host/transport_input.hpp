@@ -162,29 +162,46 @@ inline CaseRules ReadCaseRules(const std::string& path)
 	return c;
 }
 
-// ResolveBoundaryConditions (solvers/cpu/include/CaseInput.hpp:162-259), the
-// transport part plus the velocity / pressure node counts it validates.
+// ResolveBoundaryConditions (solvers/cpu/include/CaseInput.hpp:162-259): the
+// transport constraints, and the velocity / pressure Dirichlet data of the
+// legacy flow input (velocity: wall zero or the rule's velocity, inlet the
+// reference profile times vplus / the rule's scale or velocity; pressure:
+// outlets zero or the rule's pressure).
 struct TransportBoundaries {
 	std::vector<std::int32_t> constrained;
 	std::vector<double> n0, nplus;
+	std::vector<std::int32_t> velocity_mask, pressure_mask;
+	std::vector<double> velocity, pressure;   // [3N], [N]
 	std::size_t velocity_nodes = 0, pressure_nodes = 0, transport_nodes = 0;
 };
 
 inline TransportBoundaries ResolveTransportBoundaries(const CaseRules& c, const std::vector<int>& labels,
-	const TransportParameters& p)
+	const TransportParameters& p, const std::vector<std::array<double, 3>>* profile = nullptr)
 {
 	const std::size_t n = labels.size();
+	if (profile && profile->size() != n)
+		throw std::runtime_error("boundary labels and velocity field have different node counts");
 	TransportBoundaries r;
-	std::vector<int> vel(n, 0), pres(n, 0);
+	std::vector<std::int32_t>& vel = r.velocity_mask;
+	std::vector<std::int32_t>& pres = r.pressure_mask;
+	vel.assign(n, 0);
+	pres.assign(n, 0);
+	r.velocity.assign(3*n, 0.0);
+	r.pressure.assign(n, 0.0);
 	r.constrained.assign(n, 0);
 	r.n0.assign(n, 0.0);
 	r.nplus.assign(n, 0.0);
+	auto set_profile = [&](std::size_t i, double scale) {
+		if (profile)
+			for (int d = 0; d < 3; ++d) r.velocity[3*i + d] = scale*(*profile)[i][d];
+	};
 	if (!c.present || c.inherit_legacy) {
 		for (std::size_t i = 0; i < n; ++i) {
 			if (labels[i] == 0) {
 				vel[i] = 1;
 			} else if (labels[i] == 1) {
 				vel[i] = 1;
+				set_profile(i, p.vplus);
 				r.constrained[i] = 1;
 				r.n0[i] = p.n0_bc;
 				r.nplus[i] = p.nplus_bc;
@@ -202,15 +219,25 @@ inline TransportBoundaries ResolveTransportBoundaries(const CaseRules& c, const 
 			found = true;
 			vel[i] = pres[i] = r.constrained[i] = 0;
 			r.n0[i] = r.nplus[i] = 0.0;
+			for (int d = 0; d < 3; ++d) r.velocity[3*i + d] = 0.0;
+			r.pressure[i] = 0.0;
 			if (rule.type == RuleType::Wall) {
 				vel[i] = 1;
+				if (rule.has_velocity)
+					for (int d = 0; d < 3; ++d) r.velocity[3*i + d] = rule.velocity[d];
 			} else if (rule.type == RuleType::Inlet) {
 				vel[i] = 1;
+				if (rule.has_velocity) {
+					for (int d = 0; d < 3; ++d) r.velocity[3*i + d] = rule.velocity[d];
+				} else {
+					set_profile(i, rule.has_velocity_scale ? rule.velocity_scale : p.vplus);
+				}
 				r.constrained[i] = 1;
 				r.n0[i] = rule.has_transport ? rule.n0 : p.n0_bc;
 				r.nplus[i] = rule.has_transport ? rule.nplus : p.nplus_bc;
 			} else if (rule.type == RuleType::Outlet) {
 				pres[i] = 1;
+				r.pressure[i] = rule.has_pressure ? rule.pressure : 0.0;
 			}
 		}
 		if (!found)
