@@ -138,6 +138,12 @@ int tfi_bench_assembly(tfi_mesh* mesh, int tangent, int with_jacobian, int repet
  * grid-stride copy kernel), average milliseconds per launch.  No reference
  * counterpart. */
 int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milliseconds);
+/* Measured HBM stream rates on `count` doubles (even): a 16-byte-vector copy
+ * (GB/s counting read + written bytes) and a read-only sum (GB/s read), each
+ * averaged over `repetitions` launches of 16 workgroups per CU (HIP events).
+ * The measured bandwidth beside the published peak.  No reference
+ * counterpart. */
+int tfi_bench_stream(tfi_ctx* ctx, long long count, int repetitions, double* copy_gbs, double* read_gbs);
 /* Measured FP64 matrix-core throughput: a probe kernel of independent
  * v_mfma_f64_16x16x4f64 chains on every SIMD, TFLOP/s over `repetitions`
  * launches (HIP events).  The measured peak beside the published one.  No
@@ -157,7 +163,8 @@ typedef struct {
 	int preconditioner;               /* TFI_PC_BJACOBI | TFI_PC_AMG */
 	int precision;                    /* TFI_FP64 | TFI_MIXED */
 	int warm_start;                   /* TFI_WARM_* */
-	int verbose;
+	int verbose;                      /* > 0: one stderr line per Newton iteration as it ends (and one when a
+	                                     Chebyshev-smoothed solve falls back to Gauss-Seidel) */
 	int amg_smoother;                 /* TFI_SMOOTH_* on the large levels */
 	int tangent;                      /* TFI_TANGENT_* */
 	int amg_coarse_nodes;             /* block rows of the coarsest dense level (0: 256, at most 256) */

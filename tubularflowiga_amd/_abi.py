@@ -122,6 +122,7 @@ SIGNATURES = {
     "tfi_bench_spmv": (c_int, [c_void_p, c_int, c_int, P(c_double)]),
     "tfi_bench_copy": (c_int, [c_void_p, c_ll, c_int, P(c_double)]),
     "tfi_bench_mfma_fp64": (c_int, [c_void_p, c_int, P(c_double)]),
+    "tfi_bench_stream": (c_int, [c_void_p, c_ll, c_int, P(c_double), P(c_double)]),
     "tfi_bench_assembly": (c_int, [c_void_p, c_int, c_int, c_int, P(c_double)]),
     "tfi_flow_default_options": (None, [P(FlowOptions)]),
     "tfi_flow_create": (c_int, [c_void_p, P(FlowOptions), P(c_void_p)]),

@@ -892,6 +892,23 @@ def tube_velocity(case: Case) -> np.ndarray:
     return out
 
 
+def developed_tube_state(case: Case) -> np.ndarray:
+    """[N, 4] developed Poiseuille flow in a straight tube along z: the inlet
+    profile on every cross-section (``tube_velocity``) and the pressure
+    p = G (z_max - z), G = 4 mu u_max / R^2, in the case's normalised units
+    (mu = 3.5e-3 Pa s as ``flow_parameters``).  A start for pulsatile runs that
+    skips the impulsive start from rest."""
+    _, mu = flow_parameters(case)
+    L0 = case.length_unit_m
+    radius = case.manifest["radius_mm"] * 1e-3 / L0
+    u_max = case.manifest["u_max"] / L0
+    z = np.asarray(case.control_points)[:, 2]
+    out = np.zeros((case.mesh.nodes, 4))
+    out[:, :3] = tube_velocity(case)
+    out[:, 3] = 4.0 * mu * u_max / (radius * radius) * (z.max() - z)
+    return out
+
+
 def write_transport_case(case: Case, directory, params: dict, steps: int, velocity=None, case_config=None):
     """A case directory for `transport`: the flow case files plus the legacy
     simulation_parameter.txt ("key value" lines, solvers/cpu/include/CaseInput.hpp:85-119),

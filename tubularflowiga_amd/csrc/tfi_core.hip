@@ -98,6 +98,48 @@ __global__ void CopyKernel(const double* __restrict__ x, double* __restrict__ y,
 		y[i] = x[i];
 }
 
+// Stream probes for the measured HBM peak: 16-byte accesses, four in flight
+// per lane before the first store (copy) or add (read).  n2 counts double2.
+__global__ void __launch_bounds__(256) StreamCopyKernel(const double2* __restrict__ x, double2* __restrict__ y,
+	long long n2)
+{
+	const long long stride = (long long)gridDim.x*blockDim.x;
+	long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x;
+	for (; i + 3*stride < n2; i += 4*stride) {
+		const double2 a = x[i], b = x[i + stride], c = x[i + 2*stride], d = x[i + 3*stride];
+		y[i] = a;
+		y[i + stride] = b;
+		y[i + 2*stride] = c;
+		y[i + 3*stride] = d;
+	}
+	for (; i < n2; i += stride) y[i] = x[i];
+}
+
+__global__ void __launch_bounds__(256) StreamReadKernel(const double2* __restrict__ x, long long n2,
+	double* __restrict__ partial)
+{
+	const long long stride = (long long)gridDim.x*blockDim.x;
+	long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x;
+	double s0 = 0.0, s1 = 0.0;
+	for (; i + 3*stride < n2; i += 4*stride) {
+		const double2 a = x[i], b = x[i + stride], c = x[i + 2*stride], d = x[i + 3*stride];
+		s0 += (a.x + b.x) + (c.x + d.x);
+		s1 += (a.y + b.y) + (c.y + d.y);
+	}
+	for (; i < n2; i += stride) {
+		s0 += x[i].x;
+		s1 += x[i].y;
+	}
+	__shared__ double lane_sum[256];
+	lane_sum[threadIdx.x] = s0 + s1;
+	__syncthreads();
+	for (int w = blockDim.x/2; w > 0; w >>= 1) {
+		if (threadIdx.x < w) lane_sum[threadIdx.x] += lane_sum[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) partial[blockIdx.x] = lane_sum[0];
+}
+
 __global__ void FillKernel(double* __restrict__ x, double v, long long n)
 {
 	for (long long i = blockIdx.x*(long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x*blockDim.x)
@@ -175,6 +217,20 @@ void MfmaPeak(tfi_ctx* ctx, int blocks, int threads, int iterations, double* out
 {
 	hipLaunchKernelGGL(MfmaPeakKernel, dim3(blocks), dim3(threads), 0, ctx->stream, iterations, 0.5, out);
 	TFI_LAUNCH_CHECK("MfmaPeakKernel");
+}
+
+void StreamCopy(tfi_ctx* ctx, const double* x, double* y, long long n, int blocks)
+{
+	hipLaunchKernelGGL(StreamCopyKernel, dim3(blocks), dim3(kThreads), 0, ctx->stream,
+		reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y), n/2);
+	TFI_LAUNCH_CHECK("StreamCopyKernel");
+}
+
+void StreamRead(tfi_ctx* ctx, const double* x, long long n, double* partial, int blocks)
+{
+	hipLaunchKernelGGL(StreamReadKernel, dim3(blocks), dim3(kThreads), 0, ctx->stream,
+		reinterpret_cast<const double2*>(x), n/2, partial);
+	TFI_LAUNCH_CHECK("StreamReadKernel");
 }
 
 void Copy(tfi_ctx* ctx, const double* x, double* y, long long n)

@@ -105,6 +105,9 @@ KrylovResult LinearSolver::Solve(const double* b, double* x, int maxit, double r
 	Copy(ctx_, x, guess_.data(), n);
 	KrylovResult first = SolveOnce(b, x, std::min(maxit, kChebyshevCap), rtol);
 	if (first.converged || maxit <= kChebyshevCap) return first;
+	if (verbose_ > 0)
+		std::fprintf(stderr, "tfi_flow linear solve: Chebyshev-smoothed AMG stalled at %d iterations "
+			"(residual %.3e); Gauss-Seidel smoothing from here on\n", first.iterations, first.residual);
 	amg_->UseGaussSeidel();
 	Setup(values_);
 	Copy(ctx_, guess_.data(), x, n);
@@ -343,6 +346,7 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 		if (options->amg_gs_sweeps > 0) amg.gs_sweeps = options->amg_gs_sweeps;
 		f->solver = std::make_unique<LinearSolver>(mesh, options->preconditioner, options->precision,
 			options->restart, amg);
+		f->solver->set_verbose(options->verbose);
 		f->warm = std::make_unique<WarmStart>(mesh);
 		Sync(ctx);
 		*out = f.release();
@@ -466,6 +470,16 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 		const auto t_start = Clock::now();
 		const double scale = std::sqrt(static_cast<double>(n));
 		double initial = -1.0;
+		// Keep the iteration's record; with verbose > 0 also report it on stderr
+		// as it happens (a long solve is visibly alive).
+		auto push = [&](const tfi_newton_record& r, int iteration) {
+			if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = r;
+			if (f->opt.verbose > 0)
+				std::fprintf(stderr, "tfi_flow newton=%d residual=%.6e relative_mass_imbalance=%.3e "
+					"linear_iterations=%d linear_residual=%.3e step=%.3g assembly_s=%.3f linear_s=%.3f\n",
+					iteration, r.residual_l2, r.relative_mass_imbalance, r.linear_iterations, r.linear_residual,
+					r.step, r.assembly_s, r.linear_s);
+		};
 		// Residual (and tangent when jac is given) at the current state: traction
 		// load, continuity / boundary-flow measurement, Dirichlet rows.
 		auto assemble = [&](bool with_jacobian, Metrics* mt) {
@@ -650,7 +664,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			rep.final_residual = norm;
 			if (norm <= tol && mt.imbalance <= f->opt.mass_rtol) {
 				converged = true;
-				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				push(rec, it);
 				rep.newton_iterations = it;
 				break;
 			}
@@ -660,7 +674,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				Axpy(ctx, -0.5*applied, f->last_update.data(), f->state.data(), n);
 				applied *= 0.5;
 				++backtracks;
-				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				push(rec, it);
 				rep.newton_iterations = it + 1;
 				prev_norm = -1.0;  // the quadratic model does not hold across a backtrack
 				continue;
@@ -685,13 +699,13 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 				Axpy(ctx, -0.5*applied, f->last_update.data(), f->state.data(), n);
 				applied *= 0.5;
 				++backtracks;
-				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				push(rec, it);
 				rep.newton_iterations = it + 1;
 				prev_norm = -1.0;  // the quadratic model does not hold across a backtrack
 				continue;
 			}
 			if (!r.converged) {
-				if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+				push(rec, it);
 				rep.total_s = std::chrono::duration<double>(Clock::now() - t_start).count();
 				if (report) *report = rep;
 				throw Error("Navier-Stokes linear solve failed at nonlinear iteration " + std::to_string(it)
@@ -710,7 +724,7 @@ int tfi_flow_newton(tfi_flow* f, tfi_newton_report* report)
 			Copy(ctx, f->update.data(), f->last_update.data(), n);
 			applied = step;
 			norm_before = norm;
-			if (rep.records < TFI_MAX_NEWTON_RECORDS) rep.record[rep.records++] = rec;
+			push(rec, it);
 			rep.newton_iterations = it + 1;
 			prev_norm = norm;
 		}
@@ -983,6 +997,34 @@ int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milli
 		float ms = 0.0f;
 		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
 		*milliseconds = ms/repetitions;
+	});
+}
+
+int tfi_bench_stream(tfi_ctx* ctx, long long count, int repetitions, double* copy_gbs, double* read_gbs)
+{
+	return Guard(ctx, [&] {
+		if (count <= 0 || (count & 1) || repetitions <= 0 || !copy_gbs || !read_gbs)
+			throw Error("stream probe needs an even positive size, a positive repetition count and outputs");
+		const int blocks = 16*ctx->cus;
+		DevBuf<double> a(ctx, count), b(ctx, count), partial(ctx, blocks);
+		Fill(ctx, a.data(), 1.0, count);
+		Fill(ctx, b.data(), 0.0, count);
+		StreamCopy(ctx, a.data(), b.data(), count, blocks);
+		StreamRead(ctx, a.data(), count, partial.data(), blocks);
+		Sync(ctx);
+		auto time = [&](auto&& launch) {
+			Check(hipEventRecord(ctx->ev_begin, ctx->stream), "event");
+			for (int i = 0; i < repetitions; ++i) launch();
+			Check(hipEventRecord(ctx->ev_end, ctx->stream), "event");
+			Check(hipEventSynchronize(ctx->ev_end), "event sync");
+			float ms = 0.0f;
+			Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
+			return static_cast<double>(ms)/repetitions;
+		};
+		const double copy_ms = time([&] { StreamCopy(ctx, a.data(), b.data(), count, blocks); });
+		const double read_ms = time([&] { StreamRead(ctx, a.data(), count, partial.data(), blocks); });
+		*copy_gbs = 16.0*count/(copy_ms*1e-3)/1e9;
+		*read_gbs = 8.0*count/(read_ms*1e-3)/1e9;
 	});
 }
 

@@ -175,6 +175,9 @@ void Scale(tfi_ctx* ctx, double a, double* x, long long n);
 void Copy(tfi_ctx* ctx, const double* x, double* y, long long n);
 // FP64 matrix-core throughput probe (blocks x threads lanes, iterations of 4 MFMA chains)
 void MfmaPeak(tfi_ctx* ctx, int blocks, int threads, int iterations, double* out);
+// n doubles (even), `blocks` workgroups of 256; partial receives one sum per workgroup
+void StreamCopy(tfi_ctx* ctx, const double* x, double* y, long long n, int blocks);
+void StreamRead(tfi_ctx* ctx, const double* x, long long n, double* partial, int blocks);
 void Fill(tfi_ctx* ctx, double* x, double value, long long n);
 void FillInt(tfi_ctx* ctx, int* x, int value, long long n);
 void ToFloat(tfi_ctx* ctx, const double* x, float* y, long long n);

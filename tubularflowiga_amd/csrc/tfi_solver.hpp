@@ -49,6 +49,7 @@ public:
 	int singular_blocks() const { return singular_; }
 	int refinements() const { return refinements_; }
 	const Amg* amg() const { return amg_.get(); }
+	void set_verbose(int verbose) { verbose_ = verbose; }
 
 private:
 	tfi_mesh* mesh_;
@@ -62,6 +63,7 @@ private:
 	int singular_ = 0;
 	int refinements_ = 0;
 	DevBuf<double> guess_;
+	int verbose_ = 0;
 	KrylovResult SolveOnce(const double* b, double* x, int maxit, double rtol);
 };
 

@@ -47,6 +47,12 @@ class Context:
         self.check(self.lib.tfi_bench_copy(self.handle, int(count), repetitions, ctypes.byref(ms)))
         return ms.value
 
+    def bench_stream(self, count, repetitions=5):
+        """Measured HBM rates (GB/s): 16-byte-vector copy (read + write bytes) and read-only sum."""
+        c, r = ctypes.c_double(), ctypes.c_double()
+        self.check(self.lib.tfi_bench_stream(self.handle, int(count), repetitions, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
     def bench_mfma_fp64(self, repetitions=5):
         """Measured FP64 matrix-core TFLOP/s (probe kernel)."""
         t = ctypes.c_double()
