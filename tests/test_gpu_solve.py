@@ -226,11 +226,13 @@ def test_cli_solve_matches_the_oracle_and_restarts(tmp_path):
     assert np.abs(again - got).max() <= 1e-12 * np.abs(got).max()
 
 
-def test_cli_solve_with_a_velocity_series(tmp_path):
+@pytest.mark.parametrize("preconditioner", ["bjacobi", "amg"])
+def test_cli_solve_with_a_velocity_series(tmp_path, preconditioner):
     """A snapshot-series velocity source: the operators are re-assembled before
     every step for the velocity interpolated linearly at the step's end time
-    (ConfiguredTransportHost.inc:337-376); the oracle does the same with its
-    own assembly and a direct solve per step."""
+    (ConfiguredTransportHost.inc:337-376), with the block-Jacobi inverses or the
+    AMG hierarchy rebuilt from each new matrix; the oracle does the same with
+    its own assembly and a direct solve per step."""
     import json
     import subprocess
     import sys
@@ -239,7 +241,6 @@ def test_cli_solve_with_a_velocity_series(tmp_path):
     sys.path.insert(0, str(Path(__file__).resolve().parent))
     case, d, sysdef, constrained, values, start, surfaces = _cli_case(tmp_path)
     config = json.loads((d / "simulation_config.json").read_text())
-    # the AMG (set up again after every re-assembly) on this run
     for t in config["equation_systems"][0]["terms"]:
         if t["operator"] == "advection":
             t["velocity"] = "series"
@@ -256,9 +257,9 @@ def test_cli_solve_with_a_velocity_series(tmp_path):
     (d / "velocity.csv").write_text("time,file\n" + "".join(f"{t!r},v{k}.txt\n" for k, (t, _) in enumerate(snaps)))
     cli = Path(__file__).resolve().parent.parent / "tubularflowiga_amd" / "iga_hip"
     out = tmp_path / "series" / "species.txt"
-    print("case: iga_hip solve with a velocity series, 4 steps", flush=True)
+    print(f"case: iga_hip solve with a velocity series, 4 steps, {preconditioner}", flush=True)
     r = subprocess.run([str(cli), "solve", str(d / "case.ntiga"), str(d), "--output", str(out), "--linear-rtol",
-                        "1e-11", "--preconditioner", "amg"], capture_output=True, text=True, timeout=300)
+                        "1e-11", "--preconditioner", preconditioner], capture_output=True, text=True, timeout=300)
     print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
     assert r.returncode == 0
     got = _read_output(out, 3)
