@@ -359,6 +359,30 @@ def test_exact_tangent_finite_difference():
         assert np.linalg.norm(defect) / np.linalg.norm(action) <= 1e-8
 
 
+@pytest.mark.parametrize("dt", [1e-3, 0.0])
+def test_assembled_exact_tangent_finite_difference(dt):
+    """The assembled exact tangent on the tube (extraction operators, scatter)
+    matches central differences of the assembled negated residual, for a
+    backward-Euler step and for the steady equations (dt = 0, no mass term in
+    the stabilisation)."""
+    case = casegen.workload("tube_c0")
+    m = case.mesh
+    pattern = O.block_pattern(m)
+    ro, cols, _ = pattern
+    n4 = 4 * m.nodes
+    rng = np.random.default_rng(11)
+    state = 0.1 * rng.standard_normal(n4)
+    prev = state + 0.01 * rng.standard_normal(n4)
+    d = rng.standard_normal(n4)
+    jac, _ = O.ns_assemble(m, pattern, state, prev, 1.0, 0.1, dt, "exact")
+    A = sp.bsr_matrix((jac, cols, ro), shape=(n4, n4))
+    eps = 1e-6
+    _, rp = O.ns_assemble(m, pattern, state + eps * d, prev, 1.0, 0.1, dt, "exact", jacobian=False)
+    _, rm = O.ns_assemble(m, pattern, state - eps * d, prev, 1.0, 0.1, dt, "exact", jacobian=False)
+    fd = -(rp - rm) / (2 * eps)
+    assert np.linalg.norm(A @ d - fd) / np.linalg.norm(fd) <= 1e-9
+
+
 def test_legacy_tangent_equals_exact_on_uniform_flow():
     """With uniform velocity, uniform pressure, u = u_prev and vanishing
     viscosity the strong residual is zero and the exact tangent reduces to the
