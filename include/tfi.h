@@ -140,7 +140,8 @@ int tfi_bench_assembly(tfi_mesh* mesh, int tangent, int with_jacobian, int repet
 int tfi_bench_copy(tfi_ctx* ctx, long long count, int repetitions, double* milliseconds);
 /* Measured HBM stream rates on `count` doubles (even): a 16-byte-vector copy
  * (GB/s counting read + written bytes) and a read-only sum (GB/s read), each
- * averaged over `repetitions` launches of 16 workgroups per CU (HIP events).
+ * averaged over `repetitions` launches of 32 workgroups per CU, one contiguous
+ * chunk per workgroup (HIP events).
  * The measured bandwidth beside the published peak.  No reference
  * counterpart. */
 int tfi_bench_stream(tfi_ctx* ctx, long long count, int repetitions, double* copy_gbs, double* read_gbs);

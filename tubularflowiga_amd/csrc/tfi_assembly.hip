@@ -33,6 +33,7 @@ constexpr int kThreads = 256;
 constexpr size_t kLdsLimit = 160*1024;  // gfx950 LDS per workgroup
 constexpr int kQd = 56;       // per-point record in LDS
 constexpr int kBasisRec = 7;  // N, dN/dx, dN/dy, dN/dz, lap N, u.grad N, w.grad N (w = u + u')
+static_assert(kMaxBasis % 2 == 0, "the connectivity ints fill whole doubles of the LDS carve-up");
 
 __constant__ double kGp[4] = {0.06943184420297371, 0.33000947820757187, 0.6699905217924281, 0.9305681557970262};
 
@@ -45,6 +46,7 @@ __device__ inline void Bern3(double x, double b[4], double db[4], double d2b[4])
 }
 
 constexpr int kTableRec = 5;  // N, dN/dx, dN/dy, dN/dz, lap N
+constexpr int kPointGeom = 10;  // geometry fields the point record reads: measure, J^-1
 
 // Value, physical gradient and Laplacian of the basis function with Bezier
 // extraction row c[64] at element point q (Bernstein sum-factorisation in
@@ -84,17 +86,18 @@ __device__ inline void EvalBasis(const double* __restrict__ c, const double* __r
 }
 
 // Per-point record from the field sums s (u, p, grad u, grad p, lap u, u_prev
-// at the point; 22 values) and the point's geometry record gq (field stride
-// kQ): r[0] measure, r[1..2] tau_M, tau_C, r[3..5] u, r[6..8] rho (du/dt +
+// at the point; 22 values) and the point's geometry record gq (fields 0..9:
+// measure and J^-1, field stride kGs): r[0] measure, r[1..2] tau_M, tau_C, r[3..5] u, r[6..8] rho (du/dt +
 // w.grad u), r[9..17] stress rows, r[18] div u, r[19..21] u', r[22..24] w =
 // u + u', r[25..33] grad u and, with exact_tangent, r[34..54] the
 // linearisation of u' (K, R, c1) the consistent tangent needs.
+template <int kGs>
 __device__ inline void PointRecord(const double* __restrict__ s, const double* __restrict__ gq, double rho, double mu,
 	double nu, double dt, bool exact_tangent, double* __restrict__ r)
 {
 	double inv[3][3];
 	for (int a = 0; a < 3; ++a)
-		for (int x = 0; x < 3; ++x) inv[a][x] = gq[(1 + 3*a + x)*kQ];
+		for (int x = 0; x < 3; ++x) inv[a][x] = gq[(1 + 3*a + x)*kGs];
 	const double u[3] = {s[0], s[1], s[2]};
 	const double p = s[3];
 	double grad[4][3];
@@ -234,15 +237,16 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 	const double nu = mu/rho;
 	const double idt = dt > 0.0 ? 1.0/dt : 0.0;
 
-	// LDS carve-up
+	// LDS carve-up (the MFMA scatter stages the element's blocks over basis and qd)
 	const int qstride = args.qstride;
 	const double* table = args.table;
 	double* nodal = lds;                                   // [nen][8]
-	double* C = nodal + kMaxBasis*8;                        // [nen][64], only without the table
+	double* resid = nodal + kMaxBasis*8;                    // [a_tile][4]
+	int* lconn = reinterpret_cast<int*>(resid + 4*kMaxBasis);
+	double* gslab = resid + 4*kMaxBasis + kMaxBasis/2;      // [kPointGeom][kSlab] measure, J^-1 of the slab
+	double* C = gslab + (kMfma ? kPointGeom*kSlab : 0);     // [nen][64], only without the table
 	double* basis = C + (table ? 0 : (size_t)args.max_nen*64);  // [kSlab][qstride] records
 	double* qd = basis + (size_t)kSlab*qstride;             // [kSlab][kQd]
-	double* resid = qd + kSlab*kQd;                         // [a_tile][4]
-	int* lconn = reinterpret_cast<int*>(resid + 4*kMaxBasis);
 
 	for (int a = tid; a < nen; a += kT) {
 		const int node = args.conn[b0 + a];
@@ -283,6 +287,11 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 		const int q0 = slab*kSlab;
 		// ---- P1: basis values, gradients and Laplacians per (a, point): read
 		// from the precomputed table when there is one, else evaluated here
+		if (kMfma)  // the point records read the slab's measure and J^-1 from LDS
+			for (int i = tid; i < kPointGeom*kSlab; i += kT) {
+				const int f = i/kSlab, ql = i - f*kSlab;
+				gslab[i] = geo[f*kQ + q0 + ql];
+			}
 		if (kMfma || table) {  // the MFMA variants run only with the table
 			const int rec = nen*kTableRec;
 			const double* src = table + ((size_t)b0*kQ + (size_t)q0*nen)*kTableRec;
@@ -320,8 +329,10 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 			for (int i = 0; i < 22; ++i)
 #pragma unroll
 				for (int o = 1; o < kL; o <<= 1) s[i] += __shfl_xor(s[i], o, 64);
-			if (part == 0)
-				PointRecord(s, geo + q, rho, mu, nu, dt, kExact && kJac, qd + ql*kQd);
+			if (part == 0) {
+				if constexpr (kMfma) PointRecord<kSlab>(s, gslab + ql, rho, mu, nu, dt, kExact && kJac, qd + ql*kQd);
+				else PointRecord<kQ>(s, geo + q, rho, mu, nu, dt, kExact && kJac, qd + ql*kQd);
+			}
 		}
 		__syncthreads();
 		// ---- P3: u.grad N per (a, point); residual rows of this tile
@@ -568,40 +579,41 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 		__syncthreads();
 	}
 
-	// ---- scatter (exclusive by colour: plain read-modify-write), straight
-	// from the MFMA output layout (staging through LDS into 128-byte block rows
-	// measured slower: the four waves' updates of a block meet in L2)
+	// ---- scatter (exclusive by colour: plain read-modify-write).  The MFMA
+	// path stages the element's blocks in LDS (the slab buffers are free after
+	// the last slab's barrier; block stride kStageStride = 17 doubles, so that
+	// lanes writing neighbouring blocks hit different banks), then eight lanes
+	// per block read-modify-write its 128 bytes as 16-byte pieces: whole cache
+	// lines per block instead of 16 scattered 8-byte updates from four waves.
 	if (kMfma && kJac) {
+		constexpr int kStageStride = 17;
+		double* stage = basis;
 		const int wave = tid >> 6, lane = tid & 63;
-		const long long pbase = args.pair_offsets[e];
 		const int x = wave & 3, ti = wave >> 2;
 		const int y1 = (x + 1)%3, y2 = (x + 2)%3;
 		const int comp[4] = {x < 3 ? 5*x : 12, x < 3 ? 4*x + y1 : 13, x < 3 ? 4*x + y2 : 14, x < 3 ? 4*x + 3 : 15};
-		// block indices of this lane's 8 pairs, then per component all 8 loads
-		// before the 8 stores, so that the read latencies overlap
-		int slot[2][4];
 #pragma unroll
-		for (int tj = 0; tj < 2; ++tj) {
-			const int b = 16*tj + (lane & 15);
+		for (int c = 0; c < 4; ++c)
 #pragma unroll
-			for (int i = 0; i < 4; ++i) {
-				const int a = 16*ti + 4*i + (lane >> 4);
-				slot[tj][i] = (a < nen && b < nen) ? args.pair_block[pbase + (long long)a*nen + b] : -1;
+			for (int tj = 0; tj < 2; ++tj) {
+				const int b = 16*tj + (lane & 15);
+#pragma unroll
+				for (int i = 0; i < 4; ++i) {
+					const int a = 16*ti + 4*i + (lane >> 4);  // output row of element i of the lane's accumulator
+					if (a < nen && b < nen) stage[(a*nen + b)*kStageStride + comp[c]] = acc[c][tj][i];
+				}
 			}
-		}
-#pragma unroll
-		for (int c = 0; c < 4; ++c) {
-			double v[2][4];
-#pragma unroll
-			for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-				for (int i = 0; i < 4; ++i)
-					v[tj][i] = slot[tj][i] >= 0 ? args.jac[(size_t)slot[tj][i]*16 + comp[c]] : 0.0;
-#pragma unroll
-			for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-				for (int i = 0; i < 4; ++i)
-					if (slot[tj][i] >= 0) args.jac[(size_t)slot[tj][i]*16 + comp[c]] = v[tj][i] + acc[c][tj][i];
+		__syncthreads();
+		const long long pbase = args.pair_offsets[e];
+		for (int it = tid; it < nen*nen*8; it += kT) {
+			const int blk = it >> 3, part = it & 7;
+			const int slot = args.pair_block[pbase + blk];
+			double2* J = reinterpret_cast<double2*>(args.jac + (size_t)slot*16) + part;
+			const double* src = stage + blk*kStageStride + 2*part;
+			double2 v = *J;
+			v.x += src[0];
+			v.y += src[1];
+			*J = v;
 		}
 	}
 	if (pair_active) {
@@ -898,7 +910,7 @@ __global__ void __launch_bounds__(kThreads, 2) NsResidualKernel(AsmArgs args)
 			s[i] += __shfl_xor(s[i], 2, 64);
 		}
 		if (part == 0)
-			PointRecord(s, args.geometry + (size_t)e*kGeomFields*kQ + q, rho, mu, nu, dt, false, qd + q*kQr);
+			PointRecord<kQ>(s, args.geometry + (size_t)e*kGeomFields*kQ + q, rho, mu, nu, dt, false, qd + q*kQr);
 	}
 	__syncthreads();
 	if (tid < nen*8) {
@@ -979,10 +991,13 @@ namespace {
 // LDS bytes of one AssembleNsKernel workgroup (nodal records, extraction rows
 // when there is no basis table, slab basis records, point records, residual
 // rows, connectivity).
-size_t AssemblyLds(int nen_max, bool table, int slab, int qstride)
+size_t AssemblyLds(int nen_max, bool table, int slab, int qstride, bool mfma)
 {
-	return sizeof(double)*(kMaxBasis*8 + (table ? 0 : (size_t)nen_max*64) + (size_t)slab*qstride + (size_t)slab*kQd
-		+ 4*kMaxBasis) + sizeof(int)*kMaxBasis;
+	// the MFMA scatter stages nen^2 blocks of 17 doubles over the slab buffers
+	const size_t slab_part = std::max((size_t)slab*qstride + (size_t)slab*kQd,
+		mfma ? (size_t)nen_max*nen_max*17 : (size_t)0);
+	return sizeof(double)*(kMaxBasis*8 + 4*kMaxBasis + kMaxBasis/2 + (mfma ? (size_t)kPointGeom*slab : 0)
+		+ (table ? 0 : (size_t)nen_max*64) + slab_part);
 }
 
 template <bool kExact, bool kMfma, bool kJac, int kSlab>
@@ -1044,8 +1059,8 @@ void AssembleNs(tfi_mesh* m, const double* state, const double* previous, const 
 	// slab of quadrature points: the largest of 32 / 16 / 8 whose LDS fits
 	int slab = 32;
 	if (!mfma)
-		while (slab > 8 && AssemblyLds(nen_max, table, slab, qstride) > kLdsLimit) slab >>= 1;
-	const size_t lds = AssemblyLds(nen_max, table, slab, qstride);
+		while (slab > 8 && AssemblyLds(nen_max, table, slab, qstride, false) > kLdsLimit) slab >>= 1;
+	const size_t lds = AssemblyLds(nen_max, table, slab, qstride, mfma);
 	if (lds > kLdsLimit)
 		throw Error("element assembly needs " + std::to_string(lds) + " bytes of LDS for " + std::to_string(nen_max)
 			+ " basis functions per element, above the " + std::to_string(kLdsLimit) + " bytes of a workgroup");

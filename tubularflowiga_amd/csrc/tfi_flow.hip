@@ -1005,7 +1005,7 @@ int tfi_bench_stream(tfi_ctx* ctx, long long count, int repetitions, double* cop
 	return Guard(ctx, [&] {
 		if (count <= 0 || (count & 1) || repetitions <= 0 || !copy_gbs || !read_gbs)
 			throw Error("stream probe needs an even positive size, a positive repetition count and outputs");
-		const int blocks = 16*ctx->cus;
+		const int blocks = 32*ctx->cus;
 		DevBuf<double> a(ctx, count), b(ctx, count), partial(ctx, blocks);
 		Fill(ctx, a.data(), 1.0, count);
 		Fill(ctx, b.data(), 0.0, count);
