@@ -68,13 +68,14 @@ DESCRIPTIONS = {"tree_c4": "seeded vessel tree (binary, depth 4, 15 segments, ro
 
 
 def run_method(dm, case, bc, faces, rho, mu, dt, steps, warmup, precision, warm, tol, train_steps=20,
-               waveform=None, smoother=None, start=None):
+               waveform=None, smoother=None, start=None, cheb_degree=2):
     from tubularflowiga_amd import boundary, device
     vmask, pmask, vt, pt = bc
     flow = device.FlowSolver(dm, density=rho, viscosity=mu, dt=dt, nonlinear_rtol=tol["rtol"],
                              nonlinear_atol=tol["atol"], mass_rtol=tol["mass"], linear_rtol=1e-8,
                              preconditioner=device.TFI_PC_AMG, precision=precision, warm_start=warm,
                              restart=100, linear_maximum_iterations=4000, verbose=1,
+                             amg_chebyshev_degree=cheb_degree,
                              amg_smoother=device.TFI_SMOOTH_AUTO if smoother is None else smoother)
     if warm == device.TFI_WARM_LEARNED:
         flow.configure_warm_start(hidden=128, train_steps=train_steps, learning_rate=1e-3, min_snapshots=2)
@@ -501,6 +502,9 @@ def main(argv=None):
                          "400-step workload)")
     ap.add_argument("--smoother", choices=["auto", "gauss-seidel", "chebyshev", "jacobi"], default="auto",
                     help="AMG smoother of both methods (auto: Chebyshev for transient steps, Gauss-Seidel for steady)")
+    ap.add_argument("--cheb-degree", type=int, default=None,
+                    help="Chebyshev steps before and after each coarse correction (both methods); default 1 on the "
+                         "vessel tree, 2 elsewhere")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV kernel on the workload's pattern plus a 2 GiB copy (for counter passes)")
     ap.add_argument("--assembly-only", action="store_true",
@@ -552,16 +556,21 @@ def main(argv=None):
     # the same mesh gives the same), and the shortened Newton steps that
     # follow leave tangents the AMG-preconditioned FGMRES cannot solve
     start = casegen.developed_tube_state(case) if a.workload.startswith("tube") else None
+    # Chebyshev degree: on the tree a second smoothing step saves too few
+    # Krylov iterations to pay for its matrix pass (steps 4-8 from rest, plain
+    # FP64: 5.77 s per step at degree 1, 6.80 s at 2, 8.16 s at 3); on the
+    # Y-bifurcation degree 2 is the better one (0.48 s against 0.54 s at 1)
+    cheb_degree = a.cheb_degree or (1 if a.workload.startswith("tree") else 2)
     smoother = {"gauss-seidel": device.TFI_SMOOTH_GAUSS_SEIDEL, "chebyshev": device.TFI_SMOOTH_CHEBYSHEV,
                 "jacobi": device.TFI_SMOOTH_JACOBI, "auto": device.TFI_SMOOTH_AUTO}[a.smoother]
     log(f"plain FP64 (AMG with {a.smoother} smoothing, previous-step start)")
     plain, plain_all, plain_timers, plain_state, _ = run_method(
         dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_FP64, device.TFI_WARM_PREVIOUS, tol,
-        waveform=wf, smoother=smoother, start=start)
+        waveform=wf, smoother=smoother, start=start, cheb_degree=cheb_degree)
     log("mixed precision + learned warm start")
     fast, fast_all, fast_timers, fast_state, warm_stats = run_method(
         dm, case, bc, faces, rho, mu, a.dt, a.steps, a.warmup, device.TFI_MIXED, device.TFI_WARM_LEARNED, tol,
-        a.train_steps, waveform=wf, smoother=smoother, start=start)
+        a.train_steps, waveform=wf, smoother=smoother, start=start, cheb_degree=cheb_degree)
 
     ms_plain = 1e3 * float(np.mean([r["wall_s"] for r in plain]))
     ms_fast = 1e3 * float(np.mean([r["wall_s"] for r in fast]))
@@ -601,7 +610,7 @@ def main(argv=None):
                                f"{a.warmup} untimed + {a.steps} timed steps per method",
                    "nodes": mesh.nodes, "elements": mesh.elements, "dofs": 4 * mesh.nodes,
                    "block_nonzeros": int(dm.nnzb), "newton_rtol": tol["rtol"], "linear_rtol": 1e-8,
-                   "amg_smoother": a.smoother},
+                   "amg_smoother": a.smoother, "amg_chebyshev_degree": cheb_degree},
         "fp64_ms_per_step": round(ms_plain, 3),
         "speedup_vs_fp64": round(ms_plain / ms_fast, 3),
         "plain_fp64": method_summary(plain),

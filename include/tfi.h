@@ -178,6 +178,7 @@ typedef struct {
 	                                     residual 4x) before a new tangent is built.  A learned guess
 	                                     predicted within that window starts this way, with the tangent
 	                                     of an earlier step. */
+	int amg_chebyshev_degree;         /* Chebyshev steps before and after the coarse correction (0: 2) */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);

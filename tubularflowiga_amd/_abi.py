@@ -39,7 +39,8 @@ class FlowOptions(ctypes.Structure):
                 ("mass_rtol", c_double), ("linear_maximum_iterations", c_int), ("linear_rtol", c_double),
                 ("restart", c_int), ("preconditioner", c_int), ("precision", c_int), ("warm_start", c_int),
                 ("verbose", c_int), ("amg_smoother", c_int), ("tangent", c_int), ("amg_coarse_nodes", c_int),
-                ("amg_gs_min_rows", c_int), ("amg_gs_sweeps", c_int), ("chord_polish", c_int)]
+                ("amg_gs_min_rows", c_int), ("amg_gs_sweeps", c_int), ("chord_polish", c_int),
+                ("amg_chebyshev_degree", c_int)]
 
 
 class NewtonRecord(ctypes.Structure):

@@ -834,7 +834,7 @@ void Amg::Impl::Cycle(int l)
 		gs_sweep(true);
 		if (opt.gs_sweeps > 1) gs_sweep(false, true);
 	} else if (cheb) {
-		chebyshev(kPre, true);
+		chebyshev(opt.chebyshev_degree, true);
 	} else {
 		// damped block Jacobi from x = 0
 		hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv, b,
@@ -863,7 +863,7 @@ void Amg::Impl::Cycle(int l)
 		if (opt.gs_sweeps > 1) gs_sweep(true);
 		gs_sweep(false, opt.gs_sweeps > 1);
 	} else if (cheb) {
-		chebyshev(kPost, false);
+		chebyshev(opt.chebyshev_degree, false);
 	} else {
 		for (int s = 0; s < kPost; ++s) {
 			hipLaunchKernelGGL(SmoothKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, inv,

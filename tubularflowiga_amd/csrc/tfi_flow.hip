@@ -344,6 +344,9 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 		if (options->amg_coarse_nodes > 0) amg.coarse_nodes = options->amg_coarse_nodes;
 		if (options->amg_gs_min_rows > 0) amg.gs_min_rows = options->amg_gs_min_rows;
 		if (options->amg_gs_sweeps > 0) amg.gs_sweeps = options->amg_gs_sweeps;
+		if (options->amg_chebyshev_degree < 0 || options->amg_chebyshev_degree > 8)
+			throw Error("amg_chebyshev_degree must be between 0 (default: 2) and 8");
+		if (options->amg_chebyshev_degree > 0) amg.chebyshev_degree = options->amg_chebyshev_degree;
 		f->solver = std::make_unique<LinearSolver>(mesh, options->preconditioner, options->precision,
 			options->restart, amg);
 		f->solver->set_verbose(options->verbose);

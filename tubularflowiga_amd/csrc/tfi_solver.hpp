@@ -11,6 +11,7 @@ struct AmgOptions {
 	int coarse_nodes = 256;                  // block rows on the coarsest (dense direct) level
 	int gs_min_rows = 8192;                  // smaller levels use damped block Jacobi
 	int gs_sweeps = 2;                       // 2: symmetric (forward+backward) pre and post; 1: forward pre, backward post
+	int chebyshev_degree = 2;                // Chebyshev steps pre and post (the first pre step from zero reads no matrix)
 };
 
 class Amg {
