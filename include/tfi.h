@@ -179,6 +179,9 @@ typedef struct {
 	                                     predicted within that window starts this way, with the tangent
 	                                     of an earlier step. */
 	int amg_chebyshev_degree;         /* Chebyshev steps before and after the coarse correction (0: 2) */
+	int amg_half_smoother;            /* TFI_MIXED: 1 (default) = Chebyshev-smoothed levels read D^-1 A
+	                                     (block-Jacobi scaled) in FP16, half the bytes of FP32 blocks; a level
+	                                     whose D^-1 A has an entry above 60000 keeps FP32; 0 = FP32 A */
 } tfi_flow_options;
 
 void tfi_flow_default_options(tfi_flow_options* options);

@@ -40,7 +40,7 @@ class FlowOptions(ctypes.Structure):
                 ("restart", c_int), ("preconditioner", c_int), ("precision", c_int), ("warm_start", c_int),
                 ("verbose", c_int), ("amg_smoother", c_int), ("tangent", c_int), ("amg_coarse_nodes", c_int),
                 ("amg_gs_min_rows", c_int), ("amg_gs_sweeps", c_int), ("chord_polish", c_int),
-                ("amg_chebyshev_degree", c_int)]
+                ("amg_chebyshev_degree", c_int), ("amg_half_smoother", c_int)]
 
 
 class NewtonRecord(ctypes.Structure):

@@ -165,6 +165,122 @@ __global__ void __launch_bounds__(256) ResidualKernel(int rows, const int* __res
 	if (sub == 0) r[4*(size_t)row + rr] = b[4*(size_t)row + rr] - acc;
 }
 
+// Mixed precision, Chebyshev levels: the smoother reads S = D^-1 A, the
+// block-Jacobi-scaled matrix, in FP16.  Its diagonal blocks are the identity
+// and its other entries of order one, so FP16's 11-bit significand and range
+// hold it (a level whose S has an entry above 60000 keeps the FP32 matrix).
+// Half the bytes of FP32 blocks per smoothing pass.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline float RowPartialHalf(const int* __restrict__ ro, const int* __restrict__ cols,
+	const _Float16* __restrict__ s, const float* __restrict__ x, int row, int sub, int r)
+{
+	typedef float F4 __attribute__((ext_vector_type(4)));
+	const int end = ro[row + 1];
+	float acc = 0.0f;
+	constexpr int kD = 4;  // blocks in flight per lane
+	for (int k0 = ro[row] + sub; k0 < end; k0 += 16*kD) {
+		int c[kD];
+#pragma unroll
+		for (int u = 0; u < kD; ++u) c[u] = k0 + 16*u < end ? cols[k0 + 16*u] : -1;
+		half4_t a[kD];
+		F4 xv[kD];
+#pragma unroll
+		for (int u = 0; u < kD; ++u)
+			if (c[u] >= 0) {
+				a[u] = *reinterpret_cast<const half4_t*>(s + (size_t)(k0 + 16*u)*16 + 4*r);
+				xv[u] = *reinterpret_cast<const F4*>(x + 4*(size_t)c[u]);
+			}
+#pragma unroll
+		for (int u = 0; u < kD; ++u)
+			if (c[u] >= 0)
+				acc += static_cast<float>(a[u].x)*xv[u].x + static_cast<float>(a[u].y)*xv[u].y
+					+ static_cast<float>(a[u].z)*xv[u].z + static_cast<float>(a[u].w)*xv[u].w;
+	}
+	return acc;
+}
+
+// S = D^-1 A in FP16 (D^-1 from the FP64 block inverses) and D in FP32; flag
+// set when an entry of S is not finite or above 60000 in magnitude.
+__global__ void __launch_bounds__(256) ScaleBlocksKernel(int rows, const int* __restrict__ ro,
+	const int* __restrict__ diag, const double* __restrict__ vals, const double* __restrict__ inv,
+	_Float16* __restrict__ s16, float* __restrict__ dblock, int* __restrict__ flag)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 2, r = lane & 3;
+	const double* iv = inv + (size_t)row*16 + 4*r;
+	bool bad = false;
+	for (int k = ro[row] + sub; k < ro[row + 1]; k += 16) {
+		const double* a = vals + (size_t)k*16;
+#pragma unroll
+		for (int c = 0; c < 4; ++c) {
+			const double v = iv[0]*a[c] + iv[1]*a[4 + c] + iv[2]*a[8 + c] + iv[3]*a[12 + c];
+			if (!(fabs(v) <= 60000.0)) bad = true;
+			s16[(size_t)k*16 + 4*r + c] = static_cast<_Float16>(v);
+		}
+	}
+	if (sub == 0)
+		for (int c = 0; c < 4; ++c)
+			dblock[(size_t)row*16 + 4*r + c] = static_cast<float>(vals[(size_t)diag[row]*16 + 4*r + c]);
+	if (bad) atomicOr(flag, 1);
+}
+
+// The Chebyshev step of ChebyshevKernel on S: z = D^-1 b - S x_in.
+__global__ void __launch_bounds__(256) ChebyshevScaledKernel(int rows, const int* __restrict__ ro,
+	const int* __restrict__ cols, const _Float16* __restrict__ s, const float* __restrict__ inv,
+	const float* __restrict__ b, const float* __restrict__ xin, float* __restrict__ xout, float* __restrict__ d,
+	float cd, float cz)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 2, r = lane & 3;
+	float acc = 0.0f;
+	if (xin) {
+		acc = RowPartialHalf(ro, cols, s, xin, row, sub, r);
+		acc += __shfl_xor(acc, 4, 64);
+		acc += __shfl_xor(acc, 8, 64);
+		acc += __shfl_xor(acc, 16, 64);
+		acc += __shfl_xor(acc, 32, 64);
+	}
+	if (sub == 0) {
+		const size_t i = 4*(size_t)row + r;
+		const float* iv = inv + (size_t)row*16 + 4*r;
+		const float* bv = b + 4*(size_t)row;
+		const float z = iv[0]*bv[0] + iv[1]*bv[1] + iv[2]*bv[2] + iv[3]*bv[3] - acc;
+		const float dn = (cd != 0.0f ? cd*d[i] : 0.0f) + cz*z;
+		d[i] = dn;
+		xout[i] = (xin ? xin[i] : 0.0f) + dn;
+	}
+}
+
+// r = b - A x = D (D^-1 b - S x).
+__global__ void __launch_bounds__(256) ResidualScaledKernel(int rows, const int* __restrict__ ro,
+	const int* __restrict__ cols, const _Float16* __restrict__ s, const float* __restrict__ inv,
+	const float* __restrict__ dblock, const float* __restrict__ b, const float* __restrict__ x,
+	float* __restrict__ rout)
+{
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 2, r = lane & 3;
+	float acc = RowPartialHalf(ro, cols, s, x, row, sub, r);
+	acc += __shfl_xor(acc, 4, 64);
+	acc += __shfl_xor(acc, 8, 64);
+	acc += __shfl_xor(acc, 16, 64);
+	acc += __shfl_xor(acc, 32, 64);
+	const float* iv = inv + (size_t)row*16 + 4*r;
+	const float* bv = b + 4*(size_t)row;
+	const float t = iv[0]*bv[0] + iv[1]*bv[1] + iv[2]*bv[2] + iv[3]*bv[3] - acc;
+	const float t0 = __shfl(t, 0, 64), t1 = __shfl(t, 1, 64), t2 = __shfl(t, 2, 64), t3 = __shfl(t, 3, 64);
+	if (sub == 0) {
+		const float* dv = dblock + (size_t)row*16 + 4*r;
+		rout[4*(size_t)row + r] = dv[0]*t0 + dv[1]*t1 + dv[2]*t2 + dv[3]*t3;
+	}
+}
+
 // y = D^-1 (A x)  (power iteration for lambda_max of the Jacobi-preconditioned operator)
 __global__ void __launch_bounds__(256) DinvAKernel(int rows, const int* __restrict__ ro, const int* __restrict__ cols,
 	const double* __restrict__ vals, const double* __restrict__ inv, const double* __restrict__ x,
@@ -400,6 +516,10 @@ struct Level {
 	DevBuf<double> inv64;
 	DevBuf<float> vals32_own, inv32;
 	const float* vals32 = nullptr;
+	// mixed precision, Chebyshev levels: S = D^-1 A in FP16 and D in FP32 (scaled: in use)
+	DevBuf<_Float16> s16;
+	DevBuf<float> dblock32;
+	bool scaled = false;
 	// transfer to the next level
 	DevBuf<int> agg, aoff, amem;
 	DevBuf<long long> goff;       // Galerkin lists for the NEXT level's blocks
@@ -426,6 +546,7 @@ struct Amg::Impl {
 	DevBuf<double> aug, mult, dinv64, v64tmp;
 	DevBuf<float> dinv32;
 	DevBuf<int> status;
+	DevBuf<int> scale_flags;  // per level: S did not fit FP16
 
 	void Symbolic(const double* fine_values);
 	void Numeric(const double* fine_values64, const float* fine_values32);
@@ -672,6 +793,10 @@ void Amg::Impl::Symbolic(const double* fine_values)
 			v.b32.Allocate(ctx, n);
 			v.r32.Allocate(ctx, n);
 			if (l > 0) v.vals32_own.Allocate(ctx, v.nnzb*16);
+			if (opt.half_smoother && !v.gs && l + 1 < L.size() && opt.smoother == TFI_SMOOTH_CHEBYSHEV) {
+				v.s16.Allocate(ctx, v.nnzb*16);
+				v.dblock32.Allocate(ctx, 16LL*v.rows);
+			}
 		} else {
 			v.x64.Allocate(ctx, n);
 			v.y64.Allocate(ctx, n);
@@ -686,6 +811,7 @@ void Amg::Impl::Symbolic(const double* fine_values)
 	if (precision == TFI_MIXED) dinv32.Allocate(ctx, (size_t)nc*nc);
 	else dinv64.Allocate(ctx, (size_t)nc*nc);
 	status.Allocate(ctx, 1);
+	scale_flags.Allocate(ctx, L.size());
 	built = true;
 }
 
@@ -715,6 +841,22 @@ void Amg::Impl::Numeric(const double* fine64, const float* fine32)
 				v.vals32 = v.vals32_own.data();
 			}
 		}
+	}
+	// FP16 copies of D^-1 A for the Chebyshev levels of the mixed path
+	bool any_scaled = false;
+	for (size_t l = 0; l < L.size(); ++l) any_scaled |= L[l]->s16.size() > 0 && !L[l]->gs;
+	if (any_scaled) {
+		FillInt(ctx, scale_flags.data(), 0, static_cast<long long>(L.size()));
+		for (size_t l = 0; l < L.size(); ++l) {
+			Level& v = *L[l];
+			if (!(v.s16.size() > 0 && !v.gs)) continue;
+			hipLaunchKernelGGL(ScaleBlocksKernel, dim3((v.rows + 3)/4), dim3(256), 0, ctx->stream, v.rows, v.ro,
+				v.diag, v.vals64, v.inv64.data(), v.s16.data(), v.dblock32.data(), scale_flags.data() + l);
+			TFI_LAUNCH_CHECK("ScaleBlocksKernel");
+		}
+		std::vector<int> flags(L.size(), 0);
+		scale_flags.Download(flags.data(), flags.size());
+		for (size_t l = 0; l < L.size(); ++l) L[l]->scaled = L[l]->s16.size() > 0 && !L[l]->gs && flags[l] == 0;
 	}
 	// lambda_max(D^-1 A) of every Jacobi-smoothed level by power iteration (FP64)
 	for (size_t l = 0; l + 1 < L.size(); ++l) {
@@ -800,9 +942,19 @@ void Amg::Impl::Cycle(int l)
 				cz = 2.0*rn/delta;
 				rho = rn;
 			}
-			hipLaunchKernelGGL(ChebyshevKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals,
-				inv, b, (k == 0 && from_zero) ? static_cast<const T*>(nullptr) : static_cast<const T*>(x), y, r,
-				static_cast<T>(cd), static_cast<T>(cz));
+			bool done = false;
+			if constexpr (std::is_same<T, float>::value) {
+				if (v.scaled) {
+					hipLaunchKernelGGL(ChebyshevScaledKernel, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols,
+						v.s16.data(), inv, b, (k == 0 && from_zero) ? static_cast<const float*>(nullptr) : x, y, r,
+						static_cast<float>(cd), static_cast<float>(cz));
+					done = true;
+				}
+			}
+			if (!done)
+				hipLaunchKernelGGL(ChebyshevKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals,
+					inv, b, (k == 0 && from_zero) ? static_cast<const T*>(nullptr) : static_cast<const T*>(x), y, r,
+					static_cast<T>(cd), static_cast<T>(cz));
 			std::swap(x, y);
 		}
 		TFI_LAUNCH_CHECK("ChebyshevKernel");
@@ -845,7 +997,16 @@ void Amg::Impl::Cycle(int l)
 			std::swap(x, y);
 		}
 	}
-	hipLaunchKernelGGL(ResidualKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, b, x, r);
+	bool residual_done = false;
+	if constexpr (std::is_same<T, float>::value) {
+		if (cheb && v.scaled) {
+			hipLaunchKernelGGL(ResidualScaledKernel, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols,
+				v.s16.data(), inv, v.dblock32.data(), b, x, r);
+			residual_done = true;
+		}
+	}
+	if (!residual_done)
+		hipLaunchKernelGGL(ResidualKernel<T>, dim3(grid), dim3(256), 0, ctx->stream, v.rows, v.ro, v.cols, vals, b, x, r);
 	TFI_LAUNCH_CHECK("ResidualKernel");
 	Level& c = *L[l + 1];
 	T* bc = f32 ? reinterpret_cast<T*>(c.b32.data()) : reinterpret_cast<T*>(c.b64.data());

@@ -296,6 +296,7 @@ void tfi_flow_default_options(tfi_flow_options* o)
 	o->amg_gs_min_rows = 0;
 	o->amg_gs_sweeps = 0;
 	o->chord_polish = 1;
+	o->amg_half_smoother = 1;
 }
 
 int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** out)
@@ -347,6 +348,7 @@ int tfi_flow_create(tfi_mesh* mesh, const tfi_flow_options* options, tfi_flow** 
 		if (options->amg_chebyshev_degree < 0 || options->amg_chebyshev_degree > 8)
 			throw Error("amg_chebyshev_degree must be between 0 (default: 2) and 8");
 		if (options->amg_chebyshev_degree > 0) amg.chebyshev_degree = options->amg_chebyshev_degree;
+		amg.half_smoother = options->amg_half_smoother != 0 && options->precision == TFI_MIXED;
 		f->solver = std::make_unique<LinearSolver>(mesh, options->preconditioner, options->precision,
 			options->restart, amg);
 		f->solver->set_verbose(options->verbose);

@@ -12,6 +12,7 @@ struct AmgOptions {
 	int gs_min_rows = 8192;                  // smaller levels use damped block Jacobi
 	int gs_sweeps = 2;                       // 2: symmetric (forward+backward) pre and post; 1: forward pre, backward post
 	int chebyshev_degree = 2;                // Chebyshev steps pre and post (the first pre step from zero reads no matrix)
+	int half_smoother = 0;                   // mixed precision: Chebyshev levels read D^-1 A in FP16
 };
 
 class Amg {

@@ -101,7 +101,7 @@ struct Options {
 	int tangent = TFI_TANGENT_EXACT, smoother = TFI_SMOOTH_AUTO, restart_length = 200;
 	int linear_maximum_iterations = 5000;
 	double linear_rtol = 1e-8;
-	int chord_polish = 1, train_steps = 0, progress = 0, chebyshev_degree = 0;
+	int chord_polish = 1, train_steps = 0, progress = 0, chebyshev_degree = 0, half_smoother = 1;
 };
 
 int PositiveInt(const std::string& t, const std::string& opt)
@@ -149,7 +149,7 @@ const char* kUsage =
 	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|mpe|learned] "
 	"[--tangent exact|legacy] [--amg-smoother auto|chebyshev|gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
 	"[--linear-max-iterations N] [--chord-polish on|off] [--train-steps N] [--progress on|off] "
-	"[--amg-chebyshev-degree N]";
+	"[--amg-chebyshev-degree N] [--amg-half-smoother on|off]";
 
 // ParseCudaFlowOptions (iga_cuda.cu:447-498) plus the solver-path options
 Options ParseOptions(int argc, char** argv)
@@ -200,6 +200,7 @@ Options ParseOptions(int argc, char** argv)
 		else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
 		else if (arg == "--progress") o.progress = Choice(v, arg, {{"on", 1}, {"off", 0}});
 		else if (arg == "--amg-chebyshev-degree") o.chebyshev_degree = PositiveInt(v, arg);
+		else if (arg == "--amg-half-smoother") o.half_smoother = Choice(v, arg, {{"on", 1}, {"off", 0}});
 		else if (arg == "--restart-length") o.restart_length = PositiveInt(v, arg);
 		else if (arg == "--linear-max-iterations") o.linear_maximum_iterations = PositiveInt(v, arg);
 		else if (arg == "--chord-polish") o.chord_polish = Choice(v, arg, {{"on", 1}, {"off", 0}});
@@ -329,6 +330,7 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	opt.chord_polish = o.chord_polish;
 	opt.verbose = o.progress;
 	opt.amg_chebyshev_degree = o.chebyshev_degree;
+	opt.amg_half_smoother = o.half_smoother;
 	Flow flow;
 	c.Check(tfi_flow_create(m.mesh, &opt, &flow.flow), "flow solver");
 	if (opt.warm_start == TFI_WARM_LEARNED && o.train_steps > 0) {
