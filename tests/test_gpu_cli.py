@@ -82,11 +82,55 @@ def test_cli_matches_oracle_and_writes_reference_outputs(tube, tmp_path):
     assert re.search(r"^cuda_allocations scope=project_device_buffers requested_peak_bytes=(\d+) "
                      r"requested_live_bytes=(\d+)$", r.stdout, re.M)
     assert len(re.findall(r"^step=\d+ time=\S+ converged newton=", r.stdout, re.M)) == 2
-    for suffix in (".vtk", ".pressure"):
-        assert Path(str(out) + suffix).exists()
-    assert (tmp_path / "flow.vtu").exists() and (tmp_path / "flow.pvd").exists()
+    # transient runs default to the Bezier VTKHDF visualisation (VtkOutput.hpp:27-32):
+    # no legacy VTK, VTU or PVD; the geometry report and one step at the final time
+    assert Path(str(out) + ".pressure").exists()
+    assert not Path(str(out) + ".vtk").exists() and not (tmp_path / "flow.vtu").exists()
+    assert re.search(r"^bezier_geometry_points=\d+ local_point_references=\d+ geometry_report=\S+ vtkhdf=\S+$",
+                     r.stdout, re.M)
+    report = json.loads((tmp_path / "flow.bezier_geometry.json").read_text())
+    assert report["valid"] and report["elements"] == case.mesh.elements
+    steps = h5_values(tmp_path / "flow.vtkhdf", "/VTKHDF/Steps/Values")
+    assert np.allclose(steps, [2e-3], rtol=1e-12)
+    assert int(h5_values(tmp_path / "flow.vtkhdf", "/VTKHDF/NumberOfPoints")[0]) == report["unique_points"]
+
+
+def h5_values(path, dataset):
+    """Numbers of one dataset of an HDF5 file (h5dump, 17 digits)."""
+    text = subprocess.run(["/opt/conda/bin/h5dump", "-m", "%.17g", "-d", dataset, str(path)], capture_output=True,
+                          text=True, check=True).stdout
+    body = text[text.index("DATA {") + 6:]
+    body = re.sub(r"\(\d+(,\d+)*\):", " ", body)
+    return np.array([float(x) for x in re.findall(r"-?\d+(?:\.\d+)?(?:e[-+]?\d+)?", body)])
+
+
+def test_cli_vtu_visualisation_and_time_series(tube, tmp_path):
+    """--visualization-format vtu writes the legacy VTK, the VTU and the PVD;
+    with vtkhdf and --output-every 1 the file holds the initial state and every
+    step (the final write at the last step's time replaces that step)."""
+    case, d = tube
+    out = tmp_path / "vtu" / "flow.txt"
+    out.parent.mkdir()
+    r = run("navier-stokes", d / "case.ntiga", d, "--output", out, "--visualization-format", "vtu", *TIGHT)
+    assert r.returncode == 0, r.stderr
+    assert Path(str(out) + ".vtk").exists()
+    assert (out.parent / "flow.vtu").exists() and (out.parent / "flow.pvd").exists()
     vtk = Path(str(out) + ".vtk").read_text()
     assert "VECTORS Velocity double" in vtk and "SCALARS Pressure double 1" in vtk
+    out = tmp_path / "hdf" / "flow.txt"
+    out.parent.mkdir()
+    r = run("navier-stokes", d / "case.ntiga", d, "--output", out, "--output-every", "1", "--visualization-format",
+            "vtkhdf", *TIGHT)
+    assert r.returncode == 0, r.stderr
+    h5 = out.parent / "flow.vtkhdf"
+    assert np.allclose(h5_values(h5, "/VTKHDF/Steps/Values"), [0.0, 1e-3, 2e-3], rtol=1e-12, atol=1e-15)
+    n = int(h5_values(h5, "/VTKHDF/NumberOfPoints")[0])
+    pressure = h5_values(h5, "/VTKHDF/PointData/pressure")
+    assert pressure.shape == (3 * n,)
+    assert np.array_equal(h5_values(h5, "/VTKHDF/Steps/PointDataOffsets/pressure"), [0, n, 2 * n])
+    # the last step's pressure at the Bezier points spans the final control-point pressure's range
+    final = read_state(out)[:, 3]
+    assert final.min() - 1e-9 <= pressure[2 * n:].min() and pressure[2 * n:].max() <= final.max() + 1e-9
 
 
 def test_cli_restart_reproduces_the_uninterrupted_run(tube, tmp_path):

@@ -190,13 +190,29 @@ def test_cli_solve_matches_the_oracle_and_restarts(tmp_path):
     out = tmp_path / "run" / "species.txt"
     print("case: iga_hip solve, tube_c0, 3 fields, 6 steps", flush=True)
     r = subprocess.run([str(cli), "solve", str(d / "case.ntiga"), str(d), "--output", str(out), "--linear-rtol", "1e-11",
-                        "--output-every", "3"], capture_output=True, text=True, timeout=300)
+                        "--output-every", "3", "--visualization-format", "vtu"], capture_output=True, text=True,
+                       timeout=300)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
     assert "iga_cuda_solve system=species fields=3 steps=6 run_end_step=6" in r.stdout
     assert (out.parent / "species.txt.fields").read_text().split() == ["oxygen", "carrier", "tracer"]
     assert (out.parent / "species.pvd").exists() and (out.parent / "species.step000003.vtu").exists()
     assert (out.parent / "species.step000003.txt").exists() and (out.parent / "species.vtu").exists()
+    # by default the run writes the Bezier VTKHDF series instead of the VTU files:
+    # the initial state and steps 3 and 6, one point array per field
+    hout = tmp_path / "hdf" / "species.txt"
+    hout.parent.mkdir()
+    r = subprocess.run([str(cli), "solve", str(d / "case.ntiga"), str(d), "--output", str(hout), "--linear-rtol",
+                        "1e-11", "--output-every", "3"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert not (hout.parent / "species.vtu").exists() and not (hout.parent / "species.pvd").exists()
+    assert "bezier_geometry_points=" in r.stdout
+    from test_gpu_cli import h5_values
+    h5 = hout.parent / "species.vtkhdf"
+    assert np.allclose(h5_values(h5, "/VTKHDF/Steps/Values"), [0.0, 6e-3, 1.2e-2], rtol=1e-12, atol=1e-15)
+    npts = int(h5_values(h5, "/VTKHDF/NumberOfPoints")[0])
+    for name in ("oxygen", "carrier", "tracer"):
+        assert h5_values(h5, f"/VTKHDF/PointData/{name}").shape == (3 * npts,)
     manifest = (out.parent / "physiology_fields.json").read_text()
     assert '"enabled": false' in manifest and '"oxygen": {"status": "solved"}' in manifest
     got = _read_output(out, 3)

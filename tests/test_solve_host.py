@@ -212,7 +212,7 @@ def test_configuration_errors(tmp_path, name):
     (["--output-every", "2"], "--output-every requires --output"),
     (["--checkpoint-every", "2"], "--checkpoint-every requires --checkpoint"),
     (["--stop-after-step", "9"], "--stop-after-step exceeds configured transport steps"),
-    (["--visualization-format", "vtkhdf"], "vtkhdf is not available"),
+    (["--visualization-format", "xdmf"], "--visualization-format must be auto, vtu, or vtkhdf"),
     (["--linear-rtol", "-1"], "--linear-rtol requires a finite positive value"),
     (["--bogus", "1"], "unknown option: --bogus"),
     (["species", "out.txt", "vel.txt", "extra"], "too many positional arguments"),

@@ -73,6 +73,32 @@ def build_library(jobs: int | None = None, verbose: bool = False, force: bool = 
     return LIB
 
 
+HDF5_DIR = Path(os.environ.get("HDF5_DIR", "/opt/conda"))
+HDF5_LINKS = PKG / "_hdf5"
+
+
+def hdf5_flags() -> list[str]:
+    """Compile and link flags for the HDF5 C library of the image (the Bezier
+    VTKHDF output), or [] when it is absent.  The library is reached through a
+    private directory holding only a link to it (tubularflowiga_amd/_hdf5), so
+    that neither the linker nor the loader picks up the other libraries of its
+    installation (an older libstdc++ among them)."""
+    header = HDF5_DIR / "include" / "hdf5.h"
+    libs = sorted((HDF5_DIR / "lib").glob("libhdf5.so.*[0-9]"))
+    libs = [l for l in libs if l.name.count(".") == 2]   # libhdf5.so.<soversion>
+    if not header.exists() or not libs:
+        return []
+    HDF5_LINKS.mkdir(exist_ok=True)
+    target = libs[0]
+    for name in (target.name, "libhdf5.so"):
+        link = HDF5_LINKS / name
+        if link.is_symlink() or link.exists():
+            link.unlink()
+        link.symlink_to(target)
+    return ["-DTFI_HAVE_HDF5", "-I", str(HDF5_DIR / "include"), "-L", str(HDF5_LINKS), "-lhdf5",
+            "-Wl,-rpath,$ORIGIN/_hdf5", "-Wl,-rpath," + str(HDF5_LINKS)]
+
+
 def build_cli() -> Path | None:
     srcs = sorted(HOST.glob("*.cpp")) if HOST.exists() else []
     if not srcs:
@@ -80,7 +106,7 @@ def build_cli() -> Path | None:
     cxx = shutil.which("g++") or "g++"
     tmp = CLI.with_name(CLI.name + ".tmp")
     _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"), *map(str, srcs), "-o", str(tmp),
-          "-L", str(PKG), "-ltfi_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + str(PKG)])
+          "-L", str(PKG), "-ltfi_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + str(PKG), *hdf5_flags()])
     os.replace(tmp, CLI)
     return CLI
 

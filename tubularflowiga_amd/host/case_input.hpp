@@ -44,6 +44,7 @@ struct FlatDatabase {
 	std::vector<double> extraction_values, bezier;
 	std::vector<std::int32_t> labels;       // 6 per element (-1 for v3)
 	std::vector<std::uint64_t> ids;
+	std::vector<std::int32_t> owners;       // METIS owner rank per element
 	std::array<double, 5> transform{{0.0, 0.0, 0.0, 1.0, 1.0}};
 	int maximum_basis = 0;
 
@@ -140,6 +141,7 @@ inline FlatDatabase ReadDatabase(const std::string& path)
 		std::memcpy(db.bezier.data() + b0, data.data() + p, 1536);
 		db.labels.insert(db.labels.end(), lab.begin(), lab.end());
 		db.ids.push_back(id);
+		db.owners.push_back(owner);
 		db.maximum_basis = std::max(db.maximum_basis, static_cast<int>(nen));
 		db.element_offsets.push_back(static_cast<std::int32_t>(db.connectivity.size()));
 	}

@@ -9,8 +9,12 @@
 #include "solve_input.hpp"
 #include "transport_input.hpp"
 #include "tfi.h"
+#ifdef TFI_HAVE_HDF5
+#include "bezier_output.hpp"
+#endif
 
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <iostream>
@@ -145,7 +149,7 @@ const char* kUsage =
 	"[--system NAME] [--max-newton N] [--output PATH] [--output-every N] "
 	"[--checkpoint PREFIX] [--checkpoint-every N] [--restart PREFIX] "
 	"[--stop-after-step N] [--nonlinear-rtol R] [--nonlinear-atol A] "
-	"[--mass-rtol R] [--visualization-format auto|vtu] "
+	"[--mass-rtol R] [--visualization-format auto|vtu|vtkhdf] "
 	"[--preconditioner amg|bjacobi] [--precision fp64|mixed] [--warm-start previous|extrapolate|extrapolate2|extrapolate3|extrapolate4|mpe|learned] "
 	"[--tangent exact|legacy] [--amg-smoother auto|chebyshev|gauss-seidel|jacobi] [--linear-rtol R] [--restart-length N] "
 	"[--linear-max-iterations N] [--chord-polish on|off] [--train-steps N] [--progress on|off] "
@@ -182,10 +186,9 @@ Options ParseOptions(int argc, char** argv)
 		else if (arg == "--nonlinear-atol") o.nonlinear_atol = PositiveDouble(v, arg);
 		else if (arg == "--mass-rtol") o.mass_rtol = PositiveDouble(v, arg);
 		else if (arg == "--visualization-format") {
-			if (v == "vtkhdf" || v == "bezier-vtkhdf")
-				throw std::runtime_error("--visualization-format vtkhdf is not available in iga_hip; use vtu");
-			if (v != "auto" && v != "vtu") throw std::runtime_error("--visualization-format must be auto, vtu, or vtkhdf");
-			o.visualization = v;
+			if (v != "auto" && v != "vtu" && v != "vtkhdf" && v != "bezier-vtkhdf")
+				throw std::runtime_error("--visualization-format must be auto, vtu, or vtkhdf");
+			o.visualization = v == "bezier-vtkhdf" ? "vtkhdf" : v;
 		} else if (arg == "--preconditioner") o.preconditioner = Choice(v, arg, {{"amg", TFI_PC_AMG}, {"bjacobi", TFI_PC_BJACOBI}});
 		else if (arg == "--precision") o.precision = Choice(v, arg, {{"fp64", TFI_FP64}, {"mixed", TFI_MIXED}});
 		else if (arg == "--warm-start")
@@ -227,6 +230,63 @@ double Norm(const std::vector<double>& v, int first, int stride)
 	for (std::size_t i = first; i < v.size(); i += stride) s += v[i]*v[i];
 	return std::sqrt(s);
 }
+
+// The Bezier VTKHDF visualisation (iga_cuda.cu:52-70, 366-386;
+// ConfiguredTransportHost.inc:98-140): the merged Bezier mesh and its report
+// are built once, then every visualisation time appends the point arrays.
+class BezierOutput {
+public:
+	void Initialize(const FlatDatabase& db, const fs::path& output, bool resume)
+	{
+#ifdef TFI_HAVE_HDF5
+		mesh_ = std::make_unique<bezier::VisualMesh>(BuildBezierVisualMesh(db));
+		const auto report = BezierReportPath(output);
+		WriteBezierReport(report, mesh_->checks);
+		RequireValidBezierMesh(mesh_->checks);
+		series_ = std::make_unique<BezierVtkHdfSeries>(VtkHdfOutputPath(output), *mesh_, resume);
+		std::cout << "bezier_geometry_points=" << mesh_->points.size() << " local_point_references="
+			<< mesh_->checks.local_points << " geometry_report=" << report.string() << " vtkhdf="
+			<< series_->path().string() << '\n';
+#else
+		(void)db; (void)output; (void)resume;
+		throw std::runtime_error("--visualization-format vtkhdf needs the HDF5 library, which this build of iga_hip "
+			"was made without; use --visualization-format vtu");
+#endif
+	}
+	// the flow state [4N] as "velocity" and "pressure"
+	void Append(double time, const std::vector<double>& state)
+	{
+		const std::size_t n = state.size()/4;
+		std::vector<double> v(3*n), p(n);
+		for (std::size_t i = 0; i < n; ++i) {
+			for (int k = 0; k < 3; ++k) v[3*i + k] = state[4*i + k];
+			p[i] = state[4*i + 3];
+		}
+		Append(time, std::vector<PointArray>{{"velocity", 3, std::move(v)}, {"pressure", 1, std::move(p)}});
+	}
+	void Append(double time, const std::vector<PointArray>& arrays)
+	{
+#ifdef TFI_HAVE_HDF5
+		std::vector<PointField> fields;
+		for (const auto& a : arrays) fields.push_back({a.name, a.components, a.values});
+		series_->Append(time, fields);
+#else
+		(void)time; (void)arrays;
+#endif
+	}
+	void Close()
+	{
+#ifdef TFI_HAVE_HDF5
+		if (series_) series_->Close();
+#endif
+	}
+
+private:
+#ifdef TFI_HAVE_HDF5
+	std::unique_ptr<bezier::VisualMesh> mesh_;
+	std::unique_ptr<BezierVtkHdfSeries> series_;
+#endif
+};
 
 int NavierStokes(Ctx& c, int argc, char** argv)
 {
@@ -370,12 +430,23 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 
 	const fs::path mesh_vtk = o.case_dir/"controlmesh.vtk";
 	std::vector<std::pair<double, fs::path>> vtu_snaps, text_snaps;
+	// visualisation: VTU, or by default for transient runs the Bezier VTKHDF
+	// file (ResolveVisualizationFormat, include/VtkOutput.hpp:27-32)
+	const bool vtkhdf = o.visualization == "vtkhdf" || (o.visualization == "auto" && transient);
+	BezierOutput bezier_out;
+	if (!o.output.empty() && vtkhdf) bezier_out.Initialize(db, o.output, !o.restart.empty());
+	auto visualise = [&](const fs::path& vtu, double time) {
+		if (vtkhdf) {
+			bezier_out.Append(time, state);
+		} else {
+			WriteStateVtu(mesh_vtk, vtu, state, time);
+			vtu_snaps.emplace_back(time, vtu);
+		}
+	};
 	if (transient && o.output_every > 0) {
 		const auto text = TimeIndexedPath(o.output, start_step);
-		const auto vtu = VtuStepPath(o.output, start_step);
 		WriteStateText(text, state);
-		WriteStateVtu(mesh_vtk, vtu, state, start_step*dt);
-		vtu_snaps.emplace_back(start_step*dt, vtu);
+		visualise(VtuStepPath(o.output, start_step), start_step*dt);
 		text_snaps.emplace_back(start_step*dt, text);
 	}
 	long long total_linear_iterations = 0;
@@ -479,10 +550,8 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 		if (o.output_every > 0 && completed % o.output_every == 0) {
 			c.Check(tfi_flow_get_state(flow.flow, state.data()), "state download");
 			const auto text = TimeIndexedPath(o.output, completed);
-			const auto vtu = VtuStepPath(o.output, completed);
 			WriteStateText(text, state);
-			WriteStateVtu(mesh_vtk, vtu, state, completed*dt);
-			vtu_snaps.emplace_back(completed*dt, vtu);
+			visualise(VtuStepPath(o.output, completed), completed*dt);
 			text_snaps.emplace_back(completed*dt, text);
 		}
 		if (!o.checkpoint.empty() && (completed == run_end || (o.checkpoint_every > 0 && completed % o.checkpoint_every == 0))) {
@@ -502,12 +571,17 @@ int NavierStokes(Ctx& c, int argc, char** argv)
 	c.Check(tfi_flow_get_state(flow.flow, state.data()), "state download");
 	if (!o.output.empty()) {
 		WriteStateText(o.output, state);
-		WriteStateVtk(mesh_vtk, o.output.string() + ".vtk", state);
+		if (!vtkhdf) WriteStateVtk(mesh_vtk, o.output.string() + ".vtk", state);
 		const double final_time = transient ? run_end*dt : 0.0;
-		WriteStateVtu(mesh_vtk, VtuFinalPath(o.output), state, final_time);
-		if (vtu_snaps.empty()) vtu_snaps.emplace_back(final_time, VtuFinalPath(o.output));
-		WritePvd(PvdPath(o.output), vtu_snaps);
+		if (vtkhdf) {
+			bezier_out.Append(final_time, state);
+		} else {
+			WriteStateVtu(mesh_vtk, VtuFinalPath(o.output), state, final_time);
+			if (vtu_snaps.empty()) vtu_snaps.emplace_back(final_time, VtuFinalPath(o.output));
+			WritePvd(PvdPath(o.output), vtu_snaps);
+		}
 		if (!text_snaps.empty()) WriteSeries(SeriesPath(o.output), text_snaps);
+		bezier_out.Close();
 	}
 	long long peak = 0, live = 0;
 	tfi_allocation_stats(c.ctx, &peak, &live);
@@ -763,6 +837,7 @@ struct SolveOptions {
 	int output_every = 0, checkpoint_every = 0, stop_after_step = 0;
 	double linear_rtol = 1e-8;
 	int preconditioner = TFI_PC_BJACOBI;
+	std::string visualization = "vtkhdf";  // auto: the Bezier VTKHDF series (runs are transient)
 };
 
 // ParseCudaConfiguredTransportOptions (ConfiguredTransportHost.inc:13-48), plus
@@ -772,7 +847,7 @@ SolveOptions ParseSolveOptions(int argc, char** argv)
 	if (argc < 4)
 		throw std::runtime_error("usage: iga_hip solve DATABASE.ntiga CASE_DIR [SYSTEM] [OUTPUT] [VELOCITY] "
 			"[--system NAME] [--output PATH] [--velocity PATH] [--output-every N] [--checkpoint PREFIX] "
-			"[--checkpoint-every N] [--restart PREFIX] [--stop-after-step N] [--visualization-format auto|vtu] "
+			"[--checkpoint-every N] [--restart PREFIX] [--stop-after-step N] [--visualization-format auto|vtu|vtkhdf] "
 			"[--linear-rtol R] [--preconditioner bjacobi|amg]");
 	SolveOptions o;
 	o.database = argv[2];
@@ -799,8 +874,9 @@ SolveOptions ParseSolveOptions(int argc, char** argv)
 		else if (arg == "--restart") o.restart = v;
 		else if (arg == "--stop-after-step") o.stop_after_step = PositiveInt(v, arg);
 		else if (arg == "--visualization-format") {
-			if (v == "vtkhdf") throw std::runtime_error("--visualization-format vtkhdf is not available in iga_hip; use vtu");
-			if (v != "auto" && v != "vtu") throw std::runtime_error("--visualization-format must be auto, vtu, or vtkhdf");
+			if (v != "auto" && v != "vtu" && v != "vtkhdf" && v != "bezier-vtkhdf")
+				throw std::runtime_error("--visualization-format must be auto, vtu, or vtkhdf");
+			o.visualization = v == "vtu" ? "vtu" : "vtkhdf";  // auto: backward Euler runs are transient
 		} else if (arg == "--linear-rtol") o.linear_rtol = PositiveDouble(v, arg);
 		else if (arg == "--preconditioner") o.preconditioner = Choice(v, arg, {{"bjacobi", TFI_PC_BJACOBI}, {"amg", TFI_PC_AMG}});
 		else throw std::runtime_error("unknown option: " + arg);
@@ -975,12 +1051,17 @@ int Solve(Ctx& c, const SolveCase& t)
 	std::vector<double> state(t.state);
 	const fs::path mesh_vtk = o.case_dir/"controlmesh.vtk";
 	std::vector<std::pair<double, fs::path>> snapshots;
+	const bool vtkhdf = o.visualization == "vtkhdf";
+	BezierOutput bezier_out;
+	if (!o.output.empty() && vtkhdf) bezier_out.Initialize(db, o.output, !o.restart.empty());
+	std::function<void(double, const std::vector<PointArray>&)> visual;
+	if (vtkhdf) visual = [&](double time, const std::vector<PointArray>& arrays) { bezier_out.Append(time, arrays); };
 	auto write_output = [&](int step) {
 		c.Check(tfi_solve_get_state(h.s, state.data()), "configured transport state download");
 		const fs::path vtu = VtuStepPath(o.output, step);
 		WriteConfiguredOutput(TimeIndexedPath(o.output, step), s.fields, state, mesh_vtk, vtu, step*s.dt,
-			t.config.physiology);
-		snapshots.push_back({step*s.dt, vtu});
+			t.config.physiology, visual);
+		if (!vtkhdf) snapshots.push_back({step*s.dt, vtu});
 	};
 	if (o.output_every > 0) write_output(t.start_step);
 	long long total_iterations = 0;
@@ -1025,10 +1106,13 @@ int Solve(Ctx& c, const SolveCase& t)
 	norm = std::sqrt(norm);
 	if (!o.output.empty()) {
 		WriteConfiguredOutput(o.output, s.fields, state, mesh_vtk, VtuFinalPath(o.output), t.end_step*s.dt,
-			t.config.physiology);
-		if (snapshots.empty()) snapshots.push_back({t.end_step*s.dt, VtuFinalPath(o.output)});
-		WritePvd(PvdPath(o.output), snapshots);
+			t.config.physiology, visual);
+		if (!vtkhdf) {
+			if (snapshots.empty()) snapshots.push_back({t.end_step*s.dt, VtuFinalPath(o.output)});
+			WritePvd(PvdPath(o.output), snapshots);
+		}
 		WritePhysiologyManifest(o.output.parent_path()/"physiology_fields.json", t.config.physiology, s.fields);
+		bezier_out.Close();
 	}
 	long long peak = 0, live = 0;
 	c.Check(tfi_allocation_stats(c.ctx, &peak, &live), "allocation statistics");

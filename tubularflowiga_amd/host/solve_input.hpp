@@ -11,6 +11,7 @@
 #include "case_input.hpp"
 #include "tfi.h"
 
+#include <functional>
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -772,9 +773,13 @@ inline ScalarBoundaries ResolveScalarBoundaries(const LinearConfig& c, const Com
 // WriteCudaConfiguredOutput (ConfiguredTransportHost.inc:96-140): "node v0 v1 ..."
 // lines at 17 digits, the field names in PATH.fields, and a VTU with one
 // point array per field
+// The configured text output and the point arrays (fields, then the derived
+// physiology arrays) as a VTU, or handed to `visual` when it is given (the
+// Bezier VTKHDF series).
 inline void WriteConfiguredOutput(const fs::path& path, const std::vector<std::string>& fields,
 	const std::vector<double>& state, const fs::path& mesh_vtk, const fs::path& vtu, double time,
-	const PhysiologyDef& physiology)
+	const PhysiologyDef& physiology,
+	const std::function<void(double, const std::vector<PointArray>&)>& visual = nullptr)
 {
 	const std::size_t F = fields.size(), nodes = state.size()/F;
 	if (!path.parent_path().empty()) fs::create_directories(path.parent_path());
@@ -795,7 +800,8 @@ inline void WriteConfiguredOutput(const fs::path& path, const std::vector<std::s
 		arrays.push_back(std::move(a));
 	}
 	for (auto& a : PhysiologyArrays(physiology, fields, state)) arrays.push_back(std::move(a));
-	WritePointArraysVtu(mesh_vtk, vtu, arrays, time);
+	if (visual) visual(time, arrays);
+	else WritePointArraysVtu(mesh_vtk, vtu, arrays, time);
 }
 
 // raw_float64 transport checkpoint, schema 1 (include/TransportCheckpoint.hpp:18-219)
