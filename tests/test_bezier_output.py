@@ -36,7 +36,8 @@ pytestmark = pytest.mark.skipif(not build.hdf5_flags() or not Path(H5DUMP).exist
 def host_check(tmp_path_factory):
     exe = tmp_path_factory.mktemp("bezier") / "bezier_host_check"
     subprocess.run(["g++", "-std=c++17", "-O2", "-I", str(ROOT / "tubularflowiga_amd" / "host"),
-                    str(ROOT / "tests" / "bezier_host_check.cpp"), "-o", str(exe), *build.hdf5_flags()], check=True)
+                    str(ROOT / "tests" / "bezier_host_check.cpp"), "-o", str(exe), "-pthread", *build.hdf5_flags()],
+                   check=True)
     return exe
 
 

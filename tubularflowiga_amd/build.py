@@ -105,7 +105,7 @@ def build_cli() -> Path | None:
         return None
     cxx = shutil.which("g++") or "g++"
     tmp = CLI.with_name(CLI.name + ".tmp")
-    _run([cxx, "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"), *map(str, srcs), "-o", str(tmp),
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-I", str(ROOT / "include"), *map(str, srcs), "-o", str(tmp),
           "-L", str(PKG), "-ltfi_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + str(PKG), *hdf5_flags()])
     os.replace(tmp, CLI)
     return CLI

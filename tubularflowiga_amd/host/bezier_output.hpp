@@ -40,6 +40,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -316,12 +317,18 @@ inline bezier::VisualMesh BuildBezierVisualMesh(const FlatDatabase& db)
 		std::array<std::int64_t, 3> v;
 		bool operator==(const Key& o) const { return v == o.v; }
 	};
-	struct KeyHash {
+	struct KeyHash {  // splitmix64 finaliser over the three bucket indices
+		static std::uint64_t Mix(std::uint64_t z)
+		{
+			z += 0x9e3779b97f4a7c15ULL;
+			z = (z ^ (z >> 30))*0xbf58476d1ce4e5b9ULL;
+			z = (z ^ (z >> 27))*0x94d049bb133111ebULL;
+			return z ^ (z >> 31);
+		}
 		std::size_t operator()(const Key& k) const
 		{
-			std::uint64_t h = 1469598103934665603ULL;
-			for (const auto x : k.v) h = Fnv(h, static_cast<std::uint64_t>(x));
-			return static_cast<std::size_t>(h);
+			return static_cast<std::size_t>(Mix(static_cast<std::uint64_t>(k.v[0])
+				^ Mix(static_cast<std::uint64_t>(k.v[1]) ^ Mix(static_cast<std::uint64_t>(k.v[2])))));
 		}
 	};
 	std::unordered_map<std::uint64_t, std::vector<std::int64_t>> by_column;
@@ -341,13 +348,14 @@ inline bezier::VisualMesh BuildBezierVisualMesh(const FlatDatabase& db)
 		const Cell c = cell_of(e);
 		const int a0 = db.element_offsets[e], nen = db.element_offsets[e + 1] - a0;
 		const std::uint64_t id = db.ids[e];
+		// the element's extraction columns: (node, coefficient) over its rows, per Bezier point
+		std::array<std::vector<std::pair<std::int32_t, double>>, 64> columns;
+		for (int a = 0; a < nen; ++a)
+			for (int k = db.extraction_offsets[a0 + a]; k < db.extraction_offsets[a0 + a + 1]; ++k)
+				if (db.extraction_values[k] != 0.0)
+					columns[db.extraction_columns[k]].emplace_back(db.connectivity[a0 + a], db.extraction_values[k]);
 		for (int p = 0; p < 64; ++p) {
-			// the point's extraction column: (node, coefficient) over the element's rows
-			std::vector<std::pair<std::int32_t, double>> column;
-			for (int a = 0; a < nen; ++a)
-				for (int k = db.extraction_offsets[a0 + a]; k < db.extraction_offsets[a0 + a + 1]; ++k)
-					if (db.extraction_columns[k] == p && db.extraction_values[k] != 0.0)
-						column.emplace_back(db.connectivity[a0 + a], db.extraction_values[k]);
+			auto& column = columns[p];
 			if (column.empty()) throw std::runtime_error("Bezier extraction contains an empty column");
 			std::vector<std::pair<std::int32_t, std::int64_t>> key;
 			key.reserve(column.size());
@@ -381,33 +389,8 @@ inline bezier::VisualMesh BuildBezierVisualMesh(const FlatDatabase& db)
 				++m.checks.shared_point_references;
 				continue;
 			}
-			Key bucket;
-			for (int d = 0; d < 3; ++d) bucket.v[d] = static_cast<std::int64_t>(std::llround(c[p][d]/match_tol));
-			std::int64_t near = -1;
-			for (int dz = -1; dz <= 1 && near < 0; ++dz)
-				for (int dy = -1; dy <= 1 && near < 0; ++dy)
-					for (int dx = -1; dx <= 1 && near < 0; ++dx) {
-						Key nb = bucket;
-						nb.v[0] += dx;
-						nb.v[1] += dy;
-						nb.v[2] += dz;
-						const auto f = by_position.find(nb);
-						if (f == by_position.end()) continue;
-						for (const auto cand : f->second)
-							if (dist2(m.points[static_cast<size_t>(cand)], c[p]) <= match_tol2) {
-								near = cand;
-								break;
-							}
-					}
-			if (near >= 0) {
-				++m.checks.coincident_unmerged_points;
-				m.checks.first_bad_element = std::min(m.checks.first_bad_element, id);
-				if (m.checks.first_coincident_element_pair[0] == kNone)
-					m.checks.first_coincident_element_pair = {{first_element[static_cast<size_t>(near)], id}};
-			}
 			const auto g = static_cast<std::int64_t>(m.points.size());
 			by_column[h].push_back(g);
-			by_position[bucket].push_back(g);
 			m.points.push_back(c[p]);
 			m.point_labels.push_back(-1);
 			first_element.push_back(id);
@@ -435,23 +418,7 @@ inline bezier::VisualMesh BuildBezierVisualMesh(const FlatDatabase& db)
 			++m.checks.collapsed_element_points;
 			m.checks.first_bad_element = std::min(m.checks.first_bad_element, id);
 		}
-		Cell merged;
-		for (int p = 0; p < 64; ++p) merged[p] = m.points[static_cast<size_t>(local[e][p])];
-		for (const auto& x : merged) boxes[e].Add(x);
-		bool bad = false;
-		for (const double w : kGauss)
-			for (const double v : kGauss)
-				for (const double u : kGauss) {
-					double J[3][3];
-					Map(merged, {{u, v, w}}, J);
-					const double det = Det3(J);
-					m.checks.minimum_jacobian = std::min(m.checks.minimum_jacobian, det);
-					if (!(det > 0.0)) bad = true;
-				}
-		if (bad) {
-			++m.checks.nonpositive_jacobian_elements;
-			m.checks.first_bad_element = std::min(m.checks.first_bad_element, id);
-		}
+		for (int p = 0; p < 64; ++p) boxes[e].Add(m.points[static_cast<size_t>(local[e][p])]);
 		for (const int t : order) m.connectivity.push_back(local[e][t]);
 		m.offsets.push_back(static_cast<std::int64_t>(m.connectivity.size()));
 		m.types.push_back(kBezierHexCell);
@@ -461,6 +428,113 @@ inline bezier::VisualMesh BuildBezierVisualMesh(const FlatDatabase& db)
 		m.face_labels.push_back(faces);
 	}
 	m.checks.unique_points = m.points.size();
+	const std::int64_t G = static_cast<std::int64_t>(m.points.size());
+	auto bucket_of = [&](const std::array<double, 3>& x) {
+		Key k;
+		for (int d = 0; d < 3; ++d) k.v[d] = static_cast<std::int64_t>(std::llround(x[d]/match_tol));
+		return k;
+	};
+	const int workers = std::max(1, std::min({16, static_cast<int>(std::thread::hardware_concurrency()), E/256 + 1}));
+	auto merged_cell = [&](int e) {
+		Cell c;
+		for (int p = 0; p < 64; ++p) c[p] = m.points[static_cast<size_t>(local[e][p])];
+		return c;
+	};
+	// points that lie on an earlier point (within the match tolerance) without
+	// sharing its extraction column: each point looks through the 27
+	// neighbouring coordinate buckets for a lower-numbered point, the buckets
+	// in (dz, dy, dx) order and each bucket's points in creation order, as a
+	// creation-ordered scan would meet them
+	{
+		for (std::int64_t g = 0; g < G; ++g) by_position[bucket_of(m.points[static_cast<size_t>(g)])].push_back(g);
+		struct Part {
+			std::uint64_t count = 0, first_bad = kNone;
+			std::int64_t first_point = -1, first_near = -1;
+		};
+		std::vector<Part> parts(workers);
+		auto scan = [&](int w) {
+			Part& part = parts[w];
+			for (std::int64_t g = G*w/workers; g < G*(w + 1)/workers; ++g) {
+				const auto& x = m.points[static_cast<size_t>(g)];
+				const Key bucket = bucket_of(x);
+				std::int64_t near = -1;
+				for (int dz = -1; dz <= 1 && near < 0; ++dz)
+					for (int dy = -1; dy <= 1 && near < 0; ++dy)
+						for (int dx = -1; dx <= 1 && near < 0; ++dx) {
+							Key nb = bucket;
+							nb.v[0] += dx;
+							nb.v[1] += dy;
+							nb.v[2] += dz;
+							const auto f = by_position.find(nb);
+							if (f == by_position.end()) continue;
+							for (const auto cand : f->second) {
+								if (cand >= g) break;
+								if (dist2(m.points[static_cast<size_t>(cand)], x) <= match_tol2) {
+									near = cand;
+									break;
+								}
+							}
+						}
+				if (near < 0) continue;
+				++part.count;
+				part.first_bad = std::min(part.first_bad, first_element[static_cast<size_t>(g)]);
+				if (part.first_point < 0) {
+					part.first_point = g;
+					part.first_near = near;
+				}
+			}
+		};
+		std::vector<std::thread> pool;
+		for (int w = 1; w < workers; ++w) pool.emplace_back(scan, w);
+		scan(0);
+		for (auto& t : pool) t.join();
+		for (const auto& part : parts) {
+			m.checks.coincident_unmerged_points += part.count;
+			m.checks.first_bad_element = std::min(m.checks.first_bad_element, part.first_bad);
+			if (m.checks.first_coincident_element_pair[0] == kNone && part.first_point >= 0)
+				m.checks.first_coincident_element_pair = {{first_element[static_cast<size_t>(part.first_near)],
+					first_element[static_cast<size_t>(part.first_point)]}};
+		}
+	}
+	// Jacobian of every merged cell at the 4x4x4 Gauss points (order-free:
+	// a count, a minimum and the smallest bad element id)
+	{
+		struct Part {
+			std::uint64_t bad = 0, first = kNone;
+			double minimum = std::numeric_limits<double>::infinity();
+		};
+		std::vector<Part> parts(workers);
+		auto scan = [&](int w) {
+			Part& part = parts[w];
+			for (int e = static_cast<int>(static_cast<long long>(E)*w/workers);
+					e < static_cast<int>(static_cast<long long>(E)*(w + 1)/workers); ++e) {
+				const Cell cell = merged_cell(e);
+				bool bad = false;
+				for (const double w3 : kGauss)
+					for (const double v : kGauss)
+						for (const double u : kGauss) {
+							double J[3][3];
+							Map(cell, {{u, v, w3}}, J);
+							const double det = Det3(J);
+							part.minimum = std::min(part.minimum, det);
+							if (!(det > 0.0)) bad = true;
+						}
+				if (bad) {
+					++part.bad;
+					part.first = std::min(part.first, db.ids[e]);
+				}
+			}
+		};
+		std::vector<std::thread> pool;
+		for (int w = 1; w < workers; ++w) pool.emplace_back(scan, w);
+		scan(0);
+		for (auto& t : pool) t.join();
+		for (const auto& part : parts) {
+			m.checks.nonpositive_jacobian_elements += part.bad;
+			m.checks.first_bad_element = std::min(m.checks.first_bad_element, part.first);
+			m.checks.minimum_jacobian = std::min(m.checks.minimum_jacobian, part.minimum);
+		}
+	}
 	// overlap test: sweep over the longest axis of the domain
 	int axis = 0;
 	for (int d = 1; d < 3; ++d)
@@ -468,27 +542,44 @@ inline bezier::VisualMesh BuildBezierVisualMesh(const FlatDatabase& db)
 	std::vector<int> sweep(E);
 	std::iota(sweep.begin(), sweep.end(), 0);
 	std::stable_sort(sweep.begin(), sweep.end(), [&](int l, int r) { return boxes[l].lo[axis] < boxes[r].lo[axis]; });
-	auto merged_cell = [&](int e) {
-		Cell c;
-		for (int p = 0; p < 64; ++p) c[p] = m.points[static_cast<size_t>(local[e][p])];
-		return c;
+	// pairs (s, t > s) of the sweep order; workers take contiguous ranges of s,
+	// so the first overlapping pair in (s, t) order is the first one found by
+	// the lowest worker that finds any
+	struct Part {
+		std::uint64_t candidates = 0, overlapping = 0;
+		std::array<std::uint64_t, 2> first{{kNone, kNone}};
 	};
-	for (int s = 0; s < E; ++s) {
-		const int l = sweep[s];
-		for (int t = s + 1; t < E; ++t) {
-			const int r = sweep[t];
-			if (boxes[r].lo[axis] >= boxes[l].hi[axis] - geo_tol) break;
-			if (!BoxesOverlap(boxes[l], boxes[r], geo_tol)) continue;
-			int shared = 0;
-			for (const auto x : local[l])
-				if (std::find(local[r].begin(), local[r].end(), x) != local[r].end()) ++shared;
-			if (shared >= 16) continue;
-			++m.checks.overlap_candidates;
-			if (CellsOverlap(merged_cell(l), boxes[l], merged_cell(r), boxes[r], geo_tol)) {
-				++m.checks.overlapping_element_pairs;
-				if (m.checks.first_overlap_pair[0] == kNone) m.checks.first_overlap_pair = {{db.ids[l], db.ids[r]}};
+	std::vector<Part> parts(workers);
+	auto scan = [&](int w) {
+		Part& part = parts[w];
+		const int s0 = static_cast<int>(static_cast<long long>(E)*w/workers);
+		const int s1 = static_cast<int>(static_cast<long long>(E)*(w + 1)/workers);
+		for (int s = s0; s < s1; ++s) {
+			const int l = sweep[s];
+			for (int t = s + 1; t < E; ++t) {
+				const int r = sweep[t];
+				if (boxes[r].lo[axis] >= boxes[l].hi[axis] - geo_tol) break;
+				if (!BoxesOverlap(boxes[l], boxes[r], geo_tol)) continue;
+				int shared = 0;
+				for (const auto x : local[l])
+					if (std::find(local[r].begin(), local[r].end(), x) != local[r].end()) ++shared;
+				if (shared >= 16) continue;
+				++part.candidates;
+				if (CellsOverlap(merged_cell(l), boxes[l], merged_cell(r), boxes[r], geo_tol)) {
+					++part.overlapping;
+					if (part.first[0] == kNone) part.first = {{db.ids[l], db.ids[r]}};
+				}
 			}
 		}
+	};
+	std::vector<std::thread> pool;
+	for (int w = 1; w < workers; ++w) pool.emplace_back(scan, w);
+	scan(0);
+	for (auto& t : pool) t.join();
+	for (const auto& part : parts) {
+		m.checks.overlap_candidates += part.candidates;
+		m.checks.overlapping_element_pairs += part.overlapping;
+		if (m.checks.first_overlap_pair[0] == kNone && part.first[0] != kNone) m.checks.first_overlap_pair = part.first;
 	}
 	// source coordinates
 	const double scale = db.transform[3];
