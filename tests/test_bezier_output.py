@@ -113,3 +113,31 @@ def test_linear_field_is_reproduced_at_the_bezier_points(host_check, tmp_path):
     bez = case.mesh.bezier_points.reshape(-1, 64, 3)
     for e in range(len(conn)):
         assert np.max(np.abs(normalised[conn[e, :8]] - bez[e, corners])) < 1e-12
+
+
+def test_resumed_series_appends_and_checks_the_geometry(host_check, tmp_path):
+    """A restarted run opens the file and appends (TemporalVtkHdf.hpp:600-640):
+    one state, then two more resumed, dumps like the three written at once; a
+    time equal to the last one replaces that step; another geometry is refused."""
+    case = casegen.workload("tube_c0")
+    rng = np.random.default_rng(7)
+    states = [rng.standard_normal((case.mesh.nodes, 4)) for _ in range(3)]
+    _, db, files = _inputs("tube_c0", tmp_path / "case", states)
+    whole, split = tmp_path / "whole", tmp_path / "split"
+    whole.mkdir()
+    split.mkdir()
+    assert subprocess.run([str(host_check), str(db), str(whole / "flow.txt"), *map(str, files)]).returncode == 0
+    assert subprocess.run([str(host_check), str(db), str(split / "flow.txt"), str(files[0])]).returncode == 0
+    # resume at step 2 after a state written again at step 1: that step is replaced
+    assert subprocess.run([str(host_check), "--resume", "1", str(db), str(split / "flow.txt"), str(files[0]),
+                           str(files[1]), str(files[2])]).returncode == 0
+    a = _dump(whole / "flow.vtkhdf").replace(str(whole), "X")
+    b = _dump(split / "flow.vtkhdf").replace(str(split), "X")
+    assert a == b
+    other = casegen.make_case("tube", radius_mm=1.0, length_mm=10.0, n_core=1, m_ring=1, n_axial=12, u_max=0.1)
+    casegen.write_case(other, tmp_path / "other")
+    np.savetxt(tmp_path / "other" / "s.txt", np.zeros((other.mesh.nodes, 4)))
+    r = subprocess.run([str(host_check), "--resume", "4", str(tmp_path / "other" / "case.ntiga"), str(split / "flow.txt"),
+                        str(tmp_path / "other" / "s.txt")], capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "existing VTKHDF geometry does not match the current .ntiga database" in r.stderr
