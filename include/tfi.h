@@ -28,6 +28,7 @@ extern "C" {
 typedef struct tfi_ctx tfi_ctx;
 typedef struct tfi_mesh tfi_mesh;
 typedef struct tfi_flow tfi_flow;
+typedef struct tfi_amg tfi_amg;
 
 enum { TFI_OK = 0, TFI_ERROR = 1 };
 enum { TFI_PC_BJACOBI = 0, TFI_PC_AMG = 1 };
@@ -103,6 +104,17 @@ int tfi_ns_assemble(tfi_mesh* mesh, const double* state, const double* previous,
 /* y = A x on the mesh pattern (BlockSpmvKernel<4>, SparseKernels.cuh:25-40);
  * precision TFI_FP64 or TFI_MIXED (values rounded to fp32). */
 int tfi_bsr_spmv(tfi_mesh* mesh, const double* values, const double* x, double* y, int precision);
+/* Device-pointer forms of the kernels above, for a Newton loop that keeps its
+ * arrays in device memory (the reference driver's own loop, iga_cuda.cu:956-1046,
+ * keeps state, Jacobian and rhs on the GPU): the same arithmetic with no host
+ * copies (tfi_ns_assemble round-trips the nnzb*16 Jacobian).  All arrays are
+ * device pointers of the sizes above; synchronous on the context stream. */
+int tfi_ns_assemble_device(tfi_mesh* mesh, const double* state, const double* previous, double density,
+	double viscosity, double dt, double* jacobian, double* negative_residual, int tangent);
+int tfi_bsr_spmv_device(tfi_mesh* mesh, const double* values, const double* x, double* y);
+int tfi_apply_dirichlet_device(tfi_mesh* mesh, const int32_t* velocity_mask, const int32_t* pressure_mask,
+	const double* velocity_target, const double* pressure_target, const double* state, double* jacobian,
+	double* rhs);
 /* 4x4 diagonal-block inverses (BuildBlockInverseKernel<4>, SparseKernels.cuh:42-90). */
 int tfi_block_inverse(tfi_mesh* mesh, const double* values, double* inverse, int* singular);
 /* Allocation-free variant of tfi_block_inverse.  No counterpart in the
@@ -241,6 +253,26 @@ int tfi_flow_boundary_flows(tfi_flow* flow, int count, const int32_t* labels, do
 int tfi_flow_amg_levels(tfi_flow* flow, int capacity, int* levels, int* rows, long long* block_nonzeros,
 	int* colors);
 
+/* ------------------------------------------------------------------ AMG */
+/* The coupled 4x4-block aggregation AMG as a preconditioner of its own, for a
+ * caller with its own Krylov method (the reference's GMRES uses block-Jacobi
+ * scaling only, Gmres.hpp:69-213; no reference counterpart).  smoother:
+ * TFI_SMOOTH_* (AUTO = Gauss-Seidel); precision TFI_MIXED builds the levels
+ * in FP32 (Chebyshev levels read D^-1 A in FP16).  tfi_amg_setup builds the
+ * aggregation (first call) and the numeric hierarchy; tfi_amg_update rebuilds
+ * the numeric hierarchy for new values on the same pattern.  values [nnzb*16]
+ * is a DEVICE pointer the hierarchy keeps reading: it must stay valid and
+ * unchanged until the next update or tfi_amg_destroy.  tfi_amg_apply: z =
+ * M^-1 v, one V-cycle, v and z DEVICE pointers [4N].  Destroy the handle
+ * before its mesh. */
+int tfi_amg_create(tfi_mesh* mesh, int precision, int smoother, tfi_amg** out);
+int tfi_amg_setup(tfi_amg* amg, const double* values);
+int tfi_amg_update(tfi_amg* amg, const double* values);
+int tfi_amg_apply(tfi_amg* amg, const double* v, double* z);
+/* Per level: block rows, block nonzeros, Gauss-Seidel colours (0: other smoother). */
+int tfi_amg_levels(tfi_amg* amg, int capacity, int* levels, int* rows, long long* block_nonzeros, int* colors);
+void tfi_amg_destroy(tfi_amg* amg);
+
 /* Learned warm start (MLP on 27-control-point stencils).  The network
  * predicts the error of a base prediction: TFI_WARM_MPE (minimal polynomial
  * extrapolation of the step increments, the default) or TFI_WARM_EXTRAPOLATE4
@@ -270,6 +302,15 @@ int tfi_flow_warm_predict(tfi_flow* flow, int mode, double* state, int* order);
  * to resume a restarted run with the states before its checkpoint.
  * tfi_flow_set_state clears the history.  No reference counterpart. */
 int tfi_flow_warm_record(tfi_flow* flow, const double* state);
+/* One training step of the learned predictor outside a Newton solve: the
+ * state (host array [4N]) is appended to the history as tfi_flow_commit_step
+ * appends a converged state, and the network trains on the snapshot it forms
+ * (the base prediction's error at this state from the history before it),
+ * train_steps Adam steps of tfi_warm_options, once five states precede it.
+ * phase: up to 8 features (may be NULL with phase_count 0).  *loss (may be
+ * NULL) receives the last training loss (0 before the first snapshot).  No
+ * reference counterpart. */
+int tfi_flow_warm_train(tfi_flow* flow, const double* state, const double* phase, int phase_count, double* loss);
 
 /* ------------------------------------------------------------------ transport */
 /* Two-species neuron transport on the mesh pattern with 2x2 node blocks

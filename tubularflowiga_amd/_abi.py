@@ -123,6 +123,18 @@ SIGNATURES = {
     "tfi_bench_spmv": (c_int, [c_void_p, c_int, c_int, P(c_double)]),
     "tfi_bench_copy": (c_int, [c_void_p, c_ll, c_int, P(c_double)]),
     "tfi_bench_mfma_fp64": (c_int, [c_void_p, c_int, P(c_double)]),
+    "tfi_ns_assemble_device": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_void_p, c_void_p,
+                                       c_int]),
+    "tfi_bsr_spmv_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tfi_apply_dirichlet_device": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p]),
+    "tfi_amg_create": (c_int, [c_void_p, c_int, c_int, P(c_void_p)]),
+    "tfi_amg_setup": (c_int, [c_void_p, c_void_p]),
+    "tfi_amg_update": (c_int, [c_void_p, c_void_p]),
+    "tfi_amg_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "tfi_amg_levels": (c_int, [c_void_p, c_int, P(c_int), P(c_int), P(c_ll), P(c_int)]),
+    "tfi_amg_destroy": (None, [c_void_p]),
+    "tfi_flow_warm_train": (c_int, [c_void_p, _d, c_void_p, c_int, P(c_double)]),
     "tfi_bench_stream": (c_int, [c_void_p, c_ll, c_int, P(c_double), P(c_double)]),
     "tfi_bench_assembly": (c_int, [c_void_p, c_int, c_int, c_int, P(c_double)]),
     "tfi_flow_default_options": (None, [P(FlowOptions)]),

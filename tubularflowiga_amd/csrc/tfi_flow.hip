@@ -270,6 +270,32 @@ std::string Precise(double v)
 }
 }  // namespace
 
+// A standalone AMG hierarchy for callers with their own Krylov solver.
+struct tfi_amg {
+	tfi_mesh* mesh = nullptr;
+	int precision = TFI_FP64;
+	std::unique_ptr<Amg> amg;
+	DevBuf<float> values32;
+	bool ready = false;
+};
+
+namespace {
+void AmgNumeric(tfi_amg* a, const double* values)
+{
+	tfi_mesh* m = a->mesh;
+	if (!values) throw Error("the AMG set-up needs the matrix values");
+	const float* v32 = nullptr;
+	if (a->precision == TFI_MIXED) {
+		if (!a->values32.size()) a->values32.Allocate(m->ctx, m->nnzb*16);
+		ToFloat(m->ctx, values, a->values32.data(), m->nnzb*16);
+		v32 = a->values32.data();
+	}
+	a->amg->Setup(values, v32);
+	Sync(m->ctx);
+	a->ready = true;
+}
+}  // namespace
+
 extern "C" {
 
 void tfi_flow_default_options(tfi_flow_options* o)
@@ -867,6 +893,40 @@ int tfi_bsr_spmv(tfi_mesh* m, const double* values, const double* x, double* y, 
 	});
 }
 
+int tfi_ns_assemble_device(tfi_mesh* m, const double* state, const double* previous, double density,
+	double viscosity, double dt, double* jacobian, double* negative_residual, int tangent)
+{
+	return Guard(m->ctx, [&] {
+		if (!(density > 0.0) || !(viscosity > 0.0) || !(dt >= 0.0))
+			throw Error("invalid Navier-Stokes density, dynamic viscosity, or time step");
+		if (!state || !negative_residual) throw Error("tfi_ns_assemble_device needs a state and a residual array");
+		if (tangent != TFI_TANGENT_EXACT && tangent != TFI_TANGENT_LEGACY) throw Error("unknown tangent");
+		AssembleNs(m, state, previous ? previous : state, NsParams{density, viscosity, dt, tangent}, jacobian,
+			negative_residual);
+		Sync(m->ctx);
+	});
+}
+
+int tfi_bsr_spmv_device(tfi_mesh* m, const double* values, const double* x, double* y)
+{
+	return Guard(m->ctx, [&] {
+		if (!values || !x || !y) throw Error("tfi_bsr_spmv_device needs values, x and y");
+		Spmv64(m->ctx, m->view(), values, x, y);
+		Sync(m->ctx);
+	});
+}
+
+int tfi_apply_dirichlet_device(tfi_mesh* m, const int32_t* velocity_mask, const int32_t* pressure_mask,
+	const double* velocity_target, const double* pressure_target, const double* state, double* jacobian, double* rhs)
+{
+	return Guard(m->ctx, [&] {
+		if (!velocity_mask || !pressure_mask || !velocity_target || !pressure_target || !state)
+			throw Error("tfi_apply_dirichlet_device needs masks, targets and the state");
+		ApplyDirichlet(m, velocity_mask, pressure_mask, velocity_target, pressure_target, state, jacobian, rhs);
+		Sync(m->ctx);
+	});
+}
+
 int tfi_block_inverse(tfi_mesh* m, const double* values, double* inverse, int* singular)
 {
 	return Guard(m->ctx, [&] {
@@ -1078,6 +1138,87 @@ int tfi_bench_spmv(tfi_mesh* m, int precision, int repetitions, double* millisec
 		float ms = 0.0f;
 		Check(hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end), "elapsed");
 		*milliseconds = ms/std::max(1, repetitions);
+	});
+}
+
+// AMG handle and learned-predictor training (C-ABI)
+int tfi_amg_create(tfi_mesh* m, int precision, int smoother, tfi_amg** out)
+{
+	return Guard(m->ctx, [&] {
+		if (!out) throw Error("tfi_amg_create needs an output handle");
+		if (precision != TFI_FP64 && precision != TFI_MIXED) throw Error("unknown precision");
+		if (smoother < TFI_SMOOTH_GAUSS_SEIDEL || smoother > TFI_SMOOTH_AUTO) throw Error("unknown AMG smoother");
+		auto a = std::make_unique<tfi_amg>();
+		a->mesh = m;
+		a->precision = precision;
+		AmgOptions o;
+		// without a time step to judge by, AUTO takes Gauss-Seidel (any matrix)
+		o.smoother = smoother == TFI_SMOOTH_AUTO ? TFI_SMOOTH_GAUSS_SEIDEL : smoother;
+		o.half_smoother = precision == TFI_MIXED && o.smoother == TFI_SMOOTH_CHEBYSHEV;
+		a->amg = std::make_unique<Amg>(m, precision, o);
+		*out = a.release();
+	});
+}
+
+void tfi_amg_destroy(tfi_amg* a)
+{
+	delete a;
+}
+
+int tfi_amg_setup(tfi_amg* a, const double* values)
+{
+	return Guard(a->mesh->ctx, [&] { AmgNumeric(a, values); });
+}
+
+int tfi_amg_update(tfi_amg* a, const double* values)
+{
+	return Guard(a->mesh->ctx, [&] {
+		if (!a->ready) throw Error("tfi_amg_update needs a hierarchy from tfi_amg_setup");
+		AmgNumeric(a, values);
+	});
+}
+
+int tfi_amg_apply(tfi_amg* a, const double* v, double* z)
+{
+	return Guard(a->mesh->ctx, [&] {
+		if (!a->ready) throw Error("tfi_amg_apply needs a hierarchy from tfi_amg_setup");
+		if (!v || !z) throw Error("tfi_amg_apply needs v and z");
+		a->amg->Apply(v, z);
+		Sync(a->mesh->ctx);
+	});
+}
+
+int tfi_amg_levels(tfi_amg* a, int capacity, int* levels, int* rows, long long* block_nonzeros, int* colors)
+{
+	return Guard(a->mesh->ctx, [&] {
+		if (!a->ready) throw Error("tfi_amg_levels needs a hierarchy from tfi_amg_setup");
+		std::vector<int> r, c;
+		std::vector<long long> nz;
+		a->amg->Describe(r, nz, c);
+		if (levels) *levels = static_cast<int>(r.size());
+		for (int i = 0; i < capacity && i < static_cast<int>(r.size()); ++i) {
+			if (rows) rows[i] = r[i];
+			if (block_nonzeros) block_nonzeros[i] = nz[i];
+			if (colors) colors[i] = c[i];
+		}
+	});
+}
+
+int tfi_flow_warm_train(tfi_flow* f, const double* state, const double* phase, int phase_count, double* loss)
+{
+	return Guard(f->ctx, [&] {
+		if (!state) throw Error("tfi_flow_warm_train needs a state");
+		if (phase_count < 0 || phase_count > 8 || (phase_count > 0 && !phase))
+			throw Error("tfi_flow_warm_train takes at most 8 phase features");
+		const long long n = 4LL*f->mesh->nodes;
+		f->update.Upload(state, n);
+		const double none[1] = {0.0};
+		f->warm->Commit(f->update.data(), phase_count > 0 ? phase : none, phase_count, true);
+		Sync(f->ctx);
+		if (loss) {
+			int snapshots = 0, accepted = 0, rejected = 0;
+			f->warm->Stats(&snapshots, loss, &accepted, &rejected);
+		}
 	});
 }
 

@@ -234,6 +234,44 @@ class DeviceMesh:
             self.handle = None
 
 
+class AmgPreconditioner:
+    """The coupled 4x4-block aggregation AMG as a stand-alone preconditioner
+    (tfi_amg_*).  Matrix values and vectors are DEVICE pointers (int or
+    ctypes.c_void_p); the hierarchy keeps reading the values given to setup /
+    update until the next update or close."""
+
+    def __init__(self, dmesh: DeviceMesh, precision=TFI_FP64, smoother=TFI_SMOOTH_AUTO):
+        self.dmesh, self.ctx, self.lib = dmesh, dmesh.ctx, dmesh.lib
+        h = ctypes.c_void_p()
+        self.ctx.check(self.lib.tfi_amg_create(dmesh.handle, precision, smoother, ctypes.byref(h)))
+        self.handle = h
+
+    def setup(self, values):
+        self.ctx.check(self.lib.tfi_amg_setup(self.handle, values))
+
+    def update(self, values):
+        self.ctx.check(self.lib.tfi_amg_update(self.handle, values))
+
+    def apply(self, v, z):
+        """z = M^-1 v (one V-cycle)."""
+        self.ctx.check(self.lib.tfi_amg_apply(self.handle, v, z))
+
+    def levels(self):
+        """[(rows, block_nonzeros, gauss_seidel_colours)] per level."""
+        cap = 16
+        n = ctypes.c_int()
+        rows = (ctypes.c_int * cap)()
+        nnzb = (ctypes.c_longlong * cap)()
+        cols = (ctypes.c_int * cap)()
+        self.ctx.check(self.lib.tfi_amg_levels(self.handle, cap, ctypes.byref(n), rows, nnzb, cols))
+        return [(rows[i], nnzb[i], cols[i]) for i in range(min(n.value, cap))]
+
+    def close(self):
+        if self.handle:
+            self.lib.tfi_amg_destroy(self.handle)
+            self.handle = None
+
+
 class FlowSolver:
     """Device-resident backward-Euler / steady Navier-Stokes solver."""
 
@@ -321,6 +359,17 @@ class FlowSolver:
     def warm_record(self, state):
         """Append a state to the warm-start history (no training)."""
         self.ctx.check(self.lib.tfi_flow_warm_record(self.handle, np.ascontiguousarray(state, np.float64)))
+
+    def warm_train(self, state, phase=None):
+        """Append a state to the history and train the learned predictor on the
+        snapshot it forms (once five states precede it); returns the last
+        training loss (0.0 before the first snapshot)."""
+        ph = np.zeros(0) if phase is None else np.ascontiguousarray(phase, np.float64)
+        loss = ctypes.c_double()
+        self.ctx.check(self.lib.tfi_flow_warm_train(self.handle, np.ascontiguousarray(state, np.float64),
+                                                    ph.ctypes.data_as(ctypes.c_void_p) if len(ph) else None,
+                                                    len(ph), ctypes.byref(loss)))
+        return loss.value
 
     def warm_predict(self, mode):
         """(state the predicting `mode` would start the next step from, order
