@@ -24,6 +24,7 @@ HOST = PKG / "host"
 BUILD = PKG / "_build"
 LIB = PKG / "libtfi_hip.so"
 CLI = PKG / "iga_hip"
+PACK = PKG / "tfi_pack"
 
 SOURCES = ["tfi_core.hip", "tfi_mesh.hip", "tfi_assembly.hip", "tfi_sparse.hip", "tfi_krylov.hip",
            "tfi_amg.hip", "tfi_mlp.hip", "tfi_warmstart.hip", "tfi_flow.hip", "tfi_transport.hip",
@@ -100,15 +101,26 @@ def hdf5_flags() -> list[str]:
 
 
 def build_cli() -> Path | None:
-    srcs = sorted(HOST.glob("*.cpp")) if HOST.exists() else []
-    if not srcs:
+    """The solver CLI iga_hip (links libtfi_hip.so) and the database packer
+    tfi_pack (host code only)."""
+    src = HOST / "iga_hip.cpp"
+    if not src.exists():
         return None
     cxx = shutil.which("g++") or "g++"
     tmp = CLI.with_name(CLI.name + ".tmp")
-    _run([cxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-I", str(ROOT / "include"), *map(str, srcs), "-o", str(tmp),
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-I", str(ROOT / "include"), str(src), "-o", str(tmp),
           "-L", str(PKG), "-ltfi_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + str(PKG), *hdf5_flags()])
     os.replace(tmp, CLI)
+    build_pack()
     return CLI
+
+
+def build_pack() -> Path:
+    cxx = shutil.which("g++") or "g++"
+    tmp = PACK.with_name(PACK.name + ".tmp")
+    _run([cxx, "-O2", "-std=c++17", "-Wall", str(HOST / "tfi_pack.cpp"), "-o", str(tmp)])
+    os.replace(tmp, PACK)
+    return PACK
 
 
 def main(argv=None):
