@@ -302,3 +302,35 @@ def test_pack_usage_matches_reference(tmp_path, pack, args):
     assert mine.returncode == theirs.returncode
     norm = lambda s: s.replace("tfi_pack", "P").replace("iga_pack", "P")  # noqa: E731
     assert norm(mine.stderr) == norm(theirs.stderr)
+
+
+@pytest.mark.parametrize("name", ["tube_c0", "tree_small"])
+def test_pack_of_a_generated_case_is_its_database(tmp_path, pack, name):
+    """A generated case's spline data written as cmat.txt / bzpt.txt beside its
+    control mesh packs into the case's own database, byte for byte apart from
+    the 40-byte geometry transform (the control mesh holds normalised points,
+    so the packer's bounding-box transform is origin 0, extent 1): the face
+    labels the packer derives from the control cells and point labels are
+    the generator's, element by element."""
+    from tubularflowiga_amd import casegen
+    case = casegen.workload(name)
+    d = casegen.write_case(case, tmp_path / "case", steps=1)
+    m = case.mesh
+    E = m.elements
+    out = [str(E)]
+    for e in range(E):
+        conn = m.element_connectivity(e)
+        out.append(f"{e} {len(conn)} 0")
+        out.append(" ".join(map(str, conn)))
+        out += [" ".join(repr(float(x)) for x in row) for row in m.dense_extraction(e)]
+    (d / "cmat.txt").write_text("\n".join(out) + "\n")
+    bz = m.bezier_points.reshape(-1, 3)
+    (d / "bzpt.txt").write_text(f"{64 * E}\n" + "\n".join(" ".join(repr(float(x)) for x in p) for p in bz) + "\n")
+    (d / "bzmeshinfo.txt.epart.1").write_text("0\n" * E)
+    r = _run(pack, d, 1, tmp_path / "packed.ntiga")
+    assert r.returncode == 0, r.stderr
+    mine, gen = (tmp_path / "packed.ntiga").read_bytes(), (d / "case.ntiga").read_bytes()
+    assert len(mine) == len(gen) and mine[:48] == gen[:48] and mine[88:] == gen[88:]
+    lo = case.control_points.min(0)
+    assert np.array_equal(np.frombuffer(mine[48:88], "<f8"),
+                          [*lo, (case.control_points.max(0) - lo).min(), 1.0])
