@@ -295,9 +295,20 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 		if (kMfma || table) {  // the MFMA variants run only with the table
 			const int rec = nen*kTableRec;
 			const double* src = table + ((size_t)b0*kQ + (size_t)q0*nen)*kTableRec;
+			// entry i = (point ql, offset r) of this thread, advanced by kT entries
+			// per pass without a division (dr < rec, so one carry suffices)
+			const int dq = kT/rec, dr = kT - dq*rec;
+			int ql = tid/rec, r = tid - ql*rec;
+#pragma unroll 4
 			for (int i = tid; i < kSlab*rec; i += kT) {
-				const int ql = i/rec, r = i - ql*rec, a = r/kTableRec, f = r - a*kTableRec;
+				const int a = r/kTableRec, f = r - a*kTableRec;
 				Brec(basis, qstride, a, ql)[f] = src[i];
+				ql += dq;
+				r += dr;
+				if (r >= rec) {
+					r -= rec;
+					++ql;
+				}
 			}
 		} else {
 			for (int it = tid; it < nen*kSlab; it += kT) {
