@@ -615,16 +615,30 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 				}
 			}
 		__syncthreads();
+		// All of a thread's block slots, then all of its Jacobian pieces, are
+		// loaded before the first store (the blocks of one colour are disjoint),
+		// so the read-modify-write costs two memory round trips, not two per
+		// piece.  Indices past the element's pieces are clamped to its last piece
+		// for the loads and skipped for the stores.
+		constexpr int kPieces = (32*32*8 + kT - 1)/kT;  // per thread, nen <= 32
 		const long long pbase = args.pair_offsets[e];
-		for (int it = tid; it < nen*nen*8; it += kT) {
-			const int blk = it >> 3, part = it & 7;
-			const int slot = args.pair_block[pbase + blk];
-			double2* J = reinterpret_cast<double2*>(args.jac + (size_t)slot*16) + part;
-			const double* src = stage + blk*kStageStride + 2*part;
-			double2 v = *J;
-			v.x += src[0];
-			v.y += src[1];
-			*J = v;
+		const int total = nen*nen*8;
+		int slot[kPieces];
+#pragma unroll
+		for (int k = 0; k < kPieces; ++k) slot[k] = args.pair_block[pbase + (min(tid + k*kT, total - 1) >> 3)];
+		double2 v[kPieces];
+#pragma unroll
+		for (int k = 0; k < kPieces; ++k)
+			v[k] = reinterpret_cast<const double2*>(args.jac + (size_t)slot[k]*16)[min(tid + k*kT, total - 1) & 7];
+#pragma unroll
+		for (int k = 0; k < kPieces; ++k) {
+			const int it = tid + k*kT;
+			if (it < total) {
+				const double* src = stage + (it >> 3)*kStageStride + 2*(it & 7);
+				v[k].x += src[0];
+				v[k].y += src[1];
+				reinterpret_cast<double2*>(args.jac + (size_t)slot[k]*16)[it & 7] = v[k];
+			}
 		}
 	}
 	if (pair_active) {
