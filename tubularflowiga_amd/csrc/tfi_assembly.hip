@@ -248,45 +248,11 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 	double* basis = C + (table ? 0 : (size_t)args.max_nen*64);  // [kSlab][qstride] records
 	double* qd = basis + (size_t)kSlab*qstride;             // [kSlab][kQd]
 
-	for (int a = tid; a < nen; a += kT) {
-		const int node = args.conn[b0 + a];
-		lconn[a] = node;
-		for (int f = 0; f < 4; ++f) nodal[8*a + f] = args.state[4*(size_t)node + f];
-		for (int f = 0; f < 3; ++f) nodal[8*a + 4 + f] = args.previous[4*(size_t)node + f];
-	}
-	if (!kMfma && !table)
-		for (int i = tid; i < nen*64; i += kT) C[i] = 0.0;
-	for (int i = tid; i < 4*kMaxBasis; i += kT) resid[i] = 0.0;
-	if (kMfma)  // padding functions nen..31 stay zero in every slab
-		for (int i = tid; i < kSlab*(32 - nen)*kBasisRec; i += kT) {
-			const int ql = i/((32 - nen)*kBasisRec), rest = i - ql*(32 - nen)*kBasisRec;
-			Brec(basis, qstride, nen, ql)[rest] = 0.0;
-		}
-	__syncthreads();
-	if (!kMfma && !table)
-		for (int a = tid; a < nen; a += kT)
-			for (int k = args.xoff[b0 + a]; k < args.xoff[b0 + a + 1]; ++k) C[64*a + args.xcol[k]] = args.xval[k];
-	__syncthreads();
-
 	const double* geo = args.geometry + (size_t)e*kGeomFields*kQ;
-
-	// pair tile of this thread: rows ra0, ra0+1 ; columns rb0, rb0+1
-	const int tiles_b = (nen + 1)/2;
-	const int tiles_a = (a1 - a0 + 1)/2;
-	const bool pair_active = !kMfma && kJac && tid < tiles_a*tiles_b;
-	const int ta = pair_active ? tid/tiles_b : 0;
-	const int tb = pair_active ? tid - ta*tiles_b : 0;
-	const int ra[2] = {a0 + 2*ta, min(a0 + 2*ta + 1, a1 - 1)};
-	const int rb[2] = {2*tb, min(2*tb + 1, nen - 1)};
-	double dg[4] = {}, uu[4][9] = {}, i3[4][3] = {}, t3[4][3] = {}, k33[4] = {};
-	doublex4 acc[4][2];  // MFMA path: [component of this wave][trial tile tj]
-	for (int c = 0; c < 4; ++c)
-		for (int t = 0; t < 2; ++t) acc[c][t] = doublex4{0.0, 0.0, 0.0, 0.0};
-
-	for (int slab = 0; slab < kQ/kSlab; ++slab) {
+	// P1 of a slab: basis values, gradients and Laplacians per (a, point), read
+	// from the precomputed table when there is one, else evaluated here
+	auto stage = [&](int slab) {
 		const int q0 = slab*kSlab;
-		// ---- P1: basis values, gradients and Laplacians per (a, point): read
-		// from the precomputed table when there is one, else evaluated here
 		if (kMfma)  // the point records read the slab's measure and J^-1 from LDS
 			for (int i = tid; i < kPointGeom*kSlab; i += kT) {
 				const int f = i/kSlab, ql = i - f*kSlab;
@@ -316,7 +282,49 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 				EvalBasis(C + 64*a, geo + q0 + ql, q0 + ql, Brec(basis, qstride, a, ql));
 			}
 		}
+	};
+
+	for (int a = tid; a < nen; a += kT) {
+		const int node = args.conn[b0 + a];
+		lconn[a] = node;
+		for (int f = 0; f < 4; ++f) nodal[8*a + f] = args.state[4*(size_t)node + f];
+		for (int f = 0; f < 3; ++f) nodal[8*a + 4 + f] = args.previous[4*(size_t)node + f];
+	}
+	if (kMfma) stage(0);  // the first slab's loads in flight with the nodal gather
+	if (!kMfma && !table)
+		for (int i = tid; i < nen*64; i += kT) C[i] = 0.0;
+	for (int i = tid; i < 4*kMaxBasis; i += kT) resid[i] = 0.0;
+	if (kMfma)  // padding functions nen..31 stay zero in every slab
+		for (int i = tid; i < kSlab*(32 - nen)*kBasisRec; i += kT) {
+			const int ql = i/((32 - nen)*kBasisRec), rest = i - ql*(32 - nen)*kBasisRec;
+			Brec(basis, qstride, nen, ql)[rest] = 0.0;
+		}
+	__syncthreads();
+	if (!kMfma && !table) {
+		for (int a = tid; a < nen; a += kT)
+			for (int k = args.xoff[b0 + a]; k < args.xoff[b0 + a + 1]; ++k) C[64*a + args.xcol[k]] = args.xval[k];
 		__syncthreads();
+	}
+
+	// pair tile of this thread: rows ra0, ra0+1 ; columns rb0, rb0+1
+	const int tiles_b = (nen + 1)/2;
+	const int tiles_a = (a1 - a0 + 1)/2;
+	const bool pair_active = !kMfma && kJac && tid < tiles_a*tiles_b;
+	const int ta = pair_active ? tid/tiles_b : 0;
+	const int tb = pair_active ? tid - ta*tiles_b : 0;
+	const int ra[2] = {a0 + 2*ta, min(a0 + 2*ta + 1, a1 - 1)};
+	const int rb[2] = {2*tb, min(2*tb + 1, nen - 1)};
+	double dg[4] = {}, uu[4][9] = {}, i3[4][3] = {}, t3[4][3] = {}, k33[4] = {};
+	doublex4 acc[4][2];  // MFMA path: [component of this wave][trial tile tj]
+	for (int c = 0; c < 4; ++c)
+		for (int t = 0; t < 2; ++t) acc[c][t] = doublex4{0.0, 0.0, 0.0, 0.0};
+
+	for (int slab = 0; slab < kQ/kSlab; ++slab) {
+		const int q0 = slab*kSlab;
+		if (!kMfma || slab > 0) {  // ---- P1 (the MFMA path staged slab 0 above)
+			stage(slab);
+			__syncthreads();
+		}
 		// ---- P2: fields at the points (kT/kSlab lanes per point), tau, fine scales
 		{
 			constexpr int kL = kT/kSlab;
@@ -623,6 +631,12 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 		constexpr int kPieces = (32*32*8 + kT - 1)/kT;  // per thread, nen <= 32
 		const long long pbase = args.pair_offsets[e];
 		const int total = nen*nen*8;
+		// the element's residual rows (thread a < nen: row a) ride along
+		const bool row = tid < nen;
+		double* nr = args.negres + 4*(size_t)lconn[row ? tid : 0];
+		double nrv[4];
+#pragma unroll
+		for (int c = 0; c < 4; ++c) nrv[c] = nr[c];
 		int slot[kPieces];
 #pragma unroll
 		for (int k = 0; k < kPieces; ++k) slot[k] = args.pair_block[pbase + (min(tid + k*kT, total - 1) >> 3)];
@@ -640,6 +654,8 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 				reinterpret_cast<double2*>(args.jac + (size_t)slot[k]*16)[it & 7] = v[k];
 			}
 		}
+		if (row)
+			for (int c = 0; c < 4; ++c) nr[c] = nrv[c] + resid[4*tid + c];
 	}
 	if (pair_active) {
 		const long long pbase = args.pair_offsets[e];
@@ -659,10 +675,11 @@ __global__ void __launch_bounds__(kMfma ? 512 : 256) AssembleNsKernel(AsmArgs ar
 				J[15] += k33[pr];
 			}
 	}
-	for (int a = a0 + tid; a < a1; a += kT) {
-		const size_t node = lconn[a];
-		for (int c = 0; c < 4; ++c) args.negres[4*node + c] += resid[4*(a - a0) + c];
-	}
+	if (!(kMfma && kJac))
+		for (int a = a0 + tid; a < a1; a += kT) {
+			const size_t node = lconn[a];
+			for (int c = 0; c < 4; ++c) args.negres[4*node + c] += resid[4*(a - a0) + c];
+		}
 }
 
 __global__ void DirichletMatrixKernel(int nodes, const int* __restrict__ ro, const int* __restrict__ diag,
