@@ -1,4 +1,7 @@
 """End-to-end Newton / backward-Euler steps on the GPU against the CPU oracle."""
+import math
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -98,3 +101,57 @@ def test_mass_balance_reported(ctx, tube):
             oflows[lab] += O.boundary_flow(m.element_bezier(e), m.dense_extraction(e), state[conn], int(f))
     assert abs(oflows[1] - q_in) <= 1e-12 * abs(q_in)
     assert abs(oflows[2] - q_out) <= 1e-12 * abs(q_out)
+
+
+GOLDEN_Y = Path(__file__).resolve().parent / "golden" / "ybif_small_pulsatile_steps.npz"
+
+
+def _pulsatile(t):
+    """Inflow scale Q(t)/Q0 = 1 + 0.6 sin 2 pi t + 0.2 sin 4 pi t (period 1 s)."""
+    return 1.0 + 0.6 * math.sin(2 * math.pi * t) + 0.2 * math.sin(4 * math.pi * t)
+
+
+@pytest.mark.parametrize("precision,warm", [(0, "previous"), (1, "mpe"), (1, "learned")])
+def test_y_bifurcation_pulsatile_steps_match_oracle(ctx, precision, warm):
+    """The Y-bifurcation family (ybif_small) under the pulsatile inflow at
+    dt = 5 ms: eight backward-Euler steps of the plain FP64 path and of the
+    mixed-precision path started from predicted states against the oracle's
+    states after steps 4 and 8 (tests/golden/ybif_small_pulsatile_steps.npz,
+    written by tests/golden/make_ybif_steps.py: exact-tangent Newton with
+    FGMRES + block Jacobi to 1e-13, about ten seconds a step on the CPU)."""
+    from tubularflowiga_amd import device
+    gold = np.load(GOLDEN_Y)
+    dt = float(gold["dt"])
+    case = casegen.workload("ybif_small")
+    assert int(gold["nodes"]) == case.mesh.nodes
+    mode = {"previous": device.TFI_WARM_PREVIOUS, "mpe": device.TFI_WARM_MPE,
+            "learned": device.TFI_WARM_LEARNED}[warm]
+    dm = device.DeviceMesh(ctx, case.mesh)
+    vmask, pmask, vt, pt = boundary.resolve_flow_boundaries(case)
+    flow = device.FlowSolver(dm, density=RHO, viscosity=MU, dt=dt, maximum_newton=40, nonlinear_rtol=1e-13,
+                             nonlinear_atol=1e-16, mass_rtol=1e30, linear_rtol=1e-12, precision=precision,
+                             warm_start=mode)
+    if warm == "learned":
+        flow.configure_warm_start(hidden=64, train_steps=20, learning_rate=1e-3, min_snapshots=2)
+    flow.set_state(boundary.initial_state(vmask, pmask, vt * _pulsatile(0.0), pt))
+    flow.set_boundary_faces(*boundary.boundary_faces(case.mesh))
+    used = []
+    try:
+        for k in range(1, 9):
+            t = k * dt
+            flow.begin_step(np.array([math.sin(2 * math.pi * t), math.cos(2 * math.pi * t)]))
+            flow.set_boundary(vmask, pmask, vt * _pulsatile(t), pt)
+            rep = flow.newton()
+            assert rep["converged"] == 1
+            used.append(rep["warm_start_used"])
+            flow.commit_step()
+            if f"state_{k}" in gold:
+                v, p = _rel(flow.state(), gold[f"state_{k}"])
+                print(f"precision={precision} warm={warm} step={k} newton={rep['newton_iterations']} "
+                      f"rel_l2 v={v:.3e} p={p:.3e}", flush=True)
+                assert v <= 1e-10 and p <= 1e-10
+        if warm != "previous":
+            assert any(used), "no predicted start was accepted"
+    finally:
+        flow.close()
+        dm.close()
