@@ -513,8 +513,8 @@ def main(argv=None):
                     help="only the transport assembly and 5 steps, advected by the inlet profile (for counter passes)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
-              "the multi-GPU decomposition is design-only", file=sys.stderr)
+        print(f"bench.py: the time step runs on one GPU only (asked for {a.gpus}); "
+              "the multi-GPU decomposition is not implemented", file=sys.stderr)
         return 2
     from tubularflowiga_amd import boundary, casegen, device
 
