@@ -249,8 +249,8 @@ public:
 			<< series_->path().string() << '\n';
 #else
 		(void)db; (void)output; (void)resume;
-		throw std::runtime_error("--visualization-format vtkhdf needs the HDF5 library, which this build of iga_hip "
-			"was made without; use --visualization-format vtu");
+		throw std::runtime_error("--visualization-format vtkhdf needs the HDF5 library, which iga_hip was compiled "
+			"without; use --visualization-format vtu");
 #endif
 	}
 	// the flow state [4N] as "velocity" and "pressure"
