@@ -56,6 +56,40 @@ __global__ void __launch_bounds__(256) BsrSpmvKernel(int rows, const int* __rest
 	}
 }
 
+// FP32 block values: a lane holds two rows of its block (32 bytes of values,
+// as a lane of the FP64 kernel holds one row), so that a wave keeps as many
+// value bytes in flight per pass as in FP64: 32 blocks per pass, lanes (2s,
+// 2s+1) rows (0,1) and (2,3) of block s.
+template <class X, class Y>
+__global__ void __launch_bounds__(256) BsrSpmvF32Kernel(int rows, const int* __restrict__ ro,
+	const int* __restrict__ cols, const float* __restrict__ vals, const X* __restrict__ x, Y* __restrict__ y)
+{
+	typedef float F4 __attribute__((ext_vector_type(4)));
+	const int lane = threadIdx.x & 63;
+	const int row = blockIdx.x*kRowsPerBlock + (threadIdx.x >> 6);
+	if (row >= rows) return;
+	const int sub = lane >> 1, h = lane & 1;
+	const int begin = ro[row], end = ro[row + 1];
+	double acc0 = 0.0, acc1 = 0.0;
+	for (int k = begin + sub; k < end; k += 32) {
+		const F4* a = reinterpret_cast<const F4*>(vals + (size_t)k*16 + 8*h);
+		const F4 r0 = a[0], r1 = a[1];
+		const X* xv = x + 4*(size_t)cols[k];
+		const double x0 = xv[0], x1 = xv[1], x2 = xv[2], x3 = xv[3];
+		acc0 += (double)r0.x*x0 + (double)r0.y*x1 + (double)r0.z*x2 + (double)r0.w*x3;
+		acc1 += (double)r1.x*x0 + (double)r1.y*x1 + (double)r1.z*x2 + (double)r1.w*x3;
+	}
+#pragma unroll
+	for (int o = 2; o < 64; o <<= 1) {
+		acc0 += __shfl_xor(acc0, o, 64);
+		acc1 += __shfl_xor(acc1, o, 64);
+	}
+	if (sub == 0) {
+		y[4*(size_t)row + 2*h] = static_cast<Y>(acc0);
+		y[4*(size_t)row + 2*h + 1] = static_cast<Y>(acc1);
+	}
+}
+
 // Inverse of the diagonal block of `node` into inv[16 node]; returns true when
 // a pivot fell below 1e-14 and the diagonal fallback was taken.
 __device__ inline bool InvertDiagonalBlock(int node, const int* __restrict__ diag, const double* __restrict__ vals,
@@ -169,15 +203,15 @@ void Spmv64(tfi_ctx* ctx, const BsrView& a, const double* values, const double* 
 
 void Spmv32(tfi_ctx* ctx, const BsrView& a, const float* values, const double* x, double* y)
 {
-	hipLaunchKernelGGL((BsrSpmvKernel<float, double, double, false>), dim3(SpmvBlocks(a.rows)), dim3(256), 0,
-		ctx->stream, a.rows, a.row_offsets, a.columns, values, x, nullptr, y);
+	hipLaunchKernelGGL((BsrSpmvF32Kernel<double, double>), dim3(SpmvBlocks(a.rows)), dim3(256), 0,
+		ctx->stream, a.rows, a.row_offsets, a.columns, values, x, y);
 	TFI_LAUNCH_CHECK("BsrSpmvKernel fp32");
 }
 
 void Spmv32f(tfi_ctx* ctx, const BsrView& a, const float* values, const float* x, float* y)
 {
-	hipLaunchKernelGGL((BsrSpmvKernel<float, float, float, false>), dim3(SpmvBlocks(a.rows)), dim3(256), 0,
-		ctx->stream, a.rows, a.row_offsets, a.columns, values, x, nullptr, y);
+	hipLaunchKernelGGL((BsrSpmvF32Kernel<float, float>), dim3(SpmvBlocks(a.rows)), dim3(256), 0,
+		ctx->stream, a.rows, a.row_offsets, a.columns, values, x, y);
 	TFI_LAUNCH_CHECK("BsrSpmvKernel fp32/fp32");
 }
 
