@@ -170,34 +170,34 @@ __global__ void __launch_bounds__(256) ResidualKernel(int rows, const int* __res
 // and its other entries of order one, so FP16's 11-bit significand and range
 // hold it (a level whose S has an entry above 60000 keeps the FP32 matrix).
 // Half the bytes of FP32 blocks per smoothing pass.
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-
-__device__ inline float RowPartialHalf(const int* __restrict__ ro, const int* __restrict__ cols,
-	const _Float16* __restrict__ s, const float* __restrict__ x, int row, int sub, int r)
+// (S x) of one block row, by one wave: a lane takes whole blocks (all four
+// rows, 32 bytes of S as two 16-byte loads, the 64 lanes 64 blocks a pass),
+// then a butterfly over the 64 lanes; lane l returns row l % 4.
+__device__ inline float RowProductHalf(const int* __restrict__ ro, const int* __restrict__ cols,
+	const _Float16* __restrict__ s, const float* __restrict__ x, int row, int lane)
 {
 	typedef float F4 __attribute__((ext_vector_type(4)));
+	typedef _Float16 H8 __attribute__((ext_vector_type(8)));
 	const int end = ro[row + 1];
-	float acc = 0.0f;
-	constexpr int kD = 4;  // blocks in flight per lane
-	for (int k0 = ro[row] + sub; k0 < end; k0 += 16*kD) {
-		int c[kD];
-#pragma unroll
-		for (int u = 0; u < kD; ++u) c[u] = k0 + 16*u < end ? cols[k0 + 16*u] : -1;
-		half4_t a[kD];
-		F4 xv[kD];
-#pragma unroll
-		for (int u = 0; u < kD; ++u)
-			if (c[u] >= 0) {
-				a[u] = *reinterpret_cast<const half4_t*>(s + (size_t)(k0 + 16*u)*16 + 4*r);
-				xv[u] = *reinterpret_cast<const F4*>(x + 4*(size_t)c[u]);
-			}
-#pragma unroll
-		for (int u = 0; u < kD; ++u)
-			if (c[u] >= 0)
-				acc += static_cast<float>(a[u].x)*xv[u].x + static_cast<float>(a[u].y)*xv[u].y
-					+ static_cast<float>(a[u].z)*xv[u].z + static_cast<float>(a[u].w)*xv[u].w;
+	float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+	for (int k = ro[row] + lane; k < end; k += 64) {
+		const H8* blk = reinterpret_cast<const H8*>(s + (size_t)k*16);
+		const H8 lo = blk[0], hi = blk[1];  // rows 0, 1 | rows 2, 3
+		const F4 xv = *reinterpret_cast<const F4*>(x + 4*(size_t)cols[k]);
+		a0 += (float)lo[0]*xv.x + (float)lo[1]*xv.y + (float)lo[2]*xv.z + (float)lo[3]*xv.w;
+		a1 += (float)lo[4]*xv.x + (float)lo[5]*xv.y + (float)lo[6]*xv.z + (float)lo[7]*xv.w;
+		a2 += (float)hi[0]*xv.x + (float)hi[1]*xv.y + (float)hi[2]*xv.z + (float)hi[3]*xv.w;
+		a3 += (float)hi[4]*xv.x + (float)hi[5]*xv.y + (float)hi[6]*xv.z + (float)hi[7]*xv.w;
 	}
-	return acc;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		a0 += __shfl_xor(a0, o, 64);
+		a1 += __shfl_xor(a1, o, 64);
+		a2 += __shfl_xor(a2, o, 64);
+		a3 += __shfl_xor(a3, o, 64);
+	}
+	const int r = lane & 3;
+	return r == 0 ? a0 : r == 1 ? a1 : r == 2 ? a2 : a3;
 }
 
 // S = D^-1 A in FP16 (D^-1 from the FP64 block inverses) and D in FP32; flag
@@ -238,13 +238,7 @@ __global__ void __launch_bounds__(256) ChebyshevScaledKernel(int rows, const int
 	if (row >= rows) return;
 	const int sub = lane >> 2, r = lane & 3;
 	float acc = 0.0f;
-	if (xin) {
-		acc = RowPartialHalf(ro, cols, s, xin, row, sub, r);
-		acc += __shfl_xor(acc, 4, 64);
-		acc += __shfl_xor(acc, 8, 64);
-		acc += __shfl_xor(acc, 16, 64);
-		acc += __shfl_xor(acc, 32, 64);
-	}
+	if (xin) acc = RowProductHalf(ro, cols, s, xin, row, lane);
 	if (sub == 0) {
 		const size_t i = 4*(size_t)row + r;
 		const float* iv = inv + (size_t)row*16 + 4*r;
@@ -266,11 +260,7 @@ __global__ void __launch_bounds__(256) ResidualScaledKernel(int rows, const int*
 	const int row = blockIdx.x*4 + (threadIdx.x >> 6);
 	if (row >= rows) return;
 	const int sub = lane >> 2, r = lane & 3;
-	float acc = RowPartialHalf(ro, cols, s, x, row, sub, r);
-	acc += __shfl_xor(acc, 4, 64);
-	acc += __shfl_xor(acc, 8, 64);
-	acc += __shfl_xor(acc, 16, 64);
-	acc += __shfl_xor(acc, 32, 64);
+	const float acc = RowProductHalf(ro, cols, s, x, row, lane);
 	const float* iv = inv + (size_t)row*16 + 4*r;
 	const float* bv = b + 4*(size_t)row;
 	const float t = iv[0]*bv[0] + iv[1]*bv[1] + iv[2]*bv[2] + iv[3]*bv[3] - acc;
