@@ -20,8 +20,8 @@ namespace tfi {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPerThread = 2;
-constexpr int kGroup = 4;  // basis vectors per multi-dot pass
+constexpr int kPerThread = 8;  // vector entries per thread: eight 8-byte loads per basis vector in flight
+constexpr int kGroup = 8;      // basis vectors per multi-dot pass (one set of wave reductions per pass)
 
 __device__ inline double WaveSum(double v)
 {
@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(kThreads) MultiDotKernel(const double* __restr
 	}
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	for (int g = 0; g < count; g += kGroup) {
-		double acc[kGroup] = {0, 0, 0, 0};
+		double acc[kGroup] = {};
 #pragma unroll
 		for (int v = 0; v < kGroup; ++v) {
 			if (g + v >= count) break;
@@ -93,16 +93,32 @@ __global__ void __launch_bounds__(kThreads) MultiAxpyKernel(const double* __rest
 	__shared__ double lds[4];
 	for (int i = threadIdx.x; i < count; i += kThreads) c[i] = sign*coef[i];
 	__syncthreads();
-	const long long base = (long long)blockIdx.x*kThreads*kPerThread;
+	const long long base = (long long)blockIdx.x*kThreads*kPerThread + threadIdx.x;
+	// each entry accumulates over the vectors in order; the thread's entries
+	// are independent, so their loads of one vector are in flight together
+	double acc[kPerThread];
+#pragma unroll
+	for (int t = 0; t < kPerThread; ++t) {
+		const long long i = base + (long long)t*kThreads;
+		acc[t] = i < n ? w[i] : 0.0;
+	}
+	for (int v = 0; v < count; ++v) {
+		const double cv = c[v];
+		const double* Vv = V + (long long)v*ld;
+#pragma unroll
+		for (int t = 0; t < kPerThread; ++t) {
+			const long long i = base + (long long)t*kThreads;
+			if (i < n) acc[t] += cv*Vv[i];
+		}
+	}
 	double sq = 0.0;
 #pragma unroll
 	for (int t = 0; t < kPerThread; ++t) {
-		const long long i = base + (long long)t*kThreads + threadIdx.x;
-		if (i >= n) break;
-		double acc = w[i];
-		for (int v = 0; v < count; ++v) acc += c[v]*V[(long long)v*ld + i];
-		w[i] = acc;
-		sq += acc*acc;
+		const long long i = base + (long long)t*kThreads;
+		if (i < n) {
+			w[i] = acc[t];
+			sq += acc[t]*acc[t];
+		}
 	}
 	if (norm_partial) {
 		sq = WaveSum(sq);
