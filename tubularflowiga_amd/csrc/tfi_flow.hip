@@ -1147,7 +1147,8 @@ int tfi_amg_create(tfi_mesh* m, int precision, int smoother, tfi_amg** out)
 	return Guard(m->ctx, [&] {
 		if (!out) throw Error("tfi_amg_create needs an output handle");
 		if (precision != TFI_FP64 && precision != TFI_MIXED) throw Error("unknown precision");
-		if (smoother < TFI_SMOOTH_GAUSS_SEIDEL || smoother > TFI_SMOOTH_AUTO) throw Error("unknown AMG smoother");
+		if (smoother < TFI_SMOOTH_JACOBI || smoother > TFI_SMOOTH_AUTO)
+			throw Error("unknown AMG smoother " + std::to_string(smoother));
 		auto a = std::make_unique<tfi_amg>();
 		a->mesh = m;
 		a->precision = precision;
