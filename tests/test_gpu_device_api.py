@@ -186,7 +186,7 @@ def _gmres(matvec, precondition, b, rtol, restart=60, maxit=600):
     return x, its
 
 
-AMG_VARIANTS = [(0, 3), (0, 2), (1, 1), (1, 2)]   # (precision, smoother): FP64 AUTO / Chebyshev, mixed GS / Chebyshev
+AMG_VARIANTS = [(0, 3), (0, 2), (0, 0), (1, 1), (1, 2)]   # (precision, smoother): FP64 AUTO / Chebyshev / Jacobi, mixed GS / Chebyshev
 
 
 @pytest.mark.parametrize("precision,smoother", AMG_VARIANTS)
