@@ -247,7 +247,6 @@ inline double SpmvBytes(const BsrView& a, int value_bytes)
 }
 void Spmv64(tfi_ctx* ctx, const BsrView& a, const double* values, const double* x, double* y);
 void Spmv32(tfi_ctx* ctx, const BsrView& a, const float* values, const double* x, double* y);
-void Spmv32f(tfi_ctx* ctx, const BsrView& a, const float* values, const float* x, float* y);
 // r = b - A x
 void Residual64(tfi_ctx* ctx, const BsrView& a, const double* values, const double* x, const double* b, double* r);
 int BlockInverse64(tfi_ctx* ctx, const BsrView& a, const double* values, double* inverse);

@@ -205,14 +205,7 @@ void Spmv32(tfi_ctx* ctx, const BsrView& a, const float* values, const double* x
 {
 	hipLaunchKernelGGL((BsrSpmvF32Kernel<double, double>), dim3(SpmvBlocks(a.rows)), dim3(256), 0,
 		ctx->stream, a.rows, a.row_offsets, a.columns, values, x, y);
-	TFI_LAUNCH_CHECK("BsrSpmvKernel fp32");
-}
-
-void Spmv32f(tfi_ctx* ctx, const BsrView& a, const float* values, const float* x, float* y)
-{
-	hipLaunchKernelGGL((BsrSpmvF32Kernel<float, float>), dim3(SpmvBlocks(a.rows)), dim3(256), 0,
-		ctx->stream, a.rows, a.row_offsets, a.columns, values, x, y);
-	TFI_LAUNCH_CHECK("BsrSpmvKernel fp32/fp32");
+	TFI_LAUNCH_CHECK("BsrSpmvF32Kernel");
 }
 
 void Residual64(tfi_ctx* ctx, const BsrView& a, const double* values, const double* x, const double* b, double* r)
