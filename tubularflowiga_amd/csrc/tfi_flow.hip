@@ -1153,7 +1153,7 @@ int tfi_amg_create(tfi_mesh* m, int precision, int smoother, tfi_amg** out)
 		a->mesh = m;
 		a->precision = precision;
 		AmgOptions o;
-		// without a time step to judge by, AUTO takes Gauss-Seidel (any matrix)
+		// without a time step to decide by, AUTO takes Gauss-Seidel (any matrix)
 		o.smoother = smoother == TFI_SMOOTH_AUTO ? TFI_SMOOTH_GAUSS_SEIDEL : smoother;
 		o.half_smoother = precision == TFI_MIXED && o.smoother == TFI_SMOOTH_CHEBYSHEV;
 		a->amg = std::make_unique<Amg>(m, precision, o);

@@ -186,7 +186,7 @@ std::vector<std::array<std::int32_t, 6>> FaceLabels(const ControlMesh& m)
 			std::sort(en.key.begin(), en.key.end());
 			faces.push_back(en);
 		}
-	// grouped by key in ascending key order (the order in which faces are judged),
+	// grouped by key in ascending key order (the order in which faces are checked),
 	// each group in cell-then-face order
 	std::stable_sort(faces.begin(), faces.end(), [](const Entry& a, const Entry& b) { return a.key < b.key; });
 	std::vector<std::array<std::int32_t, 6>> out(m.hexes.size());
