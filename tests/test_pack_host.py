@@ -336,8 +336,13 @@ def test_pack_of_a_generated_case_is_its_database(tmp_path, pack, name):
                           [*lo, (case.control_points.max(0) - lo).min(), 1.0])
     # the drop-in's host input path reads the packed database as it reads the
     # generator's: the same boundary data, wall trace and flow records
+    # (also packed for three ranks: the single-GPU driver reads every element)
+    (d / "bzmeshinfo.txt.epart.3").write_text("".join(f"{e % 3}\n" for e in range(E)))
+    r = _run(pack, d, 3, tmp_path / "packed3.ntiga")
+    assert r.returncode == 0, r.stderr
     cli = ROOT / "tubularflowiga_amd" / "iga_hip"
     checks = [subprocess.run([str(cli), "case-check", str(db), str(d)], capture_output=True, text=True, timeout=120)
-              for db in (tmp_path / "packed.ntiga", d / "case.ntiga")]
+              for db in (tmp_path / "packed.ntiga", tmp_path / "packed3.ntiga", d / "case.ntiga")]
     assert all(c.returncode == 0 for c in checks), [c.stderr for c in checks]
-    assert checks[0].stdout == checks[1].stdout and "case_check nodes=" in checks[0].stdout
+    assert checks[0].stdout == checks[2].stdout and checks[1].stdout == checks[2].stdout
+    assert "case_check nodes=" in checks[0].stdout
