@@ -50,3 +50,19 @@ def test_product_path_refuses_to_run_without_the_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_abi, "LIB_PATH", tmp_path / "missing.so", raising=False)
     with pytest.raises((OSError, RuntimeError)):
         _abi.load()
+
+
+@pytest.mark.skipif(not LIB.exists(), reason="libtfi_hip.so is not built (run __graft_entry__.build())")
+def test_amg_handle_entry_points_refuse_null_handles():
+    """The stand-alone AMG entry points return TFI_ERROR for a null mesh or
+    handle (no device call is made)."""
+    from tubularflowiga_amd import _abi
+    lib = _abi.load()
+    out = ctypes.c_void_p()
+    assert lib.tfi_amg_create(None, 0, 3, ctypes.byref(out)) == 1
+    assert lib.tfi_amg_setup(None, None) == 1
+    assert lib.tfi_amg_update(None, None) == 1
+    assert lib.tfi_amg_apply(None, None, None) == 1
+    n = ctypes.c_int(-1)
+    assert lib.tfi_amg_levels(None, 0, ctypes.byref(n), None, None, None) == 1 and n.value == -1
+    lib.tfi_amg_destroy(None)

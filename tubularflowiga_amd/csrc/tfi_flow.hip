@@ -1144,6 +1144,7 @@ int tfi_bench_spmv(tfi_mesh* m, int precision, int repetitions, double* millisec
 // AMG handle and learned-predictor training (C-ABI)
 int tfi_amg_create(tfi_mesh* m, int precision, int smoother, tfi_amg** out)
 {
+	if (!m) return TFI_ERROR;
 	return Guard(m->ctx, [&] {
 		if (!out) throw Error("tfi_amg_create needs an output handle");
 		if (precision != TFI_FP64 && precision != TFI_MIXED) throw Error("unknown precision");
@@ -1168,11 +1169,13 @@ void tfi_amg_destroy(tfi_amg* a)
 
 int tfi_amg_setup(tfi_amg* a, const double* values)
 {
+	if (!a) return TFI_ERROR;
 	return Guard(a->mesh->ctx, [&] { AmgNumeric(a, values); });
 }
 
 int tfi_amg_update(tfi_amg* a, const double* values)
 {
+	if (!a) return TFI_ERROR;
 	return Guard(a->mesh->ctx, [&] {
 		if (!a->ready) throw Error("tfi_amg_update needs a hierarchy from tfi_amg_setup");
 		AmgNumeric(a, values);
@@ -1181,6 +1184,7 @@ int tfi_amg_update(tfi_amg* a, const double* values)
 
 int tfi_amg_apply(tfi_amg* a, const double* v, double* z)
 {
+	if (!a) return TFI_ERROR;
 	return Guard(a->mesh->ctx, [&] {
 		if (!a->ready) throw Error("tfi_amg_apply needs a hierarchy from tfi_amg_setup");
 		if (!v || !z) throw Error("tfi_amg_apply needs v and z");
@@ -1191,6 +1195,7 @@ int tfi_amg_apply(tfi_amg* a, const double* v, double* z)
 
 int tfi_amg_levels(tfi_amg* a, int capacity, int* levels, int* rows, long long* block_nonzeros, int* colors)
 {
+	if (!a) return TFI_ERROR;
 	return Guard(a->mesh->ctx, [&] {
 		if (!a->ready) throw Error("tfi_amg_levels needs a hierarchy from tfi_amg_setup");
 		std::vector<int> r, c;
